@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X XPBD time step on the 1M-tet cantilever beam.
+
+One "step" is one timestep_t::step() call (one frame = `substeps` substeps of `iterations`
+Gauss-Seidel sweeps over the 997,920 stable neo-Hookean tets, contacts against the ground plane and
+16 spheres). The timed region replays the frame as a hipGraph; inputs are resident in HBM.
+`value` is substeps per second on one GPU.
+
+After the timed region a second, instrumented region (HIP events around every launch on the
+library's stream) measures the average duration of the tet projection kernel for the roofline
+figure. The CPU baseline times the FP64 oracle in the reference's insertion order on a bounded
+sample (one frame) of the same scene.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "soft-body-simulator_amd"))
+
+METRIC = "substeps/s at 1M tets to equal constraint residual, learned+batched vs plain"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes of one tet projection: indices 16 + DmInv 36 + V0 4 + per-tet material 8
+# + multiplier read/write 8 + 4 gathers of (position 12 + inverse mass 4) + 4 scatters of 12.
+BYTES_PER_TET = 184
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(scene):
+    """FP64 oracle, reference (insertion) order, one thread, one frame of the same scene."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    t0 = time.perf_counter()
+    sim = O.Simulation()
+    scene.apply(sim)
+    sim.set_order(False)
+    setup = time.perf_counter() - t0
+    log(f"cpu baseline: scene built in {setup:.1f}s, timing one frame ({scene.substeps} substeps)")
+    t0 = time.perf_counter()
+    sim.step(scene.dt, scene.iterations, scene.substeps)
+    el = time.perf_counter() - t0
+    return {"value": scene.substeps / el, "unit": "substeps/s", "cores": 1, "kind": "port",
+            "sample": f"1 frame = {scene.substeps} substeps x {scene.iterations} iterations of {scene.name} "
+                      f"({scene.num_tets} tets), FP64 oracle in reference insertion order, single thread, "
+                      f"{el:.1f}s on {cpu_model()}",
+            "projections_per_s": scene.num_tets * scene.iterations * scene.substeps / el}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30, help="timed frames")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed frames")
+    ap.add_argument("--profile-steps", type=int, default=3, help="frames of the instrumented region")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="cfg1_1m", choices=["cfg1_1m", "cfg1_6m", "cfg0"])
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the time step runs on a single MI355X in this build; "
+              "multi-GPU sharding is a design (see DESIGN.md), not a measured path.", file=sys.stderr)
+        sys.exit(2)
+
+    import sbs_amd as S
+    from sbs_amd import scenes
+
+    scene = {"cfg1_1m": scenes.cfg1_1m, "cfg1_6m": scenes.cfg1_6m, "cfg0": scenes.cfg0}[args.workload]()
+    log(f"workload {scene.name}: {scene.num_vertices} vertices, {scene.num_tets} tets")
+    t0 = time.perf_counter()
+    sim = scenes.build_device(scene)
+    _, ncolors = sim.group_colors(0)
+    log(f"committed in {time.perf_counter() - t0:.1f}s, {ncolors} colors")
+    kind = {"green": S._lib.KERNEL_GREEN, "neohookean": S._lib.KERNEL_NEOHOOKEAN}[scene.groups[0].kind]
+
+    sim.use_graphs(True)
+    for _ in range(args.warmup):
+        sim.step(scene.dt, scene.iterations, scene.substeps)
+    sim.synchronize()
+    log(f"warmup done ({args.warmup} frames); timing {args.steps} frames")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.step(scene.dt, scene.iterations, scene.substeps)
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    substeps_per_s = args.steps * scene.substeps / elapsed
+    log(f"timed region {elapsed:.3f}s -> {substeps_per_s:.1f} substeps/s")
+
+    # instrumented region: events around every launch, stream launches (no graph)
+    sim.use_graphs(False)
+    sim.set_timing(True)
+    sim.reset_stats()
+    for _ in range(args.profile_steps):
+        sim.step(scene.dt, scene.iterations, scene.substeps)
+    launches, ms, units = sim.kernel_stats(kind)
+    per_kernel = {}
+    for k, name in enumerate(S.KERNEL_NAMES):
+        n, t, u = sim.kernel_stats(k)
+        if n:
+            per_kernel[name] = {"launches": n, "avg_us": 1e3 * t / n, "share": 0.0}
+    total = sum(v["avg_us"] * v["launches"] for v in per_kernel.values())
+    for v in per_kernel.values():
+        v["share"] = round(v["avg_us"] * v["launches"] / total, 4)
+        v["avg_us"] = round(v["avg_us"], 3)
+    sim.set_timing(False)
+    avg_launch_s = ms / 1e3 / launches
+    bytes_per_launch = BYTES_PER_TET * units / launches
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    log(f"projection kernel: {launches} launches, avg {1e6 * avg_launch_s:.2f} us, {units / launches:.0f} tets/launch, "
+        f"{achieved:.1f} GB/s algorithmic")
+    # residual after the last substep (FP64), a correctness anchor for later comparisons
+    res = float(sum(sim.residual(0, scene.dt / scene.substeps)) ** 0.5)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(scene)
+        log(f"cpu baseline {cpu['value']:.3f} substeps/s")
+
+    out = {
+        "metric": METRIC,
+        "value": round(substeps_per_s, 2),
+        "unit": "substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": scene.name,
+            "solver": "plain GPU XPBD (colored Gauss-Seidel, no learned warm start)",
+            "tets": scene.num_tets,
+            "vertices": scene.num_vertices,
+            "substeps_per_step": scene.substeps,
+            "iterations": scene.iterations,
+            "colors": ncolors,
+            "obstacles": f"ground plane + {len(scene.spheres)} spheres",
+            "launch": "hipGraph per frame",
+            "parallelism": "single GPU",
+        },
+        "constraint_projections_per_s": round(substeps_per_s * scene.iterations * scene.num_tets, 1),
+        "residual_l2_last_substep": res,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"k_{scene.groups[0].kind}",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_unit": BYTES_PER_TET,
+            "units_per_launch": round(units / launches, 1),
+            "avg_launch_us": round(1e6 * avg_launch_s, 3),
+        },
+        "kernels": per_kernel,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,356 @@
+// C entry points of the FP64 oracle, loaded with ctypes by tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg.
+//
+// TEST INFRASTRUCTURE ONLY: the oracle is the checker, never the thing measured as the product or
+// shipped in it.
+#include <cstring>
+#include <stdexcept>
+
+#include "mesh.h"
+#include "xpbd.h"
+
+using namespace oracle;
+
+extern "C" {
+
+void* orc_create() { return new simulation_t(); }
+void orc_destroy(void* h) { delete static_cast<simulation_t*>(h); }
+
+void orc_set_params(void* h, double gx, double gy, double gz, int gravity_is_force, double collision_compliance,
+                    double collision_damping)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    s->gravity = {gx, gy, gz};
+    s->gravity_is_force = gravity_is_force != 0;
+    s->collision_compliance = collision_compliance;
+    s->collision_damping = collision_damping;
+}
+
+uint32_t orc_add_body(void* h, const double* x0, const double* mass, uint32_t n)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    std::vector<particle_t> ps;
+    ps.reserve(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        particle_t p({x0[3 * i], x0[3 * i + 1], x0[3 * i + 2]});
+        if (mass) p.m = mass[i];
+        ps.push_back(p);
+    }
+    s->particles.push_back(std::move(ps));
+    s->surface.emplace_back();
+    return uint32_t(s->particles.size() - 1);
+}
+
+// Overwrites x (and xi, xn, which equal x between frames) and/or v of a body.
+void orc_set_body_state(void* h, uint32_t body, const double* x, const double* v)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    auto& P = s->particles.at(body);
+    for (std::size_t i = 0; i < P.size(); ++i) {
+        if (x) P[i].x = P[i].xi = P[i].xn = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+        if (v) P[i].v = {v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+    }
+}
+
+void orc_set_surface(void* h, uint32_t body, const uint32_t* map, uint32_t n)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    s->surface.at(body).assign(map, map + n);
+}
+
+static int add_group(simulation_t* s, int kind, std::size_t first)
+{
+    group_t g;
+    g.kind = kind;
+    g.first = first;
+    g.count = s->constraints.size() - first;
+    s->groups.push_back(g);
+    s->order_dirty = true;
+    return int(s->groups.size() - 1);
+}
+
+int orc_add_green(void* h, uint32_t body, const uint32_t* tets, uint32_t n, double alpha, double beta,
+                  double young, double poisson)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    std::size_t const first = s->constraints.size();
+    for (uint32_t t = 0; t < n; ++t)
+        s->constraints.push_back(
+            std::make_unique<green_constraint_t>(alpha, beta, *s, body, tets + 4 * t, young, poisson));
+    return add_group(s, 0, first);
+}
+
+int orc_add_neohookean(void* h, uint32_t body, const uint32_t* tets, uint32_t n, double beta, double young,
+                       double poisson)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    std::size_t const first = s->constraints.size();
+    for (uint32_t t = 0; t < n; ++t)
+        s->constraints.push_back(
+            std::make_unique<neohookean_constraint_t>(beta, *s, body, tets + 4 * t, young, poisson));
+    return add_group(s, 1, first);
+}
+
+int orc_add_distance(void* h, const uint32_t* b1v1b2v2, uint32_t n, double alpha, double beta)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    std::size_t const first = s->constraints.size();
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t const* e = b1v1b2v2 + 4 * i;
+        s->constraints.push_back(std::make_unique<distance_constraint_t>(alpha, beta, *s, e[0], e[1], e[2], e[3]));
+    }
+    return add_group(s, 2, first);
+}
+
+void orc_add_plane(void* h, const double n[3], const double point[3], const double vmin[3], const double vmax[3])
+{
+    auto* s = static_cast<simulation_t*>(h);
+    sdf_t f;
+    f.kind = 0;
+    f.n = {n[0], n[1], n[2]};
+    f.offset = -dot(f.n, vec3{point[0], point[1], point[2]});
+    f.vmin = {vmin[0], vmin[1], vmin[2]};
+    f.vmax = {vmax[0], vmax[1], vmax[2]};
+    s->sdfs.push_back(f);
+}
+
+void orc_add_sphere(void* h, const double c[3], double r)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    sdf_t f;
+    f.kind = 1;
+    f.c = {c[0], c[1], c[2]};
+    f.r = r;
+    f.vmin = {c[0] - r, c[1] - r, c[2] - r};
+    f.vmax = {c[0] + r, c[1] + r, c[2] + r};
+    s->sdfs.push_back(f);
+}
+
+static void constraint_vertices(simulation_t const& s, group_t const& g, std::vector<uint64_t>& keys, uint32_t* arity)
+{
+    keys.clear();
+    if (g.kind == 2) {
+        *arity = 2;
+        for (std::size_t i = 0; i < g.count; ++i) {
+            auto const* d = static_cast<distance_constraint_t const*>(s.constraints[g.first + i].get());
+            keys.push_back((uint64_t(d->b1) << 32) | d->v1);
+            keys.push_back((uint64_t(d->b2) << 32) | d->v2);
+        }
+    } else {
+        *arity = 4;
+        for (std::size_t i = 0; i < g.count; ++i) {
+            constraint_t const* c = s.constraints[g.first + i].get();
+            uint32_t bi;
+            uint32_t const* v;
+            if (g.kind == 0) {
+                auto const* t = static_cast<green_constraint_t const*>(c);
+                bi = t->bi;
+                v = t->v;
+            } else {
+                auto const* t = static_cast<neohookean_constraint_t const*>(c);
+                bi = t->bi;
+                v = t->v;
+            }
+            for (int k = 0; k < 4; ++k) keys.push_back((uint64_t(bi) << 32) | v[k]);
+        }
+    }
+}
+
+// colored != 0 selects the colored visiting order (and computes the coloring of every group).
+void orc_set_order(void* h, int colored, uint32_t recolor_passes)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    s->colored_order = colored != 0;
+    s->recolor_passes = recolor_passes;
+    if (s->colored_order) {
+        std::vector<uint64_t> keys;
+        for (auto& g : s->groups) {
+            uint32_t arity;
+            constraint_vertices(*s, g, keys, &arity);
+            g.color.assign(g.count, 0);
+            g.num_colors = g.count ? greedy_color(uint32_t(g.count), arity, keys.data(), recolor_passes, g.color.data()) : 0;
+        }
+    }
+    s->order_dirty = true;
+}
+
+uint32_t orc_group_colors(void* h, uint32_t group, uint32_t* color_out)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    auto const& g = s->groups.at(group);
+    if (color_out && !g.color.empty()) std::memcpy(color_out, g.color.data(), g.count * sizeof(uint32_t));
+    return g.num_colors;
+}
+
+void orc_step(void* h, double dt, uint32_t iterations, uint32_t substeps)
+{
+    static_cast<simulation_t*>(h)->step(dt, iterations, substeps);
+}
+
+void orc_read_body(void* h, uint32_t body, double* x, double* v)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    auto const& P = s->particles.at(body);
+    for (std::size_t i = 0; i < P.size(); ++i) {
+        if (x) {
+            x[3 * i] = P[i].x.x;
+            x[3 * i + 1] = P[i].x.y;
+            x[3 * i + 2] = P[i].x.z;
+        }
+        if (v) {
+            v[3 * i] = P[i].v.x;
+            v[3 * i + 1] = P[i].v.y;
+            v[3 * i + 2] = P[i].v.z;
+        }
+    }
+}
+
+uint32_t orc_detect_contacts(void* h)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    s->detect_contacts();
+    return uint32_t(s->contacts.size());
+}
+
+// ids: 4 per contact (body, surface vertex, body vertex, sdf); qn: 6 per contact (q, n).
+uint32_t orc_read_contacts(void* h, uint32_t* ids, double* qn, uint32_t cap)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    uint32_t const n = uint32_t(std::min<std::size_t>(cap, s->contacts.size()));
+    for (uint32_t i = 0; i < n; ++i) {
+        auto const& c = s->contacts[i];
+        if (ids) {
+            ids[4 * i] = c.body;
+            ids[4 * i + 1] = c.surface_vertex;
+            ids[4 * i + 2] = c.vertex;
+            ids[4 * i + 3] = c.sdf;
+        }
+        if (qn) {
+            double const v[6] = {c.q.x, c.q.y, c.q.z, c.n.x, c.n.y, c.n.z};
+            std::memcpy(qn + 6 * i, v, sizeof v);
+        }
+    }
+    return uint32_t(s->contacts.size());
+}
+
+// Squared-residual sums of one group's constraints, per color (colored order) or in one bucket,
+// evaluated at the current positions with the multipliers of the last substep.
+void orc_residual(void* h, uint32_t group, double dt, double* per_color, uint32_t cap)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    auto const& g = s->groups.at(group);
+    for (uint32_t c = 0; c < cap; ++c) per_color[c] = 0;
+    for (std::size_t i = 0; i < g.count; ++i) {
+        uint32_t const c = g.color.empty() ? 0 : g.color[i];
+        if (c < cap) s->constraints[g.first + i]->residual_sq(*s, dt, &per_color[c]);
+    }
+}
+
+// out: kinetic, gravitational potential, elastic, momentum x/y/z.
+void orc_energy(void* h, double* out)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    double ke = 0, pe = 0, ee = 0, px = 0, py = 0, pz = 0;
+    for (auto const& body : s->particles)
+        for (auto const& p : body) {
+            double const m = p.m;
+            ke += 0.5 * m * norm2(p.v);
+            pe -= (s->gravity_is_force ? 1.0 : m) * dot(s->gravity, p.x);
+            px += m * p.v.x;
+            py += m * p.v.y;
+            pz += m * p.v.z;
+        }
+    for (auto const& c : s->constraints) ee += c->energy(*s);
+    double const r[6] = {ke, pe, ee, px, py, pz};
+    std::memcpy(out, r, sizeof r);
+}
+
+void orc_svd3(const double* F, double* U, double* sv, double* V)
+{
+    mat3 a, u, v;
+    for (int i = 0; i < 9; ++i) a.m[i / 3][i % 3] = F[i];
+    jacobi_svd3(a, u, sv, v);
+    for (int i = 0; i < 9; ++i) {
+        U[i] = u.m[i / 3][i % 3];
+        V[i] = v.m[i / 3][i % 3];
+    }
+}
+
+void orc_inverse3(const double* A, double* out, double* det)
+{
+    mat3 a;
+    for (int i = 0; i < 9; ++i) a.m[i / 3][i % 3] = A[i];
+    mat3 const r = inverse(a);
+    for (int i = 0; i < 9; ++i) out[i] = r.m[i / 3][i % 3];
+    *det = determinant(a);
+}
+
+// Green constraint value C = |V0| psi for a tet with rest positions x0 (12) at positions p (12).
+double orc_green_value(const double* x0, const double* p, double young, double poisson)
+{
+    simulation_t s;
+    uint32_t const vs[4] = {0, 1, 2, 3};
+    std::vector<particle_t> ps;
+    for (int i = 0; i < 4; ++i) ps.push_back(particle_t({x0[3 * i], x0[3 * i + 1], x0[3 * i + 2]}));
+    s.particles.push_back(ps);
+    green_constraint_t g(0.0, 0.0, s, 0, vs, young, poisson);
+    vec3 const q[4] = {{p[0], p[1], p[2]}, {p[3], p[4], p[5]}, {p[6], p[7], p[8]}, {p[9], p[10], p[11]}};
+    return g.evaluate(q);
+}
+
+// Green constraint value and forces f1..f4 (= -grad C, 12 doubles) for one tet.
+double orc_green_eval(const double* x0, const double* p, double young, double poisson, double* f)
+{
+    simulation_t s;
+    uint32_t const vs[4] = {0, 1, 2, 3};
+    std::vector<particle_t> ps;
+    for (int i = 0; i < 4; ++i) ps.push_back(particle_t({x0[3 * i], x0[3 * i + 1], x0[3 * i + 2]}));
+    s.particles.push_back(ps);
+    green_constraint_t g(0.0, 0.0, s, 0, vs, young, poisson);
+    vec3 const q[4] = {{p[0], p[1], p[2]}, {p[3], p[4], p[5]}, {p[6], p[7], p[8]}, {p[9], p[10], p[11]}};
+    vec3 fk[4];
+    double const C = g.evaluate_forces(q, fk);
+    for (int k = 0; k < 4; ++k) {
+        f[3 * k] = fk[k].x;
+        f[3 * k + 1] = fk[k].y;
+        f[3 * k + 2] = fk[k].z;
+    }
+    return C;
+}
+
+uint32_t orc_bar5(uint32_t w, uint32_t hh, uint32_t d, double* pos, uint32_t* tets)
+{
+    tet_mesh const m = bar5(w, hh, d);
+    if (pos) std::memcpy(pos, m.positions.data(), m.positions.size() * sizeof(double));
+    if (tets) std::memcpy(tets, m.tets.data(), m.tets.size() * sizeof(uint32_t));
+    return uint32_t(m.tets.size() / 4);
+}
+
+uint32_t orc_box6(uint32_t nx, uint32_t ny, uint32_t nz, double edge, const double* origin, double* pos, uint32_t* tets)
+{
+    tet_mesh const m = box6(nx, ny, nz, edge, origin);
+    if (pos) std::memcpy(pos, m.positions.data(), m.positions.size() * sizeof(double));
+    if (tets) std::memcpy(tets, m.tets.data(), m.tets.size() * sizeof(uint32_t));
+    return uint32_t(m.tets.size() / 4);
+}
+
+// Boundary extraction; call once with null outputs to get the sizes.
+void orc_boundary(uint32_t nv, const uint32_t* tets, uint32_t nt, uint32_t* s2t, uint32_t* tris, uint32_t* tri2face,
+                  uint32_t* nsurf, uint32_t* ntris)
+{
+    boundary const b = extract_boundary(nv, tets, nt);
+    *nsurf = uint32_t(b.surface_to_tet.size());
+    *ntris = uint32_t(b.triangle_to_face.size());
+    if (s2t) std::memcpy(s2t, b.surface_to_tet.data(), b.surface_to_tet.size() * sizeof(uint32_t));
+    if (tris) std::memcpy(tris, b.triangles.data(), b.triangles.size() * sizeof(uint32_t));
+    if (tri2face) std::memcpy(tri2face, b.triangle_to_face.data(), b.triangle_to_face.size() * sizeof(uint32_t));
+}
+
+uint32_t orc_greedy_color(uint32_t n, uint32_t arity, const uint64_t* verts, uint32_t passes, uint32_t* color)
+{
+    return greedy_color(n, arity, verts, passes, color);
+}
+
+uint32_t orc_hash32(uint32_t x) { return hash32(x); }
+
+}  // extern "C"

@@ -1,0 +1,251 @@
+"""ctypes bindings to the FP64 oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the oracle is a CPU restatement of the reference XPBD step and is used
+as the checker of the GPU path (tests/, __graft_entry__.smoke()) and as the timed CPU baseline of
+bench.py. Nothing in the product imports this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+_lib = None
+
+_d = ctypes.c_double
+_u32 = ctypes.c_uint32
+_i = ctypes.c_int
+_p = ctypes.c_void_p
+_pd = ctypes.POINTER(ctypes.c_double)
+_pu = ctypes.POINTER(ctypes.c_uint32)
+_pu64 = ctypes.POINTER(ctypes.c_uint64)
+
+_SIGS = {
+    "orc_create": (_p, []),
+    "orc_destroy": (None, [_p]),
+    "orc_set_params": (None, [_p, _d, _d, _d, _i, _d, _d]),
+    "orc_add_body": (_u32, [_p, _pd, _pd, _u32]),
+    "orc_set_body_state": (None, [_p, _u32, _pd, _pd]),
+    "orc_set_surface": (None, [_p, _u32, _pu, _u32]),
+    "orc_add_green": (_i, [_p, _u32, _pu, _u32, _d, _d, _d, _d]),
+    "orc_add_neohookean": (_i, [_p, _u32, _pu, _u32, _d, _d, _d]),
+    "orc_add_distance": (_i, [_p, _pu, _u32, _d, _d]),
+    "orc_add_plane": (None, [_p, _pd, _pd, _pd, _pd]),
+    "orc_add_sphere": (None, [_p, _pd, _d]),
+    "orc_set_order": (None, [_p, _i, _u32]),
+    "orc_group_colors": (_u32, [_p, _u32, _pu]),
+    "orc_step": (None, [_p, _d, _u32, _u32]),
+    "orc_read_body": (None, [_p, _u32, _pd, _pd]),
+    "orc_detect_contacts": (_u32, [_p]),
+    "orc_read_contacts": (_u32, [_p, _pu, _pd, _u32]),
+    "orc_residual": (None, [_p, _u32, _d, _pd, _u32]),
+    "orc_energy": (None, [_p, _pd]),
+    "orc_svd3": (None, [_pd, _pd, _pd, _pd]),
+    "orc_inverse3": (None, [_pd, _pd, _pd]),
+    "orc_green_value": (_d, [_pd, _pd, _d, _d]),
+    "orc_green_eval": (_d, [_pd, _pd, _d, _d, _pd]),
+    "orc_bar5": (_u32, [_u32, _u32, _u32, _pd, _pu]),
+    "orc_box6": (_u32, [_u32, _u32, _u32, _d, _pd, _pd, _pu]),
+    "orc_boundary": (None, [_u32, _pu, _u32, _pu, _pu, _pu, _pu, _pu]),
+    "orc_greedy_color": (_u32, [_u32, _u32, _pu64, _u32, _pu]),
+    "orc_hash32": (_u32, [_u32]),
+}
+
+
+def build():
+    """Compile the oracle with its Makefile (g++)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(_pd)
+
+
+def _up(a):
+    return None if a is None else a.ctypes.data_as(_pu)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+# ---------------------------------------------------------------- pure helpers
+
+def svd3(F):
+    F = f64(F).reshape(9)
+    U = np.zeros(9)
+    s = np.zeros(3)
+    V = np.zeros(9)
+    lib().orc_svd3(_dp(F), _dp(U), _dp(s), _dp(V))
+    return U.reshape(3, 3), s, V.reshape(3, 3)
+
+
+def inverse3(A):
+    A = f64(A).reshape(9)
+    out = np.zeros(9)
+    det = ctypes.c_double()
+    lib().orc_inverse3(_dp(A), _dp(out), ctypes.byref(det))
+    return out.reshape(3, 3), det.value
+
+
+def green_value(x0, p, young, poisson):
+    return lib().orc_green_value(_dp(f64(x0).reshape(12)), _dp(f64(p).reshape(12)), young, poisson)
+
+
+def green_eval(x0, p, young, poisson):
+    """(C, f) of one green constraint: C = |V0| psi, f (4, 3) = -grad C."""
+    f = np.zeros(12)
+    C = lib().orc_green_eval(_dp(f64(x0).reshape(12)), _dp(f64(p).reshape(12)), young, poisson, _dp(f))
+    return C, f.reshape(4, 3)
+
+
+def bar5(w, h, d):
+    nt = max(w - 1, 0) * max(h - 1, 0) * max(d - 1, 0) * 5
+    pos = np.zeros(w * h * d * 3)
+    tets = np.zeros(nt * 4, dtype=np.uint32)
+    lib().orc_bar5(w, h, d, _dp(pos), _up(tets))
+    return pos.reshape(-1, 3), tets.reshape(-1, 4)
+
+
+def box6(nx, ny, nz, edge, origin=(0.0, 0.0, 0.0)):
+    nv = (nx + 1) * (ny + 1) * (nz + 1)
+    pos = np.zeros(nv * 3)
+    tets = np.zeros(nx * ny * nz * 24, dtype=np.uint32)
+    lib().orc_box6(nx, ny, nz, edge, _dp(f64(origin)), _dp(pos), _up(tets))
+    return pos.reshape(-1, 3), tets.reshape(-1, 4)
+
+
+def boundary(nv, tets):
+    tets = u32(tets).reshape(-1)
+    ns, ntr = _u32(), _u32()
+    lib().orc_boundary(nv, _up(tets), len(tets) // 4, None, None, None, ctypes.byref(ns), ctypes.byref(ntr))
+    s2t = np.zeros(ns.value, dtype=np.uint32)
+    tris = np.zeros(ntr.value * 3, dtype=np.uint32)
+    t2f = np.zeros(ntr.value, dtype=np.uint32)
+    lib().orc_boundary(nv, _up(tets), len(tets) // 4, _up(s2t), _up(tris), _up(t2f), ctypes.byref(ns),
+                       ctypes.byref(ntr))
+    return s2t, tris.reshape(-1, 3), t2f
+
+
+def greedy_color(verts, passes):
+    """verts: (n, arity) uint64 vertex keys of each constraint."""
+    verts = np.ascontiguousarray(verts, dtype=np.uint64)
+    n, arity = verts.shape
+    color = np.zeros(n, dtype=np.uint32)
+    nc = lib().orc_greedy_color(n, arity, verts.ctypes.data_as(_pu64), passes, _up(color))
+    return color, nc
+
+
+def hash32(x):
+    return lib().orc_hash32(x)
+
+
+# ---------------------------------------------------------------- scene
+
+class Simulation:
+    """FP64 oracle scene; call sequence mirrors the GPU C-ABI (include/sbs_gpu.h)."""
+
+    def __init__(self):
+        self._L = lib()
+        self.h = self._L.orc_create()
+        self.bodies = []  # vertex counts
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._L.orc_destroy(self.h)
+            self.h = None
+
+    def set_params(self, gravity=(0.0, -9.81, 0.0), gravity_is_force=True, collision_compliance=1e-8,
+                   collision_damping=0.0):
+        self._L.orc_set_params(self.h, gravity[0], gravity[1], gravity[2], int(gravity_is_force),
+                               collision_compliance, collision_damping)
+
+    def add_body(self, x0, mass=None):
+        x0 = f64(x0).reshape(-1)
+        m = None if mass is None else f64(mass)
+        b = self._L.orc_add_body(self.h, _dp(x0), _dp(m), len(x0) // 3)
+        self.bodies.append(len(x0) // 3)
+        return b
+
+    def set_body_state(self, body, x=None, v=None):
+        x = None if x is None else f64(x).reshape(-1)
+        v = None if v is None else f64(v).reshape(-1)
+        self._L.orc_set_body_state(self.h, body, _dp(x), _dp(v))
+
+    def set_surface(self, body, s2t):
+        s2t = u32(s2t)
+        self._L.orc_set_surface(self.h, body, _up(s2t), len(s2t))
+
+    def add_green(self, body, tets, alpha, beta, young, poisson):
+        t = u32(tets).reshape(-1)
+        return self._L.orc_add_green(self.h, body, _up(t), len(t) // 4, alpha, beta, young, poisson)
+
+    def add_neohookean(self, body, tets, beta, young, poisson):
+        t = u32(tets).reshape(-1)
+        return self._L.orc_add_neohookean(self.h, body, _up(t), len(t) // 4, beta, young, poisson)
+
+    def add_distance(self, b1v1b2v2, alpha, beta):
+        e = u32(b1v1b2v2).reshape(-1)
+        return self._L.orc_add_distance(self.h, _up(e), len(e) // 4, alpha, beta)
+
+    def add_plane(self, normal, point, vmin=(-1e30,) * 3, vmax=(1e30,) * 3):
+        self._L.orc_add_plane(self.h, _dp(f64(normal)), _dp(f64(point)), _dp(f64(vmin)), _dp(f64(vmax)))
+
+    def add_sphere(self, center, radius):
+        self._L.orc_add_sphere(self.h, _dp(f64(center)), radius)
+
+    def set_order(self, colored, recolor_passes=2):
+        self._L.orc_set_order(self.h, int(colored), recolor_passes)
+
+    def group_colors(self, group, n):
+        c = np.zeros(n, dtype=np.uint32)
+        nc = self._L.orc_group_colors(self.h, group, _up(c))
+        return c, nc
+
+    def step(self, dt, iterations, substeps):
+        self._L.orc_step(self.h, dt, iterations, substeps)
+
+    def read_body(self, body):
+        n = self.bodies[body]
+        x = np.zeros(n * 3)
+        v = np.zeros(n * 3)
+        self._L.orc_read_body(self.h, body, _dp(x), _dp(v))
+        return x.reshape(-1, 3), v.reshape(-1, 3)
+
+    def detect_contacts(self):
+        n = self._L.orc_detect_contacts(self.h)
+        ids = np.zeros(n * 4, dtype=np.uint32)
+        qn = np.zeros(n * 6)
+        self._L.orc_read_contacts(self.h, _up(ids), _dp(qn), n)
+        return ids.reshape(-1, 4), qn.reshape(-1, 6)
+
+    def residual(self, group, dt, ncolors):
+        out = np.zeros(max(ncolors, 1))
+        self._L.orc_residual(self.h, group, dt, _dp(out), len(out))
+        return out
+
+    def energy(self):
+        out = np.zeros(6)
+        self._L.orc_energy(self.h, _dp(out))
+        return out

@@ -1,0 +1,456 @@
+// TEST INFRASTRUCTURE ONLY (see xpbd.h).
+#include "xpbd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+namespace oracle {
+
+namespace {
+
+mat3 rest_shape(simulation_t const& sim, uint32_t body, uint32_t const v[4])
+{
+    auto const& P = sim.particles.at(body);
+    vec3 const p4 = P.at(v[3]).x0;
+    return mat3::from_cols(P.at(v[0]).x0 - p4, P.at(v[1]).x0 - p4, P.at(v[2]).x0 - p4);
+}
+
+void lame(double young, double poisson, double* mu, double* lam)
+{
+    // green_constraint.cpp:45-46
+    *mu = young / (2. * (1 + poisson));
+    *lam = (young * poisson) / ((1 + poisson) * (1 - 2 * poisson));
+}
+
+// Green-strain constraint core (green_constraint.cpp:61-131): computes C = |V0| psi and the
+// forces f1..f4 (= -grad C) at positions p. Returns the weighted sum of squared gradients.
+struct green_eval {
+    double C, wsum;
+    vec3 f[4];
+};
+
+green_eval green_core(mat3 const& DmInv, double V0, double mu, double lam, vec3 const p[4], double const w[4])
+{
+    // signed_volume (:160-173)
+    mat3 const Dsv = mat3::from_cols(p[0] - p[3], p[1] - p[3], p[2] - p[3]);
+    double const Vsigned = (1. / 6.) * determinant(Dsv);
+    bool const is_V_positive = Vsigned >= 0.;
+    bool const is_V0_positive = V0 >= 0.;
+    bool const is_tet_inverted = (is_V_positive && !is_V0_positive) || (!is_V_positive && is_V0_positive);
+
+    mat3 const F = Dsv * DmInv;  // (:69-74)
+    mat3 U, Vm;
+    double s[3];
+    jacobi_svd3(F, U, s, Vm);  // (:81)
+    mat3 Fhat;
+    Fhat.m[0][0] = s[0];
+    Fhat.m[1][1] = s[1];
+    Fhat.m[2][2] = s[2];
+    if (is_tet_inverted) {  // (:92-96)
+        Fhat.m[2][2] = -Fhat.m[2][2];
+        U.set_col(2, -U.col(2));
+    }
+    double const min_singular_value = 0.577;  // (:99-102)
+    for (int i = 0; i < 3; ++i) Fhat.m[i][i] = std::max(Fhat.m[i][i], min_singular_value);
+
+    mat3 const I = mat3::identity();
+    mat3 const Ehat = 0.5 * (transpose(Fhat) * Fhat - I);  // (:104-106)
+    double const EhatTrace = trace(Ehat);
+    mat3 const Piolahat = Fhat * ((2. * mu * Ehat) + (lam * EhatTrace * I));
+
+    mat3 const E = U * Ehat * transpose(Vm);  // (:108-110)
+    double const Etrace = trace(E);
+    double const psi = mu * frob2(E) + 0.5 * lam * Etrace * Etrace;
+
+    mat3 const Piola = U * Piolahat * transpose(Vm);  // (:112)
+    double const V0abs = std::abs(V0);               // (:115-120)
+    mat3 const H = (-V0abs) * Piola * transpose(DmInv);
+    green_eval r;
+    r.f[0] = H.col(0);
+    r.f[1] = H.col(1);
+    r.f[2] = H.col(2);
+    r.f[3] = -(r.f[0] + r.f[1] + r.f[2]);
+    r.wsum = 0;
+    if (w)
+        r.wsum = w[0] * norm2(r.f[0]) + w[1] * norm2(r.f[1]) + w[2] * norm2(r.f[2]) + w[3] * norm2(r.f[3]);
+    r.C = V0abs * psi;  // (:133)
+    return r;
+}
+
+// One XPBD multiplier update with the reference's damping term (green_constraint.cpp:133-151,
+// distance_constraint.cpp:37-48). grad[k] is grad C at vertex k.
+double xpbd_delta(double C, double lagrange, double alpha, double beta, double dt, double wsum,
+                  double grad_dot_disp)
+{
+    double const dt2 = dt * dt;
+    double const alpha_tilde = alpha / dt2;
+    double const beta_tilde = beta * dt2;
+    double const gamma = alpha_tilde * beta_tilde / dt;
+    double const num = -(C + alpha_tilde * lagrange) + gamma * grad_dot_disp;
+    double const den = (1. + gamma) * wsum + alpha_tilde;
+    return num / den;
+}
+
+}  // namespace
+
+green_constraint_t::green_constraint_t(double alpha, double beta, simulation_t const& sim, uint32_t body,
+                                       uint32_t const vs[4], double young, double poisson)
+    : constraint_t(alpha, beta), bi(body)
+{
+    std::copy(vs, vs + 4, v);
+    mat3 const Dm = rest_shape(sim, body, v);  // green_constraint.cpp:38-44
+    DmInv = inverse(Dm);
+    V0 = (1. / 6.) * determinant(Dm);
+    lame(young, poisson, &mu, &lam);
+}
+
+double green_constraint_t::evaluate(vec3 const p[4], bool* early_out, double const w[4]) const
+{
+    green_eval const g = green_core(DmInv, V0, mu, lam, p, w);
+    if (early_out) *early_out = g.wsum < 1e-20;
+    return g.C;
+}
+
+double green_constraint_t::evaluate_forces(vec3 const p[4], vec3 f[4]) const
+{
+    green_eval const g = green_core(DmInv, V0, mu, lam, p, nullptr);
+    for (int k = 0; k < 4; ++k) f[k] = g.f[k];
+    return g.C;
+}
+
+void green_constraint_t::project_positions(simulation_t& sim, double dt)
+{
+    auto& P = sim.particles[bi];
+    particle_t* pp[4] = {&P[v[0]], &P[v[1]], &P[v[2]], &P[v[3]]};
+    vec3 const p[4] = {pp[0]->xi, pp[1]->xi, pp[2]->xi, pp[3]->xi};
+    double const w[4] = {pp[0]->invmass(), pp[1]->invmass(), pp[2]->invmass(), pp[3]->invmass()};
+    green_eval const g = green_core(DmInv, V0, mu, lam, p, w);
+    double constexpr epsilon = 1e-20;  // (:67, :130-131)
+    if (g.wsum < epsilon) return;
+    double gdd = 0;
+    for (int k = 0; k < 4; ++k) gdd += dot(g.f[k], pp[k]->xi - pp[k]->xn);  // (:140-144)
+    double const dl = xpbd_delta(g.C, lagrange, alpha, beta, dt, g.wsum, gdd);
+    lagrange += dl;  // (:152-157)
+    for (int k = 0; k < 4; ++k) pp[k]->xi += w[k] * (-g.f[k]) * dl;
+}
+
+void green_constraint_t::residual_sq(simulation_t const& sim, double dt, double* acc) const
+{
+    auto const& P = sim.particles[bi];
+    vec3 const p[4] = {P[v[0]].x, P[v[1]].x, P[v[2]].x, P[v[3]].x};
+    double const C = evaluate(p);
+    double const h = C + (alpha / (dt * dt)) * lagrange;
+    *acc += h * h;
+}
+
+double green_constraint_t::energy(simulation_t const& sim) const
+{
+    auto const& P = sim.particles[bi];
+    vec3 const p[4] = {P[v[0]].x, P[v[1]].x, P[v[2]].x, P[v[3]].x};
+    return evaluate(p);
+}
+
+neohookean_constraint_t::neohookean_constraint_t(double beta, simulation_t const& sim, uint32_t body,
+                                                 uint32_t const vs[4], double young, double poisson)
+    : constraint_t(0.0, beta), bi(body)
+{
+    std::copy(vs, vs + 4, v);
+    mat3 const Dm = rest_shape(sim, body, v);
+    DmInv = inverse(Dm);
+    V0 = (1. / 6.) * determinant(Dm);
+    lame(young, poisson, &mu, &lam);
+}
+
+namespace {
+
+// grad_x C for vertices 1..3 = (dC/dF) * (row k of DmInv)^T, vertex 4 = -sum.
+void tet_gradients(mat3 const& dCdF, mat3 const& DmInv, vec3 g[4])
+{
+    for (int k = 0; k < 3; ++k) g[k] = dCdF * DmInv.row(k);
+    g[3] = -(g[0] + g[1] + g[2]);
+}
+
+mat3 cofactor(mat3 const& F)
+{
+    vec3 const c0 = F.col(0), c1 = F.col(1), c2 = F.col(2);
+    return mat3::from_cols(cross(c1, c2), cross(c2, c0), cross(c0, c1));
+}
+
+}  // namespace
+
+void neohookean_constraint_t::project_positions(simulation_t& sim, double dt)
+{
+    auto& P = sim.particles[bi];
+    particle_t* pp[4] = {&P[v[0]], &P[v[1]], &P[v[2]], &P[v[3]]};
+    double const w[4] = {pp[0]->invmass(), pp[1]->invmass(), pp[2]->invmass(), pp[3]->invmass()};
+    double const V0abs = std::abs(V0);
+    double constexpr epsilon = 1e-20;
+    for (int part = 0; part < 2; ++part) {
+        mat3 const Ds = mat3::from_cols(pp[0]->xi - pp[3]->xi, pp[1]->xi - pp[3]->xi, pp[2]->xi - pp[3]->xi);
+        mat3 const F = Ds * DmInv;
+        double C, compliance;
+        mat3 dCdF;
+        if (part == 0) {
+            double const r = std::sqrt(frob2(F));
+            C = r;
+            compliance = 1.0 / (mu * V0abs);
+            dCdF = (r > 0 ? 1.0 / r : 0.0) * F;
+        } else {
+            C = determinant(F) - (1.0 + mu / lam);
+            compliance = 1.0 / (lam * V0abs);
+            dCdF = cofactor(F);
+        }
+        vec3 g[4];
+        tet_gradients(dCdF, DmInv, g);
+        double wsum = 0, gdd = 0;
+        for (int k = 0; k < 4; ++k) {
+            wsum += w[k] * norm2(g[k]);
+            gdd += dot(g[k], pp[k]->xi - pp[k]->xn);
+        }
+        if (wsum < epsilon) continue;
+        double& lm = (part == 0) ? lagrange : lagrange_h;
+        double const dl = xpbd_delta(C, lm, compliance, beta, dt, wsum, gdd);
+        lm += dl;
+        for (int k = 0; k < 4; ++k) pp[k]->xi += w[k] * g[k] * dl;
+    }
+}
+
+void neohookean_constraint_t::residual_sq(simulation_t const& sim, double dt, double* acc) const
+{
+    auto const& P = sim.particles[bi];
+    mat3 const Ds = mat3::from_cols(P[v[0]].x - P[v[3]].x, P[v[1]].x - P[v[3]].x, P[v[2]].x - P[v[3]].x);
+    mat3 const F = Ds * DmInv;
+    double const V0abs = std::abs(V0);
+    double const dt2 = dt * dt;
+    double const hd = std::sqrt(frob2(F)) + (1.0 / (mu * V0abs)) / dt2 * lagrange;
+    double const hh = (determinant(F) - (1.0 + mu / lam)) + (1.0 / (lam * V0abs)) / dt2 * lagrange_h;
+    *acc += hd * hd + hh * hh;
+}
+
+double neohookean_constraint_t::energy(simulation_t const& sim) const
+{
+    auto const& P = sim.particles[bi];
+    mat3 const Ds = mat3::from_cols(P[v[0]].x - P[v[3]].x, P[v[1]].x - P[v[3]].x, P[v[2]].x - P[v[3]].x);
+    mat3 const F = Ds * DmInv;
+    double const V0abs = std::abs(V0);
+    double const j = determinant(F) - (1.0 + mu / lam);
+    return V0abs * (0.5 * mu * (frob2(F) - 3.0) + 0.5 * lam * j * j);
+}
+
+distance_constraint_t::distance_constraint_t(double alpha, double beta, simulation_t const& sim, uint32_t b1_,
+                                             uint32_t v1_, uint32_t b2_, uint32_t v2_)
+    : constraint_t(alpha, beta), b1(b1_), b2(b2_), v1(v1_), v2(v2_)
+{
+    d = norm(sim.particles.at(b1).at(v1).x0 - sim.particles.at(b2).at(v2).x0);  // (:18-21)
+}
+
+void distance_constraint_t::project_positions(simulation_t& sim, double dt)
+{
+    // distance_constraint.cpp:24-53 (zero length gives NaN there as well: no guard)
+    particle_t& p1 = sim.particles[b1][v1];
+    particle_t& p2 = sim.particles[b2][v2];
+    double const w1 = p1.invmass(), w2 = p2.invmass();
+    vec3 const diff = p1.xi - p2.xi;
+    double const length = norm(diff);
+    vec3 const n = diff / length;
+    double const C = length - d;
+    double const gdd = dot(n, p1.xi - p1.xn) - dot(n, p2.xi - p2.xn);
+    double const dl = xpbd_delta(C, lagrange, alpha, beta, dt, w1 + w2, gdd);
+    lagrange += dl;
+    p1.xi += w1 * n * dl;
+    p2.xi += w2 * (-n) * dl;
+}
+
+void distance_constraint_t::residual_sq(simulation_t const& sim, double dt, double* acc) const
+{
+    vec3 const diff = sim.particles[b1][v1].x - sim.particles[b2][v2].x;
+    double const h = (norm(diff) - d) + (alpha / (dt * dt)) * lagrange;
+    *acc += h * h;
+}
+
+void collision_constraint_t::project_positions(simulation_t& sim, double dt)
+{
+    // collision_constraint.cpp:21-41; beta is accepted but unused, as there
+    particle_t& p = sim.particles[bi][vi];
+    double const w = p.invmass();
+    double const C = evaluate(p.xi);
+    if (C >= 0.) return;
+    double const alpha_tilde = alpha / (dt * dt);
+    double const dl = -(C + alpha_tilde * lagrange) / (w + alpha_tilde);
+    lagrange += dl;
+    p.xi += w * n * dl;
+}
+
+void collision_constraint_t::residual_sq(simulation_t const& sim, double dt, double* acc) const
+{
+    double const C = evaluate(sim.particles[bi][vi].x);
+    double const h = std::min(C, 0.0) + (alpha / (dt * dt)) * lagrange;
+    *acc += h * h;
+}
+
+void sdf_t::evaluate(vec3 p, double* sd, vec3* grad) const
+{
+    if (kind == 0) {
+        *sd = dot(n, p) + offset;
+        *grad = n;
+    } else {
+        vec3 const d = p - c;
+        double const len = norm(d);
+        *sd = len - r;
+        *grad = len > 0 ? d / len : vec3{0.0, 1.0, 0.0};
+    }
+}
+
+// Contact generation for every (tet body, SDF) pair in order, surface vertices in surface order.
+// A surface vertex at x (the previous frame's position, bvh_model.cpp:75 reads the surface copy
+// refreshed at timestep.cpp:63) produces a contact when sd(x) < 0 (bvh_model.cpp:76-80) and x lies
+// in the SDF's englobing volume (the KD-tree traversal reaches every such vertex; vertices outside
+// the volume depend on the tree's pruning and are not produced). Normal = grad/|grad|, contact
+// point q = x + |sd| n (:82-84). The handler maps the surface vertex to its tet vertex and adds a
+// collision constraint with the collision compliance/damping (contact_handler.cpp:39-52).
+void simulation_t::detect_contacts()
+{
+    contacts.clear();
+    collision_constraints.clear();
+    for (uint32_t b = 0; b < particles.size(); ++b) {
+        if (b >= surface.size()) continue;
+        for (uint32_t k = 0; k < sdfs.size(); ++k) {
+            for (uint32_t s = 0; s < surface[b].size(); ++s) {
+                uint32_t const vi = surface[b][s];
+                vec3 const x = particles[b][vi].x;
+                double sd;
+                vec3 grad;
+                sdfs[k].evaluate(x, &sd, &grad);
+                if (!(sd < 0.)) continue;
+                if (!sdfs[k].in_volume(x)) continue;
+                vec3 const nn = grad / norm(grad);
+                vec3 const q = x + std::abs(sd) * nn;
+                contacts.push_back({b, s, vi, k, q, nn});
+                collision_constraints.push_back(std::make_unique<collision_constraint_t>(
+                    collision_compliance, collision_damping, b, vi, q, nn));
+            }
+        }
+    }
+}
+
+void simulation_t::build_order()
+{
+    order.clear();
+    if (!colored_order) {
+        for (uint32_t i = 0; i < constraints.size(); ++i) order.push_back(i);
+    } else {
+        for (auto const& g : groups)
+            for (uint32_t c = 0; c < g.num_colors; ++c)
+                for (std::size_t i = 0; i < g.count; ++i)
+                    if (g.color[i] == c) order.push_back(uint32_t(g.first + i));
+    }
+    order_dirty = false;
+}
+
+// gauss_seidel_solver.cpp:8-37, with the elastic constraints visited in `order`.
+void simulation_t::solve(double dt, std::size_t iterations)
+{
+    if (order_dirty) build_order();
+    for (auto& c : collision_constraints) c->prepare_for_projection();
+    for (auto& c : constraints) c->prepare_for_projection();
+    for (std::size_t k = 0; k < iterations; ++k) {
+        for (auto& c : collision_constraints) c->project_positions(*this, dt);
+        for (uint32_t i : order) constraints[i]->project_positions(*this, dt);
+    }
+}
+
+// timestep.cpp:20-70. Gravity: with gravity_is_force the reference's f += g; a = f w
+// (:39-40, particle.cpp:50-53); otherwise g is an acceleration applied to every particle with
+// w > 0 (lumped-mass scenes).
+void simulation_t::step(double dt_frame, std::size_t iterations, std::size_t substeps)
+{
+    double const dt = dt_frame / static_cast<double>(substeps);
+    detect_contacts();  // (:29-30)
+    for (std::size_t s = 0; s < substeps; ++s) {
+        for (auto& body : particles)
+            for (particle_t& p : body) {
+                if (gravity_is_force) {
+                    p.f += gravity;
+                    p.v = p.v + p.a() * dt;
+                } else if (p.invmass() > 0.0) {
+                    p.v = p.v + gravity * dt;
+                }
+                p.xi = p.x + p.v * dt;
+            }
+        solve(dt, iterations);  // (:45)
+        for (auto& body : particles)
+            for (particle_t& p : body) {  // (:48-57)
+                p.x = p.xi;
+                p.v = (p.x - p.xn) / dt;
+                p.xn = p.x;
+                p.f = vec3{};
+            }
+    }
+    collision_constraints.clear();  // (:68)
+}
+
+uint32_t hash32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+uint32_t greedy_color(uint32_t n, uint32_t arity, const uint64_t* verts, uint32_t passes, uint32_t* color)
+{
+    // vertex -> constraints incidence
+    std::vector<uint64_t> keys(std::size_t(n) * arity);
+    for (std::size_t i = 0; i < keys.size(); ++i) keys[i] = verts[i];
+    std::vector<uint64_t> uniq = keys;
+    std::sort(uniq.begin(), uniq.end());
+    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    std::vector<uint32_t> vid(keys.size());
+    for (std::size_t i = 0; i < keys.size(); ++i)
+        vid[i] = uint32_t(std::lower_bound(uniq.begin(), uniq.end(), keys[i]) - uniq.begin());
+    std::vector<uint32_t> off(uniq.size() + 1, 0), inc(keys.size());
+    for (uint32_t x : vid) off[x + 1]++;
+    for (std::size_t i = 0; i < uniq.size(); ++i) off[i + 1] += off[i];
+    std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+    for (uint32_t t = 0; t < n; ++t)
+        for (uint32_t a = 0; a < arity; ++a) inc[fill[vid[std::size_t(t) * arity + a]]++] = t;
+
+    std::vector<uint32_t> prev(n, 0);
+    uint32_t ncolors = 0;
+    for (uint32_t pass = 0; pass <= passes; ++pass) {
+        std::vector<uint64_t> key(n);
+        for (uint32_t t = 0; t < n; ++t) key[t] = (uint64_t(pass == 0 ? 0 : prev[t]) << 32) | hash32(t);
+        std::vector<uint32_t> ord(n);
+        for (uint32_t t = 0; t < n; ++t) ord[t] = t;
+        std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) {
+            return key[a] != key[b] ? key[a] > key[b] : a < b;
+        });
+        std::vector<int64_t> col(n, -1);
+        std::vector<uint32_t> stamp;
+        ncolors = 0;
+        for (uint32_t t : ord) {
+            std::vector<bool> used;
+            for (uint32_t a = 0; a < arity; ++a) {
+                uint32_t const x = vid[std::size_t(t) * arity + a];
+                for (uint32_t e = off[x]; e < off[x + 1]; ++e) {
+                    uint32_t const u = inc[e];
+                    if (u == t || col[u] < 0) continue;
+                    if (used.size() <= std::size_t(col[u])) used.resize(col[u] + 1, false);
+                    used[col[u]] = true;
+                }
+            }
+            uint32_t c = 0;
+            while (c < used.size() && used[c]) ++c;
+            col[t] = c;
+            ncolors = std::max(ncolors, c + 1);
+        }
+        for (uint32_t t = 0; t < n; ++t) prev[t] = uint32_t(col[t]);
+    }
+    std::copy(prev.begin(), prev.end(), color);
+    return ncolors;
+}
+
+}  // namespace oracle

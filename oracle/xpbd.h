@@ -1,0 +1,167 @@
+// FP64 restatement of the reference XPBD time step.
+//
+// TEST INFRASTRUCTURE ONLY: this is the oracle the GPU path is checked against and the CPU
+// baseline bench.py times. Product code never links it.
+//
+// Structure deliberately follows the reference (AoS particles, heap-allocated constraints with a
+// virtual project_positions, sequential Gauss-Seidel), so that its timing is representative of
+// the reference's CPU solver:
+//   particle_t                  src/physics/particle.cpp:6-53, include/sbs/physics/particle.h:46-52
+//   constraint_t                src/physics/constraint.cpp:7-16
+//   green_constraint_t          src/physics/xpbd/green_constraint.cpp:11-173
+//   distance_constraint_t       src/physics/xpbd/distance_constraint.cpp:8-53
+//   collision_constraint_t      src/physics/xpbd/collision_constraint.cpp:9-48
+//   gauss_seidel_solver_t       src/physics/gauss_seidel_solver.cpp:8-37
+//   timestep_t::step            src/physics/timestep.cpp:20-70
+//   contact generation          src/physics/collision/bvh_model.cpp:66-95, sdf_model.cpp:181-204,
+//                               src/physics/xpbd/contact_handler.cpp:14-54
+// and adds the components the reference lacks (stable neo-Hookean constraint, colored visiting
+// order, sphere SDFs, residual norms, energies), each documented where it is defined.
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "linalg.h"
+
+namespace oracle {
+
+struct particle_t {
+    vec3 x0, xi, xn, x, v, f;
+    double m = 1.0;  // particle.cpp:7 (unit mass by default)
+    explicit particle_t(vec3 p) : x0(p), xi(p), xn(p), x(p) {}
+    double invmass() const { return m > 0.0 ? 1.0 / m : 0.0; }  // particle.cpp:39-44
+    vec3 a() const { return f * invmass(); }                    // particle.cpp:50-53
+};
+
+struct simulation_t;
+
+struct constraint_t {
+    double alpha, beta, lagrange = 0.0;
+    constraint_t(double a, double b) : alpha(a), beta(b) {}
+    virtual ~constraint_t() = default;
+    void prepare_for_projection() { lagrange = 0.0; prepare_impl(); }  // constraint.cpp:12-16
+    virtual void prepare_impl() {}
+    virtual void project_positions(simulation_t& sim, double dt) = 0;
+    // Adds the squared XPBD residual h = C(x) + alpha~ * lambda of this constraint, evaluated at
+    // the particles' current x, to *acc. Returns nothing else.
+    virtual void residual_sq(simulation_t const& sim, double dt, double* acc) const = 0;
+    virtual double energy(simulation_t const& sim) const { return 0.0; }
+};
+
+struct green_constraint_t final : constraint_t {
+    uint32_t bi, v[4];
+    mat3 DmInv;
+    double V0, mu, lam;
+    green_constraint_t(double alpha, double beta, simulation_t const& sim, uint32_t body,
+                       uint32_t const vs[4], double young, double poisson);
+    void project_positions(simulation_t& sim, double dt) override;
+    void residual_sq(simulation_t const& sim, double dt, double* acc) const override;
+    double energy(simulation_t const& sim) const override;
+    // C = |V0| psi at positions p (the constraint value used by project_positions).
+    double evaluate(vec3 const p[4], bool* early_out = nullptr, double const w[4] = nullptr) const;
+    double evaluate_forces(vec3 const p[4], vec3 f[4]) const;
+};
+
+// Stable neo-Hookean as two XPBD constraints per tet (Macklin & Mueller 2021, "A Constraint-based
+// Formulation of Stable Neo-Hookean Materials"): deviatoric C_D = ||F||_F with compliance
+// 1/(mu |V0|), then hydrostatic C_H = det F - (1 + mu/lambda) with compliance 1/(lambda |V0|),
+// projected one after the other, each with its own multiplier. No reference counterpart.
+struct neohookean_constraint_t final : constraint_t {
+    uint32_t bi, v[4];
+    mat3 DmInv;
+    double V0, mu, lam;
+    double lagrange_h = 0.0;
+    neohookean_constraint_t(double beta, simulation_t const& sim, uint32_t body, uint32_t const vs[4],
+                            double young, double poisson);
+    void prepare_impl() override { lagrange_h = 0.0; }
+    void project_positions(simulation_t& sim, double dt) override;
+    void residual_sq(simulation_t const& sim, double dt, double* acc) const override;
+    double energy(simulation_t const& sim) const override;
+};
+
+struct distance_constraint_t final : constraint_t {
+    uint32_t b1, b2, v1, v2;
+    double d;
+    distance_constraint_t(double alpha, double beta, simulation_t const& sim, uint32_t b1, uint32_t v1,
+                          uint32_t b2, uint32_t v2);
+    void project_positions(simulation_t& sim, double dt) override;
+    void residual_sq(simulation_t const& sim, double dt, double* acc) const override;
+};
+
+struct collision_constraint_t final : constraint_t {
+    uint32_t bi, vi;
+    vec3 q, n;
+    collision_constraint_t(double alpha, double beta, uint32_t b, uint32_t v, vec3 q_, vec3 n_)
+        : constraint_t(alpha, beta), bi(b), vi(v), q(q_), n(n_) {}
+    double evaluate(vec3 p) const { return dot(p - q, n); }  // collision_constraint.cpp:43-48
+    void project_positions(simulation_t& sim, double dt) override;
+    void residual_sq(simulation_t const& sim, double dt, double* acc) const override;
+};
+
+// Analytic SDF obstacles. Plane: Eigen::Hyperplane(n, point) semantics used by
+// sdf_model_t::from_plane (sdf_model.cpp:181-193): sd = n.p - n.point, grad = n.
+// Sphere: sd = |p - c| - r, grad = (p - c)/|p - c| (no reference counterpart).
+struct sdf_t {
+    int kind = 0;  // 0 plane, 1 sphere
+    vec3 n;
+    double offset = 0;
+    vec3 c;
+    double r = 0;
+    vec3 vmin, vmax;  // englobing volume (bvh_model.cpp:50)
+    void evaluate(vec3 p, double* sd, vec3* grad) const;
+    bool in_volume(vec3 p) const
+    {
+        return p.x >= vmin.x && p.x <= vmax.x && p.y >= vmin.y && p.y <= vmax.y && p.z >= vmin.z &&
+               p.z <= vmax.z;
+    }
+};
+
+struct contact_t {
+    uint32_t body, surface_vertex, vertex, sdf;
+    vec3 q, n;
+};
+
+// One constraint group = one batch of constraints added together; the colored visiting order
+// processes groups in insertion order and, inside a group, colors in increasing order.
+struct group_t {
+    int kind;                // 0 green, 1 neo-Hookean, 2 distance
+    std::size_t first, count;  // range in simulation_t::constraints
+    std::vector<uint32_t> color;  // per constraint of the group
+    uint32_t num_colors = 0;
+};
+
+struct simulation_t {
+    std::vector<std::vector<particle_t>> particles;
+    std::vector<std::vector<uint32_t>> surface;  // per body: surface vertex -> body vertex
+    std::vector<std::unique_ptr<constraint_t>> constraints;
+    std::vector<std::unique_ptr<constraint_t>> collision_constraints;
+    std::vector<sdf_t> sdfs;
+    std::vector<group_t> groups;
+    std::vector<contact_t> contacts;  // contacts of the current frame, in generation order
+
+    vec3 gravity{0.0, -9.81, 0.0};
+    bool gravity_is_force = true;  // timestep.cpp:39-40 semantics
+    double collision_compliance = 1e-8, collision_damping = 0.0;  // simulation_parameters.h:26-27
+    bool colored_order = false;
+    uint32_t recolor_passes = 2;
+
+    // GS visiting order of the elastic constraints (indices into constraints)
+    std::vector<uint32_t> order;
+    bool order_dirty = true;
+    void build_order();
+
+    void detect_contacts();
+    void solve(double dt, std::size_t iterations);  // gauss_seidel_solver_t::solve
+    void step(double dt, std::size_t iterations, std::size_t substeps);  // timestep_t::step
+};
+
+// Greedy coloring of a constraint group in priority order, restating the device Jones-Plassmann
+// coloring: pass 0 visits constraints by descending hash32(index); pass r>0 visits by descending
+// (previous color, hash32(index)). Each visited constraint takes the smallest color not used by
+// an already-colored constraint sharing a vertex with it.
+uint32_t hash32(uint32_t x);
+uint32_t greedy_color(uint32_t n, uint32_t arity, const uint64_t* verts, uint32_t passes,
+                      uint32_t* color);
+
+}  // namespace oracle

@@ -1,0 +1,88 @@
+// Launch interface of the gfx950 kernels (implemented in kernels.hip). All launches go to the
+// given stream; none allocates memory.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace sbs {
+
+// Analytic SDF obstacle in device memory.
+struct sdf_dev {
+    int kind;  // 0 plane, 1 sphere
+    float a[4];  // plane: n.xyz, offset | sphere: c.xyz, r
+    float vmin[3], vmax[3];
+};
+
+// Arrays of one tet-constraint group in color order. Material layout (3 float4 per tet):
+//   m0 = (r0.x, r0.y, r0.z, r1.x), m1 = (r1.y, r1.z, r2.x, r2.y), m2 = (r2.z, ...)
+// with r_k the rows of DmInv. green:      m2 = (r2.z, V0 signed, mu, lambda)
+//                             neo-Hookean: m2 = (r2.z, 1/(mu|V0|), 1/(lambda|V0|), 1 + mu/lambda)
+struct tet_group_dev {
+    const int4* idx;
+    const float4* m0;
+    const float4* m1;
+    const float4* m2;
+    float* lambda;  // green: 1 per tet, neo-Hookean: 2 per tet (deviatoric, hydrostatic)
+};
+
+struct distance_group_dev {
+    const uint2* idx;
+    const float* rest;
+    float* lambda;
+};
+
+struct contact_dev {
+    const uint32_t* surf;  // surface vertex -> global particle
+    uint32_t S;            // surface vertex count
+    uint32_t slots;        // contact slots per surface vertex (>= number of SDFs)
+    float4* q;             // [slot * S + s]: contact point, w = sdf index (as float)
+    float4* n;             // normal
+    float* lambda;
+    uint32_t* count;       // contacts per surface vertex
+};
+
+void launch_predict(hipStream_t s, float4* xi, const float4* xn, float4* v, uint32_t n, float gx, float gy,
+                    float gz, int gravity_is_force, float dt, int block);
+void launch_commit(hipStream_t s, const float4* xi, float4* xn, float4* v, uint32_t n, float dt, int block);
+void launch_commit_predict(hipStream_t s, float4* xi, float4* xn, float4* v, uint32_t n, float gx, float gy,
+                           float gz, int gravity_is_force, float dt, int block);
+
+void launch_green(hipStream_t s, tet_group_dev g, float4* xi, const float4* xn, uint32_t begin, uint32_t count,
+                  float alpha_tilde, float gamma, int first, int block);
+void launch_neohookean(hipStream_t s, tet_group_dev g, float4* xi, const float4* xn, uint32_t begin,
+                       uint32_t count, float inv_dt2, float beta_dt, int first, int block);
+void launch_distance(hipStream_t s, distance_group_dev g, float4* xi, const float4* xn, uint32_t begin,
+                     uint32_t count, float alpha_tilde, float gamma, int first, int block);
+
+void launch_detect(hipStream_t s, contact_dev c, const float4* x, const sdf_dev* sdfs, uint32_t nsdf, int block);
+void launch_contact(hipStream_t s, contact_dev c, float4* xi, float alpha_tilde, int first, int block);
+
+// Residual partial sums: one double per block of `block` threads over [begin, begin+count).
+void launch_green_residual(hipStream_t s, tet_group_dev g, const float4* x, uint32_t begin, uint32_t count,
+                           float alpha_tilde, double* partial, int block);
+void launch_neohookean_residual(hipStream_t s, tet_group_dev g, const float4* x, uint32_t begin, uint32_t count,
+                                float inv_dt2, double* partial, int block);
+void launch_distance_residual(hipStream_t s, distance_group_dev g, const float4* x, uint32_t begin,
+                              uint32_t count, float alpha_tilde, double* partial, int block);
+
+// Energy partials: per block 6 doubles (kinetic, potential, elastic, px, py, pz).
+void launch_particle_energy(hipStream_t s, const float4* x, const float4* v, const float* mass, uint32_t n,
+                            float gx, float gy, float gz, int gravity_is_force, double* partial, int block);
+void launch_green_energy(hipStream_t s, tet_group_dev g, const float4* x, uint32_t count, double* partial,
+                         int block);
+void launch_neohookean_energy(hipStream_t s, tet_group_dev g, const float4* x, uint32_t count, double* partial,
+                              int block);
+
+// One Jones-Plassmann round over a constraint group (arity vertices per constraint).
+void launch_jp_round(hipStream_t s, uint32_t n, uint32_t arity, const uint32_t* verts, const uint32_t* inc_off,
+                     const uint32_t* inc, const uint64_t* key, const uint32_t* color_in, uint32_t* color_out,
+                     uint32_t* remaining, uint32_t* overflow, int block);
+void launch_jp_keys(hipStream_t s, uint32_t n, const uint32_t* prev, uint64_t* key, int block);
+
+// Fills a buffer with a 32-bit value (graph-safe replacement of a memset).
+void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t value, uint32_t n);
+
+constexpr uint32_t kUncolored = 0xffffffffu;
+constexpr uint32_t kMaxColors = 256;
+
+}  // namespace sbs

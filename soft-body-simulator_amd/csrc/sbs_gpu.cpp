@@ -1,0 +1,941 @@
+// C ABI of the gfx950 XPBD time step (declared in include/sbs_gpu.h).
+//
+// The handle owns one HIP stream and every device buffer. sbs_gpu_commit() colors each constraint
+// group on the device (Jones-Plassmann), lays the group's data out in color order and uploads the
+// particles; sbs_gpu_step() then issues, per frame: contact detection, and per substep the
+// prediction, `iterations` x (contact projection, one launch per color of every group), and the
+// velocity update (fused with the next substep's prediction).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "abi_util.h"
+#include "kernels.h"
+#include "sbs_gpu.h"
+
+namespace sbs {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(std::string const& msg) { g_last_error = msg; }
+
+template <class T>
+struct dev_buf {
+    T* p = nullptr;
+    std::size_t n = 0;
+    dev_buf() = default;
+    dev_buf(dev_buf const&) = delete;
+    dev_buf& operator=(dev_buf const&) = delete;
+    dev_buf(dev_buf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    dev_buf& operator=(dev_buf&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~dev_buf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(std::size_t count)
+    {
+        release();
+        if (count == 0) return;
+        hip_check(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)), "hipMalloc");
+        n = count;
+    }
+    void upload(T const* h, std::size_t count, hipStream_t s)
+    {
+        alloc(count);
+        if (count) hip_check(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+    }
+};
+
+struct body_t {
+    uint32_t offset = 0, n = 0;
+    std::vector<double> x0, mass;
+    std::vector<uint32_t> surface;
+};
+
+struct group_t {
+    int kind = 0;  // 0 green, 1 neo-Hookean, 2 distance
+    uint32_t body = 0, count = 0;
+    double alpha = 0, beta = 0, young = 0, poisson = 0;
+    std::vector<uint32_t> conn;  // 4 body-local ids per tet, or (b1, v1, b2, v2) per distance
+    // after commit
+    uint32_t ncolors = 0;
+    std::vector<uint32_t> color;          // per constraint, insertion order
+    std::vector<uint32_t> color_offsets;  // ncolors + 1, positions in color order
+    dev_buf<int4> idx;
+    dev_buf<float4> m0, m1, m2;
+    dev_buf<uint2> didx;
+    dev_buf<float> rest;
+    dev_buf<float> lambda;
+    tet_group_dev tet() const { return {idx.p, m0.p, m1.p, m2.p, lambda.p}; }
+    distance_group_dev dist() const { return {didx.p, rest.p, lambda.p}; }
+};
+
+struct timing_rec {
+    int kind;
+    uint64_t units;
+    hipEvent_t a, b;
+};
+
+}  // namespace sbs
+
+using namespace sbs;
+
+struct sbs_gpu {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    sbs_xpbd_params params{};
+    std::vector<body_t> bodies;
+    std::vector<group_t> groups;
+    std::vector<sdf_dev> sdfs;
+    bool committed = false;
+    uint32_t V = 0;
+
+    dev_buf<float4> xi, xn, vel;
+    dev_buf<float> mass;
+
+    dev_buf<uint32_t> surf;
+    uint32_t S = 0, slots = 0;
+    dev_buf<float4> cq, cn;
+    dev_buf<float> clam;
+    dev_buf<uint32_t> ccount;
+    dev_buf<sdf_dev> dsdf;
+
+    dev_buf<double> partial;
+
+    int block = 256;
+
+    bool timing = false;
+    std::vector<timing_rec> pending;
+    std::vector<hipEvent_t> event_pool;
+    uint64_t st_launches[SBS_KERNEL_COUNT] = {};
+    double st_ms[SBS_KERNEL_COUNT] = {};
+    uint64_t st_units[SBS_KERNEL_COUNT] = {};
+
+    bool use_graphs = false;
+    hipGraphExec_t graph_exec = nullptr;
+    std::array<double, 16> graph_key{};
+
+    ~sbs_gpu()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        for (auto& r : pending) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        for (auto e : event_pool) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    hipEvent_t take_event()
+    {
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        return e;
+    }
+
+    // Runs `launch` on the stream, bracketed by events when timing is on.
+    template <class F>
+    void timed(int kind, uint64_t units, F&& launch)
+    {
+        if (!timing) {
+            launch();
+            return;
+        }
+        timing_rec r{kind, units, take_event(), take_event()};
+        hip_check(hipEventRecord(r.a, stream), "hipEventRecord");
+        launch();
+        hip_check(hipEventRecord(r.b, stream), "hipEventRecord");
+        pending.push_back(r);
+    }
+
+    void harvest()
+    {
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        for (auto& r : pending) {
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, r.a, r.b), "hipEventElapsedTime");
+            st_launches[r.kind] += 1;
+            st_ms[r.kind] += ms;
+            st_units[r.kind] += r.units;
+            event_pool.push_back(r.a);
+            event_pool.push_back(r.b);
+        }
+        pending.clear();
+    }
+
+    contact_dev contacts() { return {surf.p, S, slots, cq.p, cn.p, clam.p, ccount.p}; }
+
+    void set_current() { hip_check(hipSetDevice(device), "hipSetDevice"); }
+
+    void commit(uint32_t recolor_passes);
+    void color_group(group_t& g, uint32_t recolor_passes);
+    void upload_group(group_t& g);
+    void enqueue_frame(double dt, uint32_t iterations, uint32_t substeps);
+    void detect();
+};
+
+namespace {
+
+uint32_t global_vertex(sbs_gpu const& s, uint32_t body, uint32_t v)
+{
+    require(body < s.bodies.size(), "body index out of range");
+    require(v < s.bodies[body].n, "vertex index out of range");
+    return s.bodies[body].offset + v;
+}
+
+void lame(double young, double poisson, double* mu, double* lam)
+{
+    *mu = young / (2.0 * (1.0 + poisson));
+    *lam = (young * poisson) / ((1.0 + poisson) * (1.0 - 2.0 * poisson));
+}
+
+}  // namespace
+
+// Device Jones-Plassmann coloring: pass 0 keys are hash32(index); each recolor pass keys on the
+// previous pass's color (higher first) then hash32(index), which never increases the color count.
+void sbs_gpu::color_group(group_t& g, uint32_t recolor_passes)
+{
+    uint32_t const n = g.count;
+    uint32_t const arity = g.kind == 2 ? 2 : 4;
+    std::vector<uint32_t> verts(std::size_t(n) * arity);
+    for (uint32_t c = 0; c < n; ++c) {
+        if (g.kind == 2) {
+            verts[2 * c] = global_vertex(*this, g.conn[4 * c], g.conn[4 * c + 1]);
+            verts[2 * c + 1] = global_vertex(*this, g.conn[4 * c + 2], g.conn[4 * c + 3]);
+        } else {
+            for (int k = 0; k < 4; ++k) verts[4 * std::size_t(c) + k] = global_vertex(*this, g.body, g.conn[4 * std::size_t(c) + k]);
+        }
+    }
+    // vertex -> constraints incidence (CSR)
+    std::vector<uint32_t> off(std::size_t(V) + 1, 0), inc(verts.size());
+    for (uint32_t x : verts) off[x + 1]++;
+    for (std::size_t i = 0; i < V; ++i) off[i + 1] += off[i];
+    {
+        std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+        for (uint32_t c = 0; c < n; ++c)
+            for (uint32_t a = 0; a < arity; ++a) inc[fill[verts[std::size_t(c) * arity + a]]++] = c;
+    }
+    dev_buf<uint32_t> dverts, doff, dinc, ca, cb, prev, counters;
+    dev_buf<uint64_t> key;
+    dverts.upload(verts.data(), verts.size(), stream);
+    doff.upload(off.data(), off.size(), stream);
+    dinc.upload(inc.data(), inc.size(), stream);
+    ca.alloc(n);
+    cb.alloc(n);
+    prev.alloc(n);
+    key.alloc(n);
+    counters.alloc(2);
+    for (uint32_t pass = 0; pass <= recolor_passes; ++pass) {
+        launch_jp_keys(stream, n, pass == 0 ? nullptr : prev.p, key.p, 256);
+        launch_fill_u32(stream, ca.p, kUncolored, n);
+        uint32_t h[2] = {1, 0};
+        uint32_t rounds = 0;
+        while (h[0] != 0) {
+            require(++rounds < 1000000, "coloring did not terminate", SBS_ERR_INTERNAL);
+            launch_fill_u32(stream, counters.p, 0u, 2);
+            launch_jp_round(stream, n, arity, dverts.p, doff.p, dinc.p, key.p, ca.p, cb.p, counters.p, counters.p + 1,
+                            256);
+            hip_check(hipMemcpyAsync(h, counters.p, sizeof h, hipMemcpyDeviceToHost, stream), "download counters");
+            hip_check(hipStreamSynchronize(stream), "coloring round");
+            require(h[1] == 0, "constraint group needs more than 256 colors");
+            std::swap(ca, cb);
+        }
+        hip_check(hipMemcpyAsync(prev.p, ca.p, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream), "copy colors");
+    }
+    g.color.resize(n);
+    hip_check(hipMemcpyAsync(g.color.data(), prev.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream),
+              "download colors");
+    hip_check(hipStreamSynchronize(stream), "coloring");
+    g.ncolors = 0;
+    for (uint32_t c : g.color) g.ncolors = std::max(g.ncolors, c + 1);
+}
+
+// Lays a group out in color order (stable by insertion index) and uploads it in FP32; the rest
+// shape inverse and volume are computed in FP64 first (green_constraint.cpp:38-46).
+void sbs_gpu::upload_group(group_t& g)
+{
+    uint32_t const n = g.count;
+    g.color_offsets.assign(g.ncolors + 1, 0);
+    for (uint32_t c : g.color) g.color_offsets[c + 1]++;
+    for (uint32_t c = 0; c < g.ncolors; ++c) g.color_offsets[c + 1] += g.color_offsets[c];
+    std::vector<uint32_t> order(n);
+    {
+        std::vector<uint32_t> fill(g.color_offsets.begin(), g.color_offsets.end() - 1);
+        for (uint32_t i = 0; i < n; ++i) order[fill[g.color[i]]++] = i;
+    }
+    if (g.kind == 2) {
+        std::vector<uint2> idx(n);
+        std::vector<float> rest(n);
+        for (uint32_t j = 0; j < n; ++j) {
+            uint32_t const* e = &g.conn[4 * std::size_t(order[j])];
+            uint32_t const a = global_vertex(*this, e[0], e[1]), b = global_vertex(*this, e[2], e[3]);
+            idx[j] = make_uint2(a, b);
+            double const* pa = &bodies[e[0]].x0[3 * std::size_t(e[1])];
+            double const* pb = &bodies[e[2]].x0[3 * std::size_t(e[3])];
+            double const d[3] = {pa[0] - pb[0], pa[1] - pb[1], pa[2] - pb[2]};
+            rest[j] = float(std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
+        }
+        g.didx.upload(idx.data(), n, stream);
+        g.rest.upload(rest.data(), n, stream);
+        g.lambda.alloc(n);
+    } else {
+        std::vector<int4> idx(n);
+        std::vector<float4> m0(n), m1(n), m2(n);
+        body_t const& b = bodies[g.body];
+        double mu, lam;
+        lame(g.young, g.poisson, &mu, &lam);
+        for (uint32_t j = 0; j < n; ++j) {
+            uint32_t const* t = &g.conn[4 * std::size_t(order[j])];
+            double D[3][3];  // D[r][c] = (p_c - p_4)[r]
+            for (int c = 0; c < 3; ++c)
+                for (int r = 0; r < 3; ++r) D[r][c] = b.x0[3 * std::size_t(t[c]) + r] - b.x0[3 * std::size_t(t[3]) + r];
+            double A[3][3];  // adjugate
+            A[0][0] = D[1][1] * D[2][2] - D[1][2] * D[2][1];
+            A[0][1] = D[0][2] * D[2][1] - D[0][1] * D[2][2];
+            A[0][2] = D[0][1] * D[1][2] - D[0][2] * D[1][1];
+            A[1][0] = D[1][2] * D[2][0] - D[1][0] * D[2][2];
+            A[1][1] = D[0][0] * D[2][2] - D[0][2] * D[2][0];
+            A[1][2] = D[0][2] * D[1][0] - D[0][0] * D[1][2];
+            A[2][0] = D[1][0] * D[2][1] - D[1][1] * D[2][0];
+            A[2][1] = D[0][1] * D[2][0] - D[0][0] * D[2][1];
+            A[2][2] = D[0][0] * D[1][1] - D[0][1] * D[1][0];
+            double const det = D[0][0] * A[0][0] + D[0][1] * A[1][0] + D[0][2] * A[2][0];
+            require(det != 0.0, "degenerate tet in rest shape");
+            float r[9];
+            for (int i = 0; i < 9; ++i) r[i] = float(A[i / 3][i % 3] / det);
+            double const V0 = det / 6.0;
+            idx[j] = make_int4(int(b.offset + t[0]), int(b.offset + t[1]), int(b.offset + t[2]), int(b.offset + t[3]));
+            m0[j] = make_float4(r[0], r[1], r[2], r[3]);
+            m1[j] = make_float4(r[4], r[5], r[6], r[7]);
+            if (g.kind == 0) {
+                m2[j] = make_float4(r[8], float(V0), float(mu), float(lam));
+            } else {
+                double const av = std::fabs(V0);
+                m2[j] = make_float4(r[8], float(1.0 / (mu * av)), float(1.0 / (lam * av)), float(1.0 + mu / lam));
+            }
+        }
+        g.idx.upload(idx.data(), n, stream);
+        g.m0.upload(m0.data(), n, stream);
+        g.m1.upload(m1.data(), n, stream);
+        g.m2.upload(m2.data(), n, stream);
+        g.lambda.alloc(std::size_t(n) * (g.kind == 1 ? 2 : 1));
+    }
+    hip_check(hipStreamSynchronize(stream), "upload group");
+}
+
+void sbs_gpu::commit(uint32_t recolor_passes)
+{
+    require(!committed, "sbs_gpu_commit called twice", SBS_ERR_STATE);
+    set_current();
+    V = 0;
+    for (auto& b : bodies) {
+        b.offset = V;
+        V += b.n;
+    }
+    std::vector<float4> x(V), v(V, make_float4(0, 0, 0, 0));
+    std::vector<float> m(V);
+    for (auto const& b : bodies)
+        for (uint32_t i = 0; i < b.n; ++i) {
+            double const mi = b.mass.empty() ? 1.0 : b.mass[i];
+            double const w = mi > 0.0 ? 1.0 / mi : 0.0;  // particle.cpp:39-44
+            x[b.offset + i] = make_float4(float(b.x0[3 * i]), float(b.x0[3 * i + 1]), float(b.x0[3 * i + 2]), float(w));
+            m[b.offset + i] = float(mi);
+        }
+    xi.upload(x.data(), V, stream);
+    xn.upload(x.data(), V, stream);
+    vel.upload(v.data(), V, stream);
+    mass.upload(m.data(), V, stream);
+
+    for (auto& g : groups) {
+        if (g.count == 0) {
+            g.ncolors = 0;
+            g.color_offsets.assign(1, 0);
+            continue;
+        }
+        color_group(g, recolor_passes);
+        upload_group(g);
+    }
+
+    std::vector<uint32_t> sv;
+    for (auto const& b : bodies)
+        for (uint32_t s : b.surface) sv.push_back(b.offset + s);
+    S = uint32_t(sv.size());
+    slots = uint32_t(sdfs.size());
+    if (S && slots) {
+        surf.upload(sv.data(), S, stream);
+        cq.alloc(std::size_t(S) * slots);
+        cn.alloc(std::size_t(S) * slots);
+        clam.alloc(std::size_t(S) * slots);
+        ccount.alloc(S);
+        dsdf.upload(sdfs.data(), sdfs.size(), stream);
+    }
+    hip_check(hipStreamSynchronize(stream), "commit");
+    committed = true;
+}
+
+void sbs_gpu::detect()
+{
+    if (S && slots) {
+        contact_dev const c = contacts();
+        timed(SBS_KERNEL_DETECT, S, [&] { launch_detect(stream, c, xn.p, dsdf.p, uint32_t(sdfs.size()), block); });
+    }
+}
+
+void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t substeps)
+{
+    double const dt = dt_frame / double(substeps);
+    double const dt2 = dt * dt;
+    float const g0 = float(params.gravity[0]), g1 = float(params.gravity[1]), g2 = float(params.gravity[2]);
+    int const force = params.gravity_is_force;
+    detect();  // timestep.cpp:29-30
+    bool const have_contacts = S && slots;
+    contact_dev const c = contacts();
+    float const c_at = float(params.collision_compliance / dt2);
+    for (uint32_t s = 0; s < substeps; ++s) {
+        if (s == 0)
+            timed(SBS_KERNEL_PREDICT, V, [&] { launch_predict(stream, xi.p, xn.p, vel.p, V, g0, g1, g2, force, float(dt), block); });
+        else
+            timed(SBS_KERNEL_PREDICT, V,
+                  [&] { launch_commit_predict(stream, xi.p, xn.p, vel.p, V, g0, g1, g2, force, float(dt), block); });
+        for (uint32_t it = 0; it < iterations; ++it) {
+            int const first = it == 0;
+            if (have_contacts)
+                timed(SBS_KERNEL_CONTACT, S, [&] { launch_contact(stream, c, xi.p, c_at, first, block); });
+            for (auto const& g : groups) {
+                double const at = g.alpha / dt2;
+                double const gamma = at * (g.beta * dt2) / dt;
+                for (uint32_t col = 0; col < g.ncolors; ++col) {
+                    uint32_t const b = g.color_offsets[col], n = g.color_offsets[col + 1] - b;
+                    if (g.kind == 0)
+                        timed(SBS_KERNEL_GREEN, n, [&] {
+                            launch_green(stream, g.tet(), xi.p, xn.p, b, n, float(at), float(gamma), first, block);
+                        });
+                    else if (g.kind == 1)
+                        timed(SBS_KERNEL_NEOHOOKEAN, n, [&] {
+                            launch_neohookean(stream, g.tet(), xi.p, xn.p, b, n, float(1.0 / dt2), float(g.beta * dt),
+                                              first, block);
+                        });
+                    else
+                        timed(SBS_KERNEL_DISTANCE, n, [&] {
+                            launch_distance(stream, g.dist(), xi.p, xn.p, b, n, float(at), float(gamma), first, block);
+                        });
+                }
+            }
+        }
+    }
+    timed(SBS_KERNEL_COMMIT, V, [&] { launch_commit(stream, xi.p, xn.p, vel.p, V, float(dt), block); });
+}
+
+// ---------------------------------------------------------------------------------------------
+
+extern "C" {
+
+const char* sbs_gpu_last_error(void) { return g_last_error.c_str(); }
+
+int sbs_gpu_default_params(sbs_xpbd_params* p)
+{
+    return guarded([&] {
+        require(p != nullptr, "params is null");
+        *p = sbs_xpbd_params{};
+        p->compliance = 1e-4;
+        p->damping = 0.0;
+        p->young_modulus = 1e6;
+        p->poisson_ratio = 0.30;
+        p->hooke_coefficient = 1.0;
+        p->collision_compliance = 1e-8;
+        p->collision_damping = 0.0;
+        p->gravity[0] = 0.0;
+        p->gravity[1] = -9.81;
+        p->gravity[2] = 0.0;
+        p->gravity_is_force = 1;
+    });
+}
+
+int sbs_gpu_create(int device, sbs_gpu** out)
+{
+    return guarded([&] {
+        require(out != nullptr, "out is null");
+        *out = nullptr;
+        int count = 0;
+        hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        require(count > 0, "no HIP device", SBS_ERR_NO_DEVICE);
+        require(device >= 0 && device < count, "device index out of range", SBS_ERR_NO_DEVICE);
+        auto* s = new sbs_gpu();
+        s->device = device;
+        try {
+            s->set_current();
+            hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+            sbs_gpu_default_params(&s->params);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int sbs_gpu_destroy(sbs_gpu* sim)
+{
+    return guarded([&] {
+        if (!sim) return;
+        (void)hipSetDevice(sim->device);
+        delete sim;
+    });
+}
+
+int sbs_gpu_set_params(sbs_gpu* sim, const sbs_xpbd_params* params)
+{
+    return guarded([&] {
+        require(sim && params, "null argument");
+        sim->params = *params;
+    });
+}
+
+int sbs_gpu_add_body(sbs_gpu* sim, const double* x0, const double* mass, uint32_t n, uint32_t* body_id)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(!sim->committed, "scene already committed", SBS_ERR_STATE);
+        require(x0 != nullptr || n == 0, "x0 is null");
+        body_t b;
+        b.n = n;
+        b.x0.assign(x0, x0 + 3 * std::size_t(n));
+        if (mass) {
+            b.mass.assign(mass, mass + n);
+            for (double m : b.mass) require(m >= 0.0 && std::isfinite(m), "mass must be finite and >= 0");
+        }
+        for (double v : b.x0) require(std::isfinite(v), "positions must be finite");
+        sim->bodies.push_back(std::move(b));
+        if (body_id) *body_id = uint32_t(sim->bodies.size() - 1);
+    });
+}
+
+int sbs_gpu_set_collision_surface(sbs_gpu* sim, uint32_t body, const uint32_t* surface_to_body, uint32_t n)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(!sim->committed, "scene already committed", SBS_ERR_STATE);
+        require(body < sim->bodies.size(), "body index out of range");
+        require(surface_to_body != nullptr || n == 0, "surface map is null");
+        for (uint32_t i = 0; i < n; ++i) require(surface_to_body[i] < sim->bodies[body].n, "surface vertex out of range");
+        sim->bodies[body].surface.assign(surface_to_body, surface_to_body + n);
+    });
+}
+
+static int add_tet_group(sbs_gpu* sim, int kind, uint32_t body, const uint32_t* tets, uint32_t n, double alpha,
+                         double beta, double young, double poisson, uint32_t* group_id)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(!sim->committed, "scene already committed", SBS_ERR_STATE);
+        require(body < sim->bodies.size(), "body index out of range");
+        require(tets != nullptr || n == 0, "tets is null");
+        require(poisson > -1.0 && poisson < 0.5, "poisson ratio must be in (-1, 0.5)");
+        require(young > 0.0, "young modulus must be positive");
+        for (std::size_t i = 0; i < 4 * std::size_t(n); ++i) require(tets[i] < sim->bodies[body].n, "tet vertex out of range");
+        group_t g;
+        g.kind = kind;
+        g.body = body;
+        g.count = n;
+        g.alpha = alpha;
+        g.beta = beta;
+        g.young = young;
+        g.poisson = poisson;
+        g.conn.assign(tets, tets + 4 * std::size_t(n));
+        sim->groups.push_back(std::move(g));
+        if (group_id) *group_id = uint32_t(sim->groups.size() - 1);
+    });
+}
+
+int sbs_gpu_add_green_constraints(sbs_gpu* sim, uint32_t body, const uint32_t* tets, uint32_t n, double alpha,
+                                  double beta, double young_modulus, double poisson_ratio, uint32_t* group_id)
+{
+    return add_tet_group(sim, 0, body, tets, n, alpha, beta, young_modulus, poisson_ratio, group_id);
+}
+
+int sbs_gpu_add_neohookean_constraints(sbs_gpu* sim, uint32_t body, const uint32_t* tets, uint32_t n, double beta,
+                                       double young_modulus, double poisson_ratio, uint32_t* group_id)
+{
+    return add_tet_group(sim, 1, body, tets, n, 0.0, beta, young_modulus, poisson_ratio, group_id);
+}
+
+int sbs_gpu_add_distance_constraints(sbs_gpu* sim, const uint32_t* e, uint32_t n, double alpha, double beta,
+                                     uint32_t* group_id)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(!sim->committed, "scene already committed", SBS_ERR_STATE);
+        require(e != nullptr || n == 0, "constraint array is null");
+        for (uint32_t i = 0; i < n; ++i) {
+            require(e[4 * i] < sim->bodies.size() && e[4 * i + 2] < sim->bodies.size(), "body index out of range");
+            require(e[4 * i + 1] < sim->bodies[e[4 * i]].n && e[4 * i + 3] < sim->bodies[e[4 * i + 2]].n,
+                    "vertex index out of range");
+            require(!(e[4 * i] == e[4 * i + 2] && e[4 * i + 1] == e[4 * i + 3]), "distance constraint joins a particle to itself");
+        }
+        group_t g;
+        g.kind = 2;
+        g.count = n;
+        g.alpha = alpha;
+        g.beta = beta;
+        g.conn.assign(e, e + 4 * std::size_t(n));
+        sim->groups.push_back(std::move(g));
+        if (group_id) *group_id = uint32_t(sim->groups.size() - 1);
+    });
+}
+
+int sbs_gpu_add_plane(sbs_gpu* sim, const double normal[3], const double point[3], const double vol_min[3],
+                      const double vol_max[3])
+{
+    return guarded([&] {
+        require(sim && normal && point && vol_min && vol_max, "null argument");
+        require(!sim->committed, "scene already committed", SBS_ERR_STATE);
+        sdf_dev f{};
+        f.kind = 0;
+        double const off = -(normal[0] * point[0] + normal[1] * point[1] + normal[2] * point[2]);
+        for (int i = 0; i < 3; ++i) {
+            f.a[i] = float(normal[i]);
+            f.vmin[i] = float(std::max(vol_min[i], -3.0e38));
+            f.vmax[i] = float(std::min(vol_max[i], 3.0e38));
+        }
+        f.a[3] = float(off);
+        sim->sdfs.push_back(f);
+    });
+}
+
+int sbs_gpu_add_sphere(sbs_gpu* sim, const double center[3], double radius)
+{
+    return guarded([&] {
+        require(sim && center, "null argument");
+        require(!sim->committed, "scene already committed", SBS_ERR_STATE);
+        require(radius > 0.0, "radius must be positive");
+        sdf_dev f{};
+        f.kind = 1;
+        for (int i = 0; i < 3; ++i) {
+            f.a[i] = float(center[i]);
+            f.vmin[i] = float(center[i] - radius);
+            f.vmax[i] = float(center[i] + radius);
+        }
+        f.a[3] = float(radius);
+        sim->sdfs.push_back(f);
+    });
+}
+
+int sbs_gpu_commit(sbs_gpu* sim, uint32_t recolor_passes)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(recolor_passes <= 64, "recolor_passes must be <= 64");
+        sim->commit(recolor_passes);
+    });
+}
+
+int sbs_gpu_step(sbs_gpu* sim, double dt, uint32_t iterations, uint32_t substeps)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
+        require(dt > 0.0 && std::isfinite(dt), "dt must be positive");
+        require(substeps >= 1, "substeps must be >= 1");
+        sim->set_current();
+        if (sim->use_graphs && !sim->timing) {
+            std::array<double, 16> key{};
+            key[0] = dt;
+            key[1] = iterations;
+            key[2] = substeps;
+            key[3] = sim->params.gravity[0];
+            key[4] = sim->params.gravity[1];
+            key[5] = sim->params.gravity[2];
+            key[6] = sim->params.gravity_is_force;
+            key[7] = sim->params.collision_compliance;
+            key[8] = sim->block;
+            if (!sim->graph_exec || key != sim->graph_key) {
+                if (sim->graph_exec) {
+                    hip_check(hipGraphExecDestroy(sim->graph_exec), "hipGraphExecDestroy");
+                    sim->graph_exec = nullptr;
+                }
+                hipGraph_t graph;
+                hip_check(hipStreamBeginCapture(sim->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+                try {
+                    sim->enqueue_frame(dt, iterations, substeps);
+                } catch (...) {
+                    hipGraph_t g2;
+                    (void)hipStreamEndCapture(sim->stream, &g2);
+                    if (g2) (void)hipGraphDestroy(g2);
+                    throw;
+                }
+                hip_check(hipStreamEndCapture(sim->stream, &graph), "hipStreamEndCapture");
+                hipError_t const e = hipGraphInstantiate(&sim->graph_exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                hip_check(e, "hipGraphInstantiate");
+                sim->graph_key = key;
+            }
+            hip_check(hipGraphLaunch(sim->graph_exec, sim->stream), "hipGraphLaunch");
+        } else {
+            sim->enqueue_frame(dt, iterations, substeps);
+            hip_check(hipGetLastError(), "kernel launch");
+        }
+    });
+}
+
+int sbs_gpu_synchronize(sbs_gpu* sim)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        hip_check(hipStreamSynchronize(sim->stream), "hipStreamSynchronize");
+    });
+}
+
+int sbs_gpu_use_graphs(sbs_gpu* sim, int enable)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        sim->use_graphs = enable != 0;
+    });
+}
+
+int sbs_gpu_read_body(sbs_gpu* sim, uint32_t body, double* x, double* v)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
+        require(body < sim->bodies.size(), "body index out of range");
+        body_t const& b = sim->bodies[body];
+        std::vector<float4> hx(b.n), hv(b.n);
+        if (b.n) {
+            hip_check(hipMemcpyAsync(hx.data(), sim->xn.p + b.offset, b.n * sizeof(float4), hipMemcpyDeviceToHost,
+                                     sim->stream), "read positions");
+            hip_check(hipMemcpyAsync(hv.data(), sim->vel.p + b.offset, b.n * sizeof(float4), hipMemcpyDeviceToHost,
+                                     sim->stream), "read velocities");
+        }
+        hip_check(hipStreamSynchronize(sim->stream), "read body");
+        for (uint32_t i = 0; i < b.n; ++i) {
+            if (x) {
+                x[3 * i] = hx[i].x;
+                x[3 * i + 1] = hx[i].y;
+                x[3 * i + 2] = hx[i].z;
+            }
+            if (v) {
+                v[3 * i] = hv[i].x;
+                v[3 * i + 1] = hv[i].y;
+                v[3 * i + 2] = hv[i].z;
+            }
+        }
+    });
+}
+
+int sbs_gpu_write_body(sbs_gpu* sim, uint32_t body, const double* x, const double* v)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
+        require(body < sim->bodies.size(), "body index out of range");
+        body_t const& b = sim->bodies[body];
+        if (b.n == 0) return;
+        if (x) {
+            std::vector<float4> hx(b.n);
+            hip_check(hipMemcpyAsync(hx.data(), sim->xn.p + b.offset, b.n * sizeof(float4), hipMemcpyDeviceToHost,
+                                     sim->stream), "read positions");
+            hip_check(hipStreamSynchronize(sim->stream), "write body");
+            for (uint32_t i = 0; i < b.n; ++i) {
+                require(std::isfinite(x[3 * i]) && std::isfinite(x[3 * i + 1]) && std::isfinite(x[3 * i + 2]),
+                        "positions must be finite");
+                hx[i] = make_float4(float(x[3 * i]), float(x[3 * i + 1]), float(x[3 * i + 2]), hx[i].w);
+            }
+            hip_check(hipMemcpyAsync(sim->xn.p + b.offset, hx.data(), b.n * sizeof(float4), hipMemcpyHostToDevice,
+                                     sim->stream), "write positions");
+            hip_check(hipMemcpyAsync(sim->xi.p + b.offset, hx.data(), b.n * sizeof(float4), hipMemcpyHostToDevice,
+                                     sim->stream), "write positions");
+            hip_check(hipStreamSynchronize(sim->stream), "write body");
+        }
+        if (v) {
+            std::vector<float4> hv(b.n);
+            for (uint32_t i = 0; i < b.n; ++i) hv[i] = make_float4(float(v[3 * i]), float(v[3 * i + 1]), float(v[3 * i + 2]), 0.f);
+            hip_check(hipMemcpyAsync(sim->vel.p + b.offset, hv.data(), b.n * sizeof(float4), hipMemcpyHostToDevice,
+                                     sim->stream), "write velocities");
+            hip_check(hipStreamSynchronize(sim->stream), "write body");
+        }
+    });
+}
+
+int sbs_gpu_group_colors(sbs_gpu* sim, uint32_t group, uint32_t* num_colors, uint32_t* color_of_constraint)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
+        require(group < sim->groups.size(), "group index out of range");
+        group_t const& g = sim->groups[group];
+        if (num_colors) *num_colors = g.ncolors;
+        if (color_of_constraint && !g.color.empty())
+            std::memcpy(color_of_constraint, g.color.data(), g.color.size() * sizeof(uint32_t));
+    });
+}
+
+int sbs_gpu_read_contacts(sbs_gpu* sim, uint32_t* ids, double* qn, uint32_t cap, uint32_t* count)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
+        require(count != nullptr, "count is null");
+        *count = 0;
+        if (!(sim->S && sim->slots)) return;
+        sim->detect();
+        std::vector<uint32_t> cnt(sim->S);
+        std::vector<float4> q(std::size_t(sim->S) * sim->slots), n(q.size());
+        hip_check(hipMemcpyAsync(cnt.data(), sim->ccount.p, sim->S * sizeof(uint32_t), hipMemcpyDeviceToHost, sim->stream), "read contacts");
+        hip_check(hipMemcpyAsync(q.data(), sim->cq.p, q.size() * sizeof(float4), hipMemcpyDeviceToHost, sim->stream), "read contacts");
+        hip_check(hipMemcpyAsync(n.data(), sim->cn.p, n.size() * sizeof(float4), hipMemcpyDeviceToHost, sim->stream), "read contacts");
+        hip_check(hipStreamSynchronize(sim->stream), "read contacts");
+        if (sim->timing) sim->harvest();
+        uint32_t k = 0, s = 0;
+        for (uint32_t bi = 0; bi < sim->bodies.size(); ++bi) {
+            body_t const& b = sim->bodies[bi];
+            for (uint32_t j = 0; j < b.surface.size(); ++j, ++s) {
+                for (uint32_t c = 0; c < cnt[s]; ++c, ++k) {
+                    if (k >= cap) continue;
+                    std::size_t const slot = std::size_t(c) * sim->S + s;
+                    if (ids) {
+                        ids[4 * std::size_t(k)] = bi;
+                        ids[4 * std::size_t(k) + 1] = j;
+                        ids[4 * std::size_t(k) + 2] = b.surface[j];
+                        ids[4 * std::size_t(k) + 3] = uint32_t(q[slot].w);
+                    }
+                    if (qn) {
+                        double* o = qn + 6 * std::size_t(k);
+                        o[0] = q[slot].x;
+                        o[1] = q[slot].y;
+                        o[2] = q[slot].z;
+                        o[3] = n[slot].x;
+                        o[4] = n[slot].y;
+                        o[5] = n[slot].z;
+                    }
+                }
+            }
+        }
+        *count = k;
+    });
+}
+
+int sbs_gpu_residual(sbs_gpu* sim, uint32_t group, double dt, double* per_color, uint32_t cap)
+{
+    return guarded([&] {
+        require(sim != nullptr && per_color != nullptr, "null argument");
+        require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
+        require(group < sim->groups.size(), "group index out of range");
+        require(dt > 0.0, "dt must be positive");
+        group_t const& g = sim->groups[group];
+        require(cap >= g.ncolors, "per_color has fewer entries than the group has colors");
+        double const dt2 = dt * dt;
+        for (uint32_t c = 0; c < cap; ++c) per_color[c] = 0.0;
+        for (uint32_t col = 0; col < g.ncolors; ++col) {
+            uint32_t const b = g.color_offsets[col], n = g.color_offsets[col + 1] - b;
+            uint32_t const blocks = (n + 255) / 256;
+            if (sim->partial.n < blocks) sim->partial.alloc(std::max<std::size_t>(blocks, 1024));
+            if (g.kind == 0)
+                launch_green_residual(sim->stream, g.tet(), sim->xn.p, b, n, float(g.alpha / dt2), sim->partial.p, 256);
+            else if (g.kind == 1)
+                launch_neohookean_residual(sim->stream, g.tet(), sim->xn.p, b, n, float(1.0 / dt2), sim->partial.p, 256);
+            else
+                launch_distance_residual(sim->stream, g.dist(), sim->xn.p, b, n, float(g.alpha / dt2), sim->partial.p, 256);
+            std::vector<double> h(blocks);
+            hip_check(hipMemcpyAsync(h.data(), sim->partial.p, blocks * sizeof(double), hipMemcpyDeviceToHost, sim->stream),
+                      "read residual");
+            hip_check(hipStreamSynchronize(sim->stream), "residual");
+            double acc = 0.0;
+            for (double x : h) acc += x;
+            per_color[col] = acc;
+        }
+    });
+}
+
+int sbs_gpu_energy(sbs_gpu* sim, double out[6])
+{
+    return guarded([&] {
+        require(sim != nullptr && out != nullptr, "null argument");
+        require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
+        for (int i = 0; i < 6; ++i) out[i] = 0.0;
+        auto run = [&](uint32_t n, auto&& launch) {
+            if (!n) return;
+            uint32_t const blocks = (n + 255) / 256;
+            if (sim->partial.n < std::size_t(blocks) * 6) sim->partial.alloc(std::max<std::size_t>(std::size_t(blocks) * 6, 1024));
+            launch();
+            std::vector<double> h(std::size_t(blocks) * 6);
+            hip_check(hipMemcpyAsync(h.data(), sim->partial.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, sim->stream),
+                      "read energy");
+            hip_check(hipStreamSynchronize(sim->stream), "energy");
+            for (std::size_t i = 0; i < h.size(); ++i) out[i % 6] += h[i];
+        };
+        sbs_xpbd_params const& p = sim->params;
+        run(sim->V, [&] {
+            launch_particle_energy(sim->stream, sim->xn.p, sim->vel.p, sim->mass.p, sim->V, float(p.gravity[0]),
+                                   float(p.gravity[1]), float(p.gravity[2]), p.gravity_is_force, sim->partial.p, 256);
+        });
+        for (auto const& g : sim->groups) {
+            if (g.kind == 0) run(g.count, [&] { launch_green_energy(sim->stream, g.tet(), sim->xn.p, g.count, sim->partial.p, 256); });
+            if (g.kind == 1)
+                run(g.count, [&] { launch_neohookean_energy(sim->stream, g.tet(), sim->xn.p, g.count, sim->partial.p, 256); });
+        }
+    });
+}
+
+int sbs_gpu_set_timing(sbs_gpu* sim, int enable)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        if (sim->timing && !enable) sim->harvest();
+        sim->timing = enable != 0;
+    });
+}
+
+int sbs_gpu_kernel_stats(sbs_gpu* sim, int kind, uint64_t* launches, double* total_ms, uint64_t* units)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(kind >= 0 && kind < SBS_KERNEL_COUNT, "kernel kind out of range");
+        sim->harvest();
+        if (launches) *launches = sim->st_launches[kind];
+        if (total_ms) *total_ms = sim->st_ms[kind];
+        if (units) *units = sim->st_units[kind];
+    });
+}
+
+int sbs_gpu_reset_stats(sbs_gpu* sim)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        sim->harvest();
+        for (int k = 0; k < SBS_KERNEL_COUNT; ++k) {
+            sim->st_launches[k] = 0;
+            sim->st_ms[k] = 0;
+            sim->st_units[k] = 0;
+        }
+    });
+}
+
+}  // extern "C"

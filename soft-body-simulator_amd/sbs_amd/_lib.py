@@ -1,0 +1,303 @@
+"""ctypes bindings of the C ABI in include/sbs_gpu.h (libsbs_gpu.so).
+
+This module is plumbing for tests and benchmarks; the product is the native library. Loading
+fails loudly when the library has not been built -- there is no CPU fallback.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REPO_DIR = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "_build", "libsbs_gpu.so")
+HEADER_PATH = os.path.join(REPO_DIR, "include", "sbs_gpu.h")
+
+SBS_OK = 0
+KERNEL_PREDICT, KERNEL_CONTACT, KERNEL_GREEN, KERNEL_NEOHOOKEAN, KERNEL_DISTANCE, KERNEL_COMMIT, KERNEL_DETECT = range(7)
+KERNEL_NAMES = ["predict", "contact", "green", "neohookean", "distance", "commit", "detect"]
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+_u32 = ctypes.c_uint32
+_p = ctypes.c_void_p
+_pd = ctypes.POINTER(ctypes.c_double)
+_pu = ctypes.POINTER(ctypes.c_uint32)
+_pu64 = ctypes.POINTER(ctypes.c_uint64)
+
+
+class XpbdParams(ctypes.Structure):
+    _fields_ = [
+        ("compliance", ctypes.c_double),
+        ("damping", ctypes.c_double),
+        ("young_modulus", ctypes.c_double),
+        ("poisson_ratio", ctypes.c_double),
+        ("hooke_coefficient", ctypes.c_double),
+        ("collision_compliance", ctypes.c_double),
+        ("collision_damping", ctypes.c_double),
+        ("gravity", ctypes.c_double * 3),
+        ("gravity_is_force", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+_SIGS = {
+    "sbs_gpu_default_params": [ctypes.POINTER(XpbdParams)],
+    "sbs_gpu_create": [_i, ctypes.POINTER(_p)],
+    "sbs_gpu_destroy": [_p],
+    "sbs_gpu_set_params": [_p, ctypes.POINTER(XpbdParams)],
+    "sbs_gpu_add_body": [_p, _pd, _pd, _u32, _pu],
+    "sbs_gpu_set_collision_surface": [_p, _u32, _pu, _u32],
+    "sbs_gpu_add_green_constraints": [_p, _u32, _pu, _u32, _d, _d, _d, _d, _pu],
+    "sbs_gpu_add_neohookean_constraints": [_p, _u32, _pu, _u32, _d, _d, _d, _pu],
+    "sbs_gpu_add_distance_constraints": [_p, _pu, _u32, _d, _d, _pu],
+    "sbs_gpu_add_plane": [_p, _pd, _pd, _pd, _pd],
+    "sbs_gpu_add_sphere": [_p, _pd, _d],
+    "sbs_gpu_commit": [_p, _u32],
+    "sbs_gpu_step": [_p, _d, _u32, _u32],
+    "sbs_gpu_synchronize": [_p],
+    "sbs_gpu_use_graphs": [_p, _i],
+    "sbs_gpu_read_body": [_p, _u32, _pd, _pd],
+    "sbs_gpu_write_body": [_p, _u32, _pd, _pd],
+    "sbs_gpu_group_colors": [_p, _u32, _pu, _pu],
+    "sbs_gpu_read_contacts": [_p, _pu, _pd, _u32, _pu],
+    "sbs_gpu_residual": [_p, _u32, _d, _pd, _u32],
+    "sbs_gpu_energy": [_p, _pd],
+    "sbs_gpu_set_timing": [_p, _i],
+    "sbs_gpu_kernel_stats": [_p, _i, _pu64, _pd, _pu64],
+    "sbs_gpu_reset_stats": [_p],
+    "sbs_mesh_bar5": [_u32, _u32, _u32, _pd, _pu],
+    "sbs_mesh_box6": [_u32, _u32, _u32, _d, _pd, _pd, _pu],
+    "sbs_mesh_boundary": [_u32, _pu, _u32, _pu, _pu, _pu, _pu],
+    "sbs_mesh_lumped_mass": [_u32, _pd, _pu, _u32, _d, _pd],
+}
+
+_lib = None
+
+
+class SbsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"sbs error {code}: {msg}")
+        self.code = code
+
+
+def header_functions():
+    """Names of the functions include/sbs_gpu.h declares."""
+    text = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(sbs_\w+)\s*\(", text, re.M)))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(
+                f"{LIB_PATH} is missing: build the HIP library first (python -c 'import __graft_entry__ as g; g.build()')")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = _i
+            f.argtypes = args
+        L.sbs_gpu_last_error.restype = ctypes.c_char_p
+        L.sbs_gpu_last_error.argtypes = []
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != SBS_OK:
+        raise SbsError(code, lib().sbs_gpu_last_error().decode())
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(_pd)
+
+
+def _up(a):
+    return None if a is None else a.ctypes.data_as(_pu)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+# ------------------------------------------------------------------ host mesh utilities
+
+def bar5(w, h, d):
+    nt = max(w - 1, 0) * max(h - 1, 0) * max(d - 1, 0) * 5
+    pos = np.zeros(w * h * d * 3)
+    tets = np.zeros(nt * 4, dtype=np.uint32)
+    check(lib().sbs_mesh_bar5(w, h, d, _dp(pos), _up(tets)))
+    return pos.reshape(-1, 3), tets.reshape(-1, 4)
+
+
+def box6(nx, ny, nz, edge, origin=(0.0, 0.0, 0.0)):
+    nv = (nx + 1) * (ny + 1) * (nz + 1)
+    pos = np.zeros(nv * 3)
+    tets = np.zeros(nx * ny * nz * 24, dtype=np.uint32)
+    check(lib().sbs_mesh_box6(nx, ny, nz, edge, _dp(f64(origin)), _dp(pos), _up(tets)))
+    return pos.reshape(-1, 3), tets.reshape(-1, 4)
+
+
+def boundary(nv, tets):
+    tets = u32(tets).reshape(-1)
+    ns, ntr = _u32(), _u32()
+    check(lib().sbs_mesh_boundary(nv, _up(tets), len(tets) // 4, None, None, ctypes.byref(ns), ctypes.byref(ntr)))
+    s2t = np.zeros(ns.value, dtype=np.uint32)
+    tris = np.zeros(ntr.value * 3, dtype=np.uint32)
+    check(lib().sbs_mesh_boundary(nv, _up(tets), len(tets) // 4, _up(s2t), _up(tris), ctypes.byref(ns),
+                                  ctypes.byref(ntr)))
+    return s2t, tris.reshape(-1, 3)
+
+
+def lumped_mass(pos, tets, density):
+    pos = f64(pos).reshape(-1)
+    tets = u32(tets).reshape(-1)
+    m = np.zeros(len(pos) // 3)
+    check(lib().sbs_mesh_lumped_mass(len(m), _dp(pos), _up(tets), len(tets) // 4, density, _dp(m)))
+    return m
+
+
+# ------------------------------------------------------------------ simulation handle
+
+class Simulation:
+    """A device simulation behind the C ABI (one sbs_gpu handle)."""
+
+    def __init__(self, device=0):
+        self._L = lib()
+        h = _p()
+        check(self._L.sbs_gpu_create(device, ctypes.byref(h)))
+        self.h = h
+        self.bodies = []
+        self.groups = []  # constraint counts
+        self.params = XpbdParams()
+        check(self._L.sbs_gpu_default_params(ctypes.byref(self.params)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(self._L.sbs_gpu_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, gravity=(0.0, -9.81, 0.0), gravity_is_force=True, collision_compliance=1e-8,
+                   collision_damping=0.0):
+        p = self.params
+        p.gravity[0], p.gravity[1], p.gravity[2] = gravity
+        p.gravity_is_force = int(gravity_is_force)
+        p.collision_compliance = collision_compliance
+        p.collision_damping = collision_damping
+        check(self._L.sbs_gpu_set_params(self.h, ctypes.byref(p)))
+
+    def add_body(self, x0, mass=None):
+        x0 = f64(x0).reshape(-1)
+        m = None if mass is None else f64(mass)
+        b = _u32()
+        check(self._L.sbs_gpu_add_body(self.h, _dp(x0), _dp(m), len(x0) // 3, ctypes.byref(b)))
+        self.bodies.append(len(x0) // 3)
+        return b.value
+
+    def set_surface(self, body, s2t):
+        s2t = u32(s2t)
+        check(self._L.sbs_gpu_set_collision_surface(self.h, body, _up(s2t), len(s2t)))
+
+    def _group(self, code, out, n):
+        check(code)
+        self.groups.append(n)
+        return out.value
+
+    def add_green(self, body, tets, alpha, beta, young, poisson):
+        t = u32(tets).reshape(-1)
+        g = _u32()
+        return self._group(self._L.sbs_gpu_add_green_constraints(self.h, body, _up(t), len(t) // 4, alpha, beta, young,
+                                                                 poisson, ctypes.byref(g)), g, len(t) // 4)
+
+    def add_neohookean(self, body, tets, beta, young, poisson):
+        t = u32(tets).reshape(-1)
+        g = _u32()
+        return self._group(self._L.sbs_gpu_add_neohookean_constraints(self.h, body, _up(t), len(t) // 4, beta, young,
+                                                                      poisson, ctypes.byref(g)), g, len(t) // 4)
+
+    def add_distance(self, b1v1b2v2, alpha, beta):
+        e = u32(b1v1b2v2).reshape(-1)
+        g = _u32()
+        return self._group(self._L.sbs_gpu_add_distance_constraints(self.h, _up(e), len(e) // 4, alpha, beta,
+                                                                    ctypes.byref(g)), g, len(e) // 4)
+
+    def add_plane(self, normal, point, vmin=(-1e30,) * 3, vmax=(1e30,) * 3):
+        check(self._L.sbs_gpu_add_plane(self.h, _dp(f64(normal)), _dp(f64(point)), _dp(f64(vmin)), _dp(f64(vmax))))
+
+    def add_sphere(self, center, radius):
+        check(self._L.sbs_gpu_add_sphere(self.h, _dp(f64(center)), radius))
+
+    def commit(self, recolor_passes=2):
+        check(self._L.sbs_gpu_commit(self.h, recolor_passes))
+
+    def use_graphs(self, enable=True):
+        check(self._L.sbs_gpu_use_graphs(self.h, int(enable)))
+
+    def step(self, dt, iterations, substeps):
+        check(self._L.sbs_gpu_step(self.h, dt, iterations, substeps))
+
+    def synchronize(self):
+        check(self._L.sbs_gpu_synchronize(self.h))
+
+    def read_body(self, body):
+        n = self.bodies[body]
+        x = np.zeros(n * 3)
+        v = np.zeros(n * 3)
+        check(self._L.sbs_gpu_read_body(self.h, body, _dp(x), _dp(v)))
+        return x.reshape(-1, 3), v.reshape(-1, 3)
+
+    def write_body(self, body, x=None, v=None):
+        x = None if x is None else f64(x).reshape(-1)
+        v = None if v is None else f64(v).reshape(-1)
+        check(self._L.sbs_gpu_write_body(self.h, body, _dp(x), _dp(v)))
+
+    def group_colors(self, group):
+        nc = _u32()
+        c = np.zeros(self.groups[group], dtype=np.uint32)
+        check(self._L.sbs_gpu_group_colors(self.h, group, ctypes.byref(nc), _up(c)))
+        return c, nc.value
+
+    def read_contacts(self):
+        cnt = _u32()
+        check(self._L.sbs_gpu_read_contacts(self.h, None, None, 0, ctypes.byref(cnt)))
+        n = cnt.value
+        ids = np.zeros(max(n, 1) * 4, dtype=np.uint32)
+        qn = np.zeros(max(n, 1) * 6)
+        check(self._L.sbs_gpu_read_contacts(self.h, _up(ids), _dp(qn), n, ctypes.byref(cnt)))
+        return ids[:4 * n].reshape(-1, 4), qn[:6 * n].reshape(-1, 6)
+
+    def residual(self, group, dt):
+        _, nc = self.group_colors(group)
+        out = np.zeros(max(nc, 1))
+        check(self._L.sbs_gpu_residual(self.h, group, dt, _dp(out), len(out)))
+        return out[:nc]
+
+    def energy(self):
+        out = np.zeros(6)
+        check(self._L.sbs_gpu_energy(self.h, _dp(out)))
+        return out
+
+    def set_timing(self, enable):
+        check(self._L.sbs_gpu_set_timing(self.h, int(enable)))
+
+    def reset_stats(self):
+        check(self._L.sbs_gpu_reset_stats(self.h))
+
+    def kernel_stats(self, kind):
+        launches = ctypes.c_uint64()
+        ms = ctypes.c_double()
+        units = ctypes.c_uint64()
+        check(self._L.sbs_gpu_kernel_stats(self.h, kind, ctypes.byref(launches), ctypes.byref(ms), ctypes.byref(units)))
+        return launches.value, ms.value, units.value

@@ -1,0 +1,153 @@
+"""Scene descriptions: the reference demo scene and the benchmark workloads.
+
+A Scene is plain data (numpy arrays and numbers). `Scene.apply(sim)` issues the scene through any
+object with the simulation call set of include/sbs_gpu.h (add_body, set_surface, add_green, ...),
+so the same description drives the device library and, in tests, the CPU oracle.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+
+
+@dataclass
+class Body:
+    x0: np.ndarray          # (n, 3) rest positions
+    tets: np.ndarray        # (t, 4) body-local tets
+    mass: np.ndarray = None  # (n,) or None for unit mass
+    surface: np.ndarray = None  # surface vertex -> body vertex
+
+
+@dataclass
+class Group:
+    kind: str               # "green" | "neohookean" | "distance"
+    body: int
+    conn: np.ndarray        # tets (t, 4) or (b1, v1, b2, v2) rows
+    alpha: float = 0.0
+    beta: float = 0.0
+    young: float = 1e6
+    poisson: float = 0.3
+
+
+@dataclass
+class Scene:
+    name: str
+    bodies: list = field(default_factory=list)
+    groups: list = field(default_factory=list)
+    planes: list = field(default_factory=list)   # (normal, point, vmin, vmax)
+    spheres: list = field(default_factory=list)  # (center, radius)
+    gravity: tuple = (0.0, -9.81, 0.0)
+    gravity_is_force: bool = True
+    collision_compliance: float = 1e-8
+    dt: float = 1.0 / 60.0
+    iterations: int = 10
+    substeps: int = 1
+
+    @property
+    def num_tets(self):
+        return sum(len(g.conn) for g in self.groups if g.kind != "distance")
+
+    @property
+    def num_vertices(self):
+        return sum(len(b.x0) for b in self.bodies)
+
+    def apply(self, sim):
+        sim.set_params(gravity=self.gravity, gravity_is_force=self.gravity_is_force,
+                       collision_compliance=self.collision_compliance)
+        for b in self.bodies:
+            bi = sim.add_body(b.x0, b.mass)
+            if b.surface is not None:
+                sim.set_surface(bi, b.surface)
+        for g in self.groups:
+            if g.kind == "green":
+                sim.add_green(g.body, g.conn, g.alpha, g.beta, g.young, g.poisson)
+            elif g.kind == "neohookean":
+                sim.add_neohookean(g.body, g.conn, g.beta, g.young, g.poisson)
+            elif g.kind == "distance":
+                sim.add_distance(g.conn, g.alpha, g.beta)
+            else:
+                raise ValueError(g.kind)
+        for n, p, lo, hi in self.planes:
+            sim.add_plane(n, p, lo, hi)
+        for c, r in self.spheres:
+            sim.add_sphere(c, r)
+
+    def bbox_diagonal(self):
+        x = np.concatenate([b.x0 for b in self.bodies])
+        return float(np.linalg.norm(x.max(0) - x.min(0)))
+
+
+def build_device(scene, device=0, recolor_passes=2):
+    sim = _lib.Simulation(device)
+    scene.apply(sim)
+    sim.commit(recolor_passes)
+    return sim
+
+
+def _rotation(axis, angle):
+    a = np.asarray(axis, dtype=np.float64)
+    a = a / np.linalg.norm(a)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + np.sin(angle) * K + (1 - np.cos(angle)) * (K @ K)
+
+
+def reference_demo():
+    """main.cpp:22-124: get_simple_bar_model(4, 4, 12), moved by T(-10, 5, -1) R(pi/2 about
+    (0, 1, 0.2)) S(1, 0.8, 2), one green constraint per tet with simulation_parameters_t defaults,
+    a floor plane through the origin with normal +y and volume [-20,-5,-20]-[20,5,20], unit masses,
+    gravity as a 9.81 N force, dt 0.016, 5 iterations, 1 substep."""
+    pos, tets = _lib.bar5(4, 4, 12)
+    R = _rotation((0.0, 1.0, 0.2), 3.14159 / 2.0)
+    x0 = (R @ (pos * np.array([1.0, 0.8, 2.0])).T).T + np.array([-10.0, 5.0, -1.0])
+    s2t, _ = _lib.boundary(len(pos), tets)
+    sc = Scene("reference_demo", dt=0.016, iterations=5, substeps=1, gravity_is_force=True)
+    sc.bodies.append(Body(x0=x0, tets=tets, surface=s2t))
+    sc.groups.append(Group("green", 0, tets, alpha=1e-4, beta=0.0, young=1e6, poisson=0.30))
+    sc.planes.append(((0.0, 1.0, 0.0), (0.0, 0.0, 0.0), (-20.0, -5.0, -20.0), (20.0, 5.0, 20.0)))
+    return sc
+
+
+def cantilever(nx, ny, nz, edge=0.1, model="green", young=1e6, poisson=0.45, density=1000.0, alpha=1e-4,
+               beta=0.0, ground_y=-1.0, num_spheres=0, seed=0x5B5, dt=1.0 / 60.0, iterations=10, substeps=1,
+               name=None):
+    """Box of nx*ny*nz hex cells of edge `edge` from the origin, 6 tets per cell, lumped masses of
+    the given density, the x = 0 face pinned (mass 0), gravity as an acceleration, a ground plane
+    at y = ground_y and `num_spheres` static spheres of radius U[0.05, 0.2] m under the beam."""
+    pos, tets = _lib.box6(nx, ny, nz, edge)
+    mass = _lib.lumped_mass(pos, tets, density)
+    mass[pos[:, 0] <= pos[:, 0].min()] = 0.0
+    s2t, _ = _lib.boundary(len(pos), tets)
+    sc = Scene(name or f"cantilever_{nx}x{ny}x{nz}_{model}", dt=dt, iterations=iterations, substeps=substeps,
+               gravity_is_force=False)
+    sc.bodies.append(Body(x0=pos, tets=tets, mass=mass, surface=s2t))
+    sc.groups.append(Group(model, 0, tets, alpha=alpha, beta=beta, young=young, poisson=poisson))
+    sc.planes.append(((0.0, 1.0, 0.0), (0.0, ground_y, 0.0), (-1e30,) * 3, (1e30,) * 3))
+    if num_spheres:
+        rng = np.random.default_rng(seed)
+        L, H, W = nx * edge, ny * edge, nz * edge
+        for _ in range(num_spheres):
+            r = rng.uniform(0.05, 0.2)
+            c = (rng.uniform(0.2 * L, L), rng.uniform(ground_y + r, -0.05), rng.uniform(0.0, W))
+            sc.spheres.append((c, r))
+    return sc
+
+
+def cfg0():
+    """Plain XPBD cantilever: 10x3x3 cells of 0.1 m, 6 tets each (540 tets, 176 vertices), left
+    face pinned, green constraints E=1e6, nu=0.45, density 1000, ground at y=-1, dt 1/60,
+    1 substep x 10 iterations."""
+    return cantilever(10, 3, 3, 0.1, "green", 1e6, 0.45, 1000.0, iterations=10, substeps=1, name="cfg0_beam_540")
+
+
+def cfg1_1m():
+    """Large beam, 1M-tet variant: 220x28x27 cells (997,920 tets, 179,452 vertices), stable
+    neo-Hookean E=5e5, nu=0.45, 4 substeps x 8 iterations, ground plane and 16 static spheres."""
+    return cantilever(220, 28, 27, 0.1, "neohookean", 5e5, 0.45, 1000.0, num_spheres=16, iterations=8, substeps=4,
+                      name="cfg1_beam_1M_neohookean")
+
+
+def cfg1_6m():
+    """Large beam: 400x50x50 cells (6.0M tets, 1,043,001 vertices), same material and solver."""
+    return cantilever(400, 50, 50, 0.1, "neohookean", 5e5, 0.45, 1000.0, num_spheres=16, iterations=8, substeps=4,
+                      name="cfg1_beam_6M_neohookean")

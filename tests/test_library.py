@@ -1,0 +1,70 @@
+"""CPU checks of the product library: it loads, exports every symbol include/sbs_gpu.h declares,
+its host mesh utilities match the oracle bit for bit, and argument errors come back as status
+codes. No device work is done here."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as O
+import sbs_amd as S
+from sbs_amd import scenes
+
+
+def test_library_exports_every_header_symbol():
+    L = S.lib()
+    names = S.header_functions()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(L, n), n
+
+
+@pytest.mark.parametrize("dims", [(12, 4, 4), (5, 2, 2), (2, 2, 2), (7, 3, 5), (1, 3, 3)])
+def test_bar5_matches_oracle(dims):
+    a, b = S.bar5(*dims), O.bar5(*dims)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("dims", [(10, 3, 3), (4, 5, 6), (1, 1, 1)])
+def test_box6_and_boundary_match_oracle(dims):
+    a = S.box6(*dims, 0.1, (0.3, -0.2, 0.1))
+    b = O.box6(*dims, 0.1, (0.3, -0.2, 0.1))
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    s1, t1 = S.boundary(len(a[0]), a[1])
+    s2, t2, _ = O.boundary(len(a[0]), a[1])
+    assert np.array_equal(s1, s2) and np.array_equal(t1, t2)
+
+
+def test_boundary_of_reference_bar_matches_oracle():
+    p, t = S.bar5(12, 4, 4)
+    s1, t1 = S.boundary(len(p), t)
+    s2, t2, _ = O.boundary(len(p), t)
+    assert np.array_equal(s1, s2) and np.array_equal(t1, t2)
+
+
+def test_lumped_mass_totals_density_times_volume():
+    p, t = S.box6(5, 4, 3, 0.2)
+    m = S.lumped_mass(p, t, 750.0)
+    assert np.isclose(m.sum(), 750.0 * 1.0 * 0.8 * 0.6, rtol=1e-12)
+    # a corner vertex of the lattice belongs to 1 or 2 tets of its cell (the diagonal corners to 6)
+    assert m.min() > 0
+
+
+def test_mesh_errors_are_status_codes():
+    L = S.lib()
+    tets = np.array([0, 1, 2, 9], dtype=np.uint32)
+    ns, nt = ctypes.c_uint32(), ctypes.c_uint32()
+    code = L.sbs_mesh_boundary(4, tets.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), 1, None, None,
+                               ctypes.byref(ns), ctypes.byref(nt))
+    assert code == 1
+    assert b"out of range" in L.sbs_gpu_last_error()
+
+
+def test_benchmark_scene_sizes():
+    sc = scenes.cfg0()
+    assert (sc.num_vertices, sc.num_tets, len(sc.bodies[0].surface)) == (176, 540, 140)
+    pinned = sc.bodies[0].mass == 0
+    assert pinned.sum() == 4 * 4 and np.all(sc.bodies[0].x0[pinned, 0] == 0.0)
+    assert np.isclose(sc.bodies[0].mass.sum() + 0, 1000.0 * 1.0 * 0.3 * 0.3 - 0, rtol=0.2)
+    demo = scenes.reference_demo()
+    assert (demo.num_vertices, demo.num_tets) == (192, 495)
