@@ -106,8 +106,26 @@ int sbs_gpu_add_plane(sbs_gpu* sim, const double normal[3], const double point[3
 /* Analytic sphere obstacle, sd = |p - c| - r (an analytic sdf_model_t, sdf_model.cpp:149-153). */
 int sbs_gpu_add_sphere(sbs_gpu* sim, const double center[3], double radius);
 
-/* Colors every constraint group on the device and uploads the scene. recolor_passes >= 0 extra
- * Jones-Plassmann passes (0 = hash order only). After this, no more scene additions. */
+/* Partitions a tet group's constraints (in the order they were added) into clusters of
+ * consecutive constraints: cluster i is [cluster_offsets[i], cluster_offsets[i+1]), with
+ * cluster_offsets[0] = 0 and cluster_offsets[num_clusters] = the group's constraint count. One
+ * device thread runs a cluster's constraints one after the other; clusters (not constraints) are
+ * colored, so the Gauss-Seidel sweep visits colors, then clusters, then the cluster's constraints
+ * in order. A cluster may touch at most 8 particles (e.g. the 6 tets of a hex cell). No reference
+ * counterpart: it only reorders the reference's sequential sweep (gauss_seidel_solver.cpp:28-35). */
+int sbs_gpu_set_group_clusters(sbs_gpu* sim, uint32_t group, const uint32_t* cluster_offsets, uint32_t num_clusters);
+
+/* Coloring order used by sbs_gpu_commit: */
+#define SBS_COLORING_HASH 0  /* device Jones-Plassmann, priorities hash32(index) (default) */
+#define SBS_COLORING_INDEX 1 /* sequential first-fit greedy in index order (host, at commit) */
+int sbs_gpu_set_coloring(sbs_gpu* sim, int order);
+
+/* Threads per block of the per-constraint kernels (64/128/256) and of the cluster kernels
+ * (64/128). Default 256 and 64. */
+int sbs_gpu_set_launch_config(sbs_gpu* sim, int block, int cluster_block);
+
+/* Colors every constraint group and uploads the scene. recolor_passes: extra Jones-Plassmann
+ * passes of SBS_COLORING_HASH (0 = hash order only). After this, no more scene additions. */
 int sbs_gpu_commit(sbs_gpu* sim, uint32_t recolor_passes);
 
 /* ---- time stepping ---------------------------------------------------------------------------- */

@@ -3,6 +3,7 @@
 //
 // TEST INFRASTRUCTURE ONLY: the oracle is the checker, never the thing measured as the product or
 // shipped in it.
+#include <algorithm>
 #include <cstring>
 #include <stdexcept>
 
@@ -156,6 +157,19 @@ static void constraint_vertices(simulation_t const& s, group_t const& g, std::ve
     }
 }
 
+void orc_set_clusters(void* h, uint32_t group, const uint32_t* offsets, uint32_t n)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    auto& g = s->groups.at(group);
+    if (n == 0)
+        g.cluster_off.clear();
+    else
+        g.cluster_off.assign(offsets, offsets + n + 1);
+    s->order_dirty = true;
+}
+
+void orc_set_coloring(void* h, int order) { static_cast<simulation_t*>(h)->coloring = order; }
+
 // colored != 0 selects the colored visiting order (and computes the coloring of every group).
 void orc_set_order(void* h, int colored, uint32_t recolor_passes)
 {
@@ -168,7 +182,31 @@ void orc_set_order(void* h, int colored, uint32_t recolor_passes)
             uint32_t arity;
             constraint_vertices(*s, g, keys, &arity);
             g.color.assign(g.count, 0);
-            g.num_colors = g.count ? greedy_color(uint32_t(g.count), arity, keys.data(), recolor_passes, g.color.data()) : 0;
+            g.num_colors = 0;
+            if (!g.count) continue;
+            // elements: clusters, or single constraints; element vertex lists without duplicates
+            std::size_t const ne = g.cluster_off.empty() ? g.count : g.cluster_off.size() - 1;
+            std::vector<uint32_t> eoff(ne + 1, 0);
+            std::vector<uint64_t> ev;
+            for (std::size_t e = 0; e < ne; ++e) {
+                std::size_t const b = g.cluster_off.empty() ? e : g.cluster_off[e];
+                std::size_t const en = g.cluster_off.empty() ? e + 1 : g.cluster_off[e + 1];
+                std::size_t const start = ev.size();
+                for (std::size_t c = b; c < en; ++c)
+                    for (uint32_t a = 0; a < arity; ++a) {
+                        uint64_t const k = keys[c * arity + a];
+                        if (std::find(ev.begin() + start, ev.end(), k) == ev.end()) ev.push_back(k);
+                    }
+                eoff[e + 1] = uint32_t(ev.size());
+            }
+            std::vector<uint32_t> ec(ne);
+            g.num_colors = greedy_color_csr(uint32_t(ne), eoff.data(), ev.data(), recolor_passes, s->coloring == 1,
+                                            ec.data());
+            for (std::size_t e = 0; e < ne; ++e) {
+                std::size_t const b = g.cluster_off.empty() ? e : g.cluster_off[e];
+                std::size_t const en = g.cluster_off.empty() ? e + 1 : g.cluster_off[e + 1];
+                for (std::size_t c = b; c < en; ++c) g.color[c] = ec[e];
+            }
         }
     }
     s->order_dirty = true;

@@ -36,6 +36,8 @@ _SIGS = {
     "orc_add_plane": (None, [_p, _pd, _pd, _pd, _pd]),
     "orc_add_sphere": (None, [_p, _pd, _d]),
     "orc_set_order": (None, [_p, _i, _u32]),
+    "orc_set_clusters": (None, [_p, _u32, _pu, _u32]),
+    "orc_set_coloring": (None, [_p, _i]),
     "orc_group_colors": (_u32, [_p, _u32, _pu]),
     "orc_step": (None, [_p, _d, _u32, _u32]),
     "orc_read_body": (None, [_p, _u32, _pd, _pd]),
@@ -214,6 +216,14 @@ class Simulation:
 
     def add_sphere(self, center, radius):
         self._L.orc_add_sphere(self.h, _dp(f64(center)), radius)
+
+    def set_clusters(self, group, offsets):
+        off = u32(offsets)
+        self._L.orc_set_clusters(self.h, group, _up(off), len(off) - 1)
+
+    def set_coloring(self, order):
+        """order: "hash" or "index" (sequential greedy in index order)."""
+        self._L.orc_set_coloring(self.h, {"hash": 0, "index": 1}[order])
 
     def set_order(self, colored, recolor_passes=2):
         self._L.orc_set_order(self.h, int(colored), recolor_passes)

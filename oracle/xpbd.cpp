@@ -340,10 +340,16 @@ void simulation_t::build_order()
     if (!colored_order) {
         for (uint32_t i = 0; i < constraints.size(); ++i) order.push_back(i);
     } else {
-        for (auto const& g : groups)
+        for (auto const& g : groups) {
+            std::size_t const ne = g.cluster_off.empty() ? g.count : g.cluster_off.size() - 1;
             for (uint32_t c = 0; c < g.num_colors; ++c)
-                for (std::size_t i = 0; i < g.count; ++i)
-                    if (g.color[i] == c) order.push_back(uint32_t(g.first + i));
+                for (std::size_t e = 0; e < ne; ++e) {
+                    std::size_t const b = g.cluster_off.empty() ? e : g.cluster_off[e];
+                    std::size_t const en = g.cluster_off.empty() ? e + 1 : g.cluster_off[e + 1];
+                    if (g.color[b] != c) continue;
+                    for (std::size_t i = b; i < en; ++i) order.push_back(uint32_t(g.first + i));
+                }
+        }
     }
     order_dirty = false;
 }
@@ -400,41 +406,43 @@ uint32_t hash32(uint32_t x)
     return x;
 }
 
-uint32_t greedy_color(uint32_t n, uint32_t arity, const uint64_t* verts, uint32_t passes, uint32_t* color)
+uint32_t greedy_color_csr(uint32_t n, const uint32_t* eoff, const uint64_t* verts, uint32_t passes, bool index_order,
+                          uint32_t* color)
 {
-    // vertex -> constraints incidence
-    std::vector<uint64_t> keys(std::size_t(n) * arity);
-    for (std::size_t i = 0; i < keys.size(); ++i) keys[i] = verts[i];
-    std::vector<uint64_t> uniq = keys;
+    std::size_t const nv = eoff[n];
+    // vertex keys -> dense ids, then vertex -> elements incidence
+    std::vector<uint64_t> uniq(verts, verts + nv);
     std::sort(uniq.begin(), uniq.end());
     uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
-    std::vector<uint32_t> vid(keys.size());
-    for (std::size_t i = 0; i < keys.size(); ++i)
-        vid[i] = uint32_t(std::lower_bound(uniq.begin(), uniq.end(), keys[i]) - uniq.begin());
-    std::vector<uint32_t> off(uniq.size() + 1, 0), inc(keys.size());
+    std::vector<uint32_t> vid(nv);
+    for (std::size_t i = 0; i < nv; ++i)
+        vid[i] = uint32_t(std::lower_bound(uniq.begin(), uniq.end(), verts[i]) - uniq.begin());
+    std::vector<uint32_t> off(uniq.size() + 1, 0), inc(nv);
     for (uint32_t x : vid) off[x + 1]++;
     for (std::size_t i = 0; i < uniq.size(); ++i) off[i + 1] += off[i];
     std::vector<uint32_t> fill(off.begin(), off.end() - 1);
     for (uint32_t t = 0; t < n; ++t)
-        for (uint32_t a = 0; a < arity; ++a) inc[fill[vid[std::size_t(t) * arity + a]]++] = t;
+        for (uint32_t a = eoff[t]; a < eoff[t + 1]; ++a) inc[fill[vid[a]]++] = t;
 
     std::vector<uint32_t> prev(n, 0);
     uint32_t ncolors = 0;
-    for (uint32_t pass = 0; pass <= passes; ++pass) {
-        std::vector<uint64_t> key(n);
-        for (uint32_t t = 0; t < n; ++t) key[t] = (uint64_t(pass == 0 ? 0 : prev[t]) << 32) | hash32(t);
+    uint32_t const npass = index_order ? 1 : passes + 1;
+    for (uint32_t pass = 0; pass < npass; ++pass) {
         std::vector<uint32_t> ord(n);
         for (uint32_t t = 0; t < n; ++t) ord[t] = t;
-        std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) {
-            return key[a] != key[b] ? key[a] > key[b] : a < b;
-        });
+        if (!index_order) {
+            std::vector<uint64_t> key(n);
+            for (uint32_t t = 0; t < n; ++t) key[t] = (uint64_t(pass == 0 ? 0 : prev[t]) << 32) | hash32(t);
+            std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) {
+                return key[a] != key[b] ? key[a] > key[b] : a < b;
+            });
+        }
         std::vector<int64_t> col(n, -1);
-        std::vector<uint32_t> stamp;
         ncolors = 0;
         for (uint32_t t : ord) {
             std::vector<bool> used;
-            for (uint32_t a = 0; a < arity; ++a) {
-                uint32_t const x = vid[std::size_t(t) * arity + a];
+            for (uint32_t a = eoff[t]; a < eoff[t + 1]; ++a) {
+                uint32_t const x = vid[a];
                 for (uint32_t e = off[x]; e < off[x + 1]; ++e) {
                     uint32_t const u = inc[e];
                     if (u == t || col[u] < 0) continue;
@@ -451,6 +459,13 @@ uint32_t greedy_color(uint32_t n, uint32_t arity, const uint64_t* verts, uint32_
     }
     std::copy(prev.begin(), prev.end(), color);
     return ncolors;
+}
+
+uint32_t greedy_color(uint32_t n, uint32_t arity, const uint64_t* verts, uint32_t passes, uint32_t* color)
+{
+    std::vector<uint32_t> eoff(std::size_t(n) + 1);
+    for (uint32_t t = 0; t <= n; ++t) eoff[t] = t * arity;
+    return greedy_color_csr(n, eoff.data(), verts, passes, false, color);
 }
 
 }  // namespace oracle

@@ -124,9 +124,13 @@ struct contact_t {
 
 // One constraint group = one batch of constraints added together; the colored visiting order
 // processes groups in insertion order and, inside a group, colors in increasing order.
+// A group may be partitioned into clusters of consecutive constraints; the colored order then
+// colors clusters and visits colors, clusters of a color in index order, and the constraints of a
+// cluster in order.
 struct group_t {
     int kind;                // 0 green, 1 neo-Hookean, 2 distance
     std::size_t first, count;  // range in simulation_t::constraints
+    std::vector<uint32_t> cluster_off;  // empty: one cluster per constraint
     std::vector<uint32_t> color;  // per constraint of the group
     uint32_t num_colors = 0;
 };
@@ -145,6 +149,7 @@ struct simulation_t {
     double collision_compliance = 1e-8, collision_damping = 0.0;  // simulation_parameters.h:26-27
     bool colored_order = false;
     uint32_t recolor_passes = 2;
+    int coloring = 0;  // 0: hash priorities (+ recolor passes), 1: index order
 
     // GS visiting order of the elastic constraints (indices into constraints)
     std::vector<uint32_t> order;
@@ -156,11 +161,14 @@ struct simulation_t {
     void step(double dt, std::size_t iterations, std::size_t substeps);  // timestep_t::step
 };
 
-// Greedy coloring of a constraint group in priority order, restating the device Jones-Plassmann
-// coloring: pass 0 visits constraints by descending hash32(index); pass r>0 visits by descending
-// (previous color, hash32(index)). Each visited constraint takes the smallest color not used by
-// an already-colored constraint sharing a vertex with it.
+// Greedy coloring of elements (constraints or clusters) in priority order, restating the device
+// Jones-Plassmann coloring: pass 0 visits elements by descending hash32(index); pass r>0 visits by
+// descending (previous color, hash32(index)). With index_order the single pass visits elements in
+// increasing index. Each visited element takes the smallest color not used by an already-colored
+// element sharing a vertex with it. Element e touches verts[eoff[e] .. eoff[e+1]).
 uint32_t hash32(uint32_t x);
+uint32_t greedy_color_csr(uint32_t n, const uint32_t* eoff, const uint64_t* verts, uint32_t passes, bool index_order,
+                          uint32_t* color);
 uint32_t greedy_color(uint32_t n, uint32_t arity, const uint64_t* verts, uint32_t passes,
                       uint32_t* color);
 

@@ -25,6 +25,18 @@ struct tet_group_dev {
     float* lambda;  // green: 1 per tet, neo-Hookean: 2 per tet (deviatoric, hydrostatic)
 };
 
+// Clusters of a tet group: cluster c runs constraints [off[c], off[c+1]) one after the other in
+// one thread; its (at most 8) particles are v[c].ids (-1 = unused slot) and every constraint names
+// its 4 particles by local slot (loc: 4 x 8 bits). Constraint arrays are in execution order.
+struct cluster_dev {
+    const uint32_t* off;
+    const int4* v0;  // slots 0..3
+    const int4* v1;  // slots 4..7
+    const uint32_t* loc;
+};
+
+constexpr int kClusterSlots = 8;
+
 struct distance_group_dev {
     const uint2* idx;
     const float* rest;
@@ -51,6 +63,12 @@ void launch_green(hipStream_t s, tet_group_dev g, float4* xi, const float4* xn, 
                   float alpha_tilde, float gamma, int first, int block);
 void launch_neohookean(hipStream_t s, tet_group_dev g, float4* xi, const float4* xn, uint32_t begin,
                        uint32_t count, float inv_dt2, float beta_dt, int first, int block);
+// One thread per cluster of clusters [begin, begin+count).
+void launch_green_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, float4* xi, const float4* xn,
+                           uint32_t begin, uint32_t count, float alpha_tilde, float gamma, int first, int block);
+void launch_neohookean_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, float4* xi, const float4* xn,
+                                uint32_t begin, uint32_t count, float inv_dt2, float beta_dt, int first, int block);
+
 void launch_distance(hipStream_t s, distance_group_dev g, float4* xi, const float4* xn, uint32_t begin,
                      uint32_t count, float alpha_tilde, float gamma, int first, int block);
 
@@ -73,8 +91,10 @@ void launch_green_energy(hipStream_t s, tet_group_dev g, const float4* x, uint32
 void launch_neohookean_energy(hipStream_t s, tet_group_dev g, const float4* x, uint32_t count, double* partial,
                               int block);
 
-// One Jones-Plassmann round over a constraint group (arity vertices per constraint).
-void launch_jp_round(hipStream_t s, uint32_t n, uint32_t arity, const uint32_t* verts, const uint32_t* inc_off,
+// One Jones-Plassmann round over n elements; element t touches vertices
+// verts[elem_off[t] .. elem_off[t+1]) (elem_off == nullptr: `arity` vertices each).
+void launch_jp_round(hipStream_t s, uint32_t n, uint32_t arity, const uint32_t* elem_off, const uint32_t* verts,
+                     const uint32_t* inc_off,
                      const uint32_t* inc, const uint64_t* key, const uint32_t* color_in, uint32_t* color_out,
                      uint32_t* remaining, uint32_t* overflow, int block);
 void launch_jp_keys(hipStream_t s, uint32_t n, const uint32_t* prev, uint64_t* key, int block);

@@ -232,6 +232,121 @@ __global__ void __launch_bounds__(256) k_neohookean(tet_group_dev g, float4* xi,
     xi[id.w] = P[3];
 }
 
+// ---------------------------------------------------------------- clustered Gauss-Seidel
+
+// One thread per cluster: the cluster's particles are staged in LDS (slot-major, so that the
+// threads of a wave hit consecutive 16-byte words), its constraints run in order against the
+// staged copies, and the particles are written back once. Clusters of one color share no
+// particle, so this equals the sequential sweep over the color's clusters.
+template <int BLOCK>
+struct cluster_stage {
+    float4 p[kClusterSlots][BLOCK];
+};
+
+template <int BLOCK, class Project>
+__device__ __forceinline__ void run_cluster(tet_group_dev const& g, cluster_dev const& cl, float4* xi,
+                                            uint32_t begin, uint32_t count, Project&& project)
+{
+    __shared__ cluster_stage<BLOCK> st;
+    uint32_t const i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= count) return;
+    uint32_t const c = begin + i;
+    int4 const a = cl.v0[c], b = cl.v1[c];
+    int const ids[kClusterSlots] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    float4 (*P)[BLOCK] = st.p;
+#pragma unroll
+    for (int k = 0; k < kClusterSlots; ++k)
+        if (ids[k] >= 0) P[k][threadIdx.x] = xi[ids[k]];
+    uint32_t const t0 = cl.off[c], t1 = cl.off[c + 1];
+    for (uint32_t t = t0; t < t1; ++t) {
+        uint32_t const loc = cl.loc[t];
+        uint32_t const s0 = loc & 0xffu, s1 = (loc >> 8) & 0xffu, s2 = (loc >> 16) & 0xffu, s3 = loc >> 24;
+        float4 Q[4] = {P[s0][threadIdx.x], P[s1][threadIdx.x], P[s2][threadIdx.x], P[s3][threadIdx.x]};
+        project(t, Q, s0, s1, s2, s3, ids);
+        P[s0][threadIdx.x] = Q[0];
+        P[s1][threadIdx.x] = Q[1];
+        P[s2][threadIdx.x] = Q[2];
+        P[s3][threadIdx.x] = Q[3];
+    }
+#pragma unroll
+    for (int k = 0; k < kClusterSlots; ++k)
+        if (ids[k] >= 0) xi[ids[k]] = P[k][threadIdx.x];
+}
+
+__device__ __forceinline__ int slot_id(int const ids[kClusterSlots], uint32_t s)
+{
+    int r = ids[0];
+#pragma unroll
+    for (int k = 1; k < kClusterSlots; ++k) r = (s == uint32_t(k)) ? ids[k] : r;
+    return r;
+}
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_green_clusters(tet_group_dev g, cluster_dev cl, float4* xi,
+                                                          const float4* __restrict__ xn, uint32_t begin,
+                                                          uint32_t count, float alpha_tilde, float gamma, int first)
+{
+    run_cluster<BLOCK>(g, cl, xi, begin, count,
+                       [&](uint32_t t, float4 P[4], uint32_t s0, uint32_t s1, uint32_t s2, uint32_t s3,
+                           int const ids[kClusterSlots]) {
+        float4 const a = g.m0[t], b = g.m1[t], c = g.m2[t];
+        float const lm = first ? 0.0f : g.lambda[t];
+        tet_rows const R = load_rows(a, b, c);
+        v3 f[4];
+        float const C = green_core(xyz(P[0]), xyz(P[1]), xyz(P[2]), xyz(P[3]), R, c.y, c.z, c.w, f);
+        float const wsum = P[0].w * dot(f[0], f[0]) + P[1].w * dot(f[1], f[1]) + P[2].w * dot(f[2], f[2]) +
+                           P[3].w * dot(f[3], f[3]);
+        if (wsum < 1e-20f) {
+            if (first) g.lambda[t] = 0.0f;
+            return;
+        }
+        float gdd = 0.0f;
+        if (gamma != 0.0f) {
+            uint32_t const sl[4] = {s0, s1, s2, s3};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) gdd += dot(f[k], xyz(P[k]) - xyz(xn[slot_id(ids, sl[k])]));
+        }
+        float const dl = (-(C + alpha_tilde * lm) + gamma * gdd) / ((1.0f + gamma) * wsum + alpha_tilde);
+        g.lambda[t] = lm + dl;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) P[k] = pack(xyz(P[k]) - (P[k].w * dl) * f[k], P[k].w);
+    });
+}
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_neohookean_clusters(tet_group_dev g, cluster_dev cl, float4* xi,
+                                                               const float4* __restrict__ xn, uint32_t begin,
+                                                               uint32_t count, float inv_dt2, float beta_dt,
+                                                               int first)
+{
+    run_cluster<BLOCK>(g, cl, xi, begin, count,
+                       [&](uint32_t t, float4 P[4], uint32_t s0, uint32_t s1, uint32_t s2, uint32_t s3,
+                           int const ids[kClusterSlots]) {
+        float4 const a = g.m0[t], b = g.m1[t], c = g.m2[t];
+        float2 lm = first ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(g.lambda)[t];
+        tet_rows const R = load_rows(a, b, c);
+        int4 const id = make_int4(slot_id(ids, s0), slot_id(ids, s1), slot_id(ids, s2), slot_id(ids, s3));
+        {
+            m3 const F = deformation_gradient(xyz(P[0]) - xyz(P[3]), xyz(P[1]) - xyz(P[3]), xyz(P[2]) - xyz(P[3]),
+                                              R.r0, R.r1, R.r2);
+            float const C = sqrtf(dot(F.c0, F.c0) + dot(F.c1, F.c1) + dot(F.c2, F.c2));
+            float const inv = C > 0.0f ? 1.0f / C : 0.0f;
+            m3 const G = {inv * F.c0, inv * F.c1, inv * F.c2};
+            float const at = c.y * inv_dt2;
+            tet_update(P, xn, id, G, R, C, at, at * beta_dt, lm.x);
+        }
+        {
+            m3 const F = deformation_gradient(xyz(P[0]) - xyz(P[3]), xyz(P[1]) - xyz(P[3]), xyz(P[2]) - xyz(P[3]),
+                                              R.r0, R.r1, R.r2);
+            m3 const G = cofactor(F);
+            float const C = dot(F.c0, G.c0) - c.w;
+            float const at = c.z * inv_dt2;
+            tet_update(P, xn, id, G, R, C, at, at * beta_dt, lm.y);
+        }
+        reinterpret_cast<float2*>(g.lambda)[t] = lm;
+    });
+}
+
 // ---------------------------------------------------------------- distance constraint
 
 // distance_constraint.cpp:24-53 (no guard against zero length, as there).
@@ -509,7 +624,8 @@ __global__ void k_jp_keys(uint32_t n, const uint32_t* __restrict__ prev, uint64_
 // A constraint whose key exceeds the keys of all its uncolored neighbours takes the smallest color
 // unused by its colored neighbours. Over all rounds this equals sequential greedy coloring in
 // descending key order: when a constraint is colored, exactly its higher-key neighbours are.
-__global__ void k_jp_round(uint32_t n, uint32_t arity, const uint32_t* __restrict__ verts,
+__global__ void k_jp_round(uint32_t n, uint32_t arity, const uint32_t* __restrict__ elem_off,
+                           const uint32_t* __restrict__ verts,
                            const uint32_t* __restrict__ inc_off, const uint32_t* __restrict__ inc,
                            const uint64_t* __restrict__ key, const uint32_t* __restrict__ color_in,
                            uint32_t* __restrict__ color_out, uint32_t* remaining, uint32_t* overflow)
@@ -523,8 +639,10 @@ __global__ void k_jp_round(uint32_t n, uint32_t arity, const uint32_t* __restric
     }
     uint64_t const kt = key[t];
     uint64_t used[kMaxColors / 64] = {0, 0, 0, 0};
-    for (uint32_t a = 0; a < arity; ++a) {
-        uint32_t const v = verts[size_t(t) * arity + a];
+    size_t const a0 = elem_off ? elem_off[t] : size_t(t) * arity;
+    size_t const a1 = elem_off ? elem_off[t + 1] : a0 + arity;
+    for (size_t a = a0; a < a1; ++a) {
+        uint32_t const v = verts[a];
         uint32_t const e1 = inc_off[v + 1];
         for (uint32_t e = inc_off[v]; e < e1; ++e) {
             uint32_t const u = inc[e];
@@ -646,13 +764,34 @@ void launch_neohookean_energy(hipStream_t s, tet_group_dev g, const float4* x, u
     if (count) k_neohookean_energy<<<grid_for(count, kRedBlock), kRedBlock, 0, s>>>(g, x, count, partial);
 }
 
-void launch_jp_round(hipStream_t s, uint32_t n, uint32_t arity, const uint32_t* verts, const uint32_t* inc_off,
-                     const uint32_t* inc, const uint64_t* key, const uint32_t* color_in, uint32_t* color_out,
-                     uint32_t* remaining, uint32_t* overflow, int block)
+void launch_jp_round(hipStream_t s, uint32_t n, uint32_t arity, const uint32_t* elem_off, const uint32_t* verts,
+                     const uint32_t* inc_off, const uint32_t* inc, const uint64_t* key, const uint32_t* color_in,
+                     uint32_t* color_out, uint32_t* remaining, uint32_t* overflow, int block)
 {
     if (n)
-        k_jp_round<<<grid_for(n, block), block, 0, s>>>(n, arity, verts, inc_off, inc, key, color_in, color_out,
-                                                        remaining, overflow);
+        k_jp_round<<<grid_for(n, block), block, 0, s>>>(n, arity, elem_off, verts, inc_off, inc, key, color_in,
+                                                        color_out, remaining, overflow);
+}
+
+void launch_green_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, float4* xi, const float4* xn,
+                           uint32_t begin, uint32_t count, float alpha_tilde, float gamma, int first, int block)
+{
+    if (!count) return;
+    if (block <= 64)
+        k_green_clusters<64><<<grid_for(count, 64), 64, 0, s>>>(g, cl, xi, xn, begin, count, alpha_tilde, gamma, first);
+    else
+        k_green_clusters<128><<<grid_for(count, 128), 128, 0, s>>>(g, cl, xi, xn, begin, count, alpha_tilde, gamma, first);
+}
+
+void launch_neohookean_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, float4* xi, const float4* xn,
+                                uint32_t begin, uint32_t count, float inv_dt2, float beta_dt, int first, int block)
+{
+    if (!count) return;
+    if (block <= 64)
+        k_neohookean_clusters<64><<<grid_for(count, 64), 64, 0, s>>>(g, cl, xi, xn, begin, count, inv_dt2, beta_dt, first);
+    else
+        k_neohookean_clusters<128><<<grid_for(count, 128), 128, 0, s>>>(g, cl, xi, xn, begin, count, inv_dt2, beta_dt,
+                                                                       first);
 }
 
 void launch_jp_keys(hipStream_t s, uint32_t n, const uint32_t* prev, uint64_t* key, int block)

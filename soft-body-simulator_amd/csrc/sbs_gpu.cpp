@@ -78,22 +78,29 @@ struct group_t {
     uint32_t body = 0, count = 0;
     double alpha = 0, beta = 0, young = 0, poisson = 0;
     std::vector<uint32_t> conn;  // 4 body-local ids per tet, or (b1, v1, b2, v2) per distance
+    std::vector<uint32_t> cluster_off;  // empty: every constraint is its own cluster
     // after commit
     uint32_t ncolors = 0;
     std::vector<uint32_t> color;          // per constraint, insertion order
-    std::vector<uint32_t> color_offsets;  // ncolors + 1, positions in color order
+    std::vector<uint32_t> color_offsets;  // ncolors + 1, constraint positions in execution order
+    std::vector<uint32_t> cluster_color_offsets;  // ncolors + 1, cluster positions in execution order
+    dev_buf<uint32_t> cl_off, cl_loc;
+    dev_buf<int4> cl_v0, cl_v1;
     dev_buf<int4> idx;
     dev_buf<float4> m0, m1, m2;
     dev_buf<uint2> didx;
     dev_buf<float> rest;
     dev_buf<float> lambda;
+    bool clustered() const { return !cluster_off.empty(); }
+    uint32_t num_elements() const { return clustered() ? uint32_t(cluster_off.size() - 1) : count; }
     tet_group_dev tet() const { return {idx.p, m0.p, m1.p, m2.p, lambda.p}; }
+    cluster_dev clusters() const { return {cl_off.p, cl_v0.p, cl_v1.p, cl_loc.p}; }
     distance_group_dev dist() const { return {didx.p, rest.p, lambda.p}; }
 };
 
 struct timing_rec {
     int kind;
-    uint64_t units;
+    uint64_t launches, units;
     hipEvent_t a, b;
 };
 
@@ -124,6 +131,8 @@ struct sbs_gpu {
     dev_buf<double> partial;
 
     int block = 256;
+    int cluster_block = 64;
+    int coloring_order = SBS_COLORING_HASH;
 
     bool timing = false;
     std::vector<timing_rec> pending;
@@ -160,15 +169,16 @@ struct sbs_gpu {
         return e;
     }
 
-    // Runs `launch` on the stream, bracketed by events when timing is on.
+    // Runs `launch` (which issues `launches` back-to-back kernels of one kind) on the stream,
+    // bracketed by one pair of events when timing is on.
     template <class F>
-    void timed(int kind, uint64_t units, F&& launch)
+    void timed(int kind, uint64_t units, F&& launch, uint64_t launches = 1)
     {
         if (!timing) {
             launch();
             return;
         }
-        timing_rec r{kind, units, take_event(), take_event()};
+        timing_rec r{kind, launches, units, take_event(), take_event()};
         hip_check(hipEventRecord(r.a, stream), "hipEventRecord");
         launch();
         hip_check(hipEventRecord(r.b, stream), "hipEventRecord");
@@ -181,7 +191,7 @@ struct sbs_gpu {
         for (auto& r : pending) {
             float ms = 0;
             hip_check(hipEventElapsedTime(&ms, r.a, r.b), "hipEventElapsedTime");
-            st_launches[r.kind] += 1;
+            st_launches[r.kind] += r.launches;
             st_ms[r.kind] += ms;
             st_units[r.kind] += r.units;
             event_pool.push_back(r.a);
@@ -218,63 +228,102 @@ void lame(double young, double poisson, double* mu, double* lam)
 
 }  // namespace
 
-// Device Jones-Plassmann coloring: pass 0 keys are hash32(index); each recolor pass keys on the
-// previous pass's color (higher first) then hash32(index), which never increases the color count.
+// Colors the elements of a group (its clusters, or its constraints when it has none) so that
+// elements of one color share no particle. SBS_COLORING_HASH: device Jones-Plassmann, pass 0 keyed
+// on hash32(index), each recolor pass keyed on (previous color, hash32(index)) -- never more
+// colors than the pass before. SBS_COLORING_INDEX: sequential first-fit greedy in index order
+// (inherently serial; computed on the host at commit).
 void sbs_gpu::color_group(group_t& g, uint32_t recolor_passes)
 {
-    uint32_t const n = g.count;
-    uint32_t const arity = g.kind == 2 ? 2 : 4;
-    std::vector<uint32_t> verts(std::size_t(n) * arity);
-    for (uint32_t c = 0; c < n; ++c) {
+    uint32_t const ne = g.num_elements();
+    // element -> particles (CSR), duplicates within an element removed
+    std::vector<uint32_t> eoff(std::size_t(ne) + 1, 0), verts;
+    verts.reserve(std::size_t(g.count) * 4);
+    auto constraint_particles = [&](uint32_t c, uint32_t out[4]) -> int {
         if (g.kind == 2) {
-            verts[2 * c] = global_vertex(*this, g.conn[4 * c], g.conn[4 * c + 1]);
-            verts[2 * c + 1] = global_vertex(*this, g.conn[4 * c + 2], g.conn[4 * c + 3]);
-        } else {
-            for (int k = 0; k < 4; ++k) verts[4 * std::size_t(c) + k] = global_vertex(*this, g.body, g.conn[4 * std::size_t(c) + k]);
+            out[0] = global_vertex(*this, g.conn[4 * c], g.conn[4 * c + 1]);
+            out[1] = global_vertex(*this, g.conn[4 * c + 2], g.conn[4 * c + 3]);
+            return 2;
         }
+        for (int k = 0; k < 4; ++k) out[k] = global_vertex(*this, g.body, g.conn[4 * std::size_t(c) + k]);
+        return 4;
+    };
+    for (uint32_t e = 0; e < ne; ++e) {
+        uint32_t const c0 = g.clustered() ? g.cluster_off[e] : e;
+        uint32_t const c1 = g.clustered() ? g.cluster_off[e + 1] : e + 1;
+        std::size_t const start = verts.size();
+        for (uint32_t c = c0; c < c1; ++c) {
+            uint32_t pv[4];
+            int const np = constraint_particles(c, pv);
+            for (int k = 0; k < np; ++k)
+                if (std::find(verts.begin() + start, verts.end(), pv[k]) == verts.end()) verts.push_back(pv[k]);
+        }
+        eoff[e + 1] = uint32_t(verts.size());
     }
-    // vertex -> constraints incidence (CSR)
+    // particle -> elements incidence (CSR)
     std::vector<uint32_t> off(std::size_t(V) + 1, 0), inc(verts.size());
     for (uint32_t x : verts) off[x + 1]++;
     for (std::size_t i = 0; i < V; ++i) off[i + 1] += off[i];
     {
         std::vector<uint32_t> fill(off.begin(), off.end() - 1);
-        for (uint32_t c = 0; c < n; ++c)
-            for (uint32_t a = 0; a < arity; ++a) inc[fill[verts[std::size_t(c) * arity + a]]++] = c;
+        for (uint32_t e = 0; e < ne; ++e)
+            for (uint32_t a = eoff[e]; a < eoff[e + 1]; ++a) inc[fill[verts[a]]++] = e;
     }
-    dev_buf<uint32_t> dverts, doff, dinc, ca, cb, prev, counters;
-    dev_buf<uint64_t> key;
-    dverts.upload(verts.data(), verts.size(), stream);
-    doff.upload(off.data(), off.size(), stream);
-    dinc.upload(inc.data(), inc.size(), stream);
-    ca.alloc(n);
-    cb.alloc(n);
-    prev.alloc(n);
-    key.alloc(n);
-    counters.alloc(2);
-    for (uint32_t pass = 0; pass <= recolor_passes; ++pass) {
-        launch_jp_keys(stream, n, pass == 0 ? nullptr : prev.p, key.p, 256);
-        launch_fill_u32(stream, ca.p, kUncolored, n);
-        uint32_t h[2] = {1, 0};
-        uint32_t rounds = 0;
-        while (h[0] != 0) {
-            require(++rounds < 1000000, "coloring did not terminate", SBS_ERR_INTERNAL);
-            launch_fill_u32(stream, counters.p, 0u, 2);
-            launch_jp_round(stream, n, arity, dverts.p, doff.p, dinc.p, key.p, ca.p, cb.p, counters.p, counters.p + 1,
-                            256);
-            hip_check(hipMemcpyAsync(h, counters.p, sizeof h, hipMemcpyDeviceToHost, stream), "download counters");
-            hip_check(hipStreamSynchronize(stream), "coloring round");
-            require(h[1] == 0, "constraint group needs more than 256 colors");
-            std::swap(ca, cb);
+    std::vector<uint32_t> ecolor(ne);
+    if (coloring_order == SBS_COLORING_INDEX) {
+        std::vector<uint32_t> stamp(kMaxColors + 1, UINT32_MAX);
+        for (uint32_t e = 0; e < ne; ++e) {
+            for (uint32_t a = eoff[e]; a < eoff[e + 1]; ++a)
+                for (uint32_t j = off[verts[a]]; j < off[verts[a] + 1]; ++j) {
+                    uint32_t const u = inc[j];
+                    if (u < e) stamp[ecolor[u]] = e;
+                }
+            uint32_t c = 0;
+            while (stamp[c] == e) ++c;
+            require(c < kMaxColors, "constraint group needs more than 256 colors");
+            ecolor[e] = c;
         }
-        hip_check(hipMemcpyAsync(prev.p, ca.p, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream), "copy colors");
+    } else {
+        dev_buf<uint32_t> deoff, dverts, doff, dinc, ca, cb, prev, counters;
+        dev_buf<uint64_t> key;
+        deoff.upload(eoff.data(), eoff.size(), stream);
+        dverts.upload(verts.data(), verts.size(), stream);
+        doff.upload(off.data(), off.size(), stream);
+        dinc.upload(inc.data(), inc.size(), stream);
+        ca.alloc(ne);
+        cb.alloc(ne);
+        prev.alloc(ne);
+        key.alloc(ne);
+        counters.alloc(2);
+        for (uint32_t pass = 0; pass <= recolor_passes; ++pass) {
+            launch_jp_keys(stream, ne, pass == 0 ? nullptr : prev.p, key.p, 256);
+            launch_fill_u32(stream, ca.p, kUncolored, ne);
+            uint32_t h[2] = {1, 0};
+            uint32_t rounds = 0;
+            while (h[0] != 0) {
+                require(++rounds < 1000000, "coloring did not terminate", SBS_ERR_INTERNAL);
+                launch_fill_u32(stream, counters.p, 0u, 2);
+                launch_jp_round(stream, ne, 0, deoff.p, dverts.p, doff.p, dinc.p, key.p, ca.p, cb.p, counters.p,
+                                counters.p + 1, 256);
+                hip_check(hipMemcpyAsync(h, counters.p, sizeof h, hipMemcpyDeviceToHost, stream), "download counters");
+                hip_check(hipStreamSynchronize(stream), "coloring round");
+                require(h[1] == 0, "constraint group needs more than 256 colors");
+                std::swap(ca, cb);
+            }
+            hip_check(hipMemcpyAsync(prev.p, ca.p, ne * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream), "copy colors");
+        }
+        hip_check(hipMemcpyAsync(ecolor.data(), prev.p, ne * sizeof(uint32_t), hipMemcpyDeviceToHost, stream),
+                  "download colors");
+        hip_check(hipStreamSynchronize(stream), "coloring");
     }
-    g.color.resize(n);
-    hip_check(hipMemcpyAsync(g.color.data(), prev.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream),
-              "download colors");
-    hip_check(hipStreamSynchronize(stream), "coloring");
     g.ncolors = 0;
-    for (uint32_t c : g.color) g.ncolors = std::max(g.ncolors, c + 1);
+    for (uint32_t c : ecolor) g.ncolors = std::max(g.ncolors, c + 1);
+    g.color.resize(g.count);
+    for (uint32_t e = 0; e < ne; ++e) {
+        uint32_t const c0 = g.clustered() ? g.cluster_off[e] : e;
+        uint32_t const c1 = g.clustered() ? g.cluster_off[e + 1] : e + 1;
+        for (uint32_t c = c0; c < c1; ++c) g.color[c] = ecolor[e];
+    }
 }
 
 // Lays a group out in color order (stable by insertion index) and uploads it in FP32; the rest
@@ -282,13 +331,62 @@ void sbs_gpu::color_group(group_t& g, uint32_t recolor_passes)
 void sbs_gpu::upload_group(group_t& g)
 {
     uint32_t const n = g.count;
+    uint32_t const ne = g.num_elements();
+    auto e_begin = [&](uint32_t e) { return g.clustered() ? g.cluster_off[e] : e; };
+    auto e_end = [&](uint32_t e) { return g.clustered() ? g.cluster_off[e + 1] : e + 1; };
+    // execution order: elements by color (stable by index), constraints of an element in order
+    g.cluster_color_offsets.assign(g.ncolors + 1, 0);
     g.color_offsets.assign(g.ncolors + 1, 0);
-    for (uint32_t c : g.color) g.color_offsets[c + 1]++;
-    for (uint32_t c = 0; c < g.ncolors; ++c) g.color_offsets[c + 1] += g.color_offsets[c];
-    std::vector<uint32_t> order(n);
+    for (uint32_t e = 0; e < ne; ++e) {
+        uint32_t const c = g.color[e_begin(e)];
+        g.cluster_color_offsets[c + 1]++;
+        g.color_offsets[c + 1] += e_end(e) - e_begin(e);
+    }
+    for (uint32_t c = 0; c < g.ncolors; ++c) {
+        g.cluster_color_offsets[c + 1] += g.cluster_color_offsets[c];
+        g.color_offsets[c + 1] += g.color_offsets[c];
+    }
+    std::vector<uint32_t> eorder(ne);
     {
-        std::vector<uint32_t> fill(g.color_offsets.begin(), g.color_offsets.end() - 1);
-        for (uint32_t i = 0; i < n; ++i) order[fill[g.color[i]]++] = i;
+        std::vector<uint32_t> fill(g.cluster_color_offsets.begin(), g.cluster_color_offsets.end() - 1);
+        for (uint32_t e = 0; e < ne; ++e) eorder[fill[g.color[e_begin(e)]]++] = e;
+    }
+    std::vector<uint32_t> order;
+    order.reserve(n);
+    for (uint32_t e : eorder)
+        for (uint32_t c = e_begin(e); c < e_end(e); ++c) order.push_back(c);
+    if (g.clustered()) {
+        std::vector<uint32_t> off(ne + 1, 0), loc(n);
+        std::vector<int4> v0(ne), v1(ne);
+        uint32_t pos = 0;
+        body_t const& b = bodies[g.body];
+        for (uint32_t j = 0; j < ne; ++j) {
+            uint32_t const e = eorder[j];
+            int slots[kClusterSlots];
+            int used = 0;
+            for (uint32_t c = e_begin(e); c < e_end(e); ++c, ++pos) {
+                uint32_t l = 0;
+                for (int k = 0; k < 4; ++k) {
+                    int const gv = int(b.offset + g.conn[4 * std::size_t(c) + k]);
+                    int s = 0;
+                    while (s < used && slots[s] != gv) ++s;
+                    if (s == used) {
+                        require(used < kClusterSlots, "a cluster touches more than 8 particles");
+                        slots[used++] = gv;
+                    }
+                    l |= uint32_t(s) << (8 * k);
+                }
+                loc[pos] = l;
+            }
+            for (int s = used; s < kClusterSlots; ++s) slots[s] = -1;
+            v0[j] = make_int4(slots[0], slots[1], slots[2], slots[3]);
+            v1[j] = make_int4(slots[4], slots[5], slots[6], slots[7]);
+            off[j + 1] = pos;
+        }
+        g.cl_off.upload(off.data(), off.size(), stream);
+        g.cl_loc.upload(loc.data(), loc.size(), stream);
+        g.cl_v0.upload(v0.data(), v0.size(), stream);
+        g.cl_v1.upload(v1.data(), v1.size(), stream);
     }
     if (g.kind == 2) {
         std::vector<uint2> idx(n);
@@ -431,22 +529,28 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
             for (auto const& g : groups) {
                 double const at = g.alpha / dt2;
                 double const gamma = at * (g.beta * dt2) / dt;
-                for (uint32_t col = 0; col < g.ncolors; ++col) {
-                    uint32_t const b = g.color_offsets[col], n = g.color_offsets[col + 1] - b;
-                    if (g.kind == 0)
-                        timed(SBS_KERNEL_GREEN, n, [&] {
+                int const kind = g.kind == 0 ? SBS_KERNEL_GREEN : (g.kind == 1 ? SBS_KERNEL_NEOHOOKEAN : SBS_KERNEL_DISTANCE);
+                // the colors of a group run back to back; timing brackets the whole run
+                timed(kind, g.count, [&] {
+                    for (uint32_t col = 0; col < g.ncolors; ++col) {
+                        uint32_t const b = g.color_offsets[col], n = g.color_offsets[col + 1] - b;
+                        uint32_t const cb = g.cluster_color_offsets[col];
+                        uint32_t const cn = g.cluster_color_offsets[col + 1] - cb;
+                        if (g.clustered() && g.kind == 0)
+                            launch_green_clusters(stream, g.tet(), g.clusters(), xi.p, xn.p, cb, cn, float(at),
+                                                  float(gamma), first, cluster_block);
+                        else if (g.clustered() && g.kind == 1)
+                            launch_neohookean_clusters(stream, g.tet(), g.clusters(), xi.p, xn.p, cb, cn,
+                                                       float(1.0 / dt2), float(g.beta * dt), first, cluster_block);
+                        else if (g.kind == 0)
                             launch_green(stream, g.tet(), xi.p, xn.p, b, n, float(at), float(gamma), first, block);
-                        });
-                    else if (g.kind == 1)
-                        timed(SBS_KERNEL_NEOHOOKEAN, n, [&] {
+                        else if (g.kind == 1)
                             launch_neohookean(stream, g.tet(), xi.p, xn.p, b, n, float(1.0 / dt2), float(g.beta * dt),
                                               first, block);
-                        });
-                    else
-                        timed(SBS_KERNEL_DISTANCE, n, [&] {
+                        else
                             launch_distance(stream, g.dist(), xi.p, xn.p, b, n, float(at), float(gamma), first, block);
-                        });
-                }
+                    }
+                }, g.ncolors);
             }
         }
     }
@@ -647,6 +751,49 @@ int sbs_gpu_add_sphere(sbs_gpu* sim, const double center[3], double radius)
     });
 }
 
+int sbs_gpu_set_group_clusters(sbs_gpu* sim, uint32_t group, const uint32_t* cluster_offsets, uint32_t num_clusters)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(!sim->committed, "scene already committed", SBS_ERR_STATE);
+        require(group < sim->groups.size(), "group index out of range");
+        group_t& g = sim->groups[group];
+        require(g.kind != 2, "clusters are supported for tet constraint groups");
+        if (num_clusters == 0) {
+            require(g.count == 0 || cluster_offsets == nullptr, "a non-empty group needs at least one cluster");
+            g.cluster_off.clear();
+            return;
+        }
+        require(cluster_offsets != nullptr, "cluster_offsets is null");
+        require(cluster_offsets[0] == 0 && cluster_offsets[num_clusters] == g.count,
+                "cluster offsets must start at 0 and end at the group's constraint count");
+        for (uint32_t i = 0; i < num_clusters; ++i)
+            require(cluster_offsets[i + 1] > cluster_offsets[i], "clusters must be non-empty and increasing");
+        g.cluster_off.assign(cluster_offsets, cluster_offsets + num_clusters + 1);
+    });
+}
+
+int sbs_gpu_set_coloring(sbs_gpu* sim, int order)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(!sim->committed, "scene already committed", SBS_ERR_STATE);
+        require(order == SBS_COLORING_HASH || order == SBS_COLORING_INDEX, "unknown coloring order");
+        sim->coloring_order = order;
+    });
+}
+
+int sbs_gpu_set_launch_config(sbs_gpu* sim, int block, int cluster_block)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(block == 64 || block == 128 || block == 256, "block must be 64, 128 or 256");
+        require(cluster_block == 64 || cluster_block == 128, "cluster_block must be 64 or 128");
+        sim->block = block;
+        sim->cluster_block = cluster_block;
+    });
+}
+
 int sbs_gpu_commit(sbs_gpu* sim, uint32_t recolor_passes)
 {
     return guarded([&] {
@@ -675,6 +822,7 @@ int sbs_gpu_step(sbs_gpu* sim, double dt, uint32_t iterations, uint32_t substeps
             key[6] = sim->params.gravity_is_force;
             key[7] = sim->params.collision_compliance;
             key[8] = sim->block;
+            key[9] = sim->cluster_block;
             if (!sim->graph_exec || key != sim->graph_key) {
                 if (sim->graph_exec) {
                     hip_check(hipGraphExecDestroy(sim->graph_exec), "hipGraphExecDestroy");

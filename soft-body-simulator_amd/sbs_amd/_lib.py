@@ -54,6 +54,9 @@ _SIGS = {
     "sbs_gpu_add_distance_constraints": [_p, _pu, _u32, _d, _d, _pu],
     "sbs_gpu_add_plane": [_p, _pd, _pd, _pd, _pd],
     "sbs_gpu_add_sphere": [_p, _pd, _d],
+    "sbs_gpu_set_group_clusters": [_p, _u32, _pu, _u32],
+    "sbs_gpu_set_coloring": [_p, _i],
+    "sbs_gpu_set_launch_config": [_p, _i, _i],
     "sbs_gpu_commit": [_p, _u32],
     "sbs_gpu_step": [_p, _d, _u32, _u32],
     "sbs_gpu_synchronize": [_p],
@@ -238,6 +241,17 @@ class Simulation:
 
     def add_sphere(self, center, radius):
         check(self._L.sbs_gpu_add_sphere(self.h, _dp(f64(center)), radius))
+
+    def set_clusters(self, group, offsets):
+        off = u32(offsets)
+        check(self._L.sbs_gpu_set_group_clusters(self.h, group, _up(off), len(off) - 1))
+
+    def set_coloring(self, order):
+        """order: "hash" (device Jones-Plassmann) or "index" (sequential greedy in index order)."""
+        check(self._L.sbs_gpu_set_coloring(self.h, {"hash": 0, "index": 1}[order]))
+
+    def set_launch_config(self, block=256, cluster_block=64):
+        check(self._L.sbs_gpu_set_launch_config(self.h, block, cluster_block))
 
     def commit(self, recolor_passes=2):
         check(self._L.sbs_gpu_commit(self.h, recolor_passes))

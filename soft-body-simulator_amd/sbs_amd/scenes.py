@@ -28,6 +28,7 @@ class Group:
     beta: float = 0.0
     young: float = 1e6
     poisson: float = 0.3
+    clusters: np.ndarray = None  # cluster offsets over conn, or None
 
 
 @dataclass
@@ -43,6 +44,7 @@ class Scene:
     dt: float = 1.0 / 60.0
     iterations: int = 10
     substeps: int = 1
+    coloring: str = "hash"  # "hash" (device Jones-Plassmann) or "index" (sequential greedy)
 
     @property
     def num_tets(self):
@@ -55,11 +57,12 @@ class Scene:
     def apply(self, sim):
         sim.set_params(gravity=self.gravity, gravity_is_force=self.gravity_is_force,
                        collision_compliance=self.collision_compliance)
+        sim.set_coloring(self.coloring)
         for b in self.bodies:
             bi = sim.add_body(b.x0, b.mass)
             if b.surface is not None:
                 sim.set_surface(bi, b.surface)
-        for g in self.groups:
+        for gi, g in enumerate(self.groups):
             if g.kind == "green":
                 sim.add_green(g.body, g.conn, g.alpha, g.beta, g.young, g.poisson)
             elif g.kind == "neohookean":
@@ -68,6 +71,8 @@ class Scene:
                 sim.add_distance(g.conn, g.alpha, g.beta)
             else:
                 raise ValueError(g.kind)
+            if g.clusters is not None:
+                sim.set_clusters(gi, g.clusters)
         for n, p, lo, hi in self.planes:
             sim.add_plane(n, p, lo, hi)
         for c, r in self.spheres:
@@ -108,20 +113,27 @@ def reference_demo():
     return sc
 
 
+def hex_clusters(num_tets):
+    """Cluster offsets grouping the 6 tets of each hex cell of a box6 mesh."""
+    return np.arange(0, num_tets + 1, 6, dtype=np.uint32)
+
+
 def cantilever(nx, ny, nz, edge=0.1, model="green", young=1e6, poisson=0.45, density=1000.0, alpha=1e-4,
                beta=0.0, ground_y=-1.0, num_spheres=0, seed=0x5B5, dt=1.0 / 60.0, iterations=10, substeps=1,
-               name=None):
+               name=None, cluster=False, coloring="hash"):
     """Box of nx*ny*nz hex cells of edge `edge` from the origin, 6 tets per cell, lumped masses of
     the given density, the x = 0 face pinned (mass 0), gravity as an acceleration, a ground plane
-    at y = ground_y and `num_spheres` static spheres of radius U[0.05, 0.2] m under the beam."""
+    at y = ground_y and `num_spheres` static spheres of radius U[0.05, 0.2] m under the beam.
+    cluster=True runs the 6 tets of a hex cell as one cluster."""
     pos, tets = _lib.box6(nx, ny, nz, edge)
     mass = _lib.lumped_mass(pos, tets, density)
     mass[pos[:, 0] <= pos[:, 0].min()] = 0.0
     s2t, _ = _lib.boundary(len(pos), tets)
     sc = Scene(name or f"cantilever_{nx}x{ny}x{nz}_{model}", dt=dt, iterations=iterations, substeps=substeps,
-               gravity_is_force=False)
+               gravity_is_force=False, coloring=coloring)
     sc.bodies.append(Body(x0=pos, tets=tets, mass=mass, surface=s2t))
-    sc.groups.append(Group(model, 0, tets, alpha=alpha, beta=beta, young=young, poisson=poisson))
+    sc.groups.append(Group(model, 0, tets, alpha=alpha, beta=beta, young=young, poisson=poisson,
+                           clusters=hex_clusters(len(tets)) if cluster else None))
     sc.planes.append(((0.0, 1.0, 0.0), (0.0, ground_y, 0.0), (-1e30,) * 3, (1e30,) * 3))
     if num_spheres:
         rng = np.random.default_rng(seed)
@@ -140,14 +152,16 @@ def cfg0():
     return cantilever(10, 3, 3, 0.1, "green", 1e6, 0.45, 1000.0, iterations=10, substeps=1, name="cfg0_beam_540")
 
 
-def cfg1_1m():
+def cfg1_1m(cluster=True, coloring="index"):
     """Large beam, 1M-tet variant: 220x28x27 cells (997,920 tets, 179,452 vertices), stable
-    neo-Hookean E=5e5, nu=0.45, 4 substeps x 8 iterations, ground plane and 16 static spheres."""
+    neo-Hookean E=5e5, nu=0.45, 4 substeps x 8 iterations, ground plane and 16 static spheres.
+    By default the 6 tets of a cell form one cluster and clusters are colored in index order
+    (8 colors on the lattice)."""
     return cantilever(220, 28, 27, 0.1, "neohookean", 5e5, 0.45, 1000.0, num_spheres=16, iterations=8, substeps=4,
-                      name="cfg1_beam_1M_neohookean")
+                      name="cfg1_beam_1M_neohookean", cluster=cluster, coloring=coloring)
 
 
-def cfg1_6m():
+def cfg1_6m(cluster=True, coloring="index"):
     """Large beam: 400x50x50 cells (6.0M tets, 1,043,001 vertices), same material and solver."""
     return cantilever(400, 50, 50, 0.1, "neohookean", 5e5, 0.45, 1000.0, num_spheres=16, iterations=8, substeps=4,
-                      name="cfg1_beam_6M_neohookean")
+                      name="cfg1_beam_6M_neohookean", cluster=cluster, coloring=coloring)

@@ -219,8 +219,36 @@ def test_distance_cloth_parity():
     assert errs[1] <= TOL and errs[60] <= TOL
 
 
+@pytest.mark.parametrize("coloring,passes", [("hash", 0), ("hash", 2), ("index", 0)])
+def test_cluster_coloring_bit_exact(coloring, passes):
+    sc = scenes.cantilever(16, 6, 5, model="neohookean", cluster=True, coloring=coloring)
+    say(f"cluster coloring {coloring} passes={passes}")
+    dev = scenes.build_device(sc, recolor_passes=passes)
+    orc = build_oracle(sc, True, passes)
+    cd, nd = dev.group_colors(0)
+    co, no = orc.group_colors(0, sc.num_tets)
+    assert nd == no and np.array_equal(cd, co)
+    if coloring == "index":
+        assert nd == 8
+
+
+@pytest.mark.parametrize("model", ["neohookean", "green"])
+def test_clustered_beam_parity(model):
+    if model == "neohookean":
+        sc = scenes.cantilever(12, 3, 3, 0.1, "neohookean", 5e5, 0.45, 1000.0, num_spheres=4, iterations=8,
+                               substeps=4, cluster=True, coloring="index", name="nh_clustered")
+    else:
+        sc = scenes.cantilever(10, 3, 3, 0.1, "green", 1e6, 0.30, 1000.0, iterations=10, substeps=1, cluster=True,
+                               coloring="hash", name="green_clustered")
+    say(f"{sc.name} 60 frames")
+    errs, dev, orc = run_pair(sc, 60, check_frames=(1, 60))
+    print("clustered errors", model, errs, flush=True)
+    assert errs[1] <= TOL and errs[60] <= TOL
+
+
 def test_graph_replay_is_bit_identical_to_stream_launches():
-    sc = scenes.cantilever(10, 3, 3, 0.1, "neohookean", 5e5, 0.45, num_spheres=3, iterations=6, substeps=3)
+    sc = scenes.cantilever(10, 3, 3, 0.1, "neohookean", 5e5, 0.45, num_spheres=3, iterations=6, substeps=3,
+                           cluster=True)
     say("graph vs stream")
     a = scenes.build_device(sc)
     b = scenes.build_device(sc)
@@ -273,12 +301,15 @@ def test_million_tet_beam_properties():
     dev = scenes.build_device(sc)
     colors, nc = dev.group_colors(0)
     t = sc.groups[0].conn
-    # coloring validity: within each color every vertex appears once
-    order = np.argsort(colors, kind="stable")
-    bounds = np.searchsorted(colors[order], np.arange(nc + 1))
+    assert nc == 8  # lattice parity of the hex-cell clusters
+    # coloring validity: clusters (hex cells) of one color share no particle
+    cell_colors = colors[::6]
+    assert np.all(colors.reshape(-1, 6) == cell_colors[:, None])
     for c in range(nc):
-        v = t[order[bounds[c]:bounds[c + 1]]].ravel()
-        assert len(np.unique(v)) == len(v)
+        cells = np.nonzero(cell_colors == c)[0]
+        per_cell = [np.unique(t[6 * i:6 * i + 6]) for i in cells[:4000]]
+        allv = np.concatenate(per_cell)
+        assert len(np.unique(allv)) == len(allv)
     print("1M beam colors", nc, flush=True)
     for _ in range(2):
         dev.step(sc.dt, sc.iterations, sc.substeps)

@@ -469,3 +469,60 @@ def test_colored_and_insertion_order_agree_when_constraints_are_disjoint():
             sim.step(0.01, 4, 2)
         out.append(sim.read_body(0)[0])
     assert np.array_equal(out[0], out[1])
+
+
+# ------------------------------------------------------------------ clusters
+
+def test_index_order_coloring_of_hex_clusters_is_the_lattice_parity():
+    # sequential first-fit in index order over the cells of a box colors a cell by the parity of
+    # its lattice coordinates: 8 colors, cells sharing a vertex never equal
+    nx, ny, nz = 5, 4, 3
+    p, t = O.box6(nx, ny, nz, 0.1)
+    sim = O.Simulation()
+    sim.add_body(p)
+    g = sim.add_green(0, t, 1e-4, 0.0, 1e6, 0.3)
+    sim.set_clusters(g, np.arange(0, len(t) + 1, 6))
+    sim.set_coloring("index")
+    sim.set_order(True)
+    color, nc = sim.group_colors(g, len(t))
+    assert nc == 8
+    cell = np.array([(i, j, k) for i in range(nx) for j in range(ny) for k in range(nz)])
+    cc = color[::6]
+    assert np.all(color.reshape(-1, 6) == cc[:, None])
+    # cells with equal parity vector have equal colors
+    par = (cell % 2) @ np.array([4, 2, 1])
+    for q in range(8):
+        assert len(set(cc[par == q])) == 1
+
+
+def test_cluster_hash_coloring_is_greedy_over_clusters():
+    p, t = O.box6(4, 3, 3, 0.1)
+    sim = O.Simulation()
+    sim.add_body(p)
+    g = sim.add_green(0, t, 1e-4, 0.0, 1e6, 0.3)
+    sim.set_clusters(g, np.arange(0, len(t) + 1, 6))
+    sim.set_order(True, 1)
+    color, nc = sim.group_colors(g, len(t))
+    cells = [np.unique(t[6 * i:6 * i + 6]).astype(np.uint64) for i in range(len(t) // 6)]
+    width = max(len(c) for c in cells)
+    verts = np.array([np.concatenate([c, c[:1].repeat(width - len(c))]) for c in cells])
+    _greedy_property(verts, color[::6], 1)
+
+
+def test_singleton_clusters_equal_the_unclustered_order():
+    p, t = O.box6(3, 2, 2, 0.1)
+    out = []
+    for clustered in (False, True):
+        sim = O.Simulation()
+        sim.set_params(gravity=(0, -9.81, 0), gravity_is_force=False)
+        m = np.ones(len(p))
+        m[p[:, 0] == 0] = 0
+        sim.add_body(p, m)
+        g = sim.add_neohookean(0, t, 0.0, 5e5, 0.45)
+        if clustered:
+            sim.set_clusters(g, np.arange(len(t) + 1))
+        sim.set_order(True, 2)
+        for _ in range(3):
+            sim.step(1 / 60, 4, 2)
+        out.append(sim.read_body(0)[0])
+    assert np.array_equal(out[0], out[1])
