@@ -20,6 +20,14 @@ __device__ __forceinline__ v3 cross(v3 a, v3 b)
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
+// Hardware square root / reciprocal / reciprocal square root (v_sqrt_f32, v_rcp_f32, v_rsq_f32,
+// about 1 ulp) instead of the multi-instruction correctly rounded sequences; the projection math
+// is latency bound, one instruction per 4 cycles per wave.
+__device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float hw_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float hw_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 // Columns of a 3x3 matrix.
 struct m3 {
     v3 c0, c1, c2;
@@ -51,9 +59,9 @@ __device__ __forceinline__ void jacobi_rotate(float& spp, float& sqq, float& spq
                                               v3& vq)
 {
     if (fabsf(spq) < 1e-30f) return;
-    float const theta = (sqq - spp) / (2.0f * spq);
-    float const t = copysignf(1.0f, theta) / (fabsf(theta) + sqrtf(fmaf(theta, theta, 1.0f)));
-    float const c = 1.0f / sqrtf(fmaf(t, t, 1.0f));
+    float const theta = hw_div(sqq - spp, 2.0f * spq);
+    float const t = copysignf(hw_rcp(fabsf(theta) + hw_sqrt(fmaf(theta, theta, 1.0f))), theta);
+    float const c = hw_rsq(fmaf(t, t, 1.0f));
     float const s = t * c;
     spp = spp - t * spq;
     sqq = sqq + t * spq;
@@ -101,23 +109,23 @@ __device__ __forceinline__ void svd3(m3 const& F, m3& U, float s[3], m3& V)
     V.c1 = v1;
     V.c2 = v2;
     v3 const b0 = mul(F, v0), b1 = mul(F, v1), b2 = mul(F, v2);
-    float const n0 = sqrtf(dot(b0, b0));
+    float const n0 = hw_sqrt(dot(b0, b0));
     v3 u0;
     if (n0 > 1e-30f) {
-        u0 = (1.0f / n0) * b0;
+        u0 = hw_rcp(n0) * b0;
     } else {
         u0 = mk(1, 0, 0);
     }
     v3 w1 = b1 - dot(u0, b1) * u0;
-    float n1 = sqrtf(dot(w1, w1));
+    float n1 = hw_sqrt(dot(w1, w1));
     v3 u1;
     if (n1 > 1e-30f) {
-        u1 = (1.0f / n1) * w1;
+        u1 = hw_rcp(n1) * w1;
     } else {
         // any unit vector orthogonal to u0
         v3 const a = fabsf(u0.x) < 0.57f ? mk(1, 0, 0) : (fabsf(u0.y) < 0.57f ? mk(0, 1, 0) : mk(0, 0, 1));
         v3 const c = cross(u0, a);
-        u1 = (1.0f / sqrtf(dot(c, c))) * c;
+        u1 = hw_rsq(dot(c, c)) * c;
         n1 = 0.0f;
     }
     v3 const u2 = cross(u0, u1);

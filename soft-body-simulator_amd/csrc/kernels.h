@@ -69,6 +69,16 @@ void launch_green_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, float
 void launch_neohookean_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, float4* xi, const float4* xn,
                                 uint32_t begin, uint32_t count, float inv_dt2, float beta_dt, int first, int block);
 
+// Clusters that are the 6 tets of one hex cell of the box split, in its slot pattern
+// {0123, 0413, 0253, 0563, 0743, 0673}: particles held in registers. The color's constraints
+// start at tbase, slot-major: tet k of the color's i-th cluster is tbase + k * count + i.
+void launch_green_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, float4* xi, const float4* xn, uint32_t begin,
+                      uint32_t count, uint32_t tbase, float alpha_tilde, float gamma, int first, int block);
+void launch_neohookean_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, float4* xi, const float4* xn,
+                           uint32_t begin, uint32_t count, uint32_t tbase, float inv_dt2, float beta_dt, int first,
+                           int block);
+constexpr uint32_t kHexPatternLoc[6] = {0x03020100u, 0x03010400u, 0x03050200u, 0x03060500u, 0x03040700u, 0x03070600u};
+
 void launch_distance(hipStream_t s, distance_group_dev g, float4* xi, const float4* xn, uint32_t begin,
                      uint32_t count, float alpha_tilde, float gamma, int first, int block);
 

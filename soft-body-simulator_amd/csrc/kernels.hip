@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(256) k_green(tet_group_dev g, float4* xi, cons
 #pragma unroll
         for (int k = 0; k < 4; ++k) gdd += dot(f[k], xyz(P[k]) - xyz(xn[ids[k]]));
     }
-    float const dl = (-(C + alpha_tilde * lm) + gamma * gdd) / ((1.0f + gamma) * wsum + alpha_tilde);
+    float const dl = hw_div(-(C + alpha_tilde * lm) + gamma * gdd, (1.0f + gamma) * wsum + alpha_tilde);
     g.lambda[t] = lm + dl;
     // xi += w (-f) dl (:154-157)
     xi[id.x] = pack(xyz(P[0]) - (P[0].w * dl) * f[0], P[0].w);
@@ -182,7 +182,7 @@ __device__ __forceinline__ void tet_update(float4 P[4], const float4* __restrict
 #pragma unroll
         for (int k = 0; k < 4; ++k) gdd += dot(gk[k], xyz(P[k]) - xyz(xn[ids[k]]));
     }
-    float const dl = (-(C + alpha_tilde * lm) + gamma * gdd) / ((1.0f + gamma) * wsum + alpha_tilde);
+    float const dl = hw_div(-(C + alpha_tilde * lm) + gamma * gdd, (1.0f + gamma) * wsum + alpha_tilde);
     lm += dl;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -211,8 +211,8 @@ __global__ void __launch_bounds__(256) k_neohookean(tet_group_dev g, float4* xi,
     {  // deviatoric: C = ||F||_F, dC/dF = F / C, compliance 1/(mu |V0|)
         m3 const F = deformation_gradient(xyz(P[0]) - xyz(P[3]), xyz(P[1]) - xyz(P[3]), xyz(P[2]) - xyz(P[3]),
                                           R.r0, R.r1, R.r2);
-        float const C = sqrtf(dot(F.c0, F.c0) + dot(F.c1, F.c1) + dot(F.c2, F.c2));
-        float const inv = C > 0.0f ? 1.0f / C : 0.0f;
+        float const C = hw_sqrt(dot(F.c0, F.c0) + dot(F.c1, F.c1) + dot(F.c2, F.c2));
+        float const inv = C > 0.0f ? hw_rcp(C) : 0.0f;
         m3 const G = {inv * F.c0, inv * F.c1, inv * F.c2};
         float const at = c.y * inv_dt2;
         tet_update(P, xn, id, G, R, C, at, at * beta_dt, lm.x);
@@ -251,8 +251,8 @@ __device__ __forceinline__ void run_cluster(tet_group_dev const& g, cluster_dev 
     uint32_t const i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= count) return;
     uint32_t const c = begin + i;
-    int4 const a = cl.v0[c], b = cl.v1[c];
-    int const ids[kClusterSlots] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    int4 const va = cl.v0[c], vb = cl.v1[c];
+    int const ids[kClusterSlots] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
     float4 (*P)[BLOCK] = st.p;
 #pragma unroll
     for (int k = 0; k < kClusterSlots; ++k)
@@ -262,7 +262,7 @@ __device__ __forceinline__ void run_cluster(tet_group_dev const& g, cluster_dev 
         uint32_t const loc = cl.loc[t];
         uint32_t const s0 = loc & 0xffu, s1 = (loc >> 8) & 0xffu, s2 = (loc >> 16) & 0xffu, s3 = loc >> 24;
         float4 Q[4] = {P[s0][threadIdx.x], P[s1][threadIdx.x], P[s2][threadIdx.x], P[s3][threadIdx.x]};
-        project(t, Q, s0, s1, s2, s3, ids);
+        project(t, Q, s0, s1, s2, s3, va, vb);
         P[s0][threadIdx.x] = Q[0];
         P[s1][threadIdx.x] = Q[1];
         P[s2][threadIdx.x] = Q[2];
@@ -273,12 +273,11 @@ __device__ __forceinline__ void run_cluster(tet_group_dev const& g, cluster_dev 
         if (ids[k] >= 0) xi[ids[k]] = P[k][threadIdx.x];
 }
 
-__device__ __forceinline__ int slot_id(int const ids[kClusterSlots], uint32_t s)
+__device__ __forceinline__ int slot_id(int4 va, int4 vb, uint32_t s)
 {
-    int r = ids[0];
-#pragma unroll
-    for (int k = 1; k < kClusterSlots; ++k) r = (s == uint32_t(k)) ? ids[k] : r;
-    return r;
+    int const lo = s == 0 ? va.x : (s == 1 ? va.y : (s == 2 ? va.z : va.w));
+    int const hi = s == 4 ? vb.x : (s == 5 ? vb.y : (s == 6 ? vb.z : vb.w));
+    return s < 4 ? lo : hi;
 }
 
 template <int BLOCK>
@@ -287,8 +286,8 @@ __global__ void __launch_bounds__(BLOCK) k_green_clusters(tet_group_dev g, clust
                                                           uint32_t count, float alpha_tilde, float gamma, int first)
 {
     run_cluster<BLOCK>(g, cl, xi, begin, count,
-                       [&](uint32_t t, float4 P[4], uint32_t s0, uint32_t s1, uint32_t s2, uint32_t s3,
-                           int const ids[kClusterSlots]) {
+                       [&](uint32_t t, float4 P[4], uint32_t s0, uint32_t s1, uint32_t s2, uint32_t s3, int4 va,
+                           int4 vb) {
         float4 const a = g.m0[t], b = g.m1[t], c = g.m2[t];
         float const lm = first ? 0.0f : g.lambda[t];
         tet_rows const R = load_rows(a, b, c);
@@ -304,9 +303,9 @@ __global__ void __launch_bounds__(BLOCK) k_green_clusters(tet_group_dev g, clust
         if (gamma != 0.0f) {
             uint32_t const sl[4] = {s0, s1, s2, s3};
 #pragma unroll
-            for (int k = 0; k < 4; ++k) gdd += dot(f[k], xyz(P[k]) - xyz(xn[slot_id(ids, sl[k])]));
+            for (int k = 0; k < 4; ++k) gdd += dot(f[k], xyz(P[k]) - xyz(xn[slot_id(va, vb, sl[k])]));
         }
-        float const dl = (-(C + alpha_tilde * lm) + gamma * gdd) / ((1.0f + gamma) * wsum + alpha_tilde);
+        float const dl = hw_div(-(C + alpha_tilde * lm) + gamma * gdd, (1.0f + gamma) * wsum + alpha_tilde);
         g.lambda[t] = lm + dl;
 #pragma unroll
         for (int k = 0; k < 4; ++k) P[k] = pack(xyz(P[k]) - (P[k].w * dl) * f[k], P[k].w);
@@ -320,17 +319,17 @@ __global__ void __launch_bounds__(BLOCK) k_neohookean_clusters(tet_group_dev g, 
                                                                int first)
 {
     run_cluster<BLOCK>(g, cl, xi, begin, count,
-                       [&](uint32_t t, float4 P[4], uint32_t s0, uint32_t s1, uint32_t s2, uint32_t s3,
-                           int const ids[kClusterSlots]) {
+                       [&](uint32_t t, float4 P[4], uint32_t s0, uint32_t s1, uint32_t s2, uint32_t s3, int4 va,
+                           int4 vb) {
         float4 const a = g.m0[t], b = g.m1[t], c = g.m2[t];
         float2 lm = first ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(g.lambda)[t];
         tet_rows const R = load_rows(a, b, c);
-        int4 const id = make_int4(slot_id(ids, s0), slot_id(ids, s1), slot_id(ids, s2), slot_id(ids, s3));
+        int4 const id = make_int4(slot_id(va, vb, s0), slot_id(va, vb, s1), slot_id(va, vb, s2), slot_id(va, vb, s3));
         {
             m3 const F = deformation_gradient(xyz(P[0]) - xyz(P[3]), xyz(P[1]) - xyz(P[3]), xyz(P[2]) - xyz(P[3]),
                                               R.r0, R.r1, R.r2);
-            float const C = sqrtf(dot(F.c0, F.c0) + dot(F.c1, F.c1) + dot(F.c2, F.c2));
-            float const inv = C > 0.0f ? 1.0f / C : 0.0f;
+            float const C = hw_sqrt(dot(F.c0, F.c0) + dot(F.c1, F.c1) + dot(F.c2, F.c2));
+            float const inv = C > 0.0f ? hw_rcp(C) : 0.0f;
             m3 const G = {inv * F.c0, inv * F.c1, inv * F.c2};
             float const at = c.y * inv_dt2;
             tet_update(P, xn, id, G, R, C, at, at * beta_dt, lm.x);
@@ -345,6 +344,121 @@ __global__ void __launch_bounds__(BLOCK) k_neohookean_clusters(tet_group_dev g, 
         }
         reinterpret_cast<float2*>(g.lambda)[t] = lm;
     });
+}
+
+// ---------------------------------------------------------------- hex-cell clusters
+
+// Slot pattern shared by every cluster made of the 6 tets of one hex cell of the box split (slots
+// numbered by first appearance). With the pattern known at compile time the 8 particles live in
+// registers and every constraint's material and multipliers are loaded before the first one runs.
+constexpr int kHexTets = 6;
+constexpr int kHexSlots[kHexTets][4] = {{0, 1, 2, 3}, {0, 4, 1, 3}, {0, 2, 5, 3}, {0, 5, 6, 3}, {0, 7, 4, 3}, {0, 6, 7, 3}};
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_neohookean_hex(tet_group_dev g, cluster_dev cl, float4* xi,
+                                                          const float4* __restrict__ xn, uint32_t begin,
+                                                          uint32_t count, uint32_t tbase, float inv_dt2, float beta_dt,
+                                                          int first)
+{
+    uint32_t const i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= count) return;
+    uint32_t const c = begin + i;
+    int4 const va = cl.v0[c], vb = cl.v1[c];
+    int const ids[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
+    float4 A[kHexTets], B[kHexTets], Cm[kHexTets];
+    float2 L[kHexTets];
+#pragma unroll
+    for (int k = 0; k < kHexTets; ++k) {
+        uint32_t const t = tbase + uint32_t(k) * count + i;
+        A[k] = g.m0[t];
+        B[k] = g.m1[t];
+        Cm[k] = g.m2[t];
+        L[k] = first ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(g.lambda)[t];
+    }
+    float4 P[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) P[k] = xi[ids[k]];
+#pragma unroll
+    for (int k = 0; k < kHexTets; ++k) {
+        int const s0 = kHexSlots[k][0], s1 = kHexSlots[k][1], s2 = kHexSlots[k][2], s3 = kHexSlots[k][3];
+        float4 Q[4] = {P[s0], P[s1], P[s2], P[s3]};
+        int4 const id = make_int4(ids[s0], ids[s1], ids[s2], ids[s3]);
+        tet_rows const R = load_rows(A[k], B[k], Cm[k]);
+        {
+            m3 const F = deformation_gradient(xyz(Q[0]) - xyz(Q[3]), xyz(Q[1]) - xyz(Q[3]), xyz(Q[2]) - xyz(Q[3]),
+                                              R.r0, R.r1, R.r2);
+            float const C = hw_sqrt(dot(F.c0, F.c0) + dot(F.c1, F.c1) + dot(F.c2, F.c2));
+            float const inv = C > 0.0f ? hw_rcp(C) : 0.0f;
+            m3 const G = {inv * F.c0, inv * F.c1, inv * F.c2};
+            float const at = Cm[k].y * inv_dt2;
+            tet_update(Q, xn, id, G, R, C, at, at * beta_dt, L[k].x);
+        }
+        {
+            m3 const F = deformation_gradient(xyz(Q[0]) - xyz(Q[3]), xyz(Q[1]) - xyz(Q[3]), xyz(Q[2]) - xyz(Q[3]),
+                                              R.r0, R.r1, R.r2);
+            m3 const G = cofactor(F);
+            float const C = dot(F.c0, G.c0) - Cm[k].w;
+            float const at = Cm[k].z * inv_dt2;
+            tet_update(Q, xn, id, G, R, C, at, at * beta_dt, L[k].y);
+        }
+        P[s0] = Q[0];
+        P[s1] = Q[1];
+        P[s2] = Q[2];
+        P[s3] = Q[3];
+    }
+#pragma unroll
+    for (int k = 0; k < kHexTets; ++k) reinterpret_cast<float2*>(g.lambda)[tbase + uint32_t(k) * count + i] = L[k];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) xi[ids[k]] = P[k];
+}
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_green_hex(tet_group_dev g, cluster_dev cl, float4* xi,
+                                                     const float4* __restrict__ xn, uint32_t begin, uint32_t count,
+                                                     uint32_t tbase, float alpha_tilde, float gamma, int first)
+{
+    uint32_t const i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= count) return;
+    uint32_t const c = begin + i;
+    int4 const va = cl.v0[c], vb = cl.v1[c];
+    int const ids[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
+    float4 A[kHexTets], B[kHexTets], Cm[kHexTets];
+    float L[kHexTets];
+#pragma unroll
+    for (int k = 0; k < kHexTets; ++k) {
+        uint32_t const t = tbase + uint32_t(k) * count + i;
+        A[k] = g.m0[t];
+        B[k] = g.m1[t];
+        Cm[k] = g.m2[t];
+        L[k] = first ? 0.0f : g.lambda[t];
+    }
+    float4 P[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) P[k] = xi[ids[k]];
+#pragma unroll
+    for (int k = 0; k < kHexTets; ++k) {
+        int const s[4] = {kHexSlots[k][0], kHexSlots[k][1], kHexSlots[k][2], kHexSlots[k][3]};
+        v3 f[4];
+        float const C = green_core(xyz(P[s[0]]), xyz(P[s[1]]), xyz(P[s[2]]), xyz(P[s[3]]), load_rows(A[k], B[k], Cm[k]),
+                                   Cm[k].y, Cm[k].z, Cm[k].w, f);
+        float wsum = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) wsum += P[s[q]].w * dot(f[q], f[q]);
+        if (wsum < 1e-20f) continue;  // lambda stays as loaded (0 on the first iteration)
+        float gdd = 0.0f;
+        if (gamma != 0.0f) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) gdd += dot(f[q], xyz(P[s[q]]) - xyz(xn[ids[s[q]]]));
+        }
+        float const dl = hw_div(-(C + alpha_tilde * L[k]) + gamma * gdd, (1.0f + gamma) * wsum + alpha_tilde);
+        L[k] += dl;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) P[s[q]] = pack(xyz(P[s[q]]) - (P[s[q]].w * dl) * f[q], P[s[q]].w);
+    }
+#pragma unroll
+    for (int k = 0; k < kHexTets; ++k) g.lambda[tbase + uint32_t(k) * count + i] = L[k];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) xi[ids[k]] = P[k];
 }
 
 // ---------------------------------------------------------------- distance constraint
@@ -781,6 +895,31 @@ void launch_green_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, float
         k_green_clusters<64><<<grid_for(count, 64), 64, 0, s>>>(g, cl, xi, xn, begin, count, alpha_tilde, gamma, first);
     else
         k_green_clusters<128><<<grid_for(count, 128), 128, 0, s>>>(g, cl, xi, xn, begin, count, alpha_tilde, gamma, first);
+}
+
+void launch_neohookean_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, float4* xi, const float4* xn,
+                           uint32_t begin, uint32_t count, uint32_t tbase, float inv_dt2, float beta_dt, int first,
+                           int block)
+{
+    if (!count) return;
+    if (block <= 64)
+        k_neohookean_hex<64><<<grid_for(count, 64), 64, 0, s>>>(g, cl, xi, xn, begin, count, tbase, inv_dt2, beta_dt,
+                                                                first);
+    else
+        k_neohookean_hex<128><<<grid_for(count, 128), 128, 0, s>>>(g, cl, xi, xn, begin, count, tbase, inv_dt2,
+                                                                   beta_dt, first);
+}
+
+void launch_green_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, float4* xi, const float4* xn, uint32_t begin,
+                      uint32_t count, uint32_t tbase, float alpha_tilde, float gamma, int first, int block)
+{
+    if (!count) return;
+    if (block <= 64)
+        k_green_hex<64><<<grid_for(count, 64), 64, 0, s>>>(g, cl, xi, xn, begin, count, tbase, alpha_tilde, gamma,
+                                                           first);
+    else
+        k_green_hex<128><<<grid_for(count, 128), 128, 0, s>>>(g, cl, xi, xn, begin, count, tbase, alpha_tilde, gamma,
+                                                              first);
 }
 
 void launch_neohookean_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, float4* xi, const float4* xn,

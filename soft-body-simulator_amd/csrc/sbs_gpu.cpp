@@ -86,6 +86,7 @@ struct group_t {
     std::vector<uint32_t> cluster_color_offsets;  // ncolors + 1, cluster positions in execution order
     dev_buf<uint32_t> cl_off, cl_loc;
     dev_buf<int4> cl_v0, cl_v1;
+    bool hex = false;  // every cluster is a hex cell in the fixed slot pattern
     dev_buf<int4> idx;
     dev_buf<float4> m0, m1, m2;
     dev_buf<uint2> didx;
@@ -383,6 +384,23 @@ void sbs_gpu::upload_group(group_t& g)
             v1[j] = make_int4(slots[4], slots[5], slots[6], slots[7]);
             off[j + 1] = pos;
         }
+        g.hex = true;
+        for (uint32_t j = 0; j < ne && g.hex; ++j) {
+            if (off[j + 1] - off[j] != 6) g.hex = false;
+            for (uint32_t k = 0; k < 6 && g.hex; ++k)
+                if (loc[off[j] + k] != kHexPatternLoc[k]) g.hex = false;
+            if (g.hex && v1[j].w < 0) g.hex = false;
+        }
+        if (g.hex) {
+            // slot-major constraint layout inside each color: the k-th tet of the color's j-th
+            // cluster sits at color_offsets[c] + k * n_c + j, so lane j's loads coalesce
+            order.clear();
+            for (uint32_t c = 0; c < g.ncolors; ++c) {
+                uint32_t const cb = g.cluster_color_offsets[c], ce = g.cluster_color_offsets[c + 1];
+                for (uint32_t k = 0; k < 6; ++k)
+                    for (uint32_t j = cb; j < ce; ++j) order.push_back(e_begin(eorder[j]) + k);
+            }
+        }
         g.cl_off.upload(off.data(), off.size(), stream);
         g.cl_loc.upload(loc.data(), loc.size(), stream);
         g.cl_v0.upload(v0.data(), v0.size(), stream);
@@ -536,7 +554,13 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
                         uint32_t const b = g.color_offsets[col], n = g.color_offsets[col + 1] - b;
                         uint32_t const cb = g.cluster_color_offsets[col];
                         uint32_t const cn = g.cluster_color_offsets[col + 1] - cb;
-                        if (g.clustered() && g.kind == 0)
+                        if (g.hex && g.kind == 0)
+                            launch_green_hex(stream, g.tet(), g.clusters(), xi.p, xn.p, cb, cn, b, float(at),
+                                             float(gamma), first, cluster_block);
+                        else if (g.hex && g.kind == 1)
+                            launch_neohookean_hex(stream, g.tet(), g.clusters(), xi.p, xn.p, cb, cn, b,
+                                                  float(1.0 / dt2), float(g.beta * dt), first, cluster_block);
+                        else if (g.clustered() && g.kind == 0)
                             launch_green_clusters(stream, g.tet(), g.clusters(), xi.p, xn.p, cb, cn, float(at),
                                                   float(gamma), first, cluster_block);
                         else if (g.clustered() && g.kind == 1)

@@ -246,6 +246,26 @@ def test_clustered_beam_parity(model):
     assert errs[1] <= TOL and errs[60] <= TOL
 
 
+def test_generic_cluster_kernel_parity():
+    # the reference bar's 5-tet cells as clusters: not the hex slot pattern, so the LDS-staged
+    # cluster kernel runs them
+    sc = scenes.reference_demo()
+    sc.groups[0].clusters = np.arange(0, sc.num_tets + 1, 5, dtype=np.uint32)
+    say(f"{sc.name} with 5-tet clusters, 60 frames")
+    errs, dev, orc = run_pair(sc, 60, check_frames=(1, 60))
+    print("generic cluster errors", errs, flush=True)
+    assert errs[1] <= TOL and errs[60] <= TOL
+
+
+def test_damped_hex_clusters_parity():
+    sc = scenes.cantilever(6, 3, 3, 0.1, "green", 1e6, 0.30, 1000.0, beta=0.02, iterations=6, substeps=2,
+                           cluster=True, name="damped_hex")
+    say(f"{sc.name} 30 frames")
+    errs, dev, orc = run_pair(sc, 30, check_frames=(1, 30))
+    print("damped hex errors", errs, flush=True)
+    assert errs[1] <= TOL and errs[30] <= TOL
+
+
 def test_graph_replay_is_bit_identical_to_stream_launches():
     sc = scenes.cantilever(10, 3, 3, 0.1, "neohookean", 5e5, 0.45, num_spheres=3, iterations=6, substeps=3,
                            cluster=True)
