@@ -42,6 +42,24 @@ def cpu_model():
     return platform.processor()
 
 
+def copy_calibration(nbytes=256 << 20, reps=4):
+    """Device-to-device copies of a known size (the runtime's copy kernel), used to scale the
+    FETCH_SIZE / WRITE_SIZE counters of a --pmc profile of this command."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {reps} device copies of {nbytes} bytes")
+
+
 def cpu_baseline(scene):
     """FP64 oracle, reference (insertion) order, one thread, one frame of the same scene."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -71,6 +89,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=3, help="frames of the instrumented region")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="cfg1_1m", choices=["cfg1_1m", "cfg1_6m", "cfg0"])
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run device copies of known size (to scale PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the time step runs on a single MI355X in this build; "
@@ -79,6 +99,9 @@ def main():
 
     import sbs_amd as S
     from sbs_amd import scenes
+
+    if args.copy_calibration:
+        copy_calibration()
 
     scene = {"cfg1_1m": scenes.cfg1_1m, "cfg1_6m": scenes.cfg1_6m, "cfg0": scenes.cfg0}[args.workload]()
     log(f"workload {scene.name}: {scene.num_vertices} vertices, {scene.num_tets} tets")
@@ -126,6 +149,14 @@ def main():
     # residual after the last substep (FP64), a correctness anchor for later comparisons
     res = float(sum(sim.residual(0, scene.dt / scene.substeps)) ** 0.5)
 
+    # HBM bytes per launch measured by the rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs recorded
+    # under profiles/ (scaled by a device copy of known size); null for other workloads
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"{'neohookean_hex' if kind == S._lib.KERNEL_NEOHOOKEAN else 'green'}_"
+                                           f"{scene.name.replace('_neohookean', '')}_traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath))["bytes_per_launch"]
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene)
@@ -165,7 +196,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "bytes_per_unit": BYTES_PER_TET,
             "units_per_launch": round(units / launches, 1),
             "avg_launch_us": round(1e6 * avg_launch_s, 3),

@@ -299,6 +299,23 @@ def test_residual_and_energy_match_oracle():
     assert np.allclose(ed[3:], eo[3:], rtol=1e-3, atol=1e-6)
 
 
+def test_cpp_mirror_reference_demo_matches_oracle():
+    # the reference's main.cpp scene through the C++ simulation_t / timestep_t mirror
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(S.LIB_PATH), "reference_demo")
+    say("C++ mirror reference demo 60 frames")
+    out = subprocess.run([exe, "60"], check=True, capture_output=True, text=True, timeout=120).stdout
+    xd = np.array([[float(v) for v in line.split()[1:]] for line in out.strip().split("\n")])
+    sc = scenes.reference_demo()
+    orc = build_oracle(sc)
+    for _ in range(60):
+        orc.step(sc.dt, sc.iterations, sc.substeps)
+    xo, _ = orc.read_body(0)
+    assert xd.shape == xo.shape
+    assert np.abs(xd - xo).max() / sc.bbox_diagonal() <= TOL
+
+
 def test_errors_are_reported_not_thrown():
     say("error paths")
     sim = S.Simulation()
