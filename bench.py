@@ -60,25 +60,28 @@ def copy_calibration(nbytes=256 << 20, reps=4):
     log(f"copy calibration: {reps} device copies of {nbytes} bytes")
 
 
-def cpu_baseline(scene):
-    """FP64 oracle, reference (insertion) order, one thread, one frame of the same scene."""
+def cpu_baseline(scene, sample=None):
+    """FP64 oracle, reference (insertion) order, one thread, one frame of the same scene (or of a
+    sample scene, scaled to the full scene by tet count)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
+    run = sample or scene
     t0 = time.perf_counter()
     sim = O.Simulation()
-    scene.apply(sim)
+    run.apply(sim)
     sim.set_order(False)
     setup = time.perf_counter() - t0
-    log(f"cpu baseline: scene built in {setup:.1f}s, timing one frame ({scene.substeps} substeps)")
+    log(f"cpu baseline: {run.name} built in {setup:.1f}s, timing one frame ({run.substeps} substeps)")
     t0 = time.perf_counter()
-    sim.step(scene.dt, scene.iterations, scene.substeps)
+    sim.step(run.dt, run.iterations, run.substeps)
     el = time.perf_counter() - t0
-    return {"value": scene.substeps / el, "unit": "substeps/s", "cores": 1, "kind": "port",
-            "sample": f"1 frame = {scene.substeps} substeps x {scene.iterations} iterations of {scene.name} "
-                      f"({scene.num_tets} tets), FP64 oracle in reference insertion order, single thread, "
-                      f"{el:.1f}s on {cpu_model()}",
-            "projections_per_s": scene.num_tets * scene.iterations * scene.substeps / el}
+    scale = scene.num_tets / run.num_tets
+    what = (f"1 frame = {run.substeps} substeps x {run.iterations} iterations of {run.name} ({run.num_tets} tets)"
+            + (f", scaled by tet count to {scene.num_tets} tets" if sample else ""))
+    return {"value": scene.substeps / (el * scale), "unit": "substeps/s", "cores": 1, "kind": "port",
+            "sample": f"{what}; FP64 oracle in reference insertion order, single thread, {el:.1f}s on {cpu_model()}",
+            "projections_per_s": run.num_tets * run.iterations * run.substeps / el}
 
 
 def main():
@@ -88,7 +91,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5, help="untimed frames")
     ap.add_argument("--profile-steps", type=int, default=3, help="frames of the instrumented region")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="cfg1_1m", choices=["cfg1_1m", "cfg1_6m", "cfg0"])
+    ap.add_argument("--workload", default="cfg1_1m", choices=["cfg1_1m", "cfg1_6m", "cfg0", "cfg2"])
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run device copies of known size (to scale PMC byte counters)")
     args = ap.parse_args()
@@ -103,7 +106,8 @@ def main():
     if args.copy_calibration:
         copy_calibration()
 
-    scene = {"cfg1_1m": scenes.cfg1_1m, "cfg1_6m": scenes.cfg1_6m, "cfg0": scenes.cfg0}[args.workload]()
+    scene = {"cfg1_1m": scenes.cfg1_1m, "cfg1_6m": scenes.cfg1_6m, "cfg0": scenes.cfg0,
+             "cfg2": scenes.cfg2}[args.workload]()
     log(f"workload {scene.name}: {scene.num_vertices} vertices, {scene.num_tets} tets")
     t0 = time.perf_counter()
     sim = scenes.build_device(scene)
@@ -159,7 +163,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(scene)
+        cpu = cpu_baseline(scene, scenes.batch(16) if args.workload == "cfg2" else None)
         log(f"cpu baseline {cpu['value']:.3f} substeps/s")
 
     out = {

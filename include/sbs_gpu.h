@@ -93,6 +93,17 @@ int sbs_gpu_add_green_constraints(sbs_gpu* sim, uint32_t body, const uint32_t* t
 int sbs_gpu_add_neohookean_constraints(sbs_gpu* sim, uint32_t body, const uint32_t* tets, uint32_t n, double beta,
                                        double young_modulus, double poisson_ratio, uint32_t* group_id);
 
+/* Batched forms: one group of n tet constraints spanning many bodies -- tet c lies in body
+ * bodies[c] with body-local indices tets[4c..4c+3] and its own Young modulus and Poisson ratio.
+ * Bodies share no particle, so the group's colors (and launches) cover all of them at once; the
+ * result equals adding one group per body. */
+int sbs_gpu_add_green_constraints_batched(sbs_gpu* sim, const uint32_t* bodies, const uint32_t* tets, uint32_t n,
+                                          double alpha, double beta, const double* young_modulus,
+                                          const double* poisson_ratio, uint32_t* group_id);
+int sbs_gpu_add_neohookean_constraints_batched(sbs_gpu* sim, const uint32_t* bodies, const uint32_t* tets, uint32_t n,
+                                               double beta, const double* young_modulus, const double* poisson_ratio,
+                                               uint32_t* group_id);
+
 /* n distance_constraint_t, 4 indices each (b1, v1, b2, v2)
  * (src/physics/xpbd/distance_constraint.cpp:8-22). */
 int sbs_gpu_add_distance_constraints(sbs_gpu* sim, const uint32_t* b1_v1_b2_v2, uint32_t n, double alpha,
@@ -193,6 +204,12 @@ int sbs_mesh_box6(uint32_t nx, uint32_t ny, uint32_t nz, double edge, const doub
  * numbering of src/physics/topology.cpp:335-342,904-956). Call with NULL outputs to get the sizes. */
 int sbs_mesh_boundary(uint32_t num_vertices, const uint32_t* tets, uint32_t num_tets, uint32_t* surface_to_tet,
                       uint32_t* triangles, uint32_t* num_surface_vertices, uint32_t* num_triangles);
+
+/* Ellipsoid of radii (rx, ry, rz) centred on the origin: the cells of edge `edge` whose centres lie
+ * inside it, each split like sbs_mesh_box6; vertices are the used lattice corners in lattice order.
+ * Call with NULL pos/tets to get the sizes. */
+int sbs_mesh_ellipsoid6(double rx, double ry, double rz, double edge, double* pos, uint32_t* tets,
+                        uint32_t* num_vertices, uint32_t* num_tets);
 
 /* Lumped mass: every tet adds density*|V|/4 to each of its vertices. */
 int sbs_mesh_lumped_mass(uint32_t num_vertices, const double* pos, const uint32_t* tets, uint32_t num_tets,

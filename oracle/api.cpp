@@ -92,6 +92,29 @@ int orc_add_neohookean(void* h, uint32_t body, const uint32_t* tets, uint32_t n,
     return add_group(s, 1, first);
 }
 
+// One group of tets spanning several bodies, each tet with its own body and material.
+int orc_add_green_batched(void* h, const uint32_t* bodies, const uint32_t* tets, uint32_t n, double alpha, double beta,
+                          const double* young, const double* poisson)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    std::size_t const first = s->constraints.size();
+    for (uint32_t t = 0; t < n; ++t)
+        s->constraints.push_back(
+            std::make_unique<green_constraint_t>(alpha, beta, *s, bodies[t], tets + 4 * t, young[t], poisson[t]));
+    return add_group(s, 0, first);
+}
+
+int orc_add_neohookean_batched(void* h, const uint32_t* bodies, const uint32_t* tets, uint32_t n, double beta,
+                               const double* young, const double* poisson)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    std::size_t const first = s->constraints.size();
+    for (uint32_t t = 0; t < n; ++t)
+        s->constraints.push_back(
+            std::make_unique<neohookean_constraint_t>(beta, *s, bodies[t], tets + 4 * t, young[t], poisson[t]));
+    return add_group(s, 1, first);
+}
+
 int orc_add_distance(void* h, const uint32_t* b1v1b2v2, uint32_t n, double alpha, double beta)
 {
     auto* s = static_cast<simulation_t*>(h);

@@ -33,6 +33,8 @@ _SIGS = {
     "orc_add_green": (_i, [_p, _u32, _pu, _u32, _d, _d, _d, _d]),
     "orc_add_neohookean": (_i, [_p, _u32, _pu, _u32, _d, _d, _d]),
     "orc_add_distance": (_i, [_p, _pu, _u32, _d, _d]),
+    "orc_add_green_batched": (_i, [_p, _pu, _pu, _u32, _d, _d, _pd, _pd]),
+    "orc_add_neohookean_batched": (_i, [_p, _pu, _pu, _u32, _d, _pd, _pd]),
     "orc_add_plane": (None, [_p, _pd, _pd, _pd, _pd]),
     "orc_add_sphere": (None, [_p, _pd, _d]),
     "orc_set_order": (None, [_p, _i, _u32]),
@@ -206,6 +208,16 @@ class Simulation:
     def add_neohookean(self, body, tets, beta, young, poisson):
         t = u32(tets).reshape(-1)
         return self._L.orc_add_neohookean(self.h, body, _up(t), len(t) // 4, beta, young, poisson)
+
+    def add_green_batched(self, bodies, tets, alpha, beta, young, poisson):
+        b, t = u32(bodies), u32(tets).reshape(-1)
+        return self._L.orc_add_green_batched(self.h, _up(b), _up(t), len(b), alpha, beta, _dp(f64(young)),
+                                             _dp(f64(poisson)))
+
+    def add_neohookean_batched(self, bodies, tets, beta, young, poisson):
+        b, t = u32(bodies), u32(tets).reshape(-1)
+        return self._L.orc_add_neohookean_batched(self.h, _up(b), _up(t), len(b), beta, _dp(f64(young)),
+                                                  _dp(f64(poisson)))
 
     def add_distance(self, b1v1b2v2, alpha, beta):
         e = u32(b1v1b2v2).reshape(-1)

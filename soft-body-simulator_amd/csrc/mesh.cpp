@@ -95,6 +95,77 @@ extern "C" int sbs_mesh_box6(uint32_t nx, uint32_t ny, uint32_t nz, double edge,
     });
 }
 
+// Cells of an (nx, ny, nz) lattice of edge `edge` centred on the origin whose centres satisfy
+// (x/rx)^2 + (y/ry)^2 + (z/rz)^2 <= 1, each split like sbs_mesh_box6; vertices are the used lattice
+// corners, numbered in lattice order.
+extern "C" int sbs_mesh_ellipsoid6(double rx, double ry, double rz, double edge, double* pos, uint32_t* tets,
+                                   uint32_t* num_vertices, uint32_t* num_tets)
+{
+    return sbs::guarded([&] {
+        sbs::require(rx > 0 && ry > 0 && rz > 0 && edge > 0, "radii and edge must be positive");
+        sbs::require(num_vertices && num_tets, "size outputs are null");
+        uint32_t const n[3] = {uint32_t(std::ceil(2 * rx / edge)), uint32_t(std::ceil(2 * ry / edge)),
+                               uint32_t(std::ceil(2 * rz / edge))};
+        double const r[3] = {rx, ry, rz};
+        double o[3];
+        for (int a = 0; a < 3; ++a) o[a] = -0.5 * double(n[a]) * edge;
+        uint32_t const vy = n[1] + 1, vz = n[2] + 1;
+        std::size_t const lattice = std::size_t(n[0] + 1) * vy * vz;
+        auto inside = [&](uint32_t i, uint32_t j, uint32_t k) {
+            double const c[3] = {o[0] + (i + 0.5) * edge, o[1] + (j + 0.5) * edge, o[2] + (k + 0.5) * edge};
+            double s2 = 0;
+            for (int a = 0; a < 3; ++a) s2 += (c[a] / r[a]) * (c[a] / r[a]);
+            return s2 <= 1.0;
+        };
+        std::vector<uint32_t> id(lattice, UINT32_MAX);
+        uint32_t ncells = 0;
+        for (uint32_t i = 0; i < n[0]; ++i)
+            for (uint32_t j = 0; j < n[1]; ++j)
+                for (uint32_t k = 0; k < n[2]; ++k)
+                    if (inside(i, j, k)) {
+                        ++ncells;
+                        for (int c = 0; c < 8; ++c) id[((i + (c & 1)) * vy + (j + ((c >> 1) & 1))) * vz + (k + (c >> 2))] = 0;
+                    }
+        uint32_t nv = 0;
+        for (std::size_t v = 0; v < lattice; ++v)
+            if (id[v] == 0) {
+                id[v] = nv;
+                if (pos) {
+                    uint32_t const i = uint32_t(v / (std::size_t(vy) * vz)), j = uint32_t((v / vz) % vy), k = uint32_t(v % vz);
+                    pos[3 * std::size_t(nv) + 0] = o[0] + double(i) * edge;
+                    pos[3 * std::size_t(nv) + 1] = o[1] + double(j) * edge;
+                    pos[3 * std::size_t(nv) + 2] = o[2] + double(k) * edge;
+                }
+                ++nv;
+            }
+        *num_vertices = nv;
+        *num_tets = ncells * 6;
+        if (!tets) return;
+        static const int order[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+        uint32_t* out = tets;
+        for (uint32_t i = 0; i < n[0]; ++i)
+            for (uint32_t j = 0; j < n[1]; ++j)
+                for (uint32_t k = 0; k < n[2]; ++k) {
+                    if (!inside(i, j, k)) continue;
+                    for (int p = 0; p < 6; ++p) {
+                        uint32_t c[3] = {i, j, k};
+                        uint32_t v[4];
+                        v[0] = id[(std::size_t(c[0]) * vy + c[1]) * vz + c[2]];
+                        for (int s = 0; s < 3; ++s) {
+                            ++c[order[p][s]];
+                            v[s + 1] = id[(std::size_t(c[0]) * vy + c[1]) * vz + c[2]];
+                        }
+                        bool const odd = (p == 1 || p == 2 || p == 5);
+                        out[0] = v[0];
+                        out[1] = odd ? v[2] : v[1];
+                        out[2] = odd ? v[1] : v[2];
+                        out[3] = v[3];
+                        out += 4;
+                    }
+                }
+    });
+}
+
 extern "C" int sbs_mesh_boundary(uint32_t num_vertices, const uint32_t* tets, uint32_t num_tets,
                                  uint32_t* surface_to_tet, uint32_t* triangles, uint32_t* num_surface_vertices,
                                  uint32_t* num_triangles)

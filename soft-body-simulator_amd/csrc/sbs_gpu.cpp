@@ -78,6 +78,8 @@ struct group_t {
     uint32_t body = 0, count = 0;
     double alpha = 0, beta = 0, young = 0, poisson = 0;
     std::vector<uint32_t> conn;  // 4 body-local ids per tet, or (b1, v1, b2, v2) per distance
+    std::vector<uint32_t> cbody;        // per-tet body of a batched group (empty: `body` for all)
+    std::vector<double> cyoung, cpoisson;  // per-tet material of a batched group (empty: uniform)
     std::vector<uint32_t> cluster_off;  // empty: every constraint is its own cluster
     // after commit
     uint32_t ncolors = 0;
@@ -93,6 +95,9 @@ struct group_t {
     dev_buf<float> rest;
     dev_buf<float> lambda;
     bool clustered() const { return !cluster_off.empty(); }
+    uint32_t body_of(uint32_t c) const { return cbody.empty() ? body : cbody[c]; }
+    double young_of(uint32_t c) const { return cyoung.empty() ? young : cyoung[c]; }
+    double poisson_of(uint32_t c) const { return cpoisson.empty() ? poisson : cpoisson[c]; }
     uint32_t num_elements() const { return clustered() ? uint32_t(cluster_off.size() - 1) : count; }
     tet_group_dev tet() const { return {idx.p, m0.p, m1.p, m2.p, lambda.p}; }
     cluster_dev clusters() const { return {cl_off.p, cl_v0.p, cl_v1.p, cl_loc.p}; }
@@ -246,7 +251,7 @@ void sbs_gpu::color_group(group_t& g, uint32_t recolor_passes)
             out[1] = global_vertex(*this, g.conn[4 * c + 2], g.conn[4 * c + 3]);
             return 2;
         }
-        for (int k = 0; k < 4; ++k) out[k] = global_vertex(*this, g.body, g.conn[4 * std::size_t(c) + k]);
+        for (int k = 0; k < 4; ++k) out[k] = global_vertex(*this, g.body_of(c), g.conn[4 * std::size_t(c) + k]);
         return 4;
     };
     for (uint32_t e = 0; e < ne; ++e) {
@@ -360,13 +365,13 @@ void sbs_gpu::upload_group(group_t& g)
         std::vector<uint32_t> off(ne + 1, 0), loc(n);
         std::vector<int4> v0(ne), v1(ne);
         uint32_t pos = 0;
-        body_t const& b = bodies[g.body];
         for (uint32_t j = 0; j < ne; ++j) {
             uint32_t const e = eorder[j];
             int slots[kClusterSlots];
             int used = 0;
             for (uint32_t c = e_begin(e); c < e_end(e); ++c, ++pos) {
                 uint32_t l = 0;
+                body_t const& b = bodies[g.body_of(c)];
                 for (int k = 0; k < 4; ++k) {
                     int const gv = int(b.offset + g.conn[4 * std::size_t(c) + k]);
                     int s = 0;
@@ -424,11 +429,11 @@ void sbs_gpu::upload_group(group_t& g)
     } else {
         std::vector<int4> idx(n);
         std::vector<float4> m0(n), m1(n), m2(n);
-        body_t const& b = bodies[g.body];
-        double mu, lam;
-        lame(g.young, g.poisson, &mu, &lam);
         for (uint32_t j = 0; j < n; ++j) {
             uint32_t const* t = &g.conn[4 * std::size_t(order[j])];
+            body_t const& b = bodies[g.body_of(order[j])];
+            double mu, lam;
+            lame(g.young_of(order[j]), g.poisson_of(order[j]), &mu, &lam);
             double D[3][3];  // D[r][c] = (p_c - p_4)[r]
             for (int c = 0; c < 3; ++c)
                 for (int r = 0; r < 3; ++r) D[r][c] = b.x0[3 * std::size_t(t[c]) + r] - b.x0[3 * std::size_t(t[3]) + r];
@@ -700,6 +705,51 @@ static int add_tet_group(sbs_gpu* sim, int kind, uint32_t body, const uint32_t* 
         sim->groups.push_back(std::move(g));
         if (group_id) *group_id = uint32_t(sim->groups.size() - 1);
     });
+}
+
+static int add_tet_group_batched(sbs_gpu* sim, int kind, const uint32_t* bodies, const uint32_t* tets, uint32_t n,
+                                 double alpha, double beta, const double* young, const double* poisson,
+                                 uint32_t* group_id)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(!sim->committed, "scene already committed", SBS_ERR_STATE);
+        require(n == 0 || (bodies && tets && young && poisson), "null argument");
+        for (uint32_t c = 0; c < n; ++c) {
+            require(bodies[c] < sim->bodies.size(), "body index out of range");
+            require(poisson[c] > -1.0 && poisson[c] < 0.5, "poisson ratio must be in (-1, 0.5)");
+            require(young[c] > 0.0, "young modulus must be positive");
+            for (int k = 0; k < 4; ++k) require(tets[4 * std::size_t(c) + k] < sim->bodies[bodies[c]].n, "tet vertex out of range");
+        }
+        group_t g;
+        g.kind = kind;
+        g.body = n ? bodies[0] : 0;
+        g.count = n;
+        g.alpha = alpha;
+        g.beta = beta;
+        g.young = n ? young[0] : 1.0;
+        g.poisson = n ? poisson[0] : 0.0;
+        g.conn.assign(tets, tets + 4 * std::size_t(n));
+        g.cbody.assign(bodies, bodies + n);
+        g.cyoung.assign(young, young + n);
+        g.cpoisson.assign(poisson, poisson + n);
+        sim->groups.push_back(std::move(g));
+        if (group_id) *group_id = uint32_t(sim->groups.size() - 1);
+    });
+}
+
+int sbs_gpu_add_green_constraints_batched(sbs_gpu* sim, const uint32_t* bodies, const uint32_t* tets, uint32_t n,
+                                          double alpha, double beta, const double* young_modulus,
+                                          const double* poisson_ratio, uint32_t* group_id)
+{
+    return add_tet_group_batched(sim, 0, bodies, tets, n, alpha, beta, young_modulus, poisson_ratio, group_id);
+}
+
+int sbs_gpu_add_neohookean_constraints_batched(sbs_gpu* sim, const uint32_t* bodies, const uint32_t* tets, uint32_t n,
+                                               double beta, const double* young_modulus, const double* poisson_ratio,
+                                               uint32_t* group_id)
+{
+    return add_tet_group_batched(sim, 1, bodies, tets, n, 0.0, beta, young_modulus, poisson_ratio, group_id);
 }
 
 int sbs_gpu_add_green_constraints(sbs_gpu* sim, uint32_t body, const uint32_t* tets, uint32_t n, double alpha,

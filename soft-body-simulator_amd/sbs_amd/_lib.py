@@ -52,6 +52,9 @@ _SIGS = {
     "sbs_gpu_add_green_constraints": [_p, _u32, _pu, _u32, _d, _d, _d, _d, _pu],
     "sbs_gpu_add_neohookean_constraints": [_p, _u32, _pu, _u32, _d, _d, _d, _pu],
     "sbs_gpu_add_distance_constraints": [_p, _pu, _u32, _d, _d, _pu],
+    "sbs_gpu_add_green_constraints_batched": [_p, _pu, _pu, _u32, _d, _d, _pd, _pd, _pu],
+    "sbs_gpu_add_neohookean_constraints_batched": [_p, _pu, _pu, _u32, _d, _pd, _pd, _pu],
+    "sbs_mesh_ellipsoid6": [_d, _d, _d, _d, _pd, _pu, _pu, _pu],
     "sbs_gpu_add_plane": [_p, _pd, _pd, _pd, _pd],
     "sbs_gpu_add_sphere": [_p, _pd, _d],
     "sbs_gpu_set_group_clusters": [_p, _u32, _pu, _u32],
@@ -158,6 +161,16 @@ def boundary(nv, tets):
     return s2t, tris.reshape(-1, 3)
 
 
+def ellipsoid6(rx, ry, rz, edge):
+    """Hex cells (edge `edge`) whose centers lie inside the ellipsoid, 6 tets each."""
+    nv, nt = _u32(), _u32()
+    check(lib().sbs_mesh_ellipsoid6(rx, ry, rz, edge, None, None, ctypes.byref(nv), ctypes.byref(nt)))
+    pos = np.zeros(nv.value * 3)
+    tets = np.zeros(nt.value * 4, dtype=np.uint32)
+    check(lib().sbs_mesh_ellipsoid6(rx, ry, rz, edge, _dp(pos), _up(tets), ctypes.byref(nv), ctypes.byref(nt)))
+    return pos.reshape(-1, 3), tets.reshape(-1, 4)
+
+
 def lumped_mass(pos, tets, density):
     pos = f64(pos).reshape(-1)
     tets = u32(tets).reshape(-1)
@@ -229,6 +242,18 @@ class Simulation:
         g = _u32()
         return self._group(self._L.sbs_gpu_add_neohookean_constraints(self.h, body, _up(t), len(t) // 4, beta, young,
                                                                       poisson, ctypes.byref(g)), g, len(t) // 4)
+
+    def add_green_batched(self, bodies, tets, alpha, beta, young, poisson):
+        b, t = u32(bodies), u32(tets).reshape(-1)
+        g = _u32()
+        return self._group(self._L.sbs_gpu_add_green_constraints_batched(
+            self.h, _up(b), _up(t), len(b), alpha, beta, _dp(f64(young)), _dp(f64(poisson)), ctypes.byref(g)), g, len(b))
+
+    def add_neohookean_batched(self, bodies, tets, beta, young, poisson):
+        b, t = u32(bodies), u32(tets).reshape(-1)
+        g = _u32()
+        return self._group(self._L.sbs_gpu_add_neohookean_constraints_batched(
+            self.h, _up(b), _up(t), len(b), beta, _dp(f64(young)), _dp(f64(poisson)), ctypes.byref(g)), g, len(b))
 
     def add_distance(self, b1v1b2v2, alpha, beta):
         e = u32(b1v1b2v2).reshape(-1)

@@ -22,12 +22,12 @@ class Body:
 @dataclass
 class Group:
     kind: str               # "green" | "neohookean" | "distance"
-    body: int
+    body: object            # body index, or per-constraint body array (batched group)
     conn: np.ndarray        # tets (t, 4) or (b1, v1, b2, v2) rows
     alpha: float = 0.0
     beta: float = 0.0
-    young: float = 1e6
-    poisson: float = 0.3
+    young: object = 1e6     # scalar, or per-constraint array (batched group)
+    poisson: object = 0.3
     clusters: np.ndarray = None  # cluster offsets over conn, or None
 
 
@@ -63,7 +63,12 @@ class Scene:
             if b.surface is not None:
                 sim.set_surface(bi, b.surface)
         for gi, g in enumerate(self.groups):
-            if g.kind == "green":
+            if np.ndim(g.body) > 0:
+                if g.kind == "green":
+                    sim.add_green_batched(g.body, g.conn, g.alpha, g.beta, g.young, g.poisson)
+                else:
+                    sim.add_neohookean_batched(g.body, g.conn, g.beta, g.young, g.poisson)
+            elif g.kind == "green":
                 sim.add_green(g.body, g.conn, g.alpha, g.beta, g.young, g.poisson)
             elif g.kind == "neohookean":
                 sim.add_neohookean(g.body, g.conn, g.beta, g.young, g.poisson)
@@ -165,3 +170,67 @@ def cfg1_6m(cluster=True, coloring="index"):
     """Large beam: 400x50x50 cells (6.0M tets, 1,043,001 vertices), same material and solver."""
     return cantilever(400, 50, 50, 0.1, "neohookean", 5e5, 0.45, 1000.0, num_spheres=16, iterations=8, substeps=4,
                       name="cfg1_beam_6M_neohookean", cluster=cluster, coloring=coloring)
+
+
+def _random_rotation(rng):
+    q = rng.standard_normal(4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def batch(num_bodies=512, tet_range=(2e3, 2e5), edge=0.05, seed=0xB0D1E5, density=1000.0, iterations=8,
+          substeps=4, name=None):
+    """Batched bodies: `num_bodies` boxes and ellipsoids (alternating) of 6-tet hex cells with tet
+    counts log-uniform in tet_range, random rotations, dropped from heights U[0.5, 3] m onto a
+    ground plane at y = 0, Young's modulus log-uniform in [1e4, 1e6], Poisson ratio U[0.3, 0.48],
+    stable neo-Hookean constraints in one batched group with hex-cell clusters (index-order
+    coloring), 4 substeps x 8 iterations."""
+    rng = np.random.default_rng(seed)
+    sc = Scene(name or f"batch_{num_bodies}_bodies", gravity_is_force=False, iterations=iterations,
+               substeps=substeps, coloring="index")
+    lo, hi = np.log(tet_range[0]), np.log(tet_range[1])
+    meshes = []
+    for b in range(num_bodies):
+        target = float(np.exp(rng.uniform(lo, hi)))
+        cells = max(target / 6.0, 1.0)
+        aspect = np.exp(rng.uniform(np.log(0.5), np.log(2.0), size=3))
+        aspect /= np.cbrt(np.prod(aspect))
+        if b % 2 == 0:
+            n = np.maximum(np.round(aspect * np.cbrt(cells)).astype(int), 1)
+            pos, tets = _lib.box6(int(n[0]), int(n[1]), int(n[2]), edge)
+            pos = pos - pos.mean(0)
+        else:
+            r = aspect * np.cbrt(cells * edge ** 3 * 3.0 / (4.0 * np.pi))
+            r = np.maximum(r, 1.5 * edge)
+            pos, tets = _lib.ellipsoid6(r[0], r[1], r[2], edge)
+        meshes.append((pos @ _random_rotation(rng).T, tets))
+    extent = max(float(np.ptp(p, axis=0).max()) for p, _ in meshes) + 0.2
+    side = int(np.ceil(np.sqrt(num_bodies)))
+    bodies, bconn, young, poisson, clusters = [], [], [], [], [0]
+    for b, (pos, tets) in enumerate(meshes):
+        gx, gz = b % side, b // side
+        drop = rng.uniform(0.5, 3.0)
+        x0 = pos + np.array([gx * extent, drop - pos[:, 1].min(), gz * extent])
+        mass = _lib.lumped_mass(x0, tets, density)
+        s2t, _ = _lib.boundary(len(x0), tets)
+        sc.bodies.append(Body(x0=x0, tets=tets, mass=mass, surface=s2t))
+        bodies.append(np.full(len(tets), b, dtype=np.uint32))
+        bconn.append(tets)
+        E = float(np.exp(rng.uniform(np.log(1e4), np.log(1e6))))
+        nu = float(rng.uniform(0.3, 0.48))
+        young.append(np.full(len(tets), E))
+        poisson.append(np.full(len(tets), nu))
+        clusters.append(clusters[-1] + len(tets))
+    conn = np.concatenate(bconn)
+    sc.groups.append(Group("neohookean", np.concatenate(bodies), conn, young=np.concatenate(young),
+                           poisson=np.concatenate(poisson), clusters=hex_clusters(len(conn))))
+    sc.planes.append(((0.0, 1.0, 0.0), (0.0, 0.0, 0.0), (-1e30,) * 3, (1e30,) * 3))
+    return sc
+
+
+def cfg2(num_bodies=512):
+    """Config 2: 512 batched bodies (about 22M tets)."""
+    return batch(num_bodies, name=f"cfg2_batch_{num_bodies}_bodies")

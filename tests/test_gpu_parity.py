@@ -266,6 +266,44 @@ def test_damped_hex_clusters_parity():
     assert errs[1] <= TOL and errs[30] <= TOL
 
 
+def test_batched_group_is_bit_identical_to_per_body_groups():
+    # index-order coloring colors each body's cells as it would alone, and a cluster's arithmetic
+    # does not depend on which launch runs it
+    sc = scenes.batch(5, tet_range=(2e3, 2e4))
+    g = sc.groups[0]
+    per_body = scenes.Scene("per_body", bodies=sc.bodies, planes=sc.planes, gravity_is_force=False,
+                            iterations=sc.iterations, substeps=sc.substeps, coloring="index")
+    start = 0
+    for b, body in enumerate(sc.bodies):
+        n = len(body.tets)
+        per_body.groups.append(scenes.Group("neohookean", b, body.tets, young=float(g.young[start]),
+                                            poisson=float(g.poisson[start]), clusters=scenes.hex_clusters(n)))
+        start += n
+    say("batched vs per-body groups")
+    a = scenes.build_device(sc)
+    c = scenes.build_device(per_body)
+    for f in range(20):
+        a.step(sc.dt, sc.iterations, sc.substeps)
+        c.step(sc.dt, sc.iterations, sc.substeps)
+    for b in range(len(sc.bodies)):
+        assert np.array_equal(a.read_body(b)[0], c.read_body(b)[0])
+
+
+def test_batched_bodies_parity():
+    sc = scenes.batch(4, tet_range=(2e3, 6e3))
+    say(f"{sc.name} 60 frames")
+    dev = scenes.build_device(sc)
+    orc = build_oracle(sc)
+    diag = max(float(np.linalg.norm(np.ptp(b.x0, axis=0))) for b in sc.bodies)
+    for f in range(1, 61):
+        dev.step(sc.dt, sc.iterations, sc.substeps)
+        orc.step(sc.dt, sc.iterations, sc.substeps)
+        if f in (1, 60):
+            err = max(float(np.abs(dev.read_body(b)[0] - orc.read_body(b)[0]).max()) for b in range(len(sc.bodies)))
+            print("batched error frame", f, err / diag, flush=True)
+            assert err / diag <= TOL
+
+
 def test_graph_replay_is_bit_identical_to_stream_launches():
     sc = scenes.cantilever(10, 3, 3, 0.1, "neohookean", 5e5, 0.45, num_spheres=3, iterations=6, substeps=3,
                            cluster=True)

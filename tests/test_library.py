@@ -83,3 +83,35 @@ def test_cpp_mirror_example_builds_against_the_c_abi():
                         f"{pkg}/host/examples/reference_demo.cpp", "-o", out, f"-L{pkg}/_build", "-lsbs_gpu",
                         f"-Wl,-rpath,{pkg}/_build"], check=True)
         assert os.path.exists(out)
+
+
+def test_ellipsoid6_cells_volume_and_conformity():
+    import itertools
+    rx, ry, rz, h = 0.33, 0.21, 0.27, 0.05
+    pos, tets = S.ellipsoid6(rx, ry, rz, h)
+    # brute force: cells of the centred lattice whose centres are inside
+    n = [int(np.ceil(2 * r / h)) for r in (rx, ry, rz)]
+    o = [-0.5 * k * h for k in n]
+    inside = 0
+    for i, j, k in itertools.product(*(range(a) for a in n)):
+        c = np.array([o[0] + (i + 0.5) * h, o[1] + (j + 0.5) * h, o[2] + (k + 0.5) * h])
+        inside += np.sum((c / np.array([rx, ry, rz])) ** 2) <= 1.0
+    assert len(tets) == 6 * inside
+    e = pos[tets[:, :3]] - pos[tets[:, 3:4]]
+    V = np.linalg.det(np.transpose(e, (0, 2, 1))) / 6.0
+    assert np.all(V < 0) and np.isclose(-V.sum(), inside * h ** 3)
+    faces = {}
+    for t in tets:
+        for f in itertools.combinations(sorted(t), 3):
+            faces[f] = faces.get(f, 0) + 1
+    assert max(faces.values()) == 2
+    assert len(np.unique(tets)) == len(pos)  # every vertex is used
+
+
+def test_batch_scene_shape():
+    sc = scenes.batch(6, tet_range=(2e3, 1e4))
+    g = sc.groups[0]
+    assert len(sc.bodies) == 6 and len(g.conn) == sum(len(b.tets) for b in sc.bodies)
+    assert np.all(np.diff(g.body.astype(np.int64)) >= 0)
+    for b in sc.bodies:
+        assert b.x0[:, 1].min() >= 0.5 - 1e-9

@@ -526,3 +526,29 @@ def test_singleton_clusters_equal_the_unclustered_order():
             sim.step(1 / 60, 4, 2)
         out.append(sim.read_body(0)[0])
     assert np.array_equal(out[0], out[1])
+
+
+def test_batched_group_equals_one_group_per_body():
+    # bodies share no particle, so visiting one batched group equals visiting per-body groups
+    meshes = [O.box6(3, 2, 2, 0.1, (0.0, 0.5, 0.0)), O.box6(2, 2, 3, 0.1, (1.0, 0.7, 0.0))]
+    E, nu = [3e5, 8e5], [0.35, 0.45]
+    out = []
+    for batched in (False, True):
+        sim = O.Simulation()
+        sim.set_params(gravity=(0, -9.81, 0), gravity_is_force=False)
+        for p, t in meshes:
+            sim.add_body(p, np.full(len(p), 0.05))
+        if batched:
+            bodies = np.concatenate([np.full(len(t), b) for b, (p, t) in enumerate(meshes)])
+            tets = np.concatenate([t for p, t in meshes])
+            sim.add_neohookean_batched(bodies, tets, 0.0, np.repeat(E, [len(t) for p, t in meshes]),
+                                       np.repeat(nu, [len(t) for p, t in meshes]))
+        else:
+            for b, (p, t) in enumerate(meshes):
+                sim.add_neohookean(b, t, 0.0, E[b], nu[b])
+        sim.set_order(False)
+        for _ in range(4):
+            sim.step(1 / 60, 5, 2)
+        out.append([sim.read_body(b)[0] for b in range(2)])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
