@@ -6,6 +6,9 @@
 
 namespace sbs {
 
+// Particle positions: x, y, z and the inverse mass in w, in FP64 (see kernels.hip).
+using pos4 = double4;
+
 // Analytic SDF obstacle in device memory.
 struct sdf_dev {
     int kind;  // 0 plane, 1 sphere
@@ -47,58 +50,58 @@ struct contact_dev {
     const uint32_t* surf;  // surface vertex -> global particle
     uint32_t S;            // surface vertex count
     uint32_t slots;        // contact slots per surface vertex (>= number of SDFs)
-    float4* q;             // [slot * S + s]: contact point, w = sdf index (as float)
+    pos4* q;               // [slot * S + s]: contact point, w = sdf index
     float4* n;             // normal
     float* lambda;
     uint32_t* count;       // contacts per surface vertex
 };
 
-void launch_predict(hipStream_t s, float4* xi, const float4* xn, float4* v, uint32_t n, float gx, float gy,
+void launch_predict(hipStream_t s, pos4* xi, const pos4* xn, float4* v, uint32_t n, float gx, float gy,
                     float gz, int gravity_is_force, float dt, int block);
-void launch_commit(hipStream_t s, const float4* xi, float4* xn, float4* v, uint32_t n, float dt, int block);
-void launch_commit_predict(hipStream_t s, float4* xi, float4* xn, float4* v, uint32_t n, float gx, float gy,
+void launch_commit(hipStream_t s, const pos4* xi, pos4* xn, float4* v, uint32_t n, float dt, int block);
+void launch_commit_predict(hipStream_t s, pos4* xi, pos4* xn, float4* v, uint32_t n, float gx, float gy,
                            float gz, int gravity_is_force, float dt, int block);
 
-void launch_green(hipStream_t s, tet_group_dev g, float4* xi, const float4* xn, uint32_t begin, uint32_t count,
+void launch_green(hipStream_t s, tet_group_dev g, pos4* xi, const pos4* xn, uint32_t begin, uint32_t count,
                   float alpha_tilde, float gamma, int first, int block);
-void launch_neohookean(hipStream_t s, tet_group_dev g, float4* xi, const float4* xn, uint32_t begin,
+void launch_neohookean(hipStream_t s, tet_group_dev g, pos4* xi, const pos4* xn, uint32_t begin,
                        uint32_t count, float inv_dt2, float beta_dt, int first, int block);
 // One thread per cluster of clusters [begin, begin+count).
-void launch_green_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, float4* xi, const float4* xn,
+void launch_green_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn,
                            uint32_t begin, uint32_t count, float alpha_tilde, float gamma, int first, int block);
-void launch_neohookean_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, float4* xi, const float4* xn,
+void launch_neohookean_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn,
                                 uint32_t begin, uint32_t count, float inv_dt2, float beta_dt, int first, int block);
 
 // Clusters that are the 6 tets of one hex cell of the box split, in its slot pattern
 // {0123, 0413, 0253, 0563, 0743, 0673}: particles held in registers. The color's constraints
 // start at tbase, slot-major: tet k of the color's i-th cluster is tbase + k * count + i.
-void launch_green_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, float4* xi, const float4* xn, uint32_t begin,
+void launch_green_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn, uint32_t begin,
                       uint32_t count, uint32_t tbase, float alpha_tilde, float gamma, int first, int block);
-void launch_neohookean_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, float4* xi, const float4* xn,
+void launch_neohookean_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn,
                            uint32_t begin, uint32_t count, uint32_t tbase, float inv_dt2, float beta_dt, int first,
                            int block);
 constexpr uint32_t kHexPatternLoc[6] = {0x03020100u, 0x03010400u, 0x03050200u, 0x03060500u, 0x03040700u, 0x03070600u};
 
-void launch_distance(hipStream_t s, distance_group_dev g, float4* xi, const float4* xn, uint32_t begin,
+void launch_distance(hipStream_t s, distance_group_dev g, pos4* xi, const pos4* xn, uint32_t begin,
                      uint32_t count, float alpha_tilde, float gamma, int first, int block);
 
-void launch_detect(hipStream_t s, contact_dev c, const float4* x, const sdf_dev* sdfs, uint32_t nsdf, int block);
-void launch_contact(hipStream_t s, contact_dev c, float4* xi, float alpha_tilde, int first, int block);
+void launch_detect(hipStream_t s, contact_dev c, const pos4* x, const sdf_dev* sdfs, uint32_t nsdf, int block);
+void launch_contact(hipStream_t s, contact_dev c, pos4* xi, float alpha_tilde, int first, int block);
 
 // Residual partial sums: one double per block of `block` threads over [begin, begin+count).
-void launch_green_residual(hipStream_t s, tet_group_dev g, const float4* x, uint32_t begin, uint32_t count,
+void launch_green_residual(hipStream_t s, tet_group_dev g, const pos4* x, uint32_t begin, uint32_t count,
                            float alpha_tilde, double* partial, int block);
-void launch_neohookean_residual(hipStream_t s, tet_group_dev g, const float4* x, uint32_t begin, uint32_t count,
+void launch_neohookean_residual(hipStream_t s, tet_group_dev g, const pos4* x, uint32_t begin, uint32_t count,
                                 float inv_dt2, double* partial, int block);
-void launch_distance_residual(hipStream_t s, distance_group_dev g, const float4* x, uint32_t begin,
+void launch_distance_residual(hipStream_t s, distance_group_dev g, const pos4* x, uint32_t begin,
                               uint32_t count, float alpha_tilde, double* partial, int block);
 
 // Energy partials: per block 6 doubles (kinetic, potential, elastic, px, py, pz).
-void launch_particle_energy(hipStream_t s, const float4* x, const float4* v, const float* mass, uint32_t n,
+void launch_particle_energy(hipStream_t s, const pos4* x, const float4* v, const float* mass, uint32_t n,
                             float gx, float gy, float gz, int gravity_is_force, double* partial, int block);
-void launch_green_energy(hipStream_t s, tet_group_dev g, const float4* x, uint32_t count, double* partial,
+void launch_green_energy(hipStream_t s, tet_group_dev g, const pos4* x, uint32_t count, double* partial,
                          int block);
-void launch_neohookean_energy(hipStream_t s, tet_group_dev g, const float4* x, uint32_t count, double* partial,
+void launch_neohookean_energy(hipStream_t s, tet_group_dev g, const pos4* x, uint32_t count, double* partial,
                               int block);
 
 // One Jones-Plassmann round over n elements; element t touches vertices

@@ -124,12 +124,14 @@ struct sbs_gpu {
     bool committed = false;
     uint32_t V = 0;
 
-    dev_buf<float4> xi, xn, vel;
+    dev_buf<pos4> xi, xn;
+    dev_buf<float4> vel;
     dev_buf<float> mass;
 
     dev_buf<uint32_t> surf;
     uint32_t S = 0, slots = 0;
-    dev_buf<float4> cq, cn;
+    dev_buf<pos4> cq;
+    dev_buf<float4> cn;
     dev_buf<float> clam;
     dev_buf<uint32_t> ccount;
     dev_buf<sdf_dev> dsdf;
@@ -480,13 +482,14 @@ void sbs_gpu::commit(uint32_t recolor_passes)
         b.offset = V;
         V += b.n;
     }
-    std::vector<float4> x(V), v(V, make_float4(0, 0, 0, 0));
+    std::vector<pos4> x(V);
+    std::vector<float4> v(V, make_float4(0, 0, 0, 0));
     std::vector<float> m(V);
     for (auto const& b : bodies)
         for (uint32_t i = 0; i < b.n; ++i) {
             double const mi = b.mass.empty() ? 1.0 : b.mass[i];
             double const w = mi > 0.0 ? 1.0 / mi : 0.0;  // particle.cpp:39-44
-            x[b.offset + i] = make_float4(float(b.x0[3 * i]), float(b.x0[3 * i + 1]), float(b.x0[3 * i + 2]), float(w));
+            x[b.offset + i] = make_double4(b.x0[3 * i], b.x0[3 * i + 1], b.x0[3 * i + 2], w);
             m[b.offset + i] = float(mi);
         }
     xi.upload(x.data(), V, stream);
@@ -949,9 +952,10 @@ int sbs_gpu_read_body(sbs_gpu* sim, uint32_t body, double* x, double* v)
         require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
         require(body < sim->bodies.size(), "body index out of range");
         body_t const& b = sim->bodies[body];
-        std::vector<float4> hx(b.n), hv(b.n);
+        std::vector<pos4> hx(b.n);
+        std::vector<float4> hv(b.n);
         if (b.n) {
-            hip_check(hipMemcpyAsync(hx.data(), sim->xn.p + b.offset, b.n * sizeof(float4), hipMemcpyDeviceToHost,
+            hip_check(hipMemcpyAsync(hx.data(), sim->xn.p + b.offset, b.n * sizeof(pos4), hipMemcpyDeviceToHost,
                                      sim->stream), "read positions");
             hip_check(hipMemcpyAsync(hv.data(), sim->vel.p + b.offset, b.n * sizeof(float4), hipMemcpyDeviceToHost,
                                      sim->stream), "read velocities");
@@ -981,18 +985,18 @@ int sbs_gpu_write_body(sbs_gpu* sim, uint32_t body, const double* x, const doubl
         body_t const& b = sim->bodies[body];
         if (b.n == 0) return;
         if (x) {
-            std::vector<float4> hx(b.n);
-            hip_check(hipMemcpyAsync(hx.data(), sim->xn.p + b.offset, b.n * sizeof(float4), hipMemcpyDeviceToHost,
+            std::vector<pos4> hx(b.n);
+            hip_check(hipMemcpyAsync(hx.data(), sim->xn.p + b.offset, b.n * sizeof(pos4), hipMemcpyDeviceToHost,
                                      sim->stream), "read positions");
             hip_check(hipStreamSynchronize(sim->stream), "write body");
             for (uint32_t i = 0; i < b.n; ++i) {
                 require(std::isfinite(x[3 * i]) && std::isfinite(x[3 * i + 1]) && std::isfinite(x[3 * i + 2]),
                         "positions must be finite");
-                hx[i] = make_float4(float(x[3 * i]), float(x[3 * i + 1]), float(x[3 * i + 2]), hx[i].w);
+                hx[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], hx[i].w);
             }
-            hip_check(hipMemcpyAsync(sim->xn.p + b.offset, hx.data(), b.n * sizeof(float4), hipMemcpyHostToDevice,
+            hip_check(hipMemcpyAsync(sim->xn.p + b.offset, hx.data(), b.n * sizeof(pos4), hipMemcpyHostToDevice,
                                      sim->stream), "write positions");
-            hip_check(hipMemcpyAsync(sim->xi.p + b.offset, hx.data(), b.n * sizeof(float4), hipMemcpyHostToDevice,
+            hip_check(hipMemcpyAsync(sim->xi.p + b.offset, hx.data(), b.n * sizeof(pos4), hipMemcpyHostToDevice,
                                      sim->stream), "write positions");
             hip_check(hipStreamSynchronize(sim->stream), "write body");
         }
@@ -1029,9 +1033,10 @@ int sbs_gpu_read_contacts(sbs_gpu* sim, uint32_t* ids, double* qn, uint32_t cap,
         if (!(sim->S && sim->slots)) return;
         sim->detect();
         std::vector<uint32_t> cnt(sim->S);
-        std::vector<float4> q(std::size_t(sim->S) * sim->slots), n(q.size());
+        std::vector<pos4> q(std::size_t(sim->S) * sim->slots);
+        std::vector<float4> n(q.size());
         hip_check(hipMemcpyAsync(cnt.data(), sim->ccount.p, sim->S * sizeof(uint32_t), hipMemcpyDeviceToHost, sim->stream), "read contacts");
-        hip_check(hipMemcpyAsync(q.data(), sim->cq.p, q.size() * sizeof(float4), hipMemcpyDeviceToHost, sim->stream), "read contacts");
+        hip_check(hipMemcpyAsync(q.data(), sim->cq.p, q.size() * sizeof(pos4), hipMemcpyDeviceToHost, sim->stream), "read contacts");
         hip_check(hipMemcpyAsync(n.data(), sim->cn.p, n.size() * sizeof(float4), hipMemcpyDeviceToHost, sim->stream), "read contacts");
         hip_check(hipStreamSynchronize(sim->stream), "read contacts");
         if (sim->timing) sim->harvest();
