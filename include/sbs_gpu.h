@@ -207,7 +207,8 @@ int sbs_mesh_boundary(uint32_t num_vertices, const uint32_t* tets, uint32_t num_
 
 /* Ellipsoid of radii (rx, ry, rz) centred on the origin: the cells of edge `edge` whose centres lie
  * inside it, each split like sbs_mesh_box6; vertices are the used lattice corners in lattice order.
- * Call with NULL pos/tets to get the sizes. */
+ * Cells (6 consecutive tets each) come grouped by lattice parity class, so that SBS_COLORING_INDEX
+ * colors their clusters with 8 balanced colors. Call with NULL pos/tets to get the sizes. */
 int sbs_mesh_ellipsoid6(double rx, double ry, double rz, double edge, double* pos, uint32_t* tets,
                         uint32_t* num_vertices, uint32_t* num_tets);
 

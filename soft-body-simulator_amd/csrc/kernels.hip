@@ -198,28 +198,56 @@ __global__ void __launch_bounds__(256) k_green(tet_group_dev g, pos4* xi, const 
 
 // ---------------------------------------------------------------- stable neo-Hookean constraint
 
-// One XPBD update of a tet constraint with gradients g_k = G r_k (k < 3), g_3 = -sum.
+// A tet in FP32 relative form for a sequence of constraint updates: edges e_k = p_k - p_3, the
+// displacements dx_k = p_k - xn_k since the substep start (only with damping), the inverse masses
+// and the corrections d_k accumulated so far. The FP64 differences are taken once; each update
+// then moves the FP32 quantities, and the positions receive the sum of the corrections once.
+struct tet_frame {
+    v3 e[3], dx[4], d[4];
+    float w[4];
+};
+
 template <class Pos, class XnOf>
-__device__ __forceinline__ void tet_update(Pos P[4], XnOf&& xn_of, m3 const& G, tet_rows const& R, float C,
-                                           float alpha_tilde, float gamma, float& lm)
+__device__ __forceinline__ tet_frame make_frame(Pos const P[4], XnOf&& xn_of, bool damped)
+{
+    tet_frame f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) f.e[k] = sub(P[k], P[3]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        f.w[k] = wof(P[k]);
+        f.d[k] = mk(0.0f, 0.0f, 0.0f);
+        f.dx[k] = damped ? sub(P[k], xn_of(k)) : mk(0.0f, 0.0f, 0.0f);
+    }
+    return f;
+}
+
+// One XPBD update of a tet constraint with gradients g_k = G r_k (k < 3), g_3 = -sum.
+__device__ __forceinline__ void tet_update(tet_frame& f, m3 const& G, tet_rows const& R, float C, float alpha_tilde,
+                                           float gamma, float& lm)
 {
     v3 gk[4];
     gk[0] = mul(G, R.r0);
     gk[1] = mul(G, R.r1);
     gk[2] = mul(G, R.r2);
     gk[3] = -(gk[0] + gk[1] + gk[2]);
-    float const wsum = wof(P[0]) * dot(gk[0], gk[0]) + wof(P[1]) * dot(gk[1], gk[1]) + wof(P[2]) * dot(gk[2], gk[2]) +
-                       wof(P[3]) * dot(gk[3], gk[3]);
+    float const wsum = f.w[0] * dot(gk[0], gk[0]) + f.w[1] * dot(gk[1], gk[1]) + f.w[2] * dot(gk[2], gk[2]) +
+                       f.w[3] * dot(gk[3], gk[3]);
     if (wsum < 1e-20f) return;
     float gdd = 0.0f;
-    if (gamma != 0.0f) {
-        gdd = dot(gk[0], sub(P[0], xn_of(0))) + dot(gk[1], sub(P[1], xn_of(1))) + dot(gk[2], sub(P[2], xn_of(2))) +
-              dot(gk[3], sub(P[3], xn_of(3)));
-    }
+    if (gamma != 0.0f)
+        gdd = dot(gk[0], f.dx[0]) + dot(gk[1], f.dx[1]) + dot(gk[2], f.dx[2]) + dot(gk[3], f.dx[3]);
     float const dl = hw_div(-(C + alpha_tilde * lm) + gamma * gdd, (1.0f + gamma) * wsum + alpha_tilde);
     lm += dl;
+    v3 c[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) add_to(P[k], (wof(P[k]) * dl) * gk[k]);
+    for (int k = 0; k < 4; ++k) {
+        c[k] = (f.w[k] * dl) * gk[k];
+        f.d[k] = f.d[k] + c[k];
+        f.dx[k] = f.dx[k] + c[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) f.e[k] = f.e[k] + (c[k] - c[3]);
 }
 
 __device__ __forceinline__ m3 cofactor(m3 const& F)
@@ -228,27 +256,31 @@ __device__ __forceinline__ m3 cofactor(m3 const& F)
 }
 
 // Deviatoric then hydrostatic projection of one tet (P0..P3), multipliers lm = (deviatoric,
-// hydrostatic); c = (r2.z, 1/(mu|V0|), 1/(lambda|V0|), 1 + mu/lambda).
+// hydrostatic); c = (r2.z, 1/(mu|V0|), 1/(lambda|V0|), 1 + mu/lambda). xn_of(k) is the substep
+// start position of vertex k, read only with damping (beta_dt != 0).
 template <class Pos, class XnOf>
 __device__ __forceinline__ void neohookean_project(Pos P[4], XnOf&& xn_of, float4 a, float4 b, float4 c, float inv_dt2,
                                                    float beta_dt, float2& lm)
 {
     tet_rows const R = load_rows(a, b, c);
+    tet_frame f = make_frame(P, xn_of, beta_dt != 0.0f);
     {  // deviatoric: C = ||F||_F, dC/dF = F / C, compliance 1/(mu |V0|)
-        m3 const F = deformation_gradient(sub(P[0], P[3]), sub(P[1], P[3]), sub(P[2], P[3]), R.r0, R.r1, R.r2);
+        m3 const F = deformation_gradient(f.e[0], f.e[1], f.e[2], R.r0, R.r1, R.r2);
         float const C = hw_sqrt(dot(F.c0, F.c0) + dot(F.c1, F.c1) + dot(F.c2, F.c2));
         float const inv = C > 0.0f ? hw_rcp(C) : 0.0f;
         m3 const G = {inv * F.c0, inv * F.c1, inv * F.c2};
         float const at = c.y * inv_dt2;
-        tet_update(P, xn_of, G, R, C, at, at * beta_dt, lm.x);
+        tet_update(f, G, R, C, at, at * beta_dt, lm.x);
     }
     {  // hydrostatic: C = det F - (1 + mu/lambda), dC/dF = cof F, compliance 1/(lambda |V0|)
-        m3 const F = deformation_gradient(sub(P[0], P[3]), sub(P[1], P[3]), sub(P[2], P[3]), R.r0, R.r1, R.r2);
+        m3 const F = deformation_gradient(f.e[0], f.e[1], f.e[2], R.r0, R.r1, R.r2);
         m3 const G = cofactor(F);
         float const C = dot(F.c0, G.c0) - c.w;
         float const at = c.z * inv_dt2;
-        tet_update(P, xn_of, G, R, C, at, at * beta_dt, lm.y);
+        tet_update(f, G, R, C, at, at * beta_dt, lm.y);
     }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) add_to(P[k], f.d[k]);
 }
 
 __global__ void __launch_bounds__(256) k_neohookean(tet_group_dev g, pos4* xi, const pos4* __restrict__ xn,

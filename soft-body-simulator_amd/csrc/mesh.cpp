@@ -143,9 +143,13 @@ extern "C" int sbs_mesh_ellipsoid6(double rx, double ry, double rz, double edge,
         if (!tets) return;
         static const int order[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
         uint32_t* out = tets;
-        for (uint32_t i = 0; i < n[0]; ++i)
-            for (uint32_t j = 0; j < n[1]; ++j)
-                for (uint32_t k = 0; k < n[2]; ++k) {
+        // Cells are emitted by lattice parity class (i%2, j%2, k%2), lattice order inside a class.
+        // Cells of one class share no vertex, so a greedy coloring of the cells in this order needs
+        // exactly one color per class (8), balanced, whatever the ellipsoid cuts away.
+        for (uint32_t cls = 0; cls < 8; ++cls)
+        for (uint32_t i = cls & 1; i < n[0]; i += 2)
+            for (uint32_t j = (cls >> 1) & 1; j < n[1]; j += 2)
+                for (uint32_t k = cls >> 2; k < n[2]; k += 2) {
                     if (!inside(i, j, k)) continue;
                     for (int p = 0; p < 6; ++p) {
                         uint32_t c[3] = {i, j, k};

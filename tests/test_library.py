@@ -115,3 +115,16 @@ def test_batch_scene_shape():
     assert np.all(np.diff(g.body.astype(np.int64)) >= 0)
     for b in sc.bodies:
         assert b.x0[:, 1].min() >= 0.5 - 1e-9
+
+
+def test_batch_scene_clusters_take_eight_balanced_colors():
+    # ellipsoid cells come grouped by lattice parity, so the index-order greedy coloring of the
+    # hex clusters needs one color per parity class, for every body of the batch
+    from helpers import build_oracle
+    sc = scenes.batch(8, tet_range=(2e3, 8e3))
+    orc = build_oracle(sc)
+    orc.step(sc.dt, 1, 1)
+    col, nc = orc.group_colors(0, sc.num_tets)
+    assert nc == 8
+    per_color = np.bincount(col, minlength=8)
+    assert per_color.min() > 0.5 * per_color.max()  # odd box dimensions make parity classes uneven
