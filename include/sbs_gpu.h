@@ -181,7 +181,8 @@ int sbs_gpu_energy(sbs_gpu* sim, double out[6]);
 #define SBS_KERNEL_DISTANCE 4
 #define SBS_KERNEL_COMMIT 5
 #define SBS_KERNEL_DETECT 6
-#define SBS_KERNEL_COUNT 7
+#define SBS_KERNEL_WARM_START 7 /* features, network inference and the warm-start pass */
+#define SBS_KERNEL_COUNT 8
 
 /* 1: bracket every launch with HIP events on the simulation's stream (not with graphs). */
 int sbs_gpu_set_timing(sbs_gpu* sim, int enable);
@@ -189,6 +190,80 @@ int sbs_gpu_set_timing(sbs_gpu* sim, int enable);
  * (constraints, contacts or particles). Waits for the device. */
 int sbs_gpu_kernel_stats(sbs_gpu* sim, int kind, uint64_t* launches, double* total_ms, uint64_t* units);
 int sbs_gpu_reset_stats(sbs_gpu* sim);
+
+/* ---- learned warm start ---------------------------------------------------------------------------- */
+
+/* No reference counterpart: XPBD (as in the reference) starts every substep from the predicted
+ * positions with zero multipliers (constraint.cpp:12-16, timestep.cpp:35-43). A warm start instead
+ * begins from multipliers lambda_init and the positions they imply, x = x~ + M^-1 grad C(x~)^T lambda_init
+ * (one extra colored pass, every gradient taken at the predicted positions x~), so that positions and
+ * multipliers start consistent with the implicit step XPBD approximates. XPBD's iterate depends on
+ * its starting point, so a warm-started substep is a different (not a bit-equal) approximation of
+ * that step; the residuals measure how close each one gets. */
+
+typedef struct sbs_mlp sbs_mlp;
+
+/* Network 16 -> 128 -> 128 -> 128 (ReLU) -> 32 of which num_inputs (<= 12) inputs and num_outputs
+ * (<= 4) outputs are used; BF16 matrix-core arithmetic with FP32 accumulation, FP32 master
+ * parameters (He initialization from seed, output layer scaled by 0.1). */
+int sbs_mlp_create(int device, uint32_t num_inputs, uint32_t num_outputs, uint64_t seed, sbs_mlp** out);
+int sbs_mlp_destroy(sbs_mlp* net);
+/* Length of the flat FP32 parameter vector: W1[16][128] b1[128] W2[128][128] b2[128] W3[128][128]
+ * b3[128] W4[128][32] b4[32], W_l[i][o] multiplying input unit i into output unit o. */
+int sbs_mlp_num_params(uint32_t* count);
+int sbs_mlp_get_params(sbs_mlp* net, float* params);
+int sbs_mlp_set_params(sbs_mlp* net, const float* params);
+/* The network computes y = out_scale * f((x - mean) * inv_std); mean, inv_std have num_inputs
+ * entries, out_scale num_outputs. Defaults 0, 1, 1. */
+int sbs_mlp_set_normalization(sbs_mlp* net, const float* mean, const float* inv_std, const float* out_scale);
+int sbs_mlp_get_normalization(sbs_mlp* net, float* mean, float* inv_std, float* out_scale);
+/* y[n][num_outputs] for x[n][num_inputs] through the fused inference kernel. */
+int sbs_mlp_forward(sbs_mlp* net, const float* x, uint32_t n, float* y);
+/* Mean squared error of f(x') against t / out_scale over the n rows and num_outputs outputs, and
+ * its gradient with respect to the parameter vector (through the training kernels). */
+int sbs_mlp_loss_grad(sbs_mlp* net, const float* x, const float* t, uint32_t n, double* loss, float* grad);
+/* Device-resident training rows (num_inputs features, num_outputs targets each). */
+int sbs_mlp_dataset_reserve(sbs_mlp* net, uint64_t rows);
+int sbs_mlp_dataset_append(sbs_mlp* net, const float* x, const float* t, uint64_t rows);
+int sbs_mlp_dataset_size(sbs_mlp* net, uint64_t* rows);
+/* Rows [first, first + rows) back to the host (x: rows * num_inputs, t: rows * num_outputs; either
+ * may be NULL). */
+int sbs_mlp_dataset_read(sbs_mlp* net, uint64_t first, uint64_t rows, float* x, float* t);
+int sbs_mlp_dataset_clear(sbs_mlp* net);
+/* mean / inverse standard deviation of every input and RMS of every target over (a strided sample
+ * of at most 2^20 rows of) the dataset become the normalization. */
+int sbs_mlp_fit_normalization(sbs_mlp* net);
+/* `steps` Adam steps (beta 0.9 / 0.999, eps 1e-8) on minibatches of `batch` rows drawn with
+ * replacement by a counter hash of (seed, step); *loss (may be NULL) gets the last batch's loss. */
+int sbs_mlp_train(sbs_mlp* net, uint32_t steps, uint32_t batch, float lr, uint64_t seed, double* loss);
+
+#define SBS_WARM_START_NONE 0     /* reference behaviour: lambda_init = 0 */
+#define SBS_WARM_START_PREVIOUS 1 /* lambda_init = the previous substep's final multipliers */
+#define SBS_WARM_START_LEARNED 2  /* lambda_init = previous + net(features), net with 12 inputs, 2 outputs */
+#define SBS_WARM_START_EXPLICIT 3 /* lambda_init = -C(x~) / alpha~ (zero XPBD residual at x~) */
+/* Warm start of the neo-Hookean groups' multipliers (other groups start from zero). The network's
+ * 12 inputs per constraint are, at the predicted positions: lambda_prev (deviatoric, hydrostatic),
+ * the pair before it, C_dev, C_hyd, log sum_k w_k |grad_k C_dev|^2, the same for C_hyd,
+ * log alpha~_dev, log alpha~_hyd, and the multipliers -C / alpha~ of both constraints; its 2
+ * outputs are the change of the pair over the substep. The handle keeps
+ * `net` (not owned) until the mode changes. */
+int sbs_gpu_set_warm_start(sbs_gpu* sim, int mode, sbs_mlp* net);
+/* Factor on the warm-start multipliers (default 1): lambda_init = scale * (the mode's multipliers). */
+int sbs_gpu_set_warm_start_scale(sbs_gpu* sim, double scale);
+/* While stride > 0, every substep of sbs_gpu_step appends one training row (the 12 features at the
+ * substep start, the change of the multiplier pair over the substep) for every stride-th
+ * constraint of each neo-Hookean group to net's dataset, until its capacity is reached. Graphs
+ * are not used while capturing. stride 0 stops. */
+int sbs_gpu_capture(sbs_gpu* sim, sbs_mlp* net, uint32_t stride);
+/* Device copy of the dynamic state (positions, velocities, multipliers) and its restoration. */
+int sbs_gpu_snapshot(sbs_gpu* sim);
+int sbs_gpu_restore(sbs_gpu* sim);
+/* With a warm start and limit > 0, each sbs_gpu_step snapshots the state and, when the FP64
+ * residual norm sqrt(sum over neo-Hookean groups of sum h^2) after the frame exceeds limit, restores
+ * it and repeats the frame without warm start (the plain solver as fallback) with `iterations`
+ * iterations per substep (0: the step's own count). *fallbacks counts the repeated frames. */
+int sbs_gpu_set_fallback(sbs_gpu* sim, double limit, uint32_t iterations);
+int sbs_gpu_fallback_count(sbs_gpu* sim, uint64_t* fallbacks);
 
 /* ---- host mesh utilities (no device) ---------------------------------------------------------------- */
 

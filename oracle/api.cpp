@@ -192,6 +192,11 @@ void orc_set_clusters(void* h, uint32_t group, const uint32_t* offsets, uint32_t
 }
 
 void orc_set_coloring(void* h, int order) { static_cast<simulation_t*>(h)->coloring = order; }
+void orc_set_warm_start(void* h, int mode, double scale)
+{
+    static_cast<simulation_t*>(h)->warm_start = mode;
+    static_cast<simulation_t*>(h)->warm_scale = scale;
+}
 
 // colored != 0 selects the colored visiting order (and computes the coloring of every group).
 void orc_set_order(void* h, int colored, uint32_t recolor_passes)

@@ -216,6 +216,33 @@ void neohookean_constraint_t::project_positions(simulation_t& sim, double dt)
     }
 }
 
+void neohookean_constraint_t::add_multiplier_displacement(simulation_t const& sim, std::vector<std::vector<vec3>>& d)
+{
+    auto const& P = sim.particles[bi];
+    particle_t const* pp[4] = {&P[v[0]], &P[v[1]], &P[v[2]], &P[v[3]]};
+    double const w[4] = {pp[0]->invmass(), pp[1]->invmass(), pp[2]->invmass(), pp[3]->invmass()};
+    mat3 const Ds = mat3::from_cols(pp[0]->xi - pp[3]->xi, pp[1]->xi - pp[3]->xi, pp[2]->xi - pp[3]->xi);
+    mat3 const F = Ds * DmInv;
+    double const r = std::sqrt(frob2(F));
+    vec3 gd[4], gh[4];
+    tet_gradients((r > 0 ? 1.0 / r : 0.0) * F, DmInv, gd);
+    tet_gradients(cofactor(F), DmInv, gh);
+    double wd = 0, wh = 0;
+    for (int k = 0; k < 4; ++k) {
+        wd += w[k] * norm2(gd[k]);
+        wh += w[k] * norm2(gh[k]);
+    }
+    if (sim.warm_start == 3) {  // the multipliers that zero the XPBD residual at the predicted positions
+        double const dt2 = sim.warm_dt * sim.warm_dt, V0abs = std::abs(V0);
+        lagrange = -r / (1.0 / (mu * V0abs) / dt2);
+        lagrange_h = -(determinant(F) - (1.0 + mu / lam)) / (1.0 / (lam * V0abs) / dt2);
+    }
+    lagrange *= sim.warm_scale;
+    lagrange_h *= sim.warm_scale;
+    double const ld = wd < 1e-20 ? 0.0 : lagrange, lh = wh < 1e-20 ? 0.0 : lagrange_h;
+    for (int k = 0; k < 4; ++k) d[bi][v[k]] += w[k] * (gd[k] * ld + gh[k] * lh);
+}
+
 void neohookean_constraint_t::residual_sq(simulation_t const& sim, double dt, double* acc) const
 {
     auto const& P = sim.particles[bi];
@@ -355,11 +382,26 @@ void simulation_t::build_order()
 }
 
 // gauss_seidel_solver.cpp:8-37, with the elastic constraints visited in `order`.
+// x = x~ + M^-1 grad C(x~)^T lambda over every neo-Hookean constraint, all gradients at the
+// predicted positions (a sum, so the visiting order does not matter).
+void simulation_t::apply_warm_start()
+{
+    std::vector<std::vector<vec3>> d(particles.size());
+    for (std::size_t b = 0; b < particles.size(); ++b) d[b].assign(particles[b].size(), vec3{});
+    for (auto const& c : constraints)
+        if (auto* nh = dynamic_cast<neohookean_constraint_t*>(c.get())) nh->add_multiplier_displacement(*this, d);
+    for (std::size_t b = 0; b < particles.size(); ++b)
+        for (std::size_t i = 0; i < particles[b].size(); ++i) particles[b][i].xi += d[b][i];
+}
+
 void simulation_t::solve(double dt, std::size_t iterations)
 {
     if (order_dirty) build_order();
     for (auto& c : collision_constraints) c->prepare_for_projection();
-    for (auto& c : constraints) c->prepare_for_projection();
+    for (auto& c : constraints)
+        if (!(warm_start && dynamic_cast<neohookean_constraint_t*>(c.get()))) c->prepare_for_projection();
+    warm_dt = dt;
+    if (warm_start) apply_warm_start();
     for (std::size_t k = 0; k < iterations; ++k) {
         for (auto& c : collision_constraints) c->project_positions(*this, dt);
         for (uint32_t i : order) constraints[i]->project_positions(*this, dt);

@@ -76,6 +76,11 @@ struct neohookean_constraint_t final : constraint_t {
                             double young, double poisson);
     void prepare_impl() override { lagrange_h = 0.0; }
     void project_positions(simulation_t& sim, double dt) override;
+    // Warm start: adds M^-1 (grad C_dev lambda + grad C_hyd lambda_h) at the current positions to
+    // d[body][vertex]; multipliers unchanged.
+    // With simulation_t::warm_start == 3 the multipliers are first set to -C / alpha~; then they are
+    // multiplied by simulation_t::warm_scale.
+    void add_multiplier_displacement(simulation_t const& sim, std::vector<std::vector<vec3>>& d);
     void residual_sq(simulation_t const& sim, double dt, double* acc) const override;
     double energy(simulation_t const& sim) const override;
 };
@@ -150,6 +155,12 @@ struct simulation_t {
     bool colored_order = false;
     uint32_t recolor_passes = 2;
     int coloring = 0;  // 0: hash priorities (+ recolor passes), 1: index order
+    // Warm start of the neo-Hookean multipliers (the device's SBS_WARM_START_* numbering): 1 carries
+    // them over from the previous substep, 3 sets them to -C(x~)/alpha~; either way their
+    // displacement M^-1 grad C(x~)^T lambda is added before the iterations.
+    int warm_start = 0;
+    double warm_dt = 0.0;
+    double warm_scale = 1.0;  // factor on the warm-start multipliers
 
     // GS visiting order of the elastic constraints (indices into constraints)
     std::vector<uint32_t> order;
@@ -158,6 +169,7 @@ struct simulation_t {
 
     void detect_contacts();
     void solve(double dt, std::size_t iterations);  // gauss_seidel_solver_t::solve
+    void apply_warm_start();
     void step(double dt, std::size_t iterations, std::size_t substeps);  // timestep_t::step
 };
 

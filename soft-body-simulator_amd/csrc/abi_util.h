@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <stdexcept>
 #include <string>
 
@@ -31,6 +32,47 @@ inline void hip_check(hipError_t e, char const* what)
         throw api_error(code, std::string(what) + ": " + hipGetErrorString(e));
     }
 }
+
+// Owning device array (hipMalloc / hipFree).
+template <class T>
+struct dev_buf {
+    T* p = nullptr;
+    std::size_t n = 0;
+    dev_buf() = default;
+    dev_buf(dev_buf const&) = delete;
+    dev_buf& operator=(dev_buf const&) = delete;
+    dev_buf(dev_buf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    dev_buf& operator=(dev_buf&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~dev_buf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(std::size_t count)
+    {
+        release();
+        if (count == 0) return;
+        hip_check(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)), "hipMalloc");
+        n = count;
+    }
+    void upload(T const* h, std::size_t count, hipStream_t s)
+    {
+        alloc(count);
+        if (count) hip_check(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+    }
+};
 
 template <class F>
 int guarded(F&& f)

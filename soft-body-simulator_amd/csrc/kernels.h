@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "mlp.h"
+
 namespace sbs {
 
 // Particle positions: x, y, z and the inverse mass in w, in FP64 (see kernels.hip).
@@ -64,13 +66,20 @@ void launch_commit_predict(hipStream_t s, pos4* xi, pos4* xn, float4* v, uint32_
 
 void launch_green(hipStream_t s, tet_group_dev g, pos4* xi, const pos4* xn, uint32_t begin, uint32_t count,
                   float alpha_tilde, float gamma, int first, int block);
+// Neo-Hookean launches take a mode: kSolveFirst (first iteration, multipliers start at 0),
+// kSolveNext (multipliers as stored), or a warm-start pass (no solve): kApplyWarmStart adds
+// M^-1 grad C^T lambda of the stored multipliers to xi, kApplyExplicit first sets the multipliers
+// to -C / alpha~ and stores them; both take every gradient at the predicted positions, passed as
+// `xn` (a copy of xi made before the pass), so the pass is a sum independent of visiting order.
+// In those modes beta_dt carries the factor the multipliers are scaled by (and stored).
+constexpr int kSolveNext = 0, kSolveFirst = 1, kApplyWarmStart = 2, kApplyExplicit = 3;
 void launch_neohookean(hipStream_t s, tet_group_dev g, pos4* xi, const pos4* xn, uint32_t begin,
-                       uint32_t count, float inv_dt2, float beta_dt, int first, int block);
+                       uint32_t count, float inv_dt2, float beta_dt, int mode, int block);
 // One thread per cluster of clusters [begin, begin+count).
 void launch_green_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn,
                            uint32_t begin, uint32_t count, float alpha_tilde, float gamma, int first, int block);
 void launch_neohookean_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn,
-                                uint32_t begin, uint32_t count, float inv_dt2, float beta_dt, int first, int block);
+                                uint32_t begin, uint32_t count, float inv_dt2, float beta_dt, int mode, int block);
 
 // Clusters that are the 6 tets of one hex cell of the box split, in its slot pattern
 // {0123, 0413, 0253, 0563, 0743, 0673}: particles held in registers. The color's constraints
@@ -78,7 +87,7 @@ void launch_neohookean_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, 
 void launch_green_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn, uint32_t begin,
                       uint32_t count, uint32_t tbase, float alpha_tilde, float gamma, int first, int block);
 void launch_neohookean_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn,
-                           uint32_t begin, uint32_t count, uint32_t tbase, float inv_dt2, float beta_dt, int first,
+                           uint32_t begin, uint32_t count, uint32_t tbase, float inv_dt2, float beta_dt, int mode,
                            int block);
 constexpr uint32_t kHexPatternLoc[6] = {0x03020100u, 0x03010400u, 0x03050200u, 0x03060500u, 0x03040700u, 0x03070600u};
 
@@ -111,6 +120,22 @@ void launch_jp_round(hipStream_t s, uint32_t n, uint32_t arity, const uint32_t* 
                      const uint32_t* inc, const uint64_t* key, const uint32_t* color_in, uint32_t* color_out,
                      uint32_t* remaining, uint32_t* overflow, int block);
 void launch_jp_keys(hipStream_t s, uint32_t n, const uint32_t* prev, uint64_t* key, int block);
+
+void launch_copy_pos(hipStream_t s, const pos4* a, pos4* b, uint32_t n);
+
+// Multiplier warm start of a neo-Hookean group, per constraint t (execution order) at the
+// predicted positions xi: the 12 features
+//   lambda_prev (dev, hyd), lambda_prev2 (dev, hyd), C_dev, C_hyd, log sum w|grad C_dev|^2,
+//   log sum w|grad C_hyd|^2, log alpha~_dev, log alpha~_hyd, -C_dev / alpha~_dev, -C_hyd / alpha~_hyd
+// where lambda_prev is the multiplier pair stored in g.lambda (the previous substep's final) and
+// lambda_prev2 the pair before it (hist, which receives lambda_prev). raw gets the 12 FP32 values
+// ([count][12]) and/or X the normalized BF16 network input ([count][16]); either may be null.
+void launch_ws_features(hipStream_t s, tet_group_dev g, const pos4* xi, float2* hist, uint32_t count, float inv_dt2,
+                        float* raw, mlp::bf16* X, const mlp::norm_dev* norm, int block);
+// Training rows of constraints t = offset + i * stride, i < rows: ds_x[base + i] = raw[t],
+// ds_t[base + i] = lambda[t] - lambda_prev[t] (the multipliers' change over the substep).
+void launch_ws_capture(hipStream_t s, const float* raw, const float* lambda, uint32_t stride, uint32_t offset,
+                       float* ds_x, float* ds_t, uint64_t base, uint32_t rows);
 
 // Fills a buffer with a 32-bit value (graph-safe replacement of a memset).
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t value, uint32_t n);

@@ -283,16 +283,78 @@ __device__ __forceinline__ void neohookean_project(Pos P[4], XnOf&& xn_of, float
     for (int k = 0; k < 4; ++k) add_to(P[k], f.d[k]);
 }
 
+// Warm-start displacement of one tet at positions P (the predicted positions): both gradients at
+// P, d_k = w_k (grad_k C_dev lm.x + grad_k C_hyd lm.y). With explicit the multipliers are first
+// set to -C(P) / alpha~, the pair that zeroes the XPBD residual at P; then they are multiplied by
+// scale (which the caller stores back). A constraint whose
+// sum w|grad C|^2 is below the projection's epsilon contributes nothing, as it would not project.
+template <class Pos>
+__device__ __forceinline__ void neohookean_displacement(Pos const P[4], float4 a, float4 b, float4 c, float inv_dt2,
+                                                        bool expl, float scale, float2& lm, v3 d[4])
+{
+    tet_rows const R = load_rows(a, b, c);
+    tet_frame const f = make_frame(P, [](int) { return pos4{}; }, false);
+    m3 const F = deformation_gradient(f.e[0], f.e[1], f.e[2], R.r0, R.r1, R.r2);
+    float const Cd = hw_sqrt(dot(F.c0, F.c0) + dot(F.c1, F.c1) + dot(F.c2, F.c2));
+    float const inv = Cd > 0.0f ? hw_rcp(Cd) : 0.0f;
+    m3 const Gd = {inv * F.c0, inv * F.c1, inv * F.c2};
+    m3 const Gh = cofactor(F);
+    if (expl) {
+        float const Ch = dot(F.c0, Gh.c0) - c.w;
+        lm = make_float2(-Cd / (c.y * inv_dt2), -Ch / (c.z * inv_dt2));
+    }
+    lm = make_float2(scale * lm.x, scale * lm.y);
+    v3 gd[4], gh[4];
+    gd[0] = mul(Gd, R.r0);
+    gd[1] = mul(Gd, R.r1);
+    gd[2] = mul(Gd, R.r2);
+    gd[3] = -(gd[0] + gd[1] + gd[2]);
+    gh[0] = mul(Gh, R.r0);
+    gh[1] = mul(Gh, R.r1);
+    gh[2] = mul(Gh, R.r2);
+    gh[3] = -(gh[0] + gh[1] + gh[2]);
+    float wd = 0.0f, wh = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        wd += f.w[k] * dot(gd[k], gd[k]);
+        wh += f.w[k] * dot(gh[k], gh[k]);
+    }
+    float const ld = wd < 1e-20f ? 0.0f : lm.x, lh = wh < 1e-20f ? 0.0f : lm.y;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) d[k] = f.w[k] * (ld * gd[k] + lh * gh[k]);
+}
+
+// x += d in FP64 (read, add, write) for the x, y, z of one particle.
+__device__ __forceinline__ void add_displacement(pos4* x, v3 d)
+{
+    double* q = reinterpret_cast<double*>(x);
+    q[0] += double(d.x);
+    q[1] += double(d.y);
+    q[2] += double(d.z);
+}
+
 __global__ void __launch_bounds__(256) k_neohookean(tet_group_dev g, pos4* xi, const pos4* __restrict__ xn,
                                                     uint32_t begin, uint32_t count, float inv_dt2, float beta_dt,
-                                                    int first)
+                                                    int mode)
 {
     uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     uint32_t const t = begin + i;
     int4 const id = g.idx[t];
     float4 const a = g.m0[t], b = g.m1[t], c = g.m2[t];
-    float2 lm = first ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(g.lambda)[t];
+    float2 lm = mode == kSolveFirst ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(g.lambda)[t];
+    if (mode >= kApplyWarmStart) {  // gradients at the predicted positions (xn holds a copy of them)
+        pos4 const P[4] = {xn[id.x], xn[id.y], xn[id.z], xn[id.w]};
+        v3 d[4];
+        // in the warm-start modes beta_dt carries the multipliers' scale (no damping term applies)
+        neohookean_displacement(P, a, b, c, inv_dt2, mode == kApplyExplicit, beta_dt, lm, d);
+        reinterpret_cast<float2*>(g.lambda)[t] = lm;
+        add_displacement(&xi[id.x], d[0]);
+        add_displacement(&xi[id.y], d[1]);
+        add_displacement(&xi[id.z], d[2]);
+        add_displacement(&xi[id.w], d[3]);
+        return;
+    }
     pos4 P[4] = {xi[id.x], xi[id.y], xi[id.z], xi[id.w]};
     auto xn_of = [&](int k) { return xn[k == 0 ? id.x : (k == 1 ? id.y : (k == 2 ? id.z : id.w))]; };
     neohookean_project(P, xn_of, a, b, c, inv_dt2, beta_dt, lm);
@@ -301,6 +363,88 @@ __global__ void __launch_bounds__(256) k_neohookean(tet_group_dev g, pos4* xi, c
     xi[id.y] = P[1];
     xi[id.z] = P[2];
     xi[id.w] = P[3];
+}
+
+// ---------------------------------------------------------------- multiplier warm start
+
+__global__ void k_copy_pos(const pos4* __restrict__ a, pos4* __restrict__ b, uint32_t n)
+{
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+// Sum over the tet's vertices of w_k |grad_k C|^2 for dC/dF = G.
+__device__ __forceinline__ float grad_norm2(tet_frame const& f, m3 const& G, tet_rows const& R)
+{
+    v3 const g0 = mul(G, R.r0), g1 = mul(G, R.r1), g2 = mul(G, R.r2);
+    v3 const g3 = -(g0 + g1 + g2);
+    return f.w[0] * dot(g0, g0) + f.w[1] * dot(g1, g1) + f.w[2] * dot(g2, g2) + f.w[3] * dot(g3, g3);
+}
+
+// Features of neo-Hookean constraint t at the predicted positions (see launch_ws_features).
+__global__ void __launch_bounds__(256) k_ws_features(tet_group_dev g, const pos4* __restrict__ xi,
+                                                     float2* __restrict__ hist, uint32_t count, float inv_dt2,
+                                                     float* __restrict__ raw, mlp::bf16* __restrict__ X,
+                                                     const mlp::norm_dev* __restrict__ norm)
+{
+    uint32_t const t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    int4 const id = g.idx[t];
+    float4 const a = g.m0[t], b = g.m1[t], c = g.m2[t];
+    tet_rows const R = load_rows(a, b, c);
+    pos4 const P[4] = {xi[id.x], xi[id.y], xi[id.z], xi[id.w]};
+    tet_frame const f = make_frame(P, [](int) { return pos4{}; }, false);
+    float2 const lp = reinterpret_cast<const float2*>(g.lambda)[t];
+    float2 const lp2 = hist[t];
+    hist[t] = lp;
+    m3 const F = deformation_gradient(f.e[0], f.e[1], f.e[2], R.r0, R.r1, R.r2);
+    float const Cd = hw_sqrt(dot(F.c0, F.c0) + dot(F.c1, F.c1) + dot(F.c2, F.c2));
+    float const inv = Cd > 0.0f ? hw_rcp(Cd) : 0.0f;
+    m3 const Gd = {inv * F.c0, inv * F.c1, inv * F.c2};
+    m3 const Gh = cofactor(F);
+    float const Ch = dot(F.c0, Gh.c0) - c.w;
+    float const wd = grad_norm2(f, Gd, R), wh = grad_norm2(f, Gh, R);
+    float const atd = c.y * inv_dt2, ath = c.z * inv_dt2;
+    float const v[mlp::kMaxInputs] = {lp.x,
+                                      lp.y,
+                                      lp2.x,
+                                      lp2.y,
+                                      Cd,
+                                      Ch,
+                                      logf(wd + 1e-30f),
+                                      logf(wh + 1e-30f),
+                                      logf(atd),
+                                      logf(ath),
+                                      -Cd / atd,
+                                      -Ch / ath};
+    if (raw) {
+#pragma unroll
+        for (int k = 0; k < mlp::kMaxInputs; ++k) raw[std::size_t(t) * mlp::kMaxInputs + k] = v[k];
+    }
+    if (X) {
+        mlp::bf16* o = X + std::size_t(t) * mlp::kIn;
+#pragma unroll
+        for (int k = 0; k < mlp::kIn; ++k)
+            o[k] = static_cast<mlp::bf16>(k < mlp::kMaxInputs ? (v[k] - norm->mean[k]) * norm->inv_std[k] : 0.0f);
+    }
+}
+
+__global__ void k_ws_capture(const float* __restrict__ raw, const float2* __restrict__ lambda, uint32_t stride,
+                             uint32_t offset, float* __restrict__ ds_x, float* __restrict__ ds_t, uint64_t base,
+                             uint32_t rows)
+{
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    std::size_t const t = std::size_t(offset) + std::size_t(i) * stride;
+    std::size_t const r = base + i;
+    float const* f = raw + t * mlp::kMaxInputs;
+#pragma unroll
+    for (int k = 0; k < mlp::kMaxInputs; ++k) ds_x[r * mlp::kMaxInputs + k] = f[k];
+    float2 const lf = lambda[t];
+    ds_t[r * mlp::kMaxOutputs + 0] = lf.x - f[0];
+    ds_t[r * mlp::kMaxOutputs + 1] = lf.y - f[1];
+    ds_t[r * mlp::kMaxOutputs + 2] = 0.0f;
+    ds_t[r * mlp::kMaxOutputs + 3] = 0.0f;
 }
 
 // ---------------------------------------------------------------- clustered Gauss-Seidel
@@ -367,13 +511,39 @@ __global__ void __launch_bounds__(BLOCK) k_green_clusters(tet_group_dev g, clust
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_neohookean_clusters(tet_group_dev g, cluster_dev cl, pos4* xi,
                                                                const pos4* __restrict__ xn, uint32_t begin,
-                                                               uint32_t count, float inv_dt2, float beta_dt, int first)
+                                                               uint32_t count, float inv_dt2, float beta_dt, int mode)
 {
     run_cluster<BLOCK>(cl, xi, xn, begin, count, [&](uint32_t t, pos4 P[4], auto&& xn_of) {
-        float2 lm = first ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(g.lambda)[t];
+        float2 lm = mode == kSolveFirst ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(g.lambda)[t];
         neohookean_project(P, xn_of, g.m0[t], g.m1[t], g.m2[t], inv_dt2, beta_dt, lm);
         reinterpret_cast<float2*>(g.lambda)[t] = lm;
     });
+}
+
+// Warm-start displacement of the clusters of one color: every tet's gradients at the predicted
+// positions xt, its displacement added to xi at once (clusters of a color share no particle, and
+// one thread runs a cluster's tets in order, so the additions do not race).
+__global__ void __launch_bounds__(128) k_neohookean_clusters_apply(tet_group_dev g, cluster_dev cl, pos4* xi,
+                                                                   const pos4* __restrict__ xt, uint32_t begin,
+                                                                   uint32_t count, float inv_dt2, int expl,
+                                                                   float scale)
+{
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    uint32_t const c = begin + i;
+    int4 const va = cl.v0[c], vb = cl.v1[c];
+    for (uint32_t t = cl.off[c]; t < cl.off[c + 1]; ++t) {
+        uint32_t const loc = cl.loc[t];
+        int const id[4] = {slot_id(va, vb, loc & 0xffu), slot_id(va, vb, (loc >> 8) & 0xffu),
+                           slot_id(va, vb, (loc >> 16) & 0xffu), slot_id(va, vb, loc >> 24)};
+        pos4 const P[4] = {xt[id[0]], xt[id[1]], xt[id[2]], xt[id[3]]};
+        float2 lm = reinterpret_cast<const float2*>(g.lambda)[t];
+        v3 d[4];
+        neohookean_displacement(P, g.m0[t], g.m1[t], g.m2[t], inv_dt2, expl != 0, scale, lm, d);
+        reinterpret_cast<float2*>(g.lambda)[t] = lm;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) add_displacement(&xi[id[k]], d[k]);
+    }
 }
 
 // ---------------------------------------------------------------- hex-cell clusters
@@ -390,7 +560,7 @@ constexpr int kHexSlots[kHexTets][4] = {{0, 1, 2, 3}, {0, 4, 1, 3}, {0, 2, 5, 3}
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_neohookean_hex(tet_group_dev g, cluster_dev cl, pos4* xi,
                                                           const pos4* __restrict__ xn, uint32_t begin, uint32_t count,
-                                                          uint32_t tbase, float inv_dt2, float beta_dt, int first)
+                                                          uint32_t tbase, float inv_dt2, float beta_dt, int mode)
 {
     uint32_t const i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= count) return;
@@ -405,7 +575,7 @@ __global__ void __launch_bounds__(BLOCK) k_neohookean_hex(tet_group_dev g, clust
         A[k] = g.m0[t];
         B[k] = g.m1[t];
         Cm[k] = g.m2[t];
-        L[k] = first ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(g.lambda)[t];
+        L[k] = mode == kSolveFirst ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(g.lambda)[t];
     }
     pos4 P[8];
 #pragma unroll
@@ -426,6 +596,43 @@ __global__ void __launch_bounds__(BLOCK) k_neohookean_hex(tet_group_dev g, clust
     for (int k = 0; k < kHexTets; ++k) reinterpret_cast<float2*>(g.lambda)[tbase + uint32_t(k) * count + i] = L[k];
 #pragma unroll
     for (int k = 0; k < 8; ++k) xi[ids[k]] = P[k];
+}
+
+// Warm-start displacement of the hex clusters of one color (gradients at the predicted positions
+// xt, the 6 tets' displacements summed per particle, then added to xi).
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_neohookean_hex_apply(tet_group_dev g, cluster_dev cl, pos4* xi,
+                                                                const pos4* __restrict__ xt, uint32_t begin,
+                                                                uint32_t count, uint32_t tbase, float inv_dt2, int expl,
+                                                                float scale)
+{
+    uint32_t const i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= count) return;
+    uint32_t const c = begin + i;
+    int4 const va = cl.v0[c], vb = cl.v1[c];
+    int const ids[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
+    pos4 P[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) P[k] = xt[ids[k]];
+    v3 D[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) D[k] = mk(0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int k = 0; k < kHexTets; ++k) {
+        uint32_t const t = tbase + uint32_t(k) * count + i;
+        int const s0 = kHexSlots[k][0], s1 = kHexSlots[k][1], s2 = kHexSlots[k][2], s3 = kHexSlots[k][3];
+        pos4 const Q[4] = {P[s0], P[s1], P[s2], P[s3]};
+        float2 lm = reinterpret_cast<const float2*>(g.lambda)[t];
+        v3 d[4];
+        neohookean_displacement(Q, g.m0[t], g.m1[t], g.m2[t], inv_dt2, expl != 0, scale, lm, d);
+        reinterpret_cast<float2*>(g.lambda)[t] = lm;
+        D[s0] = D[s0] + d[0];
+        D[s1] = D[s1] + d[1];
+        D[s2] = D[s2] + d[2];
+        D[s3] = D[s3] + d[3];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) add_displacement(&xi[ids[k]], D[k]);
 }
 
 template <int BLOCK>
@@ -843,10 +1050,10 @@ void launch_green(hipStream_t s, tet_group_dev g, pos4* xi, const pos4* xn, uint
 }
 
 void launch_neohookean(hipStream_t s, tet_group_dev g, pos4* xi, const pos4* xn, uint32_t begin, uint32_t count,
-                       float inv_dt2, float beta_dt, int first, int block)
+                       float inv_dt2, float beta_dt, int mode, int block)
 {
     if (count)
-        k_neohookean<<<grid_for(count, block), block, 0, s>>>(g, xi, xn, begin, count, inv_dt2, beta_dt, first);
+        k_neohookean<<<grid_for(count, block), block, 0, s>>>(g, xi, xn, begin, count, inv_dt2, beta_dt, mode);
 }
 
 void launch_distance(hipStream_t s, distance_group_dev g, pos4* xi, const pos4* xn, uint32_t begin, uint32_t count,
@@ -867,26 +1074,36 @@ void launch_green_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4*
 }
 
 void launch_neohookean_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn,
-                                uint32_t begin, uint32_t count, float inv_dt2, float beta_dt, int first, int block)
+                                uint32_t begin, uint32_t count, float inv_dt2, float beta_dt, int mode, int block)
 {
     if (!count) return;
+    if (mode >= kApplyWarmStart) {
+        k_neohookean_clusters_apply<<<grid_for(count, 128), 128, 0, s>>>(g, cl, xi, xn, begin, count, inv_dt2,
+                                                                         mode == kApplyExplicit, beta_dt);
+        return;
+    }
     if (block <= 64)
-        k_neohookean_clusters<64><<<grid_for(count, 64), 64, 0, s>>>(g, cl, xi, xn, begin, count, inv_dt2, beta_dt, first);
+        k_neohookean_clusters<64><<<grid_for(count, 64), 64, 0, s>>>(g, cl, xi, xn, begin, count, inv_dt2, beta_dt, mode);
     else
         k_neohookean_clusters<128><<<grid_for(count, 128), 128, 0, s>>>(g, cl, xi, xn, begin, count, inv_dt2, beta_dt,
-                                                                       first);
+                                                                       mode);
 }
 
 void launch_neohookean_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn, uint32_t begin,
-                           uint32_t count, uint32_t tbase, float inv_dt2, float beta_dt, int first, int block)
+                           uint32_t count, uint32_t tbase, float inv_dt2, float beta_dt, int mode, int block)
 {
     if (!count) return;
+    if (mode >= kApplyWarmStart) {
+        k_neohookean_hex_apply<64><<<grid_for(count, 64), 64, 0, s>>>(g, cl, xi, xn, begin, count, tbase, inv_dt2,
+                                                                       mode == kApplyExplicit, beta_dt);
+        return;
+    }
     if (block <= 64)
         k_neohookean_hex<64><<<grid_for(count, 64), 64, 0, s>>>(g, cl, xi, xn, begin, count, tbase, inv_dt2, beta_dt,
-                                                                first);
+                                                                mode);
     else
         k_neohookean_hex<128><<<grid_for(count, 128), 128, 0, s>>>(g, cl, xi, xn, begin, count, tbase, inv_dt2,
-                                                                   beta_dt, first);
+                                                                   beta_dt, mode);
 }
 
 void launch_green_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn, uint32_t begin,
@@ -964,6 +1181,27 @@ void launch_jp_keys(hipStream_t s, uint32_t n, const uint32_t* prev, uint64_t* k
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t value, uint32_t n)
 {
     if (n) k_fill_u32<<<grid_for(n, 256), 256, 0, s>>>(p, value, n);
+}
+
+
+void launch_copy_pos(hipStream_t s, const pos4* a, pos4* b, uint32_t n)
+{
+    if (n) k_copy_pos<<<grid_for(n, 256), 256, 0, s>>>(a, b, n);
+}
+
+void launch_ws_features(hipStream_t s, tet_group_dev g, const pos4* xi, float2* hist, uint32_t count, float inv_dt2,
+                        float* raw, mlp::bf16* X, const mlp::norm_dev* norm, int block)
+{
+    if (!count) return;
+    k_ws_features<<<grid_for(count, 256), 256, 0, s>>>(g, xi, hist, count, inv_dt2, raw, X, norm);
+}
+
+void launch_ws_capture(hipStream_t s, const float* raw, const float* lambda, uint32_t stride, uint32_t offset,
+                       float* ds_x, float* ds_t, uint64_t base, uint32_t rows)
+{
+    if (!rows) return;
+    k_ws_capture<<<grid_for(rows, 256), 256, 0, s>>>(raw, reinterpret_cast<const float2*>(lambda), stride, offset,
+                                                     ds_x, ds_t, base, rows);
 }
 
 }  // namespace sbs

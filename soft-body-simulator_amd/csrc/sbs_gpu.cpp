@@ -17,6 +17,8 @@
 
 #include "abi_util.h"
 #include "kernels.h"
+#include "mlp.h"
+#include "mlp_handle.h"
 #include "sbs_gpu.h"
 
 namespace sbs {
@@ -26,46 +28,6 @@ thread_local std::string g_last_error;
 }
 
 void set_last_error(std::string const& msg) { g_last_error = msg; }
-
-template <class T>
-struct dev_buf {
-    T* p = nullptr;
-    std::size_t n = 0;
-    dev_buf() = default;
-    dev_buf(dev_buf const&) = delete;
-    dev_buf& operator=(dev_buf const&) = delete;
-    dev_buf(dev_buf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
-    dev_buf& operator=(dev_buf&& o) noexcept
-    {
-        if (this != &o) {
-            release();
-            p = o.p;
-            n = o.n;
-            o.p = nullptr;
-            o.n = 0;
-        }
-        return *this;
-    }
-    ~dev_buf() { release(); }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    void alloc(std::size_t count)
-    {
-        release();
-        if (count == 0) return;
-        hip_check(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)), "hipMalloc");
-        n = count;
-    }
-    void upload(T const* h, std::size_t count, hipStream_t s)
-    {
-        alloc(count);
-        if (count) hip_check(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s), "upload");
-    }
-};
 
 struct body_t {
     uint32_t offset = 0, n = 0;
@@ -94,6 +56,12 @@ struct group_t {
     dev_buf<uint2> didx;
     dev_buf<float> rest;
     dev_buf<float> lambda;
+    // multiplier warm start (neo-Hookean groups)
+    dev_buf<float2> ws_hist;     // the multiplier pairs one substep before the stored ones
+    dev_buf<float> ws_raw;       // [count][12] features at the substep start (capture)
+    dev_buf<mlp::bf16> ws_X;     // [count][16] network inputs (learned warm start)
+    dev_buf<float> snap_lambda;  // snapshot copies
+    dev_buf<float2> snap_hist;
     bool clustered() const { return !cluster_off.empty(); }
     uint32_t body_of(uint32_t c) const { return cbody.empty() ? body : cbody[c]; }
     double young_of(uint32_t c) const { return cyoung.empty() ? young : cyoung[c]; }
@@ -125,6 +93,7 @@ struct sbs_gpu {
     uint32_t V = 0;
 
     dev_buf<pos4> xi, xn;
+    dev_buf<pos4> xt;  // predicted positions of the substep (warm-start pass)
     dev_buf<float4> vel;
     dev_buf<float> mass;
 
@@ -152,6 +121,18 @@ struct sbs_gpu {
     bool use_graphs = false;
     hipGraphExec_t graph_exec = nullptr;
     std::array<double, 16> graph_key{};
+
+    int ws_mode = SBS_WARM_START_NONE;
+    double ws_scale = 1.0;
+    sbs_mlp* ws_net = nullptr;
+    sbs_mlp* cap_net = nullptr;
+    uint32_t cap_stride = 0;
+    dev_buf<pos4> snap_xn;
+    dev_buf<float4> snap_vel;
+    bool have_snapshot = false;
+    double fb_limit = 0.0;
+    uint32_t fb_iterations = 0;  // 0: the step's own iteration count
+    uint64_t fb_count = 0;
 
     ~sbs_gpu()
     {
@@ -215,8 +196,14 @@ struct sbs_gpu {
     void commit(uint32_t recolor_passes);
     void color_group(group_t& g, uint32_t recolor_passes);
     void upload_group(group_t& g);
-    void enqueue_frame(double dt, uint32_t iterations, uint32_t substeps);
+    void enqueue_frame(double dt, uint32_t iterations, uint32_t substeps, int warm);
+    void run_frame(double dt, uint32_t iterations, uint32_t substeps, int warm);
     void detect();
+    void launch_group_colors(group_t const& g, double dt, int mode, const pos4* xref);
+    void ensure_warm_buffers(bool raw, bool X);
+    void snapshot();
+    void restore();
+    double residual_norm(double dt);
 };
 
 namespace {
@@ -470,6 +457,8 @@ void sbs_gpu::upload_group(group_t& g)
         g.m2.upload(m2.data(), n, stream);
         g.lambda.alloc(std::size_t(n) * (g.kind == 1 ? 2 : 1));
     }
+    // multipliers start at zero (the "previous" multipliers of a warm-started first substep)
+    if (g.lambda.n) hip_check(hipMemsetAsync(g.lambda.p, 0, g.lambda.n * sizeof(float), stream), "clear multipliers");
     hip_check(hipStreamSynchronize(stream), "upload group");
 }
 
@@ -532,7 +521,43 @@ void sbs_gpu::detect()
     }
 }
 
-void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t substeps)
+// One launch per color of a group (mode: kSolveFirst / kSolveNext, or for neo-Hookean groups a
+// warm-start pass). xref: the substep start positions (damping), or for a warm-start pass the
+// copy of the predicted positions the gradients are taken at.
+void sbs_gpu::launch_group_colors(group_t const& g, double dt, int mode, const pos4* xref)
+{
+    double const dt2 = dt * dt;
+    double const at = g.alpha / dt2;
+    double const gamma = at * (g.beta * dt2) / dt;
+    int const first = mode == kSolveFirst;
+    // warm-start passes carry the multipliers' scale in place of the damping term
+    float const bdt = mode >= kApplyWarmStart ? float(ws_scale) : float(g.beta * dt);
+    for (uint32_t col = 0; col < g.ncolors; ++col) {
+        uint32_t const b = g.color_offsets[col], n = g.color_offsets[col + 1] - b;
+        uint32_t const cb = g.cluster_color_offsets[col];
+        uint32_t const cn = g.cluster_color_offsets[col + 1] - cb;
+        if (g.hex && g.kind == 0)
+            launch_green_hex(stream, g.tet(), g.clusters(), xi.p, xref, cb, cn, b, float(at), float(gamma), first,
+                             cluster_block);
+        else if (g.hex && g.kind == 1)
+            launch_neohookean_hex(stream, g.tet(), g.clusters(), xi.p, xref, cb, cn, b, float(1.0 / dt2),
+                                  bdt, mode, cluster_block);
+        else if (g.clustered() && g.kind == 0)
+            launch_green_clusters(stream, g.tet(), g.clusters(), xi.p, xref, cb, cn, float(at), float(gamma), first,
+                                  cluster_block);
+        else if (g.clustered() && g.kind == 1)
+            launch_neohookean_clusters(stream, g.tet(), g.clusters(), xi.p, xref, cb, cn, float(1.0 / dt2),
+                                       bdt, mode, cluster_block);
+        else if (g.kind == 0)
+            launch_green(stream, g.tet(), xi.p, xref, b, n, float(at), float(gamma), first, block);
+        else if (g.kind == 1)
+            launch_neohookean(stream, g.tet(), xi.p, xref, b, n, float(1.0 / dt2), bdt, mode, block);
+        else
+            launch_distance(stream, g.dist(), xi.p, xref, b, n, float(at), float(gamma), first, block);
+    }
+}
+
+void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t substeps, int warm)
 {
     double const dt = dt_frame / double(substeps);
     double const dt2 = dt * dt;
@@ -542,51 +567,187 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
     bool const have_contacts = S && slots;
     contact_dev const c = contacts();
     float const c_at = float(params.collision_compliance / dt2);
+    bool const capture = cap_net != nullptr && cap_stride > 0;
     for (uint32_t s = 0; s < substeps; ++s) {
         if (s == 0)
             timed(SBS_KERNEL_PREDICT, V, [&] { launch_predict(stream, xi.p, xn.p, vel.p, V, g0, g1, g2, force, float(dt), block); });
         else
             timed(SBS_KERNEL_PREDICT, V,
                   [&] { launch_commit_predict(stream, xi.p, xn.p, vel.p, V, g0, g1, g2, force, float(dt), block); });
+        // multiplier warm start of the neo-Hookean groups: lambda_init (previous, plus the network's
+        // change, or -C(x~)/alpha~), then x = x~ + M^-1 grad C(x~)^T lambda_init
+        bool warmed = false;
+        if (warm != SBS_WARM_START_NONE)
+            timed(SBS_KERNEL_WARM_START, V, [&] { launch_copy_pos(stream, xi.p, xt.p, V); });
+        for (auto& g : groups) {
+            if (g.kind != 1) continue;
+            bool const learned = warm == SBS_WARM_START_LEARNED;
+            if (capture || learned) {
+                timed(SBS_KERNEL_WARM_START, g.count, [&] {
+                    launch_ws_features(stream, g.tet(), xi.p, g.ws_hist.p, g.count, float(1.0 / dt2),
+                                       capture ? g.ws_raw.p : nullptr, learned ? g.ws_X.p : nullptr,
+                                       learned ? ws_net->norm.p : nullptr, block);
+                    if (learned)
+                        mlp::launch_infer_lambda(stream, ws_net->pack.p, ws_net->bias.p, g.ws_X.p, g.count, 2,
+                                                 ws_net->out_scale_dev(), g.lambda.p, 512);
+                });
+            }
+            if (warm != SBS_WARM_START_NONE) {
+                int const mode = warm == SBS_WARM_START_EXPLICIT ? kApplyExplicit : kApplyWarmStart;
+                timed(SBS_KERNEL_WARM_START, g.count, [&] { launch_group_colors(g, dt, mode, xt.p); }, g.ncolors);
+                warmed = true;
+            }
+        }
         for (uint32_t it = 0; it < iterations; ++it) {
             int const first = it == 0;
             if (have_contacts)
                 timed(SBS_KERNEL_CONTACT, S, [&] { launch_contact(stream, c, xi.p, c_at, first, block); });
             for (auto const& g : groups) {
-                double const at = g.alpha / dt2;
-                double const gamma = at * (g.beta * dt2) / dt;
                 int const kind = g.kind == 0 ? SBS_KERNEL_GREEN : (g.kind == 1 ? SBS_KERNEL_NEOHOOKEAN : SBS_KERNEL_DISTANCE);
+                int const mode = (first && !(warmed && g.kind == 1)) ? kSolveFirst : kSolveNext;
                 // the colors of a group run back to back; timing brackets the whole run
-                timed(kind, g.count, [&] {
-                    for (uint32_t col = 0; col < g.ncolors; ++col) {
-                        uint32_t const b = g.color_offsets[col], n = g.color_offsets[col + 1] - b;
-                        uint32_t const cb = g.cluster_color_offsets[col];
-                        uint32_t const cn = g.cluster_color_offsets[col + 1] - cb;
-                        if (g.hex && g.kind == 0)
-                            launch_green_hex(stream, g.tet(), g.clusters(), xi.p, xn.p, cb, cn, b, float(at),
-                                             float(gamma), first, cluster_block);
-                        else if (g.hex && g.kind == 1)
-                            launch_neohookean_hex(stream, g.tet(), g.clusters(), xi.p, xn.p, cb, cn, b,
-                                                  float(1.0 / dt2), float(g.beta * dt), first, cluster_block);
-                        else if (g.clustered() && g.kind == 0)
-                            launch_green_clusters(stream, g.tet(), g.clusters(), xi.p, xn.p, cb, cn, float(at),
-                                                  float(gamma), first, cluster_block);
-                        else if (g.clustered() && g.kind == 1)
-                            launch_neohookean_clusters(stream, g.tet(), g.clusters(), xi.p, xn.p, cb, cn,
-                                                       float(1.0 / dt2), float(g.beta * dt), first, cluster_block);
-                        else if (g.kind == 0)
-                            launch_green(stream, g.tet(), xi.p, xn.p, b, n, float(at), float(gamma), first, block);
-                        else if (g.kind == 1)
-                            launch_neohookean(stream, g.tet(), xi.p, xn.p, b, n, float(1.0 / dt2), float(g.beta * dt),
-                                              first, block);
-                        else
-                            launch_distance(stream, g.dist(), xi.p, xn.p, b, n, float(at), float(gamma), first, block);
-                    }
-                }, g.ncolors);
+                timed(kind, g.count, [&] { launch_group_colors(g, dt, mode, xn.p); }, g.ncolors);
+            }
+        }
+        if (capture) {
+            for (auto& g : groups) {
+                if (g.kind != 1) continue;
+                uint64_t const want = (uint64_t(g.count) + cap_stride - 1) / cap_stride;
+                uint64_t const room = cap_net->ds_cap - cap_net->ds_size;
+                uint32_t const rows = uint32_t(std::min(want, room));
+                launch_ws_capture(stream, g.ws_raw.p, g.lambda.p, cap_stride, 0, cap_net->ds_x.p, cap_net->ds_t.p,
+                                  cap_net->ds_size, rows);
+                cap_net->ds_size += rows;
             }
         }
     }
     timed(SBS_KERNEL_COMMIT, V, [&] { launch_commit(stream, xi.p, xn.p, vel.p, V, float(dt), block); });
+}
+
+void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
+{
+    if (!xt.p) xt.alloc(V);
+    for (auto& g : groups) {
+        if (g.kind != 1) continue;
+        if (!g.ws_hist.p && g.count) {
+            g.ws_hist.alloc(g.count);
+            hip_check(hipMemsetAsync(g.ws_hist.p, 0, g.count * sizeof(float2), stream), "clear history");
+        }
+        if (raw && !g.ws_raw.p) g.ws_raw.alloc(std::size_t(g.count) * mlp::kMaxInputs);
+        if (X && !g.ws_X.p) g.ws_X.alloc(std::size_t(g.count) * mlp::kIn);
+    }
+}
+
+void sbs_gpu::snapshot()
+{
+    if (!snap_xn.p) {
+        snap_xn.alloc(V);
+        snap_vel.alloc(V);
+    }
+    hip_check(hipMemcpyAsync(snap_xn.p, xn.p, V * sizeof(pos4), hipMemcpyDeviceToDevice, stream), "snapshot");
+    hip_check(hipMemcpyAsync(snap_vel.p, vel.p, V * sizeof(float4), hipMemcpyDeviceToDevice, stream), "snapshot");
+    for (auto& g : groups) {
+        if (!g.lambda.n) continue;
+        if (!g.snap_lambda.p) g.snap_lambda.alloc(g.lambda.n);
+        hip_check(hipMemcpyAsync(g.snap_lambda.p, g.lambda.p, g.lambda.n * sizeof(float), hipMemcpyDeviceToDevice, stream),
+                  "snapshot");
+        if (g.ws_hist.p) {
+            if (!g.snap_hist.p) g.snap_hist.alloc(g.ws_hist.n);
+            hip_check(hipMemcpyAsync(g.snap_hist.p, g.ws_hist.p, g.ws_hist.n * sizeof(float2), hipMemcpyDeviceToDevice,
+                                     stream), "snapshot");
+        }
+    }
+    have_snapshot = true;
+}
+
+void sbs_gpu::restore()
+{
+    require(have_snapshot, "no snapshot has been taken", SBS_ERR_STATE);
+    hip_check(hipMemcpyAsync(xn.p, snap_xn.p, V * sizeof(pos4), hipMemcpyDeviceToDevice, stream), "restore");
+    hip_check(hipMemcpyAsync(xi.p, snap_xn.p, V * sizeof(pos4), hipMemcpyDeviceToDevice, stream), "restore");
+    hip_check(hipMemcpyAsync(vel.p, snap_vel.p, V * sizeof(float4), hipMemcpyDeviceToDevice, stream), "restore");
+    for (auto& g : groups) {
+        if (g.snap_lambda.p)
+            hip_check(hipMemcpyAsync(g.lambda.p, g.snap_lambda.p, g.lambda.n * sizeof(float), hipMemcpyDeviceToDevice,
+                                     stream), "restore");
+        if (g.ws_hist.p && g.snap_hist.p)
+            hip_check(hipMemcpyAsync(g.ws_hist.p, g.snap_hist.p, g.ws_hist.n * sizeof(float2), hipMemcpyDeviceToDevice,
+                                     stream), "restore");
+    }
+}
+
+// sqrt of the summed squared FP64 residuals of every neo-Hookean group (state after the last step).
+double sbs_gpu::residual_norm(double dt)
+{
+    double const dt2 = dt * dt;
+    std::size_t total = 0;
+    for (auto const& g : groups)
+        if (g.kind == 1)
+            for (uint32_t col = 0; col < g.ncolors; ++col)
+                total += (g.color_offsets[col + 1] - g.color_offsets[col] + 255) / 256;
+    if (total == 0) return 0.0;
+    if (partial.n < total) partial.alloc(std::max<std::size_t>(total, 1024));
+    std::size_t at = 0;
+    for (auto const& g : groups) {
+        if (g.kind != 1) continue;
+        for (uint32_t col = 0; col < g.ncolors; ++col) {
+            uint32_t const b = g.color_offsets[col], n = g.color_offsets[col + 1] - b;
+            launch_neohookean_residual(stream, g.tet(), xn.p, b, n, float(1.0 / dt2), partial.p + at, 256);
+            at += (n + 255) / 256;
+        }
+    }
+    std::vector<double> h(total);
+    hip_check(hipMemcpyAsync(h.data(), partial.p, total * sizeof(double), hipMemcpyDeviceToHost, stream), "residual");
+    hip_check(hipStreamSynchronize(stream), "residual");
+    double acc = 0.0;
+    for (double x : h) acc += x;
+    return std::sqrt(acc);
+}
+
+void sbs_gpu::run_frame(double dt, uint32_t iterations, uint32_t substeps, int warm)
+{
+    bool const capture = cap_net != nullptr && cap_stride > 0;
+    if (use_graphs && !timing && !capture) {
+        std::array<double, 16> key{};
+        key[0] = dt;
+        key[1] = iterations;
+        key[2] = substeps;
+        key[3] = params.gravity[0];
+        key[4] = params.gravity[1];
+        key[5] = params.gravity[2];
+        key[6] = params.gravity_is_force;
+        key[7] = params.collision_compliance;
+        key[8] = block;
+        key[9] = cluster_block;
+        key[10] = warm;
+        key[11] = double(reinterpret_cast<uintptr_t>(warm == SBS_WARM_START_LEARNED ? ws_net : nullptr));
+        key[12] = ws_scale;
+        if (!graph_exec || key != graph_key) {
+            if (graph_exec) {
+                hip_check(hipGraphExecDestroy(graph_exec), "hipGraphExecDestroy");
+                graph_exec = nullptr;
+            }
+            hipGraph_t graph;
+            hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            try {
+                enqueue_frame(dt, iterations, substeps, warm);
+            } catch (...) {
+                hipGraph_t g2 = nullptr;
+                (void)hipStreamEndCapture(stream, &g2);
+                if (g2) (void)hipGraphDestroy(g2);
+                throw;
+            }
+            hip_check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
+            hipError_t const e = hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            hip_check(e, "hipGraphInstantiate");
+            graph_key = key;
+        }
+        hip_check(hipGraphLaunch(graph_exec, stream), "hipGraphLaunch");
+    } else {
+        enqueue_frame(dt, iterations, substeps, warm);
+        hip_check(hipGetLastError(), "kernel launch");
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -888,44 +1049,15 @@ int sbs_gpu_step(sbs_gpu* sim, double dt, uint32_t iterations, uint32_t substeps
         require(dt > 0.0 && std::isfinite(dt), "dt must be positive");
         require(substeps >= 1, "substeps must be >= 1");
         sim->set_current();
-        if (sim->use_graphs && !sim->timing) {
-            std::array<double, 16> key{};
-            key[0] = dt;
-            key[1] = iterations;
-            key[2] = substeps;
-            key[3] = sim->params.gravity[0];
-            key[4] = sim->params.gravity[1];
-            key[5] = sim->params.gravity[2];
-            key[6] = sim->params.gravity_is_force;
-            key[7] = sim->params.collision_compliance;
-            key[8] = sim->block;
-            key[9] = sim->cluster_block;
-            if (!sim->graph_exec || key != sim->graph_key) {
-                if (sim->graph_exec) {
-                    hip_check(hipGraphExecDestroy(sim->graph_exec), "hipGraphExecDestroy");
-                    sim->graph_exec = nullptr;
-                }
-                hipGraph_t graph;
-                hip_check(hipStreamBeginCapture(sim->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-                try {
-                    sim->enqueue_frame(dt, iterations, substeps);
-                } catch (...) {
-                    hipGraph_t g2;
-                    (void)hipStreamEndCapture(sim->stream, &g2);
-                    if (g2) (void)hipGraphDestroy(g2);
-                    throw;
-                }
-                hip_check(hipStreamEndCapture(sim->stream, &graph), "hipStreamEndCapture");
-                hipError_t const e = hipGraphInstantiate(&sim->graph_exec, graph, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(graph);
-                hip_check(e, "hipGraphInstantiate");
-                sim->graph_key = key;
-            }
-            hip_check(hipGraphLaunch(sim->graph_exec, sim->stream), "hipGraphLaunch");
-        } else {
-            sim->enqueue_frame(dt, iterations, substeps);
-            hip_check(hipGetLastError(), "kernel launch");
+        bool const check = sim->fb_limit > 0.0 && sim->ws_mode != SBS_WARM_START_NONE;
+        if (check) sim->snapshot();
+        sim->run_frame(dt, iterations, substeps, sim->ws_mode);
+        if (check && sim->residual_norm(dt / double(substeps)) > sim->fb_limit) {
+            sim->restore();
+            sim->run_frame(dt, sim->fb_iterations ? sim->fb_iterations : iterations, substeps, SBS_WARM_START_NONE);
+            ++sim->fb_count;
         }
+        if (sim->cap_net && sim->cap_stride) hip_check(hipStreamSynchronize(sim->stream), "capture");
     });
 }
 
@@ -1162,6 +1294,96 @@ int sbs_gpu_reset_stats(sbs_gpu* sim)
             sim->st_ms[k] = 0;
             sim->st_units[k] = 0;
         }
+    });
+}
+
+int sbs_gpu_set_warm_start(sbs_gpu* sim, int mode, sbs_mlp* net)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
+        require(mode >= SBS_WARM_START_NONE && mode <= SBS_WARM_START_EXPLICIT, "unknown warm start mode");
+        if (mode == SBS_WARM_START_LEARNED) {
+            require(net != nullptr, "the learned warm start needs a network");
+            require(net->nin == uint32_t(mlp::kMaxInputs) && net->nout == 2,
+                    "the warm-start network must have 12 inputs and 2 outputs");
+            require(net->device == sim->device, "network and simulation are on different devices");
+        }
+        sim->set_current();
+        sim->ensure_warm_buffers(false, mode == SBS_WARM_START_LEARNED);
+        sim->ws_mode = mode;
+        sim->ws_net = mode == SBS_WARM_START_LEARNED ? net : nullptr;
+        hip_check(hipStreamSynchronize(sim->stream), "warm start");
+    });
+}
+
+int sbs_gpu_capture(sbs_gpu* sim, sbs_mlp* net, uint32_t stride)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
+        sim->set_current();
+        if (stride == 0) {
+            hip_check(hipStreamSynchronize(sim->stream), "capture");
+            sim->cap_net = nullptr;
+            sim->cap_stride = 0;
+            return;
+        }
+        require(net != nullptr, "net is null");
+        require(net->nin == uint32_t(mlp::kMaxInputs) && net->nout == 2,
+                "the warm-start network must have 12 inputs and 2 outputs");
+        require(net->device == sim->device, "network and simulation are on different devices");
+        sim->ensure_warm_buffers(true, false);
+        hip_check(hipStreamSynchronize(net->stream), "capture");
+        sim->cap_net = net;
+        sim->cap_stride = stride;
+    });
+}
+
+int sbs_gpu_set_warm_start_scale(sbs_gpu* sim, double scale)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(std::isfinite(scale) && scale >= 0.0, "scale must be finite and >= 0");
+        sim->ws_scale = scale;
+    });
+}
+
+int sbs_gpu_snapshot(sbs_gpu* sim)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
+        sim->set_current();
+        sim->snapshot();
+    });
+}
+
+int sbs_gpu_restore(sbs_gpu* sim)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
+        sim->set_current();
+        sim->restore();
+    });
+}
+
+int sbs_gpu_set_fallback(sbs_gpu* sim, double limit, uint32_t iterations)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(limit >= 0.0 && std::isfinite(limit), "limit must be finite and >= 0");
+        sim->fb_limit = limit;
+        sim->fb_iterations = iterations;
+    });
+}
+
+int sbs_gpu_fallback_count(sbs_gpu* sim, uint64_t* fallbacks)
+{
+    return guarded([&] {
+        require(sim != nullptr && fallbacks != nullptr, "null argument");
+        *fallbacks = sim->fb_count;
     });
 }
 

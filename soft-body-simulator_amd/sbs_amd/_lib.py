@@ -15,8 +15,10 @@ LIB_PATH = os.path.join(PKG_DIR, "_build", "libsbs_gpu.so")
 HEADER_PATH = os.path.join(REPO_DIR, "include", "sbs_gpu.h")
 
 SBS_OK = 0
-KERNEL_PREDICT, KERNEL_CONTACT, KERNEL_GREEN, KERNEL_NEOHOOKEAN, KERNEL_DISTANCE, KERNEL_COMMIT, KERNEL_DETECT = range(7)
-KERNEL_NAMES = ["predict", "contact", "green", "neohookean", "distance", "commit", "detect"]
+(KERNEL_PREDICT, KERNEL_CONTACT, KERNEL_GREEN, KERNEL_NEOHOOKEAN, KERNEL_DISTANCE, KERNEL_COMMIT, KERNEL_DETECT,
+ KERNEL_WARM_START) = range(8)
+KERNEL_NAMES = ["predict", "contact", "green", "neohookean", "distance", "commit", "detect", "warm_start"]
+WARM_START = {"none": 0, "previous": 1, "learned": 2, "explicit": 3}
 
 _d = ctypes.c_double
 _i = ctypes.c_int
@@ -25,6 +27,9 @@ _p = ctypes.c_void_p
 _pd = ctypes.POINTER(ctypes.c_double)
 _pu = ctypes.POINTER(ctypes.c_uint32)
 _pu64 = ctypes.POINTER(ctypes.c_uint64)
+_u64 = ctypes.c_uint64
+_f = ctypes.c_float
+_pf = ctypes.POINTER(ctypes.c_float)
 
 
 class XpbdParams(ctypes.Structure):
@@ -73,6 +78,29 @@ _SIGS = {
     "sbs_gpu_set_timing": [_p, _i],
     "sbs_gpu_kernel_stats": [_p, _i, _pu64, _pd, _pu64],
     "sbs_gpu_reset_stats": [_p],
+    "sbs_mlp_create": [_i, _u32, _u32, _u64, ctypes.POINTER(_p)],
+    "sbs_mlp_destroy": [_p],
+    "sbs_mlp_num_params": [_pu],
+    "sbs_mlp_get_params": [_p, _pf],
+    "sbs_mlp_set_params": [_p, _pf],
+    "sbs_mlp_set_normalization": [_p, _pf, _pf, _pf],
+    "sbs_mlp_get_normalization": [_p, _pf, _pf, _pf],
+    "sbs_mlp_forward": [_p, _pf, _u32, _pf],
+    "sbs_mlp_loss_grad": [_p, _pf, _pf, _u32, _pd, _pf],
+    "sbs_mlp_dataset_reserve": [_p, _u64],
+    "sbs_mlp_dataset_append": [_p, _pf, _pf, _u64],
+    "sbs_mlp_dataset_size": [_p, _pu64],
+    "sbs_mlp_dataset_clear": [_p],
+    "sbs_mlp_dataset_read": [_p, _u64, _u64, _pf, _pf],
+    "sbs_mlp_fit_normalization": [_p],
+    "sbs_mlp_train": [_p, _u32, _u32, _f, _u64, _pd],
+    "sbs_gpu_set_warm_start": [_p, _i, _p],
+    "sbs_gpu_capture": [_p, _p, _u32],
+    "sbs_gpu_set_warm_start_scale": [_p, _d],
+    "sbs_gpu_snapshot": [_p],
+    "sbs_gpu_restore": [_p],
+    "sbs_gpu_set_fallback": [_p, _d, _u32],
+    "sbs_gpu_fallback_count": [_p, _pu64],
     "sbs_mesh_bar5": [_u32, _u32, _u32, _pd, _pu],
     "sbs_mesh_box6": [_u32, _u32, _u32, _d, _pd, _pd, _pu],
     "sbs_mesh_boundary": [_u32, _pu, _u32, _pu, _pu, _pu, _pu],
@@ -122,6 +150,14 @@ def _dp(a):
 
 def _up(a):
     return None if a is None else a.ctypes.data_as(_pu)
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(_pf)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
 
 
 def f64(a):
@@ -177,6 +213,97 @@ def lumped_mass(pos, tets, density):
     m = np.zeros(len(pos) // 3)
     check(lib().sbs_mesh_lumped_mass(len(m), _dp(pos), _up(tets), len(tets) // 4, density, _dp(m)))
     return m
+
+
+# ------------------------------------------------------------------ warm-start network
+
+class Mlp:
+    """The warm-start network behind the C ABI (one sbs_mlp handle)."""
+
+    def __init__(self, num_inputs=12, num_outputs=2, seed=0, device=0):
+        self._L = lib()
+        h = _p()
+        check(self._L.sbs_mlp_create(device, num_inputs, num_outputs, seed, ctypes.byref(h)))
+        self.h = h
+        self.nin, self.nout = num_inputs, num_outputs
+        n = _u32()
+        check(self._L.sbs_mlp_num_params(ctypes.byref(n)))
+        self.num_params = n.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(self._L.sbs_mlp_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(self.num_params, dtype=np.float32)
+        check(self._L.sbs_mlp_get_params(self.h, _fp(p)))
+        return p
+
+    def set_params(self, p):
+        p = f32(p)
+        assert len(p) == self.num_params
+        check(self._L.sbs_mlp_set_params(self.h, _fp(p)))
+
+    def set_normalization(self, mean, inv_std, out_scale):
+        check(self._L.sbs_mlp_set_normalization(self.h, _fp(f32(mean)), _fp(f32(inv_std)), _fp(f32(out_scale))))
+
+    def get_normalization(self):
+        m, s, o = np.zeros(self.nin, np.float32), np.zeros(self.nin, np.float32), np.zeros(self.nout, np.float32)
+        check(self._L.sbs_mlp_get_normalization(self.h, _fp(m), _fp(s), _fp(o)))
+        return m, s, o
+
+    def forward(self, x):
+        x = f32(x).reshape(-1, self.nin)
+        y = np.zeros((len(x), self.nout), dtype=np.float32)
+        check(self._L.sbs_mlp_forward(self.h, _fp(x), len(x), _fp(y)))
+        return y
+
+    def loss_grad(self, x, t):
+        x = f32(x).reshape(-1, self.nin)
+        t = f32(t).reshape(-1, self.nout)
+        loss = ctypes.c_double()
+        g = np.zeros(self.num_params, dtype=np.float32)
+        check(self._L.sbs_mlp_loss_grad(self.h, _fp(x), _fp(t), len(x), ctypes.byref(loss), _fp(g)))
+        return loss.value, g
+
+    def reserve(self, rows):
+        check(self._L.sbs_mlp_dataset_reserve(self.h, rows))
+
+    def append(self, x, t):
+        x = f32(x).reshape(-1, self.nin)
+        t = f32(t).reshape(-1, self.nout)
+        check(self._L.sbs_mlp_dataset_append(self.h, _fp(x), _fp(t), len(x)))
+
+    def dataset_size(self):
+        n = ctypes.c_uint64()
+        check(self._L.sbs_mlp_dataset_size(self.h, ctypes.byref(n)))
+        return n.value
+
+    def clear(self):
+        check(self._L.sbs_mlp_dataset_clear(self.h))
+
+    def dataset(self, first=0, rows=None):
+        """Rows of the device-resident training set: features [rows][num_inputs], targets [rows][num_outputs]."""
+        rows = self.dataset_size() - first if rows is None else rows
+        x = np.zeros((rows, self.nin), dtype=np.float32)
+        t = np.zeros((rows, self.nout), dtype=np.float32)
+        check(self._L.sbs_mlp_dataset_read(self.h, first, rows, _fp(x), _fp(t)))
+        return x, t
+
+    def fit_normalization(self):
+        check(self._L.sbs_mlp_fit_normalization(self.h))
+
+    def train(self, steps, batch, lr, seed=0):
+        loss = ctypes.c_double()
+        check(self._L.sbs_mlp_train(self.h, steps, batch, lr, seed, ctypes.byref(loss)))
+        return loss.value
 
 
 # ------------------------------------------------------------------ simulation handle
@@ -327,6 +454,32 @@ class Simulation:
         out = np.zeros(6)
         check(self._L.sbs_gpu_energy(self.h, _dp(out)))
         return out
+
+    def set_warm_start(self, mode, net=None):
+        """mode: "none", "previous", "explicit" or "learned" (needs net, an Mlp(12, 2))."""
+        check(self._L.sbs_gpu_set_warm_start(self.h, WARM_START[mode], None if net is None else net.h))
+
+    def set_warm_start_scale(self, scale):
+        check(self._L.sbs_gpu_set_warm_start_scale(self.h, scale))
+
+    def capture(self, net, stride):
+        check(self._L.sbs_gpu_capture(self.h, None if net is None else net.h, stride))
+
+    def snapshot(self):
+        check(self._L.sbs_gpu_snapshot(self.h))
+
+    def restore(self):
+        check(self._L.sbs_gpu_restore(self.h))
+
+    def set_fallback(self, limit, iterations=0):
+        """Residual limit of a warm-started frame, and the plain solver's iterations per substep
+        when it is repeated (0: the step's own count)."""
+        check(self._L.sbs_gpu_set_fallback(self.h, limit, iterations))
+
+    def fallback_count(self):
+        n = ctypes.c_uint64()
+        check(self._L.sbs_gpu_fallback_count(self.h, ctypes.byref(n)))
+        return n.value
 
     def set_timing(self, enable):
         check(self._L.sbs_gpu_set_timing(self.h, int(enable)))

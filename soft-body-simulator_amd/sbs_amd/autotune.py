@@ -1,0 +1,114 @@
+"""Host-side tuning of a device simulation: launch configuration, and the learned multiplier warm
+start (training data capture, training, and the choice of warm-start scale and iteration count
+that reaches the plain solver's residual).
+
+Every trial starts from the same device snapshot (sbs_gpu_snapshot / sbs_gpu_restore), so the
+candidates are compared on identical states. Residuals are the FP64 XPBD residual norms
+sqrt(sum h^2), h = C + alpha~ lambda, over the neo-Hookean groups after each frame's last substep.
+"""
+import math
+import time
+
+import numpy as np
+
+from ._lib import Mlp
+
+
+def frame_residual(sim, scene):
+    dt = scene.dt / scene.substeps
+    total = 0.0
+    for gi, g in enumerate(scene.groups):
+        if g.kind == "neohookean":
+            total += float(np.sum(sim.residual(gi, dt)))
+    return math.sqrt(total)
+
+
+def mean_residual(sim, scene, frames, iterations):
+    """Mean frame residual over `frames` frames from the current state (inf if any is not finite)."""
+    r = []
+    for _ in range(frames):
+        sim.step(scene.dt, iterations, scene.substeps)
+        r.append(frame_residual(sim, scene))
+    r = np.asarray(r)
+    return float(r.mean()) if np.all(np.isfinite(r)) else math.inf
+
+
+def time_frames(sim, scene, frames, iterations):
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        sim.step(scene.dt, iterations, scene.substeps)
+    sim.synchronize()
+    return time.perf_counter() - t0
+
+
+def tune_launch(sim, scene, frames=3, blocks=(128, 256), cluster_blocks=(64, 128)):
+    """Fastest (block, cluster_block) over a few graph-replayed frames; leaves it set."""
+    best = None
+    sim.snapshot()
+    for b in blocks:
+        for cb in cluster_blocks:
+            sim.restore()
+            sim.set_launch_config(b, cb)
+            sim.step(scene.dt, scene.iterations, scene.substeps)  # capture
+            t = time_frames(sim, scene, frames, scene.iterations)
+            if best is None or t < best[0]:
+                best = (t, b, cb)
+    sim.restore()
+    sim.set_launch_config(best[1], best[2])
+    return {"block": best[1], "cluster_block": best[2], "ms_per_frame": 1e3 * best[0] / frames}
+
+
+def train_warm_start(sim, scene, frames=6, iterations=24, scale=0.5, max_rows=4_000_000, steps=1500, batch=16384,
+                     lr=1e-3, seed=0, log=print):
+    """Captures training rows from a well-converged trajectory (previous-multiplier warm start at
+    `scale`, `iterations` per substep) starting at the current state, trains a 12-input, 2-output
+    network on the device and restores the state. Returns the network."""
+    rows_per_substep = sum(len(g.conn) for g in scene.groups if g.kind == "neohookean")
+    stride = max(1, -(-rows_per_substep * scene.substeps * frames // max_rows))
+    net = Mlp(12, 2, seed=seed)
+    net.reserve(max_rows)
+    sim.snapshot()
+    sim.set_warm_start("previous")
+    sim.set_warm_start_scale(scale)
+    sim.capture(net, stride)
+    for _ in range(frames):
+        sim.step(scene.dt, iterations, scene.substeps)
+    sim.capture(None, 0)
+    sim.set_warm_start("none")
+    sim.restore()
+    n = net.dataset_size()
+    net.fit_normalization()
+    t0 = time.perf_counter()
+    loss = net.train(steps, batch, lr, seed=seed + 1)
+    log(f"warm-start network: {n} rows (stride {stride}), {steps} Adam steps of {batch} in "
+        f"{time.perf_counter() - t0:.1f}s, last loss {loss:.4f} (normalized MSE)")
+    return net
+
+
+def tune_warm_start(sim, scene, net, target, frames=8, scales=(0.5, 0.75, 1.0), iterations=(2, 3, 4, 5, 6, 8),
+                    modes=("learned", "previous"), log=print):
+    """For each mode, the cheapest (iterations, scale) whose mean frame residual from the current
+    state is <= target. Returns {mode: {"iterations", "scale", "residual"} or None} and leaves the
+    simulation at the state it started from, warm start off."""
+    sim.snapshot()
+    found = {}
+    trials = []
+    for mode in modes:
+        best = None
+        for k in iterations:
+            for s in scales:
+                sim.restore()
+                sim.set_warm_start(mode, net if mode == "learned" else None)
+                sim.set_warm_start_scale(s)
+                r = mean_residual(sim, scene, frames, k)
+                trials.append({"mode": mode, "iterations": k, "scale": s, "residual": r})
+                log(f"  {mode} x{s} {k} iterations: mean residual {r:.4g} (target {target:.4g})")
+                if r <= target and (best is None or r < best["residual"]):
+                    best = {"iterations": k, "scale": s, "residual": r}
+            if best is not None:
+                break  # the smallest iteration count that reaches the target
+        found[mode] = best
+    sim.set_warm_start("none")
+    sim.restore()
+    return found, trials

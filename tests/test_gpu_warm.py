@@ -1,0 +1,233 @@
+"""GPU checks of the learned warm start: the network kernels against the NumPy restatement
+(oracle/mlp_oracle.py), training on the device, the multiplier warm-start passes against the FP64
+oracle, the captured training rows, and the plain-solver fallback."""
+import numpy as np
+import pytest
+
+import mlp_oracle as M
+import sbs_amd as S
+from helpers import build_oracle
+from sbs_amd import scenes
+from test_oracle_warm import nh_energy, random_params
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4  # max vertex error / bounding-box diagonal
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("n", [1, 33, 1000])
+def test_fused_inference_matches_oracle(n):
+    say(f"mlp forward n={n}")
+    rng = np.random.default_rng(n)
+    net = S.Mlp(12, 3, seed=5)
+    p = random_params(rng).astype(np.float32)
+    net.set_params(p)
+    mean, inv_std, scale = rng.normal(0, 1, 12), np.exp(rng.normal(0, 0.3, 12)), np.array([0.5, 2.0, 1.5])
+    net.set_normalization(mean, inv_std, scale)
+    x = rng.normal(0, 2, (n, 12))
+    y = net.forward(x)
+    yo, _ = M.forward(p, M.normalize(x, mean, inv_std, True), True)
+    yo = yo[:, :3] * scale
+    assert np.all(np.isfinite(y))
+    assert np.abs(y - yo).max() <= 1e-2 * max(1.0, np.abs(yo).max())
+
+
+def test_training_gradient_matches_oracle():
+    say("mlp loss gradient")
+    rng = np.random.default_rng(11)
+    net = S.Mlp(12, 2, seed=1)
+    p = random_params(rng).astype(np.float32)
+    net.set_params(p)
+    net.set_normalization(np.zeros(12), np.ones(12), np.array([1.0, 3.0]))
+    x = rng.normal(0, 1, (517, 12))
+    t = rng.normal(0, 1, (517, 2)) * np.array([1.0, 3.0])
+    loss, g = net.loss_grad(x, t)
+    lo, go = M.loss_grad(p, M.normalize(x, np.zeros(12), np.ones(12), True), t / np.array([1.0, 3.0]), 2, True)
+    assert loss == pytest.approx(lo, rel=1e-3)
+    for (a, b) in ((M.W1, M.B1), (M.B1, M.W2), (M.W2, M.B2), (M.B2, M.W3), (M.W3, M.B3), (M.B3, M.W4), (M.W4, M.B4),
+                   (M.B4, M.NPARAMS)):
+        ref = go[a:b]
+        assert np.linalg.norm(g[a:b] - ref) <= 1e-2 * np.linalg.norm(ref) + 1e-7, (a, b)
+
+
+def test_training_fits_a_smooth_function():
+    say("mlp training")
+    rng = np.random.default_rng(2)
+    x = rng.normal(0, 1, (20000, 12)).astype(np.float32)
+    t = np.stack([np.sin(x[:, 0]) + 0.5 * x[:, 1] * x[:, 2], np.tanh(x[:, 3] - x[:, 4])], 1).astype(np.float32)
+    net = S.Mlp(12, 2, seed=3)
+    net.reserve(len(x))
+    net.append(x, t)
+    assert net.dataset_size() == len(x)
+    net.fit_normalization()
+    mean, inv_std, scale = net.get_normalization()
+    assert np.allclose(mean, x.mean(0), atol=2e-2) and np.allclose(1 / inv_std, x.std(0), rtol=2e-2)
+    l0 = net.loss_grad(x[:2048], t[:2048])[0]
+    net.train(600, 1024, 2e-3, seed=9)
+    l1 = net.loss_grad(x[:2048], t[:2048])[0]
+    say(f"loss {l0:.4f} -> {l1:.4f}")
+    assert l1 < 0.1 * l0
+
+
+def _beam(cluster, iterations=8):
+    return scenes.cantilever(8, 3, 3, 0.1, "neohookean", 5e5, 0.45, 1000.0, num_spheres=2, iterations=iterations,
+                             substeps=2, cluster=cluster, coloring="index" if cluster else "hash")
+
+
+# The warm starts at full scale are not stable on this beam (the stable neo-Hookean pair carries
+# large opposite multipliers, and carrying them into a changed configuration amplifies; the oracle
+# and the device both blow up within a few frames). The previous-multiplier start at scale 0.5 is;
+# the explicit start (an explicit elastic step) blows up within 3 frames even at 0.5, so it is
+# compared over the first frame only.
+@pytest.mark.parametrize("mode,scale,frames", [("previous", 1.0, 2), ("previous", 0.5, 20), ("explicit", 0.5, 1)])
+@pytest.mark.parametrize("cluster", [True, False])
+def test_warm_start_matches_oracle(mode, scale, frames, cluster):
+    sc = _beam(cluster)
+    say(f"warm start {mode} x{scale} cluster={cluster}")
+    dev = scenes.build_device(sc)
+    dev.set_warm_start(mode)
+    dev.set_warm_start_scale(scale)
+    orc = build_oracle(sc)
+    orc.set_warm_start(mode, scale)
+    diag = sc.bbox_diagonal()
+    for f in range(1, frames + 1):
+        dev.step(sc.dt, sc.iterations, sc.substeps)
+        orc.step(sc.dt, sc.iterations, sc.substeps)
+        if f in (1, frames):
+            err = np.abs(dev.read_body(0)[0] - orc.read_body(0)[0]).max() / diag
+            say(f"frame {f} error {err:.3e}")
+            assert err <= TOL
+
+
+def test_device_explicit_pass_is_an_explicit_elastic_step():
+    sc = scenes.cantilever(4, 2, 2, 0.1, "neohookean", 5e5, 0.45, 1000.0, num_spheres=0, iterations=1, substeps=1,
+                           cluster=True, coloring="index")
+    say("explicit pass vs energy gradient")
+    b = sc.bodies[0]
+    tets = np.asarray(sc.groups[0].conn).reshape(-1, 4)
+    dev = scenes.build_device(sc)
+    dev.set_warm_start("explicit")
+    x = b.x0.copy()
+    x[:, 0] *= 1.05
+    dev.write_body(0, x, np.zeros_like(x))
+    dt = 1.0 / 240.0
+    dev.step(dt, 0, 1)
+    xs = dev.read_body(0)[0]
+    w = np.where(b.mass > 0, 1.0 / np.where(b.mass > 0, b.mass, 1.0), 0.0)
+    xt = x + (w > 0)[:, None] * np.array([0.0, -9.81, 0.0]) * dt * dt
+    grad = np.zeros_like(xt)
+    h = 1e-7
+    for i in range(len(xt)):
+        for k in range(3):
+            xp, xm = xt.copy(), xt.copy()
+            xp[i, k] += h
+            xm[i, k] -= h
+            grad[i, k] = (nh_energy(xp, tets, b.x0, 5e5, 0.45) - nh_energy(xm, tets, b.x0, 5e5, 0.45)) / (2 * h)
+    expect = xt - dt * dt * w[:, None] * grad
+    assert np.abs(xs - expect).max() <= 1e-4 * np.abs(xs - xt).max()
+
+
+def test_capture_rows_and_features():
+    sc = _beam(True, iterations=4)
+    say("capture")
+    dev = scenes.build_device(sc)
+    net = S.Mlp(12, 2)
+    T = sc.num_tets
+    net.reserve(10 * T)
+    dev.capture(net, 1)
+    dev.step(sc.dt, sc.iterations, sc.substeps)  # 2 substeps
+    dev.capture(None, 0)
+    assert net.dataset_size() == 2 * T
+    x, tt = net.dataset()
+    x, tt = x.astype(np.float64), tt.astype(np.float64)
+    f0, f1 = x[:T], x[T:]
+    # lambda_prev of substep 2 is lambda_prev + change of substep 1; its lambda_prev2 is substep 1's lambda_prev
+    assert np.allclose(f1[:, 0:2], f0[:, 0:2] + tt[:T], rtol=1e-6, atol=1e-12)
+    assert np.array_equal(f1[:, 2:4], f0[:, 0:2])
+    assert np.all(f0[:, 0:4] == 0)  # multipliers start at zero
+    # features of substep 1 at the predicted positions x~ = x0 + g dt^2 (gravity as acceleration)
+    b = sc.bodies[0]
+    dt = sc.dt / sc.substeps
+    w = np.where(b.mass > 0, 1.0 / np.where(b.mass > 0, b.mass, 1.0), 0.0)
+    xt = b.x0 + (w > 0)[:, None] * np.array([0.0, -9.81, 0.0]) * dt * dt
+    order = _execution_order(dev, sc)
+    tets = np.asarray(sc.groups[0].conn).reshape(-1, 4)[order]
+    mu = 5e5 / (2 * 1.45)
+    la = 5e5 * 0.45 / (1.45 * 0.1)
+    Dm = np.stack([b.x0[tets[:, k]] - b.x0[tets[:, 3]] for k in range(3)], axis=2)
+    Ds = np.stack([xt[tets[:, k]] - xt[tets[:, 3]] for k in range(3)], axis=2)
+    F = Ds @ np.linalg.inv(Dm)
+    V0 = np.abs(np.linalg.det(Dm)) / 6
+    Cd = np.sqrt(np.sum(F * F, axis=(1, 2)))
+    Ch = np.linalg.det(F) - 1 - mu / la
+    ad, ah = 1 / (mu * V0) / dt ** 2, 1 / (la * V0) / dt ** 2
+    assert np.allclose(f0[:, 4], Cd, rtol=1e-5)
+    assert np.allclose(f0[:, 5], Ch, rtol=1e-3, atol=1e-5)
+    assert np.allclose(f0[:, 8], np.log(ad), rtol=1e-5) and np.allclose(f0[:, 9], np.log(ah), rtol=1e-5)
+    assert np.allclose(f0[:, 10], -Cd / ad, rtol=1e-5) and np.allclose(f0[:, 11], -Ch / ah, rtol=1e-3, atol=1e-9)
+
+
+def _execution_order(dev, sc):
+    """Constraint indices (insertion order) in the device's execution order: colors ascending, the
+    hex clusters of a color in index order, slot-major inside the color."""
+    col, nc = dev.group_colors(0)
+    off = np.asarray(sc.groups[0].clusters)
+    ccol = col[off[:-1]]
+    order = []
+    for c in range(nc):
+        cl = np.nonzero(ccol == c)[0]
+        for k in range(6):
+            order.extend(off[cl] + k)
+    return np.asarray(order)
+
+
+def test_fallback_with_zero_limit_is_the_plain_solver():
+    sc = _beam(True)
+    say("fallback")
+    a = scenes.build_device(sc)
+    b = scenes.build_device(sc)
+    b.set_warm_start("previous")
+    b.set_fallback(1e-300)
+    for _ in range(4):
+        a.step(sc.dt, sc.iterations, sc.substeps)
+        b.step(sc.dt, sc.iterations, sc.substeps)
+    assert b.fallback_count() == 4
+    assert np.array_equal(a.read_body(0)[0], b.read_body(0)[0]) and np.array_equal(a.read_body(0)[1], b.read_body(0)[1])
+
+
+def test_learned_warm_start_trains_and_runs():
+    from sbs_amd import autotune
+    sc = _beam(True, iterations=8)
+    say("learned warm start")
+    dev = scenes.build_device(sc)
+    dev.use_graphs(True)
+    for _ in range(3):
+        dev.step(sc.dt, sc.iterations, sc.substeps)
+    dev.snapshot()
+    target = autotune.mean_residual(dev, sc, 4, sc.iterations)
+    dev.restore()
+    net = autotune.train_warm_start(dev, sc, frames=4, iterations=24, steps=300, batch=4096, log=say)
+    assert net.dataset_size() > 0
+    found, trials = autotune.tune_warm_start(dev, sc, net, target, frames=4, scales=(0.5,), iterations=(4, 8),
+                                             log=say)
+    # the tuner leaves the state where it started
+    x0 = dev.read_body(0)[0]
+    dev.restore()
+    assert np.array_equal(dev.read_body(0)[0], x0)
+    for t in trials:
+        if t["mode"] == "previous":
+            assert np.isfinite(t["residual"]), t
+    say(f"target {target:.4g}: {found}")
+    # whatever the network's quality, the residual check keeps the learned run on the plain path
+    dev.set_warm_start("learned", net)
+    dev.set_warm_start_scale(0.5)
+    dev.set_fallback(2.0 * target, sc.iterations)
+    for _ in range(4):
+        dev.step(sc.dt, 4, sc.substeps)
+    assert np.all(np.isfinite(dev.read_body(0)[0]))
+    assert autotune.frame_residual(dev, sc) <= 2.0 * target
+    say(f"fallback frames {dev.fallback_count()}")
