@@ -117,6 +117,29 @@ int sbs_gpu_add_plane(sbs_gpu* sim, const double normal[3], const double point[3
 /* Analytic sphere obstacle, sd = |p - c| - r (an analytic sdf_model_t, sdf_model.cpp:149-153). */
 int sbs_gpu_add_sphere(sbs_gpu* sim, const double center[3], double radius);
 
+/* Contact broad phase. The reference traverses a KD-tree of the surface points against every SDF's
+ * englobing volume (src/physics/collision/bvh_model.cpp:30-100); here each surface vertex tests
+ * either every SDF (brute force) or, with the spatial hash, the SDFs whose volumes overlap its grid
+ * cell plus every SDF without a finite volume. Both give the same contacts in the same order (SDF
+ * order per vertex). At most 8 contacts are kept per surface vertex; more are counted
+ * (sbs_gpu_contact_overflow). */
+#define SBS_CONTACT_HASH_AUTO 0 /* hash from 32 obstacles with finite volume on (default) */
+#define SBS_CONTACT_HASH_ON 1
+#define SBS_CONTACT_HASH_OFF 2
+/* Before commit. cell_size: edge of the hash cells (0: the median largest extent of the volumes). */
+int sbs_gpu_set_contact_hash(sbs_gpu* sim, int mode, double cell_size);
+/* The built hash: cell edge (0 when not hashed), table size 2^table_bits, the slot ranges start/end
+ * (2^table_bits words each) into the obstacle index list entries (cap words, *num_entries in total,
+ * increasing inside a slot), and the unhashed obstacles every vertex tests (global, *num_global).
+ * Any output may be NULL. Cell (i, j, k) of point p is floor(p / cell) per axis in FP64 (clamped to
+ * +-2^30); its slot is ((i*73856093) ^ (j*19349663) ^ (k*83492791)) mod 2^table_bits on 32-bit
+ * integers (Teschner et al. 2003). */
+int sbs_gpu_read_contact_hash(sbs_gpu* sim, double* cell_size, uint32_t* table_bits, uint32_t* start, uint32_t* end,
+                              uint32_t* entries, uint32_t cap, uint32_t* num_entries, uint32_t* global,
+                              uint32_t* num_global);
+/* Contacts dropped so far because a vertex had more than 8. */
+int sbs_gpu_contact_overflow(sbs_gpu* sim, uint64_t* dropped);
+
 /* Partitions a tet group's constraints (in the order they were added) into clusters of
  * consecutive constraints: cluster i is [cluster_offsets[i], cluster_offsets[i+1]), with
  * cluster_offsets[0] = 0 and cluster_offsets[num_clusters] = the group's constraint count. One
@@ -264,6 +287,14 @@ int sbs_gpu_restore(sbs_gpu* sim);
  * iterations per substep (0: the step's own count). *fallbacks counts the repeated frames. */
 int sbs_gpu_set_fallback(sbs_gpu* sim, double limit, uint32_t iterations);
 int sbs_gpu_fallback_count(sbs_gpu* sim, uint64_t* fallbacks);
+
+/* ---- device primitives of the contact hash (host arrays in and out) ----------------------------------- */
+
+/* out[i] = in[0] + ... + in[i-1] for i <= n (n + 1 words out). */
+int sbs_util_exclusive_scan(int device, const uint32_t* in, uint64_t n, uint32_t* out);
+/* Stable sort of the pairs by the low `bits` bits of the keys (LSD radix sort, 8 bits per pass). */
+int sbs_util_sort_pairs(int device, const uint32_t* keys, const uint32_t* vals, uint64_t n, int bits,
+                        uint32_t* out_keys, uint32_t* out_vals);
 
 /* ---- host mesh utilities (no device) ---------------------------------------------------------------- */
 

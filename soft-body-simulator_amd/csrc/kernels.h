@@ -11,11 +11,11 @@ namespace sbs {
 // Particle positions: x, y, z and the inverse mass in w, in FP64 (see kernels.hip).
 using pos4 = double4;
 
-// Analytic SDF obstacle in device memory.
+// Analytic SDF obstacle in device memory (FP64, like the contact positions).
 struct sdf_dev {
     int kind;  // 0 plane, 1 sphere
-    float a[4];  // plane: n.xyz, offset | sphere: c.xyz, r
-    float vmin[3], vmax[3];
+    double a[4];  // plane: n.xyz, offset | sphere: c.xyz, r
+    double vmin[3], vmax[3];
 };
 
 // Arrays of one tet-constraint group in color order. Material layout (3 float4 per tet):
@@ -51,11 +51,12 @@ struct distance_group_dev {
 struct contact_dev {
     const uint32_t* surf;  // surface vertex -> global particle
     uint32_t S;            // surface vertex count
-    uint32_t slots;        // contact slots per surface vertex (>= number of SDFs)
+    uint32_t slots;        // contact slots per surface vertex
     pos4* q;               // [slot * S + s]: contact point, w = sdf index
     float4* n;             // normal
     float* lambda;
     uint32_t* count;       // contacts per surface vertex
+    uint32_t* overflow;    // contacts dropped because a vertex had more than `slots`
 };
 
 void launch_predict(hipStream_t s, pos4* xi, const pos4* xn, float4* v, uint32_t n, float gx, float gy,

@@ -10,6 +10,7 @@
 // is stored in execution (color) order so that one color is one contiguous, coalesced range.
 #include "device_math.h"
 #include "kernels.h"
+#include "sdf.h"
 
 namespace sbs {
 
@@ -708,32 +709,9 @@ __global__ void __launch_bounds__(256) k_distance(distance_group_dev g, pos4* xi
 
 // ---------------------------------------------------------------- contacts
 
-// Signed distance and gradient in FP64 (the contact test and contact point).
-__device__ __forceinline__ void sdf_eval(sdf_dev const& f, pos4 p, double& sd, double grad[3])
-{
-    if (f.kind == 0) {
-        grad[0] = f.a[0];
-        grad[1] = f.a[1];
-        grad[2] = f.a[2];
-        sd = grad[0] * p.x + grad[1] * p.y + grad[2] * p.z + f.a[3];
-    } else {
-        double const d[3] = {p.x - f.a[0], p.y - f.a[1], p.z - f.a[2]};
-        double const len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        sd = len - f.a[3];
-        if (len > 0.0) {
-            grad[0] = d[0] / len;
-            grad[1] = d[1] / len;
-            grad[2] = d[2] / len;
-        } else {
-            grad[0] = 0.0;
-            grad[1] = 1.0;
-            grad[2] = 0.0;
-        }
-    }
-}
-
 // One thread per surface vertex: contacts against every SDF, in SDF order, from the positions at
-// the start of the frame (bvh_model.cpp:72-95, contact_handler.cpp:39-52).
+// the start of the frame (bvh_model.cpp:72-95, contact_handler.cpp:39-52). A vertex keeps at most
+// c.slots contacts; more are counted in c.overflow.
 __global__ void k_detect(contact_dev c, const pos4* __restrict__ x, const sdf_dev* __restrict__ sdfs, uint32_t nsdf)
 {
     uint32_t const s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -741,20 +719,13 @@ __global__ void k_detect(contact_dev c, const pos4* __restrict__ x, const sdf_de
     pos4 const p = x[c.surf[s]];
     uint32_t cnt = 0;
     for (uint32_t k = 0; k < nsdf; ++k) {
-        sdf_dev const f = sdfs[k];
-        double sd, grad[3];
-        sdf_eval(f, p, sd, grad);
-        if (!(sd < 0.0)) continue;
-        if (!(p.x >= f.vmin[0] && p.x <= f.vmax[0] && p.y >= f.vmin[1] && p.y <= f.vmax[1] && p.z >= f.vmin[2] &&
-              p.z <= f.vmax[2]))
+        double q[3], n[3];
+        if (!sdf_contact(sdfs[k], p, q, n)) continue;
+        if (cnt == c.slots) {
+            atomicAdd(c.overflow, 1u);
             continue;
-        double const gl = sqrt(grad[0] * grad[0] + grad[1] * grad[1] + grad[2] * grad[2]);
-        double const nn[3] = {grad[0] / gl, grad[1] / gl, grad[2] / gl};
-        double const a = fabs(sd);
-        uint32_t const slot = cnt * c.S + s;
-        c.q[slot] = make_double4(p.x + a * nn[0], p.y + a * nn[1], p.z + a * nn[2], double(k));
-        c.n[slot] = make_float4(float(nn[0]), float(nn[1]), float(nn[2]), 0.0f);
-        ++cnt;
+        }
+        store_contact(c, s, cnt++, k, q, n);
     }
     c.count[s] = cnt;
 }

@@ -16,7 +16,9 @@
 #include <vector>
 
 #include "abi_util.h"
+#include "contact_hash.h"
 #include "kernels.h"
+#include "scan_sort.h"
 #include "mlp.h"
 #include "mlp_handle.h"
 #include "sbs_gpu.h"
@@ -72,6 +74,10 @@ struct group_t {
     distance_group_dev dist() const { return {didx.p, rest.p, lambda.p}; }
 };
 
+constexpr std::size_t kMaxContactSlots = 8;   // contacts kept per surface vertex
+constexpr std::size_t kHashMinObstacles = 32;  // automatic mode hashes from this many bounded obstacles on
+constexpr uint64_t kMaxHashCells = 1u << 16;   // cells of one hashed obstacle volume, at most
+
 struct timing_rec {
     int kind;
     uint64_t launches, units;
@@ -104,6 +110,16 @@ struct sbs_gpu {
     dev_buf<float> clam;
     dev_buf<uint32_t> ccount;
     dev_buf<sdf_dev> dsdf;
+    dev_buf<uint32_t> coverflow;  // contacts dropped for want of slots (cumulative)
+    // spatial hash of the obstacles (contact broad phase)
+    int hash_mode = SBS_CONTACT_HASH_AUTO;
+    double hash_cell_size = 0.0;  // requested (0: automatic)
+    bool use_hash = false;
+    double hash_h = 0.0;
+    int hash_bits = 0;
+    uint32_t hash_pairs = 0;
+    std::vector<uint32_t> hashed_ids, global_ids;
+    dev_buf<uint32_t> hstart, hend, hentries, hcount, hglobal, hids, hws;
 
     dev_buf<double> partial;
 
@@ -189,7 +205,13 @@ struct sbs_gpu {
         pending.clear();
     }
 
-    contact_dev contacts() { return {surf.p, S, slots, cq.p, cn.p, clam.p, ccount.p}; }
+    contact_dev contacts() { return {surf.p, S, slots, cq.p, cn.p, clam.p, ccount.p, coverflow.p}; }
+    contact_hash_dev hash_view() const
+    {
+        return {hstart.p, hend.p, hentries.p, hglobal.p, uint32_t(global_ids.size()), (1u << hash_bits) - 1u,
+                1.0 / hash_h};
+    }
+    void build_hash();
 
     void set_current() { hip_check(hipSetDevice(device), "hipSetDevice"); }
 
@@ -500,14 +522,17 @@ void sbs_gpu::commit(uint32_t recolor_passes)
     for (auto const& b : bodies)
         for (uint32_t s : b.surface) sv.push_back(b.offset + s);
     S = uint32_t(sv.size());
-    slots = uint32_t(sdfs.size());
+    slots = uint32_t(std::min<std::size_t>(sdfs.size(), kMaxContactSlots));
     if (S && slots) {
         surf.upload(sv.data(), S, stream);
         cq.alloc(std::size_t(S) * slots);
         cn.alloc(std::size_t(S) * slots);
         clam.alloc(std::size_t(S) * slots);
         ccount.alloc(S);
+        coverflow.alloc(1);
+        hip_check(hipMemsetAsync(coverflow.p, 0, sizeof(uint32_t), stream), "clear overflow");
         dsdf.upload(sdfs.data(), sdfs.size(), stream);
+        build_hash();
     }
     hip_check(hipStreamSynchronize(stream), "commit");
     committed = true;
@@ -517,8 +542,74 @@ void sbs_gpu::detect()
 {
     if (S && slots) {
         contact_dev const c = contacts();
-        timed(SBS_KERNEL_DETECT, S, [&] { launch_detect(stream, c, xn.p, dsdf.p, uint32_t(sdfs.size()), block); });
+        if (use_hash)
+            timed(SBS_KERNEL_DETECT, S, [&] { launch_detect_hashed(stream, c, xn.p, dsdf.p, hash_view(), block); });
+        else
+            timed(SBS_KERNEL_DETECT, S, [&] { launch_detect(stream, c, xn.p, dsdf.p, uint32_t(sdfs.size()), block); });
     }
+}
+
+// Obstacles with a finite volume of at most kMaxHashCells cells are hashed; the others (unbounded
+// planes, huge volumes) go to the list every vertex tests. Automatic cell edge: the median of the
+// hashed volumes' largest extent. Table: the smallest power of two >= 2 x (cells over all hashed
+// volumes), at least 2^10.
+void sbs_gpu::build_hash()
+{
+    use_hash = false;
+    hashed_ids.clear();
+    global_ids.clear();
+    if (hash_mode == SBS_CONTACT_HASH_OFF) return;
+    std::vector<double> ext;
+    for (uint32_t k = 0; k < sdfs.size(); ++k) {
+        sdf_dev const& f = sdfs[k];
+        bool bounded = true;
+        double e = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            bounded = bounded && std::fabs(f.vmin[d]) < 1e30 && std::fabs(f.vmax[d]) < 1e30 && f.vmax[d] >= f.vmin[d];
+            e = std::max(e, f.vmax[d] - f.vmin[d]);
+        }
+        if (bounded) ext.push_back(e);
+    }
+    if (ext.empty() || (hash_mode == SBS_CONTACT_HASH_AUTO && ext.size() < kHashMinObstacles)) return;
+    std::vector<double> sorted = ext;
+    std::nth_element(sorted.begin(), sorted.begin() + sorted.size() / 2, sorted.end());
+    hash_h = hash_cell_size > 0.0 ? hash_cell_size : std::max(sorted[sorted.size() / 2], 1e-6);
+    double const inv_h = 1.0 / hash_h;
+    uint64_t pairs = 0;
+    for (uint32_t k = 0; k < sdfs.size(); ++k) {
+        sdf_dev const& f = sdfs[k];
+        bool bounded = true;
+        uint64_t cells = 1;
+        for (int d = 0; d < 3; ++d) {
+            bounded = bounded && std::fabs(f.vmin[d]) < 1e30 && std::fabs(f.vmax[d]) < 1e30 && f.vmax[d] >= f.vmin[d];
+            if (!bounded) break;
+            cells *= uint64_t(hash_cell(f.vmax[d], inv_h) - hash_cell(f.vmin[d], inv_h) + 1);
+        }
+        if (bounded && cells <= kMaxHashCells) {
+            hashed_ids.push_back(k);
+            pairs += cells;
+        } else {
+            global_ids.push_back(k);
+        }
+    }
+    require(pairs < (uint64_t(1) << 31), "obstacle volumes cover too many hash cells; use a larger cell size");
+    hash_pairs = uint32_t(pairs);
+    hash_bits = 10;
+    while ((uint64_t(1) << hash_bits) < 2 * pairs && hash_bits < 28) ++hash_bits;
+    std::size_t const T = std::size_t(1) << hash_bits;
+    hstart.alloc(T);
+    hend.alloc(T);
+    hentries.alloc(std::max<std::size_t>(pairs, 1));
+    hcount.alloc(1);
+    hids.upload(hashed_ids.data(), hashed_ids.size(), stream);
+    hglobal.upload(global_ids.data(), global_ids.size(), stream);
+    hws.alloc(contact_hash_workspace(uint32_t(hashed_ids.size()), pairs));
+    build_contact_hash(stream, dsdf.p, hids.p, uint32_t(hashed_ids.size()), pairs, inv_h, hash_bits, hstart.p, hend.p,
+                       hentries.p, hcount.p, hws.p);
+    hip_check(hipGetLastError(), "contact hash build");
+    hip_check(hipStreamSynchronize(stream), "contact hash build");
+    hws.release();
+    use_hash = true;
 }
 
 // One launch per color of a group (mode: kSolveFirst / kSolveNext, or for neo-Hookean groups a
@@ -962,11 +1053,11 @@ int sbs_gpu_add_plane(sbs_gpu* sim, const double normal[3], const double point[3
         f.kind = 0;
         double const off = -(normal[0] * point[0] + normal[1] * point[1] + normal[2] * point[2]);
         for (int i = 0; i < 3; ++i) {
-            f.a[i] = float(normal[i]);
-            f.vmin[i] = float(std::max(vol_min[i], -3.0e38));
-            f.vmax[i] = float(std::min(vol_max[i], 3.0e38));
+            f.a[i] = normal[i];
+            f.vmin[i] = vol_min[i];
+            f.vmax[i] = vol_max[i];
         }
-        f.a[3] = float(off);
+        f.a[3] = off;
         sim->sdfs.push_back(f);
     });
 }
@@ -980,11 +1071,11 @@ int sbs_gpu_add_sphere(sbs_gpu* sim, const double center[3], double radius)
         sdf_dev f{};
         f.kind = 1;
         for (int i = 0; i < 3; ++i) {
-            f.a[i] = float(center[i]);
-            f.vmin[i] = float(center[i] - radius);
-            f.vmax[i] = float(center[i] + radius);
+            f.a[i] = center[i];
+            f.vmin[i] = center[i] - radius;
+            f.vmax[i] = center[i] + radius;
         }
-        f.a[3] = float(radius);
+        f.a[3] = radius;
         sim->sdfs.push_back(f);
     });
 }
@@ -1152,6 +1243,118 @@ int sbs_gpu_group_colors(sbs_gpu* sim, uint32_t group, uint32_t* num_colors, uin
         if (num_colors) *num_colors = g.ncolors;
         if (color_of_constraint && !g.color.empty())
             std::memcpy(color_of_constraint, g.color.data(), g.color.size() * sizeof(uint32_t));
+    });
+}
+
+int sbs_util_exclusive_scan(int device, const uint32_t* in, uint64_t n, uint32_t* out)
+{
+    return guarded([&] {
+        require(n == 0 || (in != nullptr && out != nullptr), "null argument");
+        require(out != nullptr, "out is null");
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        hipStream_t s;
+        hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+        try {
+            dev_buf<uint32_t> din, dout, ws;
+            din.upload(in, n, s);
+            dout.alloc(n + 1);
+            ws.alloc(scan_workspace(n));
+            exclusive_scan(s, din.p, dout.p, n, ws.p);
+            hip_check(hipGetLastError(), "scan");
+            hip_check(hipMemcpyAsync(out, dout.p, (n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read scan");
+            hip_check(hipStreamSynchronize(s), "scan");
+        } catch (...) {
+            (void)hipStreamDestroy(s);
+            throw;
+        }
+        hip_check(hipStreamDestroy(s), "hipStreamDestroy");
+    });
+}
+
+int sbs_util_sort_pairs(int device, const uint32_t* keys, const uint32_t* vals, uint64_t n, int bits,
+                        uint32_t* out_keys, uint32_t* out_vals)
+{
+    return guarded([&] {
+        require(n == 0 || (keys && vals && out_keys && out_vals), "null argument");
+        require(bits >= 0 && bits <= 32, "bits must be 0..32");
+        require(n < (uint64_t(1) << 31), "too many pairs");
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        if (n == 0) return;
+        hipStream_t s;
+        hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+        try {
+            dev_buf<uint32_t> dk, dv, ws;
+            dk.upload(keys, n, s);
+            dv.upload(vals, n, s);
+            ws.alloc(sort_workspace(n));
+            radix_sort_pairs(s, dk.p, dv.p, n, bits, ws.p);
+            hip_check(hipGetLastError(), "sort");
+            hip_check(hipMemcpyAsync(out_keys, dk.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read sort");
+            hip_check(hipMemcpyAsync(out_vals, dv.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read sort");
+            hip_check(hipStreamSynchronize(s), "sort");
+        } catch (...) {
+            (void)hipStreamDestroy(s);
+            throw;
+        }
+        hip_check(hipStreamDestroy(s), "hipStreamDestroy");
+    });
+}
+
+int sbs_gpu_set_contact_hash(sbs_gpu* sim, int mode, double cell_size)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(!sim->committed, "scene already committed", SBS_ERR_STATE);
+        require(mode == SBS_CONTACT_HASH_AUTO || mode == SBS_CONTACT_HASH_ON || mode == SBS_CONTACT_HASH_OFF,
+                "unknown contact hash mode");
+        require(cell_size >= 0.0 && std::isfinite(cell_size), "cell_size must be finite and >= 0");
+        sim->hash_mode = mode;
+        sim->hash_cell_size = cell_size;
+    });
+}
+
+int sbs_gpu_read_contact_hash(sbs_gpu* sim, double* cell_size, uint32_t* table_bits, uint32_t* start, uint32_t* end,
+                              uint32_t* entries, uint32_t cap, uint32_t* num_entries, uint32_t* global,
+                              uint32_t* num_global)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
+        if (cell_size) *cell_size = sim->use_hash ? sim->hash_h : 0.0;
+        if (table_bits) *table_bits = sim->use_hash ? uint32_t(sim->hash_bits) : 0u;
+        if (num_global) *num_global = uint32_t(sim->global_ids.size());
+        if (global) std::copy(sim->global_ids.begin(), sim->global_ids.end(), global);
+        uint32_t n = 0;
+        if (sim->use_hash) {
+            hip_check(hipMemcpyAsync(&n, sim->hcount.p, sizeof(uint32_t), hipMemcpyDeviceToHost, sim->stream), "read hash");
+            std::size_t const T = std::size_t(1) << sim->hash_bits;
+            if (start)
+                hip_check(hipMemcpyAsync(start, sim->hstart.p, T * sizeof(uint32_t), hipMemcpyDeviceToHost, sim->stream),
+                          "read hash");
+            if (end)
+                hip_check(hipMemcpyAsync(end, sim->hend.p, T * sizeof(uint32_t), hipMemcpyDeviceToHost, sim->stream),
+                          "read hash");
+            hip_check(hipStreamSynchronize(sim->stream), "read hash");
+            if (entries && n)
+                hip_check(hipMemcpy(entries, sim->hentries.p, std::min(n, cap) * sizeof(uint32_t), hipMemcpyDeviceToHost),
+                          "read hash");
+        }
+        if (num_entries) *num_entries = n;
+    });
+}
+
+int sbs_gpu_contact_overflow(sbs_gpu* sim, uint64_t* dropped)
+{
+    return guarded([&] {
+        require(sim != nullptr && dropped != nullptr, "null argument");
+        require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
+        uint32_t n = 0;
+        if (sim->coverflow.p) {
+            hip_check(hipMemcpyAsync(&n, sim->coverflow.p, sizeof(uint32_t), hipMemcpyDeviceToHost, sim->stream),
+                      "read overflow");
+            hip_check(hipStreamSynchronize(sim->stream), "read overflow");
+        }
+        *dropped = n;
     });
 }
 

@@ -95,6 +95,11 @@ _SIGS = {
     "sbs_mlp_fit_normalization": [_p],
     "sbs_mlp_train": [_p, _u32, _u32, _f, _u64, _pd],
     "sbs_gpu_set_warm_start": [_p, _i, _p],
+    "sbs_gpu_set_contact_hash": [_p, _i, _d],
+    "sbs_gpu_read_contact_hash": [_p, _pd, _pu, _pu, _pu, _pu, _u32, _pu, _pu, _pu],
+    "sbs_gpu_contact_overflow": [_p, _pu64],
+    "sbs_util_exclusive_scan": [_i, _pu, _u64, _pu],
+    "sbs_util_sort_pairs": [_i, _pu, _pu, _u64, _i, _pu, _pu],
     "sbs_gpu_capture": [_p, _p, _u32],
     "sbs_gpu_set_warm_start_scale": [_p, _d],
     "sbs_gpu_snapshot": [_p],
@@ -213,6 +218,24 @@ def lumped_mass(pos, tets, density):
     m = np.zeros(len(pos) // 3)
     check(lib().sbs_mesh_lumped_mass(len(m), _dp(pos), _up(tets), len(tets) // 4, density, _dp(m)))
     return m
+
+
+# ------------------------------------------------------------------ device primitives
+
+def exclusive_scan(values, device=0):
+    """Exclusive prefix sum on the device: n + 1 words (the last is the total)."""
+    a = u32(values)
+    out = np.zeros(len(a) + 1, dtype=np.uint32)
+    check(lib().sbs_util_exclusive_scan(device, _up(a), len(a), _up(out)))
+    return out
+
+
+def sort_pairs(keys, vals, bits=32, device=0):
+    """Stable device radix sort of (keys, vals) by the low `bits` bits of the keys."""
+    k, v = u32(keys), u32(vals)
+    ok, ov = np.zeros_like(k), np.zeros_like(v)
+    check(lib().sbs_util_sort_pairs(device, _up(k), _up(v), len(k), bits, _up(ok), _up(ov)))
+    return ok, ov
 
 
 # ------------------------------------------------------------------ warm-start network
@@ -479,6 +502,27 @@ class Simulation:
     def fallback_count(self):
         n = ctypes.c_uint64()
         check(self._L.sbs_gpu_fallback_count(self.h, ctypes.byref(n)))
+        return n.value
+
+    def set_contact_hash(self, mode="auto", cell_size=0.0):
+        check(self._L.sbs_gpu_set_contact_hash(self.h, {"auto": 0, "on": 1, "off": 2}[mode], cell_size))
+
+    def read_contact_hash(self):
+        """(cell size, table bits, start, end, entries, global list) of the obstacle hash."""
+        h = ctypes.c_double()
+        bits, ne, ng = _u32(), _u32(), _u32()
+        check(self._L.sbs_gpu_read_contact_hash(self.h, ctypes.byref(h), ctypes.byref(bits), None, None, None, 0,
+                                                ctypes.byref(ne), None, ctypes.byref(ng)))
+        T = (1 << bits.value) if h.value > 0 else 0
+        start, end = np.zeros(T, np.uint32), np.zeros(T, np.uint32)
+        entries, glob = np.zeros(max(ne.value, 1), np.uint32), np.zeros(max(ng.value, 1), np.uint32)
+        check(self._L.sbs_gpu_read_contact_hash(self.h, None, None, _up(start) if T else None, _up(end) if T else None,
+                                                _up(entries), ne.value, ctypes.byref(ne), _up(glob), ctypes.byref(ng)))
+        return h.value, bits.value, start, end, entries[:ne.value], glob[:ng.value]
+
+    def contact_overflow(self):
+        n = ctypes.c_uint64()
+        check(self._L.sbs_gpu_contact_overflow(self.h, ctypes.byref(n)))
         return n.value
 
     def set_timing(self, enable):

@@ -45,6 +45,7 @@ class Scene:
     iterations: int = 10
     substeps: int = 1
     coloring: str = "hash"  # "hash" (device Jones-Plassmann) or "index" (sequential greedy)
+    contact_hash: tuple = None  # (mode "auto" | "on" | "off", cell size) of the contact broad phase
 
     @property
     def num_tets(self):
@@ -82,6 +83,8 @@ class Scene:
             sim.add_plane(n, p, lo, hi)
         for c, r in self.spheres:
             sim.add_sphere(c, r)
+        if self.contact_hash is not None and hasattr(sim, "set_contact_hash"):
+            sim.set_contact_hash(*self.contact_hash)
 
     def bbox_diagonal(self):
         x = np.concatenate([b.x0 for b in self.bodies])
@@ -115,6 +118,21 @@ def reference_demo():
     sc.bodies.append(Body(x0=x0, tets=tets, surface=s2t))
     sc.groups.append(Group("green", 0, tets, alpha=1e-4, beta=0.0, young=1e6, poisson=0.30))
     sc.planes.append(((0.0, 1.0, 0.0), (0.0, 0.0, 0.0), (-20.0, -5.0, -20.0), (20.0, 5.0, 20.0)))
+    return sc
+
+
+def obstacle_field(num_spheres=4000, nx=24, ny=6, nz=6, edge=0.1, seed=7, contact_hash=("auto", 0.0)):
+    """A neo-Hookean beam (hex clusters) among many small static spheres: sphere centres uniform in
+    the beam's box grown by 0.1 m, radii U[0.02, 0.06] m, so that many surface vertices start inside
+    one or more spheres. The broad phase of the contact detection is what the scene exercises."""
+    sc = cantilever(nx, ny, nz, edge, "neohookean", 5e5, 0.45, 1000.0, num_spheres=0, iterations=8, substeps=2,
+                    cluster=True, coloring="index", name=f"obstacle_field_{num_spheres}")
+    rng = np.random.default_rng(seed)
+    x = sc.bodies[0].x0
+    lo, hi = x.min(0) - 0.1, x.max(0) + 0.1
+    for _ in range(num_spheres):
+        sc.spheres.append((tuple(rng.uniform(lo, hi)), float(rng.uniform(0.02, 0.06))))
+    sc.contact_hash = contact_hash
     return sc
 
 
