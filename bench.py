@@ -100,6 +100,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="cfg1_1m", choices=["cfg1_1m", "cfg1_6m", "cfg0", "cfg2"])
     ap.add_argument("--no-learned", action="store_true", help="plain solver only")
+    ap.add_argument("--no-graphs", action="store_true", help="stream launches instead of hipGraph replays")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run device copies of known size (to scale PMC byte counters)")
     args = ap.parse_args()
@@ -125,7 +126,7 @@ def main():
 
     from sbs_amd import autotune
 
-    sim.use_graphs(True)
+    sim.use_graphs(not args.no_graphs)
     launch = autotune.tune_launch(sim, scene)
     log(f"launch config {launch}")
     for _ in range(args.warmup):

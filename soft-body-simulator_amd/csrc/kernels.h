@@ -134,7 +134,8 @@ void launch_copy_pos(hipStream_t s, const pos4* a, pos4* b, uint32_t n);
 void launch_ws_features(hipStream_t s, tet_group_dev g, const pos4* xi, float2* hist, uint32_t count, float inv_dt2,
                         float* raw, mlp::bf16* X, const mlp::norm_dev* norm, int block);
 // Training rows of constraints t = offset + i * stride, i < rows: ds_x[base + i] = raw[t],
-// ds_t[base + i] = lambda[t] - lambda_prev[t] (the multipliers' change over the substep).
+// ds_t[base + i] = lambda[t] - lambda_prev[t] (the multipliers' change over the substep); a row
+// holding a non-finite value is stored as zeros.
 void launch_ws_capture(hipStream_t s, const float* raw, const float* lambda, uint32_t stride, uint32_t offset,
                        float* ds_x, float* ds_t, uint64_t base, uint32_t rows);
 

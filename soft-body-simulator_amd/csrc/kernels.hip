@@ -439,11 +439,16 @@ __global__ void k_ws_capture(const float* __restrict__ raw, const float2* __rest
     std::size_t const t = std::size_t(offset) + std::size_t(i) * stride;
     std::size_t const r = base + i;
     float const* f = raw + t * mlp::kMaxInputs;
-#pragma unroll
-    for (int k = 0; k < mlp::kMaxInputs; ++k) ds_x[r * mlp::kMaxInputs + k] = f[k];
     float2 const lf = lambda[t];
-    ds_t[r * mlp::kMaxOutputs + 0] = lf.x - f[0];
-    ds_t[r * mlp::kMaxOutputs + 1] = lf.y - f[1];
+    float const d0 = lf.x - f[0], d1 = lf.y - f[1];
+    // a row with a non-finite value (a diverged constraint) is stored as zeros
+    bool ok = isfinite(d0) && isfinite(d1);
+#pragma unroll
+    for (int k = 0; k < mlp::kMaxInputs; ++k) ok = ok && isfinite(f[k]);
+#pragma unroll
+    for (int k = 0; k < mlp::kMaxInputs; ++k) ds_x[r * mlp::kMaxInputs + k] = ok ? f[k] : 0.0f;
+    ds_t[r * mlp::kMaxOutputs + 0] = ok ? d0 : 0.0f;
+    ds_t[r * mlp::kMaxOutputs + 1] = ok ? d1 : 0.0f;
     ds_t[r * mlp::kMaxOutputs + 2] = 0.0f;
     ds_t[r * mlp::kMaxOutputs + 3] = 0.0f;
 }

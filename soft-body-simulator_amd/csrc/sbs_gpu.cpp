@@ -815,6 +815,8 @@ void sbs_gpu::run_frame(double dt, uint32_t iterations, uint32_t substeps, int w
         key[12] = ws_scale;
         if (!graph_exec || key != graph_key) {
             if (graph_exec) {
+                // the previous graph may still be running on the stream: let it finish first
+                hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
                 hip_check(hipGraphExecDestroy(graph_exec), "hipGraphExecDestroy");
                 graph_exec = nullptr;
             }
