@@ -155,7 +155,10 @@ int sbs_gpu_set_group_clusters(sbs_gpu* sim, uint32_t group, const uint32_t* clu
 int sbs_gpu_set_coloring(sbs_gpu* sim, int order);
 
 /* Threads per block of the per-constraint kernels (64/128/256) and of the cluster kernels
- * (64/128). Default 256 and 64. */
+ * (64/128, or SBS_CLUSTER_BLOCK_STREAMED: 64 threads, and the neo-Hookean hex-cell kernel loads
+ * each tet's material just before projecting it instead of all six up front -- fewer registers,
+ * more waves in flight, for launches large enough to be bandwidth bound). Default 256 and 64. */
+#define SBS_CLUSTER_BLOCK_STREAMED 65
 int sbs_gpu_set_launch_config(sbs_gpu* sim, int block, int cluster_block);
 
 /* Colors every constraint group and uploads the scene. recolor_passes: extra Jones-Plassmann

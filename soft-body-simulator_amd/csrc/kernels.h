@@ -5,6 +5,7 @@
 #include <cstdint>
 
 #include "mlp.h"
+#include "sbs_gpu.h"
 
 namespace sbs {
 
@@ -90,6 +91,9 @@ void launch_green_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, 
 void launch_neohookean_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn,
                            uint32_t begin, uint32_t count, uint32_t tbase, float inv_dt2, float beta_dt, int mode,
                            int block);
+// cluster_block value selecting the hex kernel that loads each tet's material just before
+// projecting it (fewer registers, more waves) instead of prefetching all six.
+constexpr int kHexStreamed = SBS_CLUSTER_BLOCK_STREAMED;
 constexpr uint32_t kHexPatternLoc[6] = {0x03020100u, 0x03010400u, 0x03050200u, 0x03060500u, 0x03040700u, 0x03070600u};
 
 void launch_distance(hipStream_t s, distance_group_dev g, pos4* xi, const pos4* xn, uint32_t begin,

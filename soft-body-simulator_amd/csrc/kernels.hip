@@ -563,43 +563,58 @@ constexpr int kHexTets = 6;
 
 constexpr int kHexSlots[kHexTets][4] = {{0, 1, 2, 3}, {0, 4, 1, 3}, {0, 2, 5, 3}, {0, 5, 6, 3}, {0, 7, 4, 3}, {0, 6, 7, 3}};
 
-template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_neohookean_hex(tet_group_dev g, cluster_dev cl, pos4* xi,
-                                                          const pos4* __restrict__ xn, uint32_t begin, uint32_t count,
-                                                          uint32_t tbase, float inv_dt2, float beta_dt, int mode)
+// PREFETCH: every tet's material and multipliers are loaded before the first projection (one
+// memory round trip, ~240 VGPRs, 2 waves per SIMD: for launches too small to fill the chip, where
+// latency rules); otherwise each tet's are loaded just before it is projected (fewer registers,
+// more waves in flight: for launches that fill the chip, where bandwidth rules).
+template <int BLOCK, bool PREFETCH, int MIN_WAVES, bool DAMPED>
+__global__ void __launch_bounds__(BLOCK, MIN_WAVES) k_neohookean_hex(tet_group_dev g, cluster_dev cl, pos4* xi,
+                                                                     const pos4* __restrict__ xn, uint32_t begin,
+                                                                     uint32_t count, uint32_t tbase, float inv_dt2,
+                                                                     float beta_dt, int mode)
 {
     uint32_t const i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= count) return;
     uint32_t const c = begin + i;
     int4 const va = cl.v0[c], vb = cl.v1[c];
     int const ids[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
-    float4 A[kHexTets], B[kHexTets], Cm[kHexTets];
-    float2 L[kHexTets];
-#pragma unroll
-    for (int k = 0; k < kHexTets; ++k) {
+    constexpr int NP = PREFETCH ? kHexTets : 1;
+    float4 A[NP], B[NP], Cm[NP];
+    float2 L[NP];
+    auto load = [&](int k, int slot) {
         uint32_t const t = tbase + uint32_t(k) * count + i;
-        A[k] = g.m0[t];
-        B[k] = g.m1[t];
-        Cm[k] = g.m2[t];
-        L[k] = mode == kSolveFirst ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(g.lambda)[t];
+        A[slot] = g.m0[t];
+        B[slot] = g.m1[t];
+        Cm[slot] = g.m2[t];
+        L[slot] = mode == kSolveFirst ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(g.lambda)[t];
+    };
+    auto store = [&](int k, int slot) { reinterpret_cast<float2*>(g.lambda)[tbase + uint32_t(k) * count + i] = L[slot]; };
+    if (PREFETCH) {
+#pragma unroll
+        for (int k = 0; k < kHexTets; ++k) load(k, k);
     }
     pos4 P[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) P[k] = xi[ids[k]];
 #pragma unroll
     for (int k = 0; k < kHexTets; ++k) {
+        int const m = PREFETCH ? k : 0;
+        if (!PREFETCH) load(k, 0);
         int const s0 = kHexSlots[k][0], s1 = kHexSlots[k][1], s2 = kHexSlots[k][2], s3 = kHexSlots[k][3];
         pos4 Q[4] = {P[s0], P[s1], P[s2], P[s3]};
         int const id[4] = {ids[s0], ids[s1], ids[s2], ids[s3]};
         auto xn_of = [&](int q) { return xn[id[q]]; };
-        neohookean_project(Q, xn_of, A[k], B[k], Cm[k], inv_dt2, beta_dt, L[k]);
+        neohookean_project(Q, xn_of, A[m], B[m], Cm[m], inv_dt2, DAMPED ? beta_dt : 0.0f, L[m]);
         P[s0] = Q[0];
         P[s1] = Q[1];
         P[s2] = Q[2];
         P[s3] = Q[3];
+        if (!PREFETCH) store(k, 0);
     }
+    if (PREFETCH) {
 #pragma unroll
-    for (int k = 0; k < kHexTets; ++k) reinterpret_cast<float2*>(g.lambda)[tbase + uint32_t(k) * count + i] = L[k];
+        for (int k = 0; k < kHexTets; ++k) store(k, k);
+    }
 #pragma unroll
     for (int k = 0; k < 8; ++k) xi[ids[k]] = P[k];
 }
@@ -1074,12 +1089,19 @@ void launch_neohookean_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4*
                                                                        mode == kApplyExplicit, beta_dt);
         return;
     }
-    if (block <= 64)
-        k_neohookean_hex<64><<<grid_for(count, 64), 64, 0, s>>>(g, cl, xi, xn, begin, count, tbase, inv_dt2, beta_dt,
-                                                                mode);
+    // block 64 / 128: material prefetched; kHexStreamed: 64 threads, material loaded per tet, at
+    // least 3 waves per SIMD (2 with damping, which would spill at 3). Damping (beta != 0) takes the
+    // variants that carry the xn differences.
+    bool const damped = beta_dt != 0.0f;
+    auto go = [&](auto kernel, int b) {
+        kernel<<<grid_for(count, b), b, 0, s>>>(g, cl, xi, xn, begin, count, tbase, inv_dt2, beta_dt, mode);
+    };
+    if (block == kHexStreamed)
+        damped ? go(k_neohookean_hex<64, false, 2, true>, 64) : go(k_neohookean_hex<64, false, 3, false>, 64);
+    else if (block <= 64)
+        damped ? go(k_neohookean_hex<64, true, 1, true>, 64) : go(k_neohookean_hex<64, true, 1, false>, 64);
     else
-        k_neohookean_hex<128><<<grid_for(count, 128), 128, 0, s>>>(g, cl, xi, xn, begin, count, tbase, inv_dt2,
-                                                                   beta_dt, mode);
+        damped ? go(k_neohookean_hex<128, true, 1, true>, 128) : go(k_neohookean_hex<128, true, 1, false>, 128);
 }
 
 void launch_green_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn, uint32_t begin,

@@ -1119,7 +1119,8 @@ int sbs_gpu_set_launch_config(sbs_gpu* sim, int block, int cluster_block)
     return guarded([&] {
         require(sim != nullptr, "sim is null");
         require(block == 64 || block == 128 || block == 256, "block must be 64, 128 or 256");
-        require(cluster_block == 64 || cluster_block == 128, "cluster_block must be 64 or 128");
+        require(cluster_block == 64 || cluster_block == 128 || cluster_block == SBS_CLUSTER_BLOCK_STREAMED,
+                "cluster_block must be 64, 128 or SBS_CLUSTER_BLOCK_STREAMED");
         sim->block = block;
         sim->cluster_block = cluster_block;
     });

@@ -42,7 +42,10 @@ def time_frames(sim, scene, frames, iterations):
     return time.perf_counter() - t0
 
 
-def tune_launch(sim, scene, frames=3, blocks=(128, 256), cluster_blocks=(64, 128)):
+STREAMED = 65  # SBS_CLUSTER_BLOCK_STREAMED
+
+
+def tune_launch(sim, scene, frames=5, blocks=(128, 256), cluster_blocks=(64, 128, STREAMED)):
     """Fastest (block, cluster_block) over a few graph-replayed frames; leaves it set."""
     best = None
     sim.snapshot()

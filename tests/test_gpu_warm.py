@@ -231,3 +231,19 @@ def test_learned_warm_start_trains_and_runs():
     assert np.all(np.isfinite(dev.read_body(0)[0]))
     assert autotune.frame_residual(dev, sc) <= 2.0 * target
     say(f"fallback frames {dev.fallback_count()}")
+
+
+@pytest.mark.parametrize("beta", [0.0, 0.01])
+def test_hex_kernel_variants_are_bit_identical(beta):
+    # material prefetched (cluster block 64 / 128) or loaded per tet (streamed): the same projections
+    sc = scenes.cantilever(8, 3, 3, 0.1, "neohookean", 5e5, 0.45, 1000.0, beta=beta, num_spheres=2, iterations=6,
+                           substeps=2, cluster=True, coloring="index")
+    say(f"hex kernel variants beta={beta}")
+    out = []
+    for cb in (64, 128, 65):
+        dev = scenes.build_device(sc)
+        dev.set_launch_config(256, cb)
+        for _ in range(5):
+            dev.step(sc.dt, sc.iterations, sc.substeps)
+        out.append(dev.read_body(0)[0])
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
