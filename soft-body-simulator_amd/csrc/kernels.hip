@@ -656,6 +656,72 @@ __global__ void __launch_bounds__(BLOCK) k_neohookean_hex_apply(tet_group_dev g,
     for (int k = 0; k < 8; ++k) add_displacement(&xi[ids[k]], D[k]);
 }
 
+// Warm-start displacements of every hex cluster of a group in one launch (all colors): nothing
+// moves during the launch, so every gradient is taken at the predicted positions xi (a copy of
+// them, or the working positions themselves before any displacement is added); cluster ci
+// (execution order) writes the summed displacement of its slot k to disp[8 ci + k].
+__global__ void __launch_bounds__(64) k_hex_warm_disp(tet_group_dev g, cluster_dev cl, const pos4* __restrict__ xi,
+                                                      const uint32_t* __restrict__ ccolor_off,
+                                                      const uint32_t* __restrict__ color_off, uint32_t ncolors,
+                                                      uint32_t nclusters, float inv_dt2, int expl, float scale,
+                                                      float4* __restrict__ disp)
+{
+    uint32_t const ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= nclusters) return;
+    uint32_t lo = 0, hi = ncolors;  // the color: ccolor_off[col] <= ci < ccolor_off[col + 1]
+    while (hi - lo > 1) {
+        uint32_t const mid = (lo + hi) / 2;
+        if (ccolor_off[mid] <= ci)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    uint32_t const i = ci - ccolor_off[lo], count = ccolor_off[lo + 1] - ccolor_off[lo], tbase = color_off[lo];
+    int4 const va = cl.v0[ci], vb = cl.v1[ci];
+    int const ids[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
+    pos4 P[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) P[k] = xi[ids[k]];
+    v3 D[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) D[k] = mk(0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int k = 0; k < kHexTets; ++k) {
+        uint32_t const t = tbase + uint32_t(k) * count + i;
+        int const s0 = kHexSlots[k][0], s1 = kHexSlots[k][1], s2 = kHexSlots[k][2], s3 = kHexSlots[k][3];
+        pos4 const Q[4] = {P[s0], P[s1], P[s2], P[s3]};
+        float2 lm = reinterpret_cast<const float2*>(g.lambda)[t];
+        v3 d[4];
+        neohookean_displacement(Q, g.m0[t], g.m1[t], g.m2[t], inv_dt2, expl != 0, scale, lm, d);
+        reinterpret_cast<float2*>(g.lambda)[t] = lm;
+        D[s0] = D[s0] + d[0];
+        D[s1] = D[s1] + d[1];
+        D[s2] = D[s2] + d[2];
+        D[s3] = D[s3] + d[3];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) disp[std::size_t(ci) * 8 + k] = make_float4(D[k].x, D[k].y, D[k].z, 0.0f);
+}
+
+// x_p += the displacements of p's cluster slots, in FP64, in the order of the list (execution
+// order of the clusters: the order of the colored pass).
+__global__ void k_gather_disp(const uint32_t* __restrict__ off, const uint32_t* __restrict__ ent,
+                              const float4* __restrict__ disp, pos4* __restrict__ xi, uint32_t n)
+{
+    uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    uint32_t const b = off[p], e = off[p + 1];
+    if (b == e) return;
+    pos4 x = xi[p];
+    for (uint32_t j = b; j < e; ++j) {
+        float4 const d = disp[ent[j]];
+        x.x += double(d.x);
+        x.y += double(d.y);
+        x.z += double(d.z);
+    }
+    xi[p] = x;
+}
+
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_green_hex(tet_group_dev g, cluster_dev cl, pos4* xi,
                                                      const pos4* __restrict__ xn, uint32_t begin, uint32_t count,
@@ -1181,6 +1247,17 @@ void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t value, uint32_t n)
     if (n) k_fill_u32<<<grid_for(n, 256), 256, 0, s>>>(p, value, n);
 }
 
+
+void launch_hex_warm_start(hipStream_t s, tet_group_dev g, cluster_dev cl, const pos4* xt, pos4* xi,
+                           const uint32_t* ccolor_off, const uint32_t* color_off, uint32_t ncolors, uint32_t nclusters,
+                           float inv_dt2, int expl, float scale, float4* disp, const uint32_t* csr_off,
+                           const uint32_t* csr, uint32_t n)
+{
+    if (!nclusters) return;
+    k_hex_warm_disp<<<grid_for(nclusters, 64), 64, 0, s>>>(g, cl, xt, ccolor_off, color_off, ncolors, nclusters,
+                                                          inv_dt2, expl, scale, disp);
+    k_gather_disp<<<grid_for(n, 256), 256, 0, s>>>(csr_off, csr, disp, xi, n);
+}
 
 void launch_copy_pos(hipStream_t s, const pos4* a, pos4* b, uint32_t n)
 {

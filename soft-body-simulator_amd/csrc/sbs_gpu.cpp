@@ -62,6 +62,8 @@ struct group_t {
     dev_buf<float2> ws_hist;     // the multiplier pairs one substep before the stored ones
     dev_buf<float> ws_raw;       // [count][12] features at the substep start (capture)
     dev_buf<mlp::bf16> ws_X;     // [count][16] network inputs (learned warm start)
+    dev_buf<float4> ws_disp;     // hex groups: slot displacements of the warm-start pass
+    dev_buf<uint32_t> ws_csr_off, ws_csr, ws_ccolor_off, ws_color_off;  // particle -> slot list
     dev_buf<float> snap_lambda;  // snapshot copies
     dev_buf<float2> snap_hist;
     bool clustered() const { return !cluster_off.empty(); }
@@ -668,8 +670,17 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
         // multiplier warm start of the neo-Hookean groups: lambda_init (previous, plus the network's
         // change, or -C(x~)/alpha~), then x = x~ + M^-1 grad C(x~)^T lambda_init
         bool warmed = false;
-        if (warm != SBS_WARM_START_NONE)
-            timed(SBS_KERNEL_WARM_START, V, [&] { launch_copy_pos(stream, xi.p, xt.p, V); });
+        // every gradient of the pass is taken at x~: a copy of it is needed unless a single hex group
+        // (whose displacements are all computed before any is added) is the only neo-Hookean group
+        uint32_t nh = 0;
+        bool colored_pass = false;
+        for (auto const& g : groups) {
+            nh += g.kind == 1;
+            colored_pass = colored_pass || (g.kind == 1 && !g.hex);
+        }
+        bool const copy_xt = warm != SBS_WARM_START_NONE && (colored_pass || nh > 1);
+        if (copy_xt) timed(SBS_KERNEL_WARM_START, V, [&] { launch_copy_pos(stream, xi.p, xt.p, V); });
+        const pos4* const xtilde = copy_xt ? xt.p : xi.p;
         for (auto& g : groups) {
             if (g.kind != 1) continue;
             bool const learned = warm == SBS_WARM_START_LEARNED;
@@ -685,7 +696,14 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
             }
             if (warm != SBS_WARM_START_NONE) {
                 int const mode = warm == SBS_WARM_START_EXPLICIT ? kApplyExplicit : kApplyWarmStart;
-                timed(SBS_KERNEL_WARM_START, g.count, [&] { launch_group_colors(g, dt, mode, xt.p); }, g.ncolors);
+                if (g.hex)
+                    timed(SBS_KERNEL_WARM_START, g.count, [&] {
+                        launch_hex_warm_start(stream, g.tet(), g.clusters(), xtilde, xi.p, g.ws_ccolor_off.p, g.ws_color_off.p,
+                                              g.ncolors, g.num_elements(), float(1.0 / dt2), mode == kApplyExplicit,
+                                              float(ws_scale), g.ws_disp.p, g.ws_csr_off.p, g.ws_csr.p, V);
+                    }, 2);
+                else
+                    timed(SBS_KERNEL_WARM_START, g.count, [&] { launch_group_colors(g, dt, mode, xt.p); }, g.ncolors);
                 warmed = true;
             }
         }
@@ -720,6 +738,30 @@ void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
     if (!xt.p) xt.alloc(V);
     for (auto& g : groups) {
         if (g.kind != 1) continue;
+        if (g.hex && !g.ws_csr.p) {
+            // particle -> (cluster, slot) list in cluster execution order (a counting sort)
+            uint32_t const nc = g.num_elements();
+            std::vector<int4> v0(nc), v1(nc);
+            hip_check(hipMemcpy(v0.data(), g.cl_v0.p, nc * sizeof(int4), hipMemcpyDeviceToHost), "read clusters");
+            hip_check(hipMemcpy(v1.data(), g.cl_v1.p, nc * sizeof(int4), hipMemcpyDeviceToHost), "read clusters");
+            std::vector<uint32_t> off(std::size_t(V) + 1, 0), ent(std::size_t(nc) * 8);
+            auto slot = [&](uint32_t c, int k) {
+                int4 const& a = k < 4 ? v0[c] : v1[c];
+                int const j = k & 3;
+                return j == 0 ? a.x : (j == 1 ? a.y : (j == 2 ? a.z : a.w));
+            };
+            for (uint32_t c = 0; c < nc; ++c)
+                for (int k = 0; k < 8; ++k) off[std::size_t(slot(c, k)) + 1]++;
+            for (std::size_t p = 0; p < V; ++p) off[p + 1] += off[p];
+            std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+            for (uint32_t c = 0; c < nc; ++c)
+                for (int k = 0; k < 8; ++k) ent[fill[slot(c, k)]++] = c * 8 + uint32_t(k);
+            g.ws_csr_off.upload(off.data(), off.size(), stream);
+            g.ws_csr.upload(ent.data(), ent.size(), stream);
+            g.ws_ccolor_off.upload(g.cluster_color_offsets.data(), g.cluster_color_offsets.size(), stream);
+            g.ws_color_off.upload(g.color_offsets.data(), g.color_offsets.size(), stream);
+            g.ws_disp.alloc(std::size_t(nc) * 8);
+        }
         if (!g.ws_hist.p && g.count) {
             g.ws_hist.alloc(g.count);
             hip_check(hipMemsetAsync(g.ws_hist.p, 0, g.count * sizeof(float2), stream), "clear history");

@@ -89,7 +89,7 @@ def train_warm_start(sim, scene, frames=6, iterations=24, scale=0.5, max_rows=4_
     return net
 
 
-def tune_warm_start(sim, scene, net, target, frames=8, scales=(0.5, 0.75, 1.0), iterations=(2, 3, 4, 5, 6, 8),
+def tune_warm_start(sim, scene, net, target, frames=8, scales=(0.3, 0.4, 0.5, 0.6, 0.75), iterations=(2, 3, 4, 5, 6, 7),
                     modes=("learned", "previous"), log=print):
     """For each mode, the cheapest (iterations, scale) whose mean frame residual from the current
     state is <= target. Returns {mode: {"iterations", "scale", "residual"} or None} and leaves the
