@@ -140,9 +140,18 @@ def main():
 
     learned = None
     if not args.no_learned:
-        net = autotune.train_warm_start(sim, scene, log=log)
-        found, trials = autotune.tune_warm_start(sim, scene, net, target, frames=eval_frames, log=log)
-        learned = {"found": found, "trials": trials}
+        # the previous-multiplier start first; the network then learns, from that configuration's own
+        # trajectory (or a 24-iteration one when it never reaches the target), the multipliers the
+        # iterations end with, and is tuned the same way
+        found, trials = autotune.tune_warm_start(sim, scene, None, target, frames=eval_frames, modes=("previous",),
+                                                 log=log)
+        prev = found["previous"]
+        cap = (prev["iterations"], prev["scale"]) if prev else (24, 0.5)
+        net = autotune.train_warm_start(sim, scene, iterations=cap[0], scale=cap[1], log=log)
+        found_l, trials_l = autotune.tune_warm_start(sim, scene, net, target, frames=eval_frames, modes=("learned",),
+                                                     log=log)
+        found.update(found_l)
+        learned = {"found": found, "trials": trials + trials_l, "capture": {"iterations": cap[0], "scale": cap[1]}}
         sim.restore()
 
     log(f"timing {args.steps} frames, plain")

@@ -64,9 +64,9 @@ def tune_launch(sim, scene, frames=5, blocks=(128, 256), cluster_blocks=(64, 128
 
 def train_warm_start(sim, scene, frames=6, iterations=24, scale=0.5, max_rows=4_000_000, steps=1500, batch=16384,
                      lr=1e-3, seed=0, log=print):
-    """Captures training rows from a well-converged trajectory (previous-multiplier warm start at
-    `scale`, `iterations` per substep) starting at the current state, trains a 12-input, 2-output
-    network on the device and restores the state. Returns the network."""
+    """Captures training rows from a previous-multiplier warm-started trajectory (`scale`,
+    `iterations` per substep) starting at the current state, trains a 12-input, 2-output network on
+    the device and restores the state. Returns the network."""
     rows_per_substep = sum(len(g.conn) for g in scene.groups if g.kind == "neohookean")
     stride = max(1, -(-rows_per_substep * scene.substeps * frames // max_rows))
     net = Mlp(12, 2, seed=seed)
