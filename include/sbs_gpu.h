@@ -82,6 +82,11 @@ int sbs_gpu_add_body(sbs_gpu* sim, const double* x0, const double* mass, uint32_
  * that contact_handler_t uses (src/physics/xpbd/contact_handler.cpp:39-40). */
 int sbs_gpu_set_collision_surface(sbs_gpu* sim, uint32_t body, const uint32_t* surface_to_body, uint32_t n);
 
+/* Boundary triangles of a body (3 indices each into its collision surface, in the order of
+ * tetrahedral_mesh_boundary_t::triangles, as sbs_mesh_boundary returns them) for the per-frame
+ * surface refresh. */
+int sbs_gpu_set_surface_triangles(sbs_gpu* sim, uint32_t body, const uint32_t* triangles, uint32_t n);
+
 /* n green_constraint_t on body-local tets (4 indices each), as main.cpp:37-54 adds them
  * (src/physics/xpbd/green_constraint.cpp:11-47). */
 int sbs_gpu_add_green_constraints(sbs_gpu* sim, uint32_t body, const uint32_t* tets, uint32_t n, double alpha,
@@ -116,6 +121,16 @@ int sbs_gpu_add_plane(sbs_gpu* sim, const double normal[3], const double point[3
 
 /* Analytic sphere obstacle, sd = |p - c| - r (an analytic sdf_model_t, sdf_model.cpp:149-153). */
 int sbs_gpu_add_sphere(sbs_gpu* sim, const double center[3], double radius);
+
+/* 1: every frame ends with tetrahedral_body_t::update_visual_model on the device
+ * (src/physics/tetrahedral_body.cpp:157-165, timestep.cpp:61-66): the surface normals accumulate
+ * the face normals (p2 - p1) x (p3 - p1) of their triangles in triangle order and are normalized;
+ * as in tetrahedral_mesh_boundary_t::compute_normals (src/physics/tetrahedral_mesh_boundary.cpp:122-145)
+ * the accumulator keeps the previous frame's normal. Default 0. */
+int sbs_gpu_set_surface_refresh(sbs_gpu* sim, int enable);
+/* Surface vertex positions (the committed particle positions) and normals of a body, 3 doubles per
+ * surface vertex each; either may be NULL. */
+int sbs_gpu_read_surface(sbs_gpu* sim, uint32_t body, double* positions, double* normals);
 
 /* Contact broad phase. The reference traverses a KD-tree of the surface points against every SDF's
  * englobing volume (src/physics/collision/bvh_model.cpp:30-100); here each surface vertex tests

@@ -153,6 +153,12 @@ void launch_ws_features(hipStream_t s, tet_group_dev g, const pos4* xi, float2* 
 void launch_ws_capture(hipStream_t s, const float* raw, const float* lambda, uint32_t stride, uint32_t offset,
                        float* ds_x, float* ds_t, uint64_t base, uint32_t rows);
 
+// Surface refresh (surface.hip): for every surface vertex s < S (particle surf[s]), normal[s] +=
+// the face normals of its triangles tris[inc[off[s] .. off[s+1])] (surface vertex indices, in
+// triangle order), then normal[s] is normalized; positions are read from x.
+void launch_surface_refresh(hipStream_t s, const uint32_t* surf, const pos4* x, const uint4* tris, const uint32_t* off,
+                            const uint32_t* inc, double4* normal, uint32_t S);
+
 // Fills a buffer with a 32-bit value (graph-safe replacement of a memset).
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t value, uint32_t n);
 

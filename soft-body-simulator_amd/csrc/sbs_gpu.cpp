@@ -35,6 +35,7 @@ struct body_t {
     uint32_t offset = 0, n = 0;
     std::vector<double> x0, mass;
     std::vector<uint32_t> surface;
+    std::vector<uint32_t> triangles;  // boundary triangles, 3 surface vertex indices each
 };
 
 struct group_t {
@@ -107,6 +108,12 @@ struct sbs_gpu {
 
     dev_buf<uint32_t> surf;
     uint32_t S = 0, slots = 0;
+    // surface refresh: triangles (global surface indices), vertex -> triangle lists, normals
+    bool surface_refresh = false;
+    uint32_t ntris = 0;
+    dev_buf<uint4> stris;
+    dev_buf<uint32_t> stri_off, stri_inc;
+    dev_buf<double4> snormal;
     dev_buf<pos4> cq;
     dev_buf<float4> cn;
     dev_buf<float> clam;
@@ -147,6 +154,7 @@ struct sbs_gpu {
     uint32_t cap_stride = 0;
     dev_buf<pos4> snap_xn;
     dev_buf<float4> snap_vel;
+    dev_buf<double4> snap_normal;
     bool have_snapshot = false;
     double fb_limit = 0.0;
     uint32_t fb_iterations = 0;  // 0: the step's own iteration count
@@ -524,9 +532,39 @@ void sbs_gpu::commit(uint32_t recolor_passes)
     for (auto const& b : bodies)
         for (uint32_t s : b.surface) sv.push_back(b.offset + s);
     S = uint32_t(sv.size());
+    {  // boundary triangles of all bodies in global surface numbering, and each vertex's triangles
+        std::vector<uint4> tr;
+        uint32_t base = 0;
+        for (auto const& b : bodies) {
+            for (std::size_t t = 0; t + 2 < b.triangles.size(); t += 3)
+                tr.push_back(make_uint4(base + b.triangles[t], base + b.triangles[t + 1], base + b.triangles[t + 2], 0));
+            base += uint32_t(b.surface.size());
+        }
+        ntris = uint32_t(tr.size());
+        if (ntris) {
+            std::vector<uint32_t> off(std::size_t(S) + 1, 0), inc(3 * std::size_t(ntris));
+            for (auto const& t : tr) {
+                off[t.x + 1]++;
+                off[t.y + 1]++;
+                off[t.z + 1]++;
+            }
+            for (std::size_t i = 0; i < S; ++i) off[i + 1] += off[i];
+            std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+            for (uint32_t j = 0; j < ntris; ++j) {
+                inc[fill[tr[j].x]++] = j;
+                inc[fill[tr[j].y]++] = j;
+                inc[fill[tr[j].z]++] = j;
+            }
+            stris.upload(tr.data(), tr.size(), stream);
+            stri_off.upload(off.data(), off.size(), stream);
+            stri_inc.upload(inc.data(), inc.size(), stream);
+            snormal.alloc(S);
+            hip_check(hipMemsetAsync(snormal.p, 0, S * sizeof(double4), stream), "clear normals");
+        }
+    }
     slots = uint32_t(std::min<std::size_t>(sdfs.size(), kMaxContactSlots));
+    if (S) surf.upload(sv.data(), S, stream);
     if (S && slots) {
-        surf.upload(sv.data(), S, stream);
         cq.alloc(std::size_t(S) * slots);
         cn.alloc(std::size_t(S) * slots);
         clam.alloc(std::size_t(S) * slots);
@@ -731,6 +769,8 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
         }
     }
     timed(SBS_KERNEL_COMMIT, V, [&] { launch_commit(stream, xi.p, xn.p, vel.p, V, float(dt), block); });
+    if (surface_refresh && ntris)  // timestep.cpp:61-66
+        launch_surface_refresh(stream, surf.p, xn.p, stris.p, stri_off.p, stri_inc.p, snormal.p, S);
 }
 
 void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
@@ -779,6 +819,11 @@ void sbs_gpu::snapshot()
     }
     hip_check(hipMemcpyAsync(snap_xn.p, xn.p, V * sizeof(pos4), hipMemcpyDeviceToDevice, stream), "snapshot");
     hip_check(hipMemcpyAsync(snap_vel.p, vel.p, V * sizeof(float4), hipMemcpyDeviceToDevice, stream), "snapshot");
+    if (snormal.p) {
+        if (!snap_normal.p) snap_normal.alloc(S);
+        hip_check(hipMemcpyAsync(snap_normal.p, snormal.p, S * sizeof(double4), hipMemcpyDeviceToDevice, stream),
+                  "snapshot");
+    }
     for (auto& g : groups) {
         if (!g.lambda.n) continue;
         if (!g.snap_lambda.p) g.snap_lambda.alloc(g.lambda.n);
@@ -799,6 +844,9 @@ void sbs_gpu::restore()
     hip_check(hipMemcpyAsync(xn.p, snap_xn.p, V * sizeof(pos4), hipMemcpyDeviceToDevice, stream), "restore");
     hip_check(hipMemcpyAsync(xi.p, snap_xn.p, V * sizeof(pos4), hipMemcpyDeviceToDevice, stream), "restore");
     hip_check(hipMemcpyAsync(vel.p, snap_vel.p, V * sizeof(float4), hipMemcpyDeviceToDevice, stream), "restore");
+    if (snap_normal.p)
+        hip_check(hipMemcpyAsync(snormal.p, snap_normal.p, S * sizeof(double4), hipMemcpyDeviceToDevice, stream),
+                  "restore");
     for (auto& g : groups) {
         if (g.snap_lambda.p)
             hip_check(hipMemcpyAsync(g.lambda.p, g.snap_lambda.p, g.lambda.n * sizeof(float), hipMemcpyDeviceToDevice,
@@ -855,6 +903,7 @@ void sbs_gpu::run_frame(double dt, uint32_t iterations, uint32_t substeps, int w
         key[10] = warm;
         key[11] = double(reinterpret_cast<uintptr_t>(warm == SBS_WARM_START_LEARNED ? ws_net : nullptr));
         key[12] = ws_scale;
+        key[13] = surface_refresh;
         if (!graph_exec || key != graph_key) {
             if (graph_exec) {
                 // the previous graph may still be running on the stream: let it finish first
@@ -978,6 +1027,20 @@ int sbs_gpu_set_collision_surface(sbs_gpu* sim, uint32_t body, const uint32_t* s
         require(surface_to_body != nullptr || n == 0, "surface map is null");
         for (uint32_t i = 0; i < n; ++i) require(surface_to_body[i] < sim->bodies[body].n, "surface vertex out of range");
         sim->bodies[body].surface.assign(surface_to_body, surface_to_body + n);
+    });
+}
+
+int sbs_gpu_set_surface_triangles(sbs_gpu* sim, uint32_t body, const uint32_t* triangles, uint32_t n)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(!sim->committed, "scene already committed", SBS_ERR_STATE);
+        require(body < sim->bodies.size(), "body index out of range");
+        require(triangles != nullptr || n == 0, "triangles is null");
+        std::size_t const ns = sim->bodies[body].surface.size();
+        for (std::size_t i = 0; i < 3 * std::size_t(n); ++i)
+            require(triangles[i] < ns, "triangle vertex is not a surface vertex (set the collision surface first)");
+        sim->bodies[body].triangles.assign(triangles, triangles + 3 * std::size_t(n));
     });
 }
 
@@ -1342,6 +1405,52 @@ int sbs_util_sort_pairs(int device, const uint32_t* keys, const uint32_t* vals, 
             throw;
         }
         hip_check(hipStreamDestroy(s), "hipStreamDestroy");
+    });
+}
+
+int sbs_gpu_set_surface_refresh(sbs_gpu* sim, int enable)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        sim->surface_refresh = enable != 0;
+    });
+}
+
+int sbs_gpu_read_surface(sbs_gpu* sim, uint32_t body, double* positions, double* normals)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
+        require(body < sim->bodies.size(), "body index out of range");
+        uint32_t base = 0;
+        for (uint32_t b = 0; b < body; ++b) base += uint32_t(sim->bodies[b].surface.size());
+        body_t const& bd = sim->bodies[body];
+        std::size_t const n = bd.surface.size();
+        if (n == 0) return;
+        if (positions) {
+            std::vector<pos4> x(bd.n);
+            hip_check(hipMemcpyAsync(x.data(), sim->xn.p + bd.offset, bd.n * sizeof(pos4), hipMemcpyDeviceToHost,
+                                     sim->stream), "read surface");
+            hip_check(hipStreamSynchronize(sim->stream), "read surface");
+            for (std::size_t i = 0; i < n; ++i) {
+                pos4 const& p = x[bd.surface[i]];
+                positions[3 * i] = p.x;
+                positions[3 * i + 1] = p.y;
+                positions[3 * i + 2] = p.z;
+            }
+        }
+        if (normals) {
+            require(sim->snormal.p != nullptr, "the body has no surface triangles", SBS_ERR_STATE);
+            std::vector<double4> nv(n);
+            hip_check(hipMemcpyAsync(nv.data(), sim->snormal.p + base, n * sizeof(double4), hipMemcpyDeviceToHost,
+                                     sim->stream), "read normals");
+            hip_check(hipStreamSynchronize(sim->stream), "read normals");
+            for (std::size_t i = 0; i < n; ++i) {
+                normals[3 * i] = nv[i].x;
+                normals[3 * i + 1] = nv[i].y;
+                normals[3 * i + 2] = nv[i].z;
+            }
+        }
     });
 }
 

@@ -96,6 +96,9 @@ _SIGS = {
     "sbs_mlp_train": [_p, _u32, _u32, _f, _u64, _pd],
     "sbs_gpu_set_warm_start": [_p, _i, _p],
     "sbs_gpu_set_contact_hash": [_p, _i, _d],
+    "sbs_gpu_set_surface_triangles": [_p, _u32, _pu, _u32],
+    "sbs_gpu_set_surface_refresh": [_p, _i],
+    "sbs_gpu_read_surface": [_p, _u32, _pd, _pd],
     "sbs_gpu_read_contact_hash": [_p, _pd, _pu, _pu, _pu, _pu, _u32, _pu, _pu, _pu],
     "sbs_gpu_contact_overflow": [_p, _pu64],
     "sbs_util_exclusive_scan": [_i, _pu, _u64, _pu],
@@ -375,6 +378,22 @@ class Simulation:
     def set_surface(self, body, s2t):
         s2t = u32(s2t)
         check(self._L.sbs_gpu_set_collision_surface(self.h, body, _up(s2t), len(s2t)))
+        self._surface_sizes = getattr(self, "_surface_sizes", {})
+        self._surface_sizes[body] = len(s2t)
+
+    def set_surface_triangles(self, body, triangles):
+        t = u32(triangles).reshape(-1)
+        check(self._L.sbs_gpu_set_surface_triangles(self.h, body, _up(t), len(t) // 3))
+
+    def set_surface_refresh(self, enable=True):
+        check(self._L.sbs_gpu_set_surface_refresh(self.h, int(enable)))
+
+    def read_surface(self, body):
+        """(positions, normals) of the body's surface vertices, (n, 3) each."""
+        n = self._surface_sizes.get(body, 0)
+        x, nv = np.zeros(3 * n), np.zeros(3 * n)
+        check(self._L.sbs_gpu_read_surface(self.h, body, _dp(x), _dp(nv)))
+        return x.reshape(-1, 3), nv.reshape(-1, 3)
 
     def _group(self, code, out, n):
         check(code)

@@ -17,6 +17,7 @@ class Body:
     tets: np.ndarray        # (t, 4) body-local tets
     mass: np.ndarray = None  # (n,) or None for unit mass
     surface: np.ndarray = None  # surface vertex -> body vertex
+    triangles: np.ndarray = None  # (f, 3) boundary triangles in surface vertex indices
 
 
 @dataclass
@@ -63,6 +64,8 @@ class Scene:
             bi = sim.add_body(b.x0, b.mass)
             if b.surface is not None:
                 sim.set_surface(bi, b.surface)
+                if b.triangles is not None and hasattr(sim, "set_surface_triangles"):
+                    sim.set_surface_triangles(bi, b.triangles)
         for gi, g in enumerate(self.groups):
             if np.ndim(g.body) > 0:
                 if g.kind == "green":
@@ -113,9 +116,9 @@ def reference_demo():
     pos, tets = _lib.bar5(4, 4, 12)
     R = _rotation((0.0, 1.0, 0.2), 3.14159 / 2.0)
     x0 = (R @ (pos * np.array([1.0, 0.8, 2.0])).T).T + np.array([-10.0, 5.0, -1.0])
-    s2t, _ = _lib.boundary(len(pos), tets)
+    s2t, tris = _lib.boundary(len(pos), tets)
     sc = Scene("reference_demo", dt=0.016, iterations=5, substeps=1, gravity_is_force=True)
-    sc.bodies.append(Body(x0=x0, tets=tets, surface=s2t))
+    sc.bodies.append(Body(x0=x0, tets=tets, surface=s2t, triangles=tris))
     sc.groups.append(Group("green", 0, tets, alpha=1e-4, beta=0.0, young=1e6, poisson=0.30))
     sc.planes.append(((0.0, 1.0, 0.0), (0.0, 0.0, 0.0), (-20.0, -5.0, -20.0), (20.0, 5.0, 20.0)))
     return sc
@@ -151,10 +154,10 @@ def cantilever(nx, ny, nz, edge=0.1, model="green", young=1e6, poisson=0.45, den
     pos, tets = _lib.box6(nx, ny, nz, edge)
     mass = _lib.lumped_mass(pos, tets, density)
     mass[pos[:, 0] <= pos[:, 0].min()] = 0.0
-    s2t, _ = _lib.boundary(len(pos), tets)
+    s2t, tris = _lib.boundary(len(pos), tets)
     sc = Scene(name or f"cantilever_{nx}x{ny}x{nz}_{model}", dt=dt, iterations=iterations, substeps=substeps,
                gravity_is_force=False, coloring=coloring)
-    sc.bodies.append(Body(x0=pos, tets=tets, mass=mass, surface=s2t))
+    sc.bodies.append(Body(x0=pos, tets=tets, mass=mass, surface=s2t, triangles=tris))
     sc.groups.append(Group(model, 0, tets, alpha=alpha, beta=beta, young=young, poisson=poisson,
                            clusters=hex_clusters(len(tets)) if cluster else None))
     sc.planes.append(((0.0, 1.0, 0.0), (0.0, ground_y, 0.0), (-1e30,) * 3, (1e30,) * 3))
@@ -233,8 +236,8 @@ def batch(num_bodies=512, tet_range=(2e3, 2e5), edge=0.05, seed=0xB0D1E5, densit
         drop = rng.uniform(0.5, 3.0)
         x0 = pos + np.array([gx * extent, drop - pos[:, 1].min(), gz * extent])
         mass = _lib.lumped_mass(x0, tets, density)
-        s2t, _ = _lib.boundary(len(x0), tets)
-        sc.bodies.append(Body(x0=x0, tets=tets, mass=mass, surface=s2t))
+        s2t, tris = _lib.boundary(len(x0), tets)
+        sc.bodies.append(Body(x0=x0, tets=tets, mass=mass, surface=s2t, triangles=tris))
         bodies.append(np.full(len(tets), b, dtype=np.uint32))
         bconn.append(tets)
         E = float(np.exp(rng.uniform(np.log(1e4), np.log(1e6))))
