@@ -329,6 +329,13 @@ int sbs_mesh_box6(uint32_t nx, uint32_t ny, uint32_t nz, double edge, const doub
 int sbs_mesh_boundary(uint32_t num_vertices, const uint32_t* tets, uint32_t num_tets, uint32_t* surface_to_tet,
                       uint32_t* triangles, uint32_t* num_surface_vertices, uint32_t* num_triangles);
 
+/* The same boundary surface computed on GPU `device` (radix sorts and prefix sums over the 4T
+ * faces; at most 2^21 vertices). Outputs are host arrays sized by a first call with NULL outputs,
+ * or at most 3 * 4 * num_tets triangle indices and num_vertices surface vertices. */
+int sbs_gpu_extract_boundary(int device, uint32_t num_vertices, const uint32_t* tets, uint32_t num_tets,
+                             uint32_t* surface_to_tet, uint32_t* triangles, uint32_t* num_surface_vertices,
+                             uint32_t* num_triangles);
+
 /* Ellipsoid of radii (rx, ry, rz) centred on the origin: the cells of edge `edge` whose centres lie
  * inside it, each split like sbs_mesh_box6; vertices are the used lattice corners in lattice order.
  * Cells (6 consecutive tets each) come grouped by lattice parity class, so that SBS_COLORING_INDEX

@@ -103,6 +103,7 @@ _SIGS = {
     "sbs_gpu_contact_overflow": [_p, _pu64],
     "sbs_util_exclusive_scan": [_i, _pu, _u64, _pu],
     "sbs_util_sort_pairs": [_i, _pu, _pu, _u64, _i, _pu, _pu],
+    "sbs_gpu_extract_boundary": [_i, _u32, _pu, _u32, _pu, _pu, _pu, _pu],
     "sbs_gpu_capture": [_p, _p, _u32],
     "sbs_gpu_set_warm_start_scale": [_p, _d],
     "sbs_gpu_snapshot": [_p],
@@ -231,6 +232,18 @@ def exclusive_scan(values, device=0):
     out = np.zeros(len(a) + 1, dtype=np.uint32)
     check(lib().sbs_util_exclusive_scan(device, _up(a), len(a), _up(out)))
     return out
+
+
+def extract_boundary(nv, tets, device=0):
+    """Boundary surface of a tet mesh computed on the device; the same result as boundary()."""
+    tets = u32(tets).reshape(-1)
+    nt = len(tets) // 4
+    ns, ntr = _u32(), _u32()
+    s2t = np.zeros(max(nv, 1), dtype=np.uint32)
+    tris = np.zeros(max(12 * nt, 1), dtype=np.uint32)
+    check(lib().sbs_gpu_extract_boundary(device, nv, _up(tets), nt, _up(s2t), _up(tris), ctypes.byref(ns),
+                                         ctypes.byref(ntr)))
+    return s2t[:ns.value].copy(), tris[:3 * ntr.value].reshape(-1, 3).copy()
 
 
 def sort_pairs(keys, vals, bits=32, device=0):
