@@ -35,6 +35,8 @@ extern "C" {
 #define SBS_ERR_OUT_OF_MEMORY 4
 #define SBS_ERR_NO_DEVICE 5
 #define SBS_ERR_INTERNAL 6
+#define SBS_ERR_IO 7     /* a file could not be opened, read or written */
+#define SBS_ERR_FORMAT 8 /* a file is not a well-formed PLY file */
 
 typedef struct sbs_gpu sbs_gpu;
 
@@ -131,6 +133,11 @@ int sbs_gpu_set_surface_refresh(sbs_gpu* sim, int enable);
 /* Surface vertex positions (the committed particle positions) and normals of a body, 3 doubles per
  * surface vertex each; either may be NULL. */
 int sbs_gpu_read_surface(sbs_gpu* sim, uint32_t body, double* positions, double* normals);
+/* Frame dump: the body's surface mesh (surface positions, its boundary triangles and, with the
+ * surface refresh on, the accumulated normals) as a PLY triangle mesh (sbs_ply_write). The
+ * reference's surface mesh, tetrahedral_mesh_boundary_t, has no file output; this writes the same
+ * geometry record its renderer draws from (src/physics/tetrahedral_body.cpp:157-165). */
+int sbs_gpu_write_surface_ply(sbs_gpu* sim, uint32_t body, const char* path, int format);
 
 /* Contact broad phase. The reference traverses a KD-tree of the surface points against every SDF's
  * englobing volume (src/physics/collision/bvh_model.cpp:30-100); here each surface vertex tests
@@ -335,6 +342,46 @@ int sbs_mesh_boundary(uint32_t num_vertices, const uint32_t* tets, uint32_t num_
 int sbs_gpu_extract_boundary(int device, uint32_t num_vertices, const uint32_t* tets, uint32_t num_tets,
                              uint32_t* surface_to_tet, uint32_t* triangles, uint32_t* num_surface_vertices,
                              uint32_t* num_triangles);
+
+/* ---------------------------------------------------------------- PLY files
+ * The geometry record of src/io/ply.cpp (common::geometry_t, include/sbs/common/geometry.h:10-30):
+ * 3 position floats per vertex, 3 or 4 indices per triangle or tet, optional per-vertex normals
+ * (3 floats), uvs (2 floats) and colors (3 bytes). Arrays a read returns are owned by the library
+ * and freed by sbs_geometry_release. */
+#define SBS_GEOMETRY_UNSET (-1) /* the file has neither a face nor a tet element */
+#define SBS_GEOMETRY_TRIANGLE 0
+#define SBS_GEOMETRY_TETRAHEDRON 1
+#define SBS_PLY_ASCII 0 /* io::ply_format_t, include/sbs/io/ply.h:25 */
+#define SBS_PLY_BINARY_LITTLE_ENDIAN 1
+#define SBS_PLY_BINARY_BIG_ENDIAN 2
+
+typedef struct sbs_geometry {
+    float* positions;
+    uint64_t num_positions; /* floats */
+    int32_t* indices;
+    uint64_t num_indices;
+    float* normals;
+    uint64_t num_normals;
+    float* uvs;
+    uint64_t num_uvs;
+    uint8_t* colors;
+    uint64_t num_colors;
+    int32_t geometry_type; /* SBS_GEOMETRY_* */
+} sbs_geometry;
+
+/* io::read_ply(path) (src/io/ply.cpp:303-319, 321-775): ASCII, little- and big-endian binary.
+ * The path must end in ".ply". Fails with SBS_ERR_IO or SBS_ERR_FORMAT where read_ply returns no
+ * value. */
+int sbs_ply_read(const char* path, sbs_geometry* out);
+/* io::read_ply(std::istream&) over a file image in memory. */
+int sbs_ply_read_memory(const void* data, uint64_t size, sbs_geometry* out);
+/* Frees the arrays of a geometry returned by a read and clears the record. */
+int sbs_geometry_release(sbs_geometry* geometry);
+/* io::write_ply(path, geometry, format) (src/io/ply.cpp:25-39, 41-296). */
+int sbs_ply_write(const char* path, const sbs_geometry* geometry, int format);
+/* io::write_ply(std::ostream&, ...) into a caller buffer: *size gets the file size; with a NULL
+ * buffer nothing is written. */
+int sbs_ply_write_memory(const sbs_geometry* geometry, int format, void* buffer, uint64_t capacity, uint64_t* size);
 
 /* Ellipsoid of radii (rx, ry, rz) centred on the origin: the cells of edge `edge` whose centres lie
  * inside it, each split like sbs_mesh_box6; vertices are the used lattice corners in lattice order.

@@ -1416,41 +1416,77 @@ int sbs_gpu_set_surface_refresh(sbs_gpu* sim, int enable)
     });
 }
 
+namespace {
+
+void read_surface(sbs_gpu* sim, uint32_t body, double* positions, double* normals)
+{
+    require(sim != nullptr, "sim is null");
+    require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
+    require(body < sim->bodies.size(), "body index out of range");
+    uint32_t base = 0;
+    for (uint32_t b = 0; b < body; ++b) base += uint32_t(sim->bodies[b].surface.size());
+    body_t const& bd = sim->bodies[body];
+    std::size_t const n = bd.surface.size();
+    if (n == 0) return;
+    if (positions) {
+        std::vector<pos4> x(bd.n);
+        hip_check(hipMemcpyAsync(x.data(), sim->xn.p + bd.offset, bd.n * sizeof(pos4), hipMemcpyDeviceToHost,
+                                 sim->stream), "read surface");
+        hip_check(hipStreamSynchronize(sim->stream), "read surface");
+        for (std::size_t i = 0; i < n; ++i) {
+            pos4 const& p = x[bd.surface[i]];
+            positions[3 * i] = p.x;
+            positions[3 * i + 1] = p.y;
+            positions[3 * i + 2] = p.z;
+        }
+    }
+    if (normals) {
+        require(sim->snormal.p != nullptr, "the body has no surface triangles", SBS_ERR_STATE);
+        std::vector<double4> nv(n);
+        hip_check(hipMemcpyAsync(nv.data(), sim->snormal.p + base, n * sizeof(double4), hipMemcpyDeviceToHost,
+                                 sim->stream), "read normals");
+        hip_check(hipStreamSynchronize(sim->stream), "read normals");
+        for (std::size_t i = 0; i < n; ++i) {
+            normals[3 * i] = nv[i].x;
+            normals[3 * i + 1] = nv[i].y;
+            normals[3 * i + 2] = nv[i].z;
+        }
+    }
+}
+
+}  // namespace
+
 int sbs_gpu_read_surface(sbs_gpu* sim, uint32_t body, double* positions, double* normals)
 {
+    return guarded([&] { read_surface(sim, body, positions, normals); });
+}
+
+int sbs_gpu_write_surface_ply(sbs_gpu* sim, uint32_t body, const char* path, int format)
+{
     return guarded([&] {
-        require(sim != nullptr, "sim is null");
-        require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
-        require(body < sim->bodies.size(), "body index out of range");
-        uint32_t base = 0;
-        for (uint32_t b = 0; b < body; ++b) base += uint32_t(sim->bodies[b].surface.size());
+        require(path != nullptr, "path is null");
+        read_surface(sim, body, nullptr, nullptr);  // argument and state checks
         body_t const& bd = sim->bodies[body];
+        require(!bd.triangles.empty(), "the body has no surface triangles (sbs_gpu_set_surface_triangles)",
+                SBS_ERR_STATE);
         std::size_t const n = bd.surface.size();
-        if (n == 0) return;
-        if (positions) {
-            std::vector<pos4> x(bd.n);
-            hip_check(hipMemcpyAsync(x.data(), sim->xn.p + bd.offset, bd.n * sizeof(pos4), hipMemcpyDeviceToHost,
-                                     sim->stream), "read surface");
-            hip_check(hipStreamSynchronize(sim->stream), "read surface");
-            for (std::size_t i = 0; i < n; ++i) {
-                pos4 const& p = x[bd.surface[i]];
-                positions[3 * i] = p.x;
-                positions[3 * i + 1] = p.y;
-                positions[3 * i + 2] = p.z;
-            }
-        }
-        if (normals) {
-            require(sim->snormal.p != nullptr, "the body has no surface triangles", SBS_ERR_STATE);
-            std::vector<double4> nv(n);
-            hip_check(hipMemcpyAsync(nv.data(), sim->snormal.p + base, n * sizeof(double4), hipMemcpyDeviceToHost,
-                                     sim->stream), "read normals");
-            hip_check(hipStreamSynchronize(sim->stream), "read normals");
-            for (std::size_t i = 0; i < n; ++i) {
-                normals[3 * i] = nv[i].x;
-                normals[3 * i + 1] = nv[i].y;
-                normals[3 * i + 2] = nv[i].z;
-            }
-        }
+        std::vector<double> x(3 * n), nv;
+        bool const with_normals = sim->surface_refresh;
+        if (with_normals) nv.resize(3 * n);
+        read_surface(sim, body, x.data(), with_normals ? nv.data() : nullptr);
+        // the surface mesh holds float positions and normals (common::geometry_t)
+        std::vector<float> xf(x.begin(), x.end()), nf(nv.begin(), nv.end());
+        std::vector<int32_t> tri(bd.triangles.begin(), bd.triangles.end());
+        sbs_geometry g{};
+        g.positions = xf.data();
+        g.num_positions = xf.size();
+        g.indices = tri.data();
+        g.num_indices = tri.size();
+        g.normals = nf.empty() ? nullptr : nf.data();
+        g.num_normals = nf.size();
+        g.geometry_type = SBS_GEOMETRY_TRIANGLE;
+        int const code = sbs_ply_write(path, &g, format);
+        if (code != SBS_OK) throw api_error(code, sbs_gpu_last_error());
     });
 }
 

@@ -1,7 +1,7 @@
 """Python plumbing over the MI355X XPBD library (include/sbs_gpu.h, libsbs_gpu.so)."""
 from ._lib import (KERNEL_NAMES, LIB_PATH, WARM_START, Mlp, SbsError, Simulation, exclusive_scan, extract_boundary, sort_pairs, bar5, boundary, box6, ellipsoid6, header_functions, lib,
                    lumped_mass)
-from . import scenes
+from . import ply, scenes
 
 __all__ = ["KERNEL_NAMES", "LIB_PATH", "WARM_START", "Mlp", "SbsError", "Simulation", "exclusive_scan", "extract_boundary", "sort_pairs", "bar5", "boundary", "box6", "ellipsoid6", "header_functions",
-           "lib", "lumped_mass", "scenes"]
+           "lib", "lumped_mass", "ply", "scenes"]

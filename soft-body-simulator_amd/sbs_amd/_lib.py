@@ -47,7 +47,24 @@ class XpbdParams(ctypes.Structure):
     ]
 
 
+class GeometryRecord(ctypes.Structure):
+    """struct sbs_geometry of include/sbs_gpu.h."""
+    _fields_ = [("positions", ctypes.POINTER(ctypes.c_float)), ("num_positions", ctypes.c_uint64),
+                ("indices", ctypes.POINTER(ctypes.c_int32)), ("num_indices", ctypes.c_uint64),
+                ("normals", ctypes.POINTER(ctypes.c_float)), ("num_normals", ctypes.c_uint64),
+                ("uvs", ctypes.POINTER(ctypes.c_float)), ("num_uvs", ctypes.c_uint64),
+                ("colors", ctypes.POINTER(ctypes.c_uint8)), ("num_colors", ctypes.c_uint64),
+                ("geometry_type", ctypes.c_int32)]
+
+
+_pgeo = ctypes.POINTER(GeometryRecord)
+
 _SIGS = {
+    "sbs_ply_read": [ctypes.c_char_p, _pgeo],
+    "sbs_ply_read_memory": [_p, _u64, _pgeo],
+    "sbs_geometry_release": [_pgeo],
+    "sbs_ply_write": [ctypes.c_char_p, _pgeo, _i],
+    "sbs_ply_write_memory": [_pgeo, _i, _p, _u64, _pu64],
     "sbs_gpu_default_params": [ctypes.POINTER(XpbdParams)],
     "sbs_gpu_create": [_i, ctypes.POINTER(_p)],
     "sbs_gpu_destroy": [_p],
@@ -99,6 +116,7 @@ _SIGS = {
     "sbs_gpu_set_surface_triangles": [_p, _u32, _pu, _u32],
     "sbs_gpu_set_surface_refresh": [_p, _i],
     "sbs_gpu_read_surface": [_p, _u32, _pd, _pd],
+    "sbs_gpu_write_surface_ply": [_p, _u32, ctypes.c_char_p, _i],
     "sbs_gpu_read_contact_hash": [_p, _pd, _pu, _pu, _pu, _pu, _u32, _pu, _pu, _pu],
     "sbs_gpu_contact_overflow": [_p, _pu64],
     "sbs_util_exclusive_scan": [_i, _pu, _u64, _pu],
@@ -407,6 +425,13 @@ class Simulation:
         x, nv = np.zeros(3 * n), np.zeros(3 * n)
         check(self._L.sbs_gpu_read_surface(self.h, body, _dp(x), _dp(nv)))
         return x.reshape(-1, 3), nv.reshape(-1, 3)
+
+    def write_surface_ply(self, body, path, format="ascii"):
+        """Frame dump of the body's surface mesh as a PLY triangle mesh (with the refreshed normals
+        when the surface refresh is on)."""
+        fmt = {"ascii": 0, "binary_little_endian": 1, "binary_big_endian": 2}[format] if isinstance(format, str) \
+            else int(format)
+        check(self._L.sbs_gpu_write_surface_ply(self.h, body, os.fsencode(path), fmt))
 
     def _group(self, code, out, n):
         check(code)

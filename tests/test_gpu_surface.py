@@ -3,7 +3,7 @@ against a NumPy restatement of the reference's loops on the device's own positio
 import numpy as np
 import pytest
 
-from sbs_amd import scenes
+from sbs_amd import ply, scenes
 
 pytestmark = pytest.mark.gpu
 
@@ -43,3 +43,27 @@ def test_surface_refresh_matches_reference_loops(graphs):
         n = reference_normals(n, x, b.surface, b.triangles)
         assert np.array_equal(nrm, n)
     assert np.allclose(np.linalg.norm(nrm, axis=1), 1.0)
+
+
+@pytest.mark.parametrize("fmt", ["ascii", "binary_little_endian", "binary_big_endian"])
+def test_surface_frame_dump(tmp_path, fmt):
+    sc = scenes.reference_demo()
+    print(f"surface dump {fmt}", flush=True)
+    dev = scenes.build_device(sc)
+    dev.set_surface_refresh(True)
+    for _ in range(3):
+        dev.step(sc.dt, sc.iterations, sc.substeps)
+    path = tmp_path / "surface.ply"
+    dev.write_surface_ply(0, path, fmt)
+    pos, nrm = dev.read_surface(0)
+    g = ply.read_ply(path, raise_errors=True)
+    assert g.is_triangle_mesh()
+    assert np.array_equal(g.indices.reshape(-1, 3), sc.bodies[0].triangles.astype(np.int32))
+    if fmt == "ascii":  # "%f" keeps 6 decimals
+        assert np.allclose(g.positions.reshape(-1, 3), pos, rtol=0, atol=5e-7 + 1e-6 * np.abs(pos).max())
+        assert np.allclose(g.normals.reshape(-1, 3), nrm, rtol=0, atol=1e-6)
+        expect = ply.write_ply_bytes(ply.triangle_geometry(pos, sc.bodies[0].triangles, nrm), "ascii")
+        assert open(path, "rb").read() == expect
+    else:
+        assert np.array_equal(g.positions.reshape(-1, 3), pos.astype(np.float32))
+        assert np.array_equal(g.normals.reshape(-1, 3), nrm.astype(np.float32))
