@@ -124,6 +124,28 @@ int sbs_gpu_add_plane(sbs_gpu* sim, const double normal[3], const double point[3
 /* Analytic sphere obstacle, sd = |p - c| - r (an analytic sdf_model_t, sdf_model.cpp:149-153). */
 int sbs_gpu_add_sphere(sbs_gpu* sim, const double center[3], double radius);
 
+/* Grid SDF obstacles: sdf_model_t over a Discregrid::CubicLagrangeDiscreteGrid (sdf_model.cpp:17,
+ * evaluate :66-75). The grid splits the box [domain_min, domain_max] into resolution[0] x
+ * resolution[1] x resolution[2] cells of 32-node cubic Lagrange elements (nodes at the cell corners
+ * and at 1/3 and 2/3 of the cell edges, shared between cells); sbs_sdf_grid_node_count and
+ * sbs_sdf_grid_node_positions give the node layout. Outside the domain the field is DBL_MAX.
+ * sbs_gpu_add_grid_sdf takes node values (environment_body_t(simulation, id, geometry, sdf_model),
+ * environment_body.cpp:80-88) and the englobing volume used by the contact broad phase. */
+int sbs_gpu_add_grid_sdf(sbs_gpu* sim, const double domain_min[3], const double domain_max[3],
+                         const uint32_t resolution[3], const double* nodes, const double vol_min[3],
+                         const double vol_max[3]);
+/* environment_body_t(simulation, id, geometry, domain, resolution) (environment_body.cpp:12-78): a
+ * triangle mesh obstacle. The domain is extended as the reference does (the box of the domain and
+ * the vertices, grown by 1e-3 of its diagonal once per vertex), the mesh's signed distance (closest
+ * point, angle-weighted pseudonormal sign) is computed at every node on the device, and the
+ * extended domain is the englobing volume. */
+int sbs_gpu_add_mesh_obstacle(sbs_gpu* sim, const double* positions, uint32_t num_vertices, const uint32_t* triangles,
+                              uint32_t num_triangles, const double domain_min[3], const double domain_max[3],
+                              const uint32_t resolution[3]);
+/* sdf_model_t::evaluate (sdf_model.cpp:66-75) of obstacle `sdf` at n points on the device: signed
+ * distances (n) and gradients (3 n). */
+int sbs_gpu_evaluate_sdf(sbs_gpu* sim, uint32_t sdf, const double* points, uint32_t n, double* sd, double* grad);
+
 /* 1: every frame ends with tetrahedral_body_t::update_visual_model on the device
  * (src/physics/tetrahedral_body.cpp:157-165, timestep.cpp:61-66): the surface normals accumulate
  * the face normals (p2 - p1) x (p3 - p1) of their triangles in triangle order and are normalized;
@@ -382,6 +404,20 @@ int sbs_ply_write(const char* path, const sbs_geometry* geometry, int format);
 /* io::write_ply(std::ostream&, ...) into a caller buffer: *size gets the file size; with a NULL
  * buffer nothing is written. */
 int sbs_ply_write_memory(const sbs_geometry* geometry, int format, void* buffer, uint64_t capacity, uint64_t* size);
+
+/* Grid SDF helpers (no simulation handle). Node count and node positions of a cubic Lagrange grid
+ * (Discregrid's node numbering: corners, then x, y and z edge nodes), environment_body_t's extended
+ * domain (environment_body.cpp:52-64), and the node values of a triangle mesh's signed distance
+ * computed on GPU `device` (what CubicLagrangeDiscreteGrid::addFunction stores for
+ * MeshDistance::signedDistanceCached, environment_body.cpp:66-73). */
+int sbs_sdf_grid_node_count(const uint32_t resolution[3], uint64_t* count);
+int sbs_sdf_grid_node_positions(const double domain_min[3], const double domain_max[3], const uint32_t resolution[3],
+                                double* positions);
+int sbs_sdf_environment_domain(const double* positions, uint32_t num_vertices, const double domain_min[3],
+                               const double domain_max[3], double out_min[3], double out_max[3]);
+int sbs_sdf_grid_from_mesh(int device, const double* positions, uint32_t num_vertices, const uint32_t* triangles,
+                           uint32_t num_triangles, const double domain_min[3], const double domain_max[3],
+                           const uint32_t resolution[3], double* nodes);
 
 /* Ellipsoid of radii (rx, ry, rz) centred on the origin: the cells of edge `edge` whose centres lie
  * inside it, each split like sbs_mesh_box6; vertices are the used lattice corners in lattice order.

@@ -150,6 +150,81 @@ void orc_add_sphere(void* h, const double c[3], double r)
     s->sdfs.push_back(f);
 }
 
+// Grid SDF obstacle: domain, resolution (cells per axis), node values, englobing volume.
+void orc_add_grid_sdf(void* h, const double lo[3], const double hi[3], const uint32_t res[3], const double* nodes,
+                      const double vmin[3], const double vmax[3])
+{
+    auto* s = static_cast<simulation_t*>(h);
+    sdf_t f;
+    f.kind = 2;
+    f.grid = std::make_shared<grid_sdf_t>();
+    f.grid->init(vec3{lo[0], lo[1], lo[2]}, vec3{hi[0], hi[1], hi[2]}, res);
+    std::copy(nodes, nodes + f.grid->nodes.size(), f.grid->nodes.begin());
+    f.vmin = {vmin[0], vmin[1], vmin[2]};
+    f.vmax = {vmax[0], vmax[1], vmax[2]};
+    s->sdfs.push_back(f);
+}
+
+// sdf k evaluated at n points: sd (n) and grad (3 n)
+void orc_evaluate_sdf(void* h, uint32_t k, const double* pts, uint32_t n, double* sd, double* grad)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    for (uint32_t i = 0; i < n; ++i) {
+        vec3 g;
+        s->sdfs[k].evaluate(vec3{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]}, &sd[i], &g);
+        grad[3 * i] = g.x;
+        grad[3 * i + 1] = g.y;
+        grad[3 * i + 2] = g.z;
+    }
+}
+
+uint64_t orc_grid_node_count(const uint32_t res[3])
+{
+    grid_sdf_t g;
+    for (int d = 0; d < 3; ++d) g.res[d] = res[d];
+    return g.node_count();
+}
+
+void orc_grid_node_positions(const double lo[3], const double hi[3], const uint32_t res[3], double* out)
+{
+    grid_sdf_t g;
+    g.init(vec3{lo[0], lo[1], lo[2]}, vec3{hi[0], hi[1], hi[2]}, res);
+    for (uint64_t l = 0; l < g.node_count(); ++l) {
+        vec3 const x = g.node_position(l);
+        out[3 * l] = x.x;
+        out[3 * l + 1] = x.y;
+        out[3 * l + 2] = x.z;
+    }
+}
+
+// Node values of a triangle mesh's signed distance (MeshDistance at every grid node).
+void orc_grid_from_mesh(const double* pos, uint32_t nv, const uint32_t* tris, uint32_t nt, const double lo[3],
+                        const double hi[3], const uint32_t res[3], double* nodes)
+{
+    grid_sdf_t g;
+    g.init(vec3{lo[0], lo[1], lo[2]}, vec3{hi[0], hi[1], hi[2]}, res);
+    mesh_distance_t const md(pos, nv, tris, nt);
+    for (uint64_t l = 0; l < g.node_count(); ++l) nodes[l] = md.signed_distance(g.node_position(l));
+}
+
+void orc_mesh_signed_distance(const double* pos, uint32_t nv, const uint32_t* tris, uint32_t nt, const double* pts,
+                              uint32_t n, double* out)
+{
+    mesh_distance_t const md(pos, nv, tris, nt);
+    for (uint32_t i = 0; i < n; ++i) out[i] = md.signed_distance(vec3{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]});
+}
+
+void orc_environment_domain(const double* pos, uint32_t nv, const double lo[3], const double hi[3], double out_lo[3],
+                            double out_hi[3])
+{
+    vec3 a, b;
+    environment_domain(pos, nv, vec3{lo[0], lo[1], lo[2]}, vec3{hi[0], hi[1], hi[2]}, &a, &b);
+    for (int d = 0; d < 3; ++d) {
+        out_lo[d] = a[d];
+        out_hi[d] = b[d];
+    }
+}
+
 static void constraint_vertices(simulation_t const& s, group_t const& g, std::vector<uint64_t>& keys, uint32_t* arity)
 {
     keys.clear();

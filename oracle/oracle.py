@@ -57,6 +57,13 @@ _SIGS = {
     "orc_boundary": (None, [_u32, _pu, _u32, _pu, _pu, _pu, _pu, _pu]),
     "orc_greedy_color": (_u32, [_u32, _u32, _pu64, _u32, _pu]),
     "orc_hash32": (_u32, [_u32]),
+    "orc_add_grid_sdf": (None, [_p, _pd, _pd, _pu, _pd, _pd, _pd]),
+    "orc_evaluate_sdf": (None, [_p, _u32, _pd, _u32, _pd, _pd]),
+    "orc_grid_node_count": (ctypes.c_uint64, [_pu]),
+    "orc_grid_node_positions": (None, [_pd, _pd, _pu, _pd]),
+    "orc_grid_from_mesh": (None, [_pd, _u32, _pu, _u32, _pd, _pd, _pu, _pd]),
+    "orc_mesh_signed_distance": (None, [_pd, _u32, _pu, _u32, _pd, _u32, _pd]),
+    "orc_environment_domain": (None, [_pd, _u32, _pd, _pd, _pd, _pd]),
 }
 
 
@@ -230,6 +237,26 @@ class Simulation:
     def add_sphere(self, center, radius):
         self._L.orc_add_sphere(self.h, _dp(f64(center)), radius)
 
+    def add_grid_sdf(self, lo, hi, res, nodes, vmin=None, vmax=None):
+        """Grid SDF obstacle (Discregrid cubic Lagrange grid); volume defaults to the domain."""
+        nodes = f64(nodes)
+        assert len(nodes) == grid_node_count(res)
+        self._L.orc_add_grid_sdf(self.h, _dp(f64(lo)), _dp(f64(hi)), _up(u32(res)), _dp(nodes),
+                                 _dp(f64(lo if vmin is None else vmin)), _dp(f64(hi if vmax is None else vmax)))
+
+    def add_mesh_obstacle(self, positions, triangles, lo, hi, res=(10, 10, 10)):
+        """environment_body_t(geometry, domain, resolution): extended domain, mesh distance at the
+        grid nodes (on the CPU), the extended domain as englobing volume."""
+        a, b = environment_domain(positions, lo, hi)
+        self.add_grid_sdf(a, b, res, grid_from_mesh(positions, triangles, a, b, res), a, b)
+
+    def evaluate_sdf(self, k, points):
+        p = f64(points).reshape(-1)
+        n = len(p) // 3
+        sd, g = np.zeros(n), np.zeros(3 * n)
+        self._L.orc_evaluate_sdf(self.h, k, _dp(p), n, _dp(sd), _dp(g))
+        return sd, g.reshape(-1, 3)
+
     def set_clusters(self, group, offsets):
         off = u32(offsets)
         self._L.orc_set_clusters(self.h, group, _up(off), len(off) - 1)
@@ -277,3 +304,35 @@ class Simulation:
         out = np.zeros(6)
         self._L.orc_energy(self.h, _dp(out))
         return out
+
+
+def grid_node_count(res):
+    return int(lib().orc_grid_node_count(_up(u32(res))))
+
+
+def grid_node_positions(lo, hi, res):
+    out = np.zeros(3 * grid_node_count(res))
+    lib().orc_grid_node_positions(_dp(f64(lo)), _dp(f64(hi)), _up(u32(res)), _dp(out))
+    return out.reshape(-1, 3)
+
+
+def grid_from_mesh(pos, tris, lo, hi, res):
+    p, t = f64(pos).reshape(-1), u32(tris).reshape(-1)
+    out = np.zeros(grid_node_count(res))
+    lib().orc_grid_from_mesh(_dp(p), len(p) // 3, _up(t), len(t) // 3, _dp(f64(lo)), _dp(f64(hi)), _up(u32(res)),
+                             _dp(out))
+    return out
+
+
+def mesh_signed_distance(pos, tris, points):
+    p, t, q = f64(pos).reshape(-1), u32(tris).reshape(-1), f64(points).reshape(-1)
+    out = np.zeros(len(q) // 3)
+    lib().orc_mesh_signed_distance(_dp(p), len(p) // 3, _up(t), len(t) // 3, _dp(q), len(q) // 3, _dp(out))
+    return out
+
+
+def environment_domain(pos, lo, hi):
+    p = f64(pos).reshape(-1)
+    a, b = np.zeros(3), np.zeros(3)
+    lib().orc_environment_domain(_dp(p), len(p) // 3, _dp(f64(lo)), _dp(f64(hi)), _dp(a), _dp(b))
+    return a, b

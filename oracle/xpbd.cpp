@@ -321,6 +321,8 @@ void sdf_t::evaluate(vec3 p, double* sd, vec3* grad) const
     if (kind == 0) {
         *sd = dot(n, p) + offset;
         *grad = n;
+    } else if (kind == 2) {
+        *sd = grid->interpolate(p, grad);
     } else {
         vec3 const d = p - c;
         double const len = norm(d);
@@ -351,7 +353,9 @@ void simulation_t::detect_contacts()
                 sdfs[k].evaluate(x, &sd, &grad);
                 if (!(sd < 0.)) continue;
                 if (!sdfs[k].in_volume(x)) continue;
-                vec3 const nn = grad / norm(grad);
+                // Eigen's normalized(): a zero vector is returned unchanged
+                double const gl2 = norm2(grad);
+                vec3 const nn = gl2 > 0.0 ? grad / std::sqrt(gl2) : grad;
                 vec3 const q = x + std::abs(sd) * nn;
                 contacts.push_back({b, s, vi, k, q, nn});
                 collision_constraints.push_back(std::make_unique<collision_constraint_t>(

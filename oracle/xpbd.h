@@ -22,6 +22,7 @@
 #include <memory>
 #include <vector>
 
+#include "grid_sdf.h"
 #include "linalg.h"
 
 namespace oracle {
@@ -107,12 +108,14 @@ struct collision_constraint_t final : constraint_t {
 // Analytic SDF obstacles. Plane: Eigen::Hyperplane(n, point) semantics used by
 // sdf_model_t::from_plane (sdf_model.cpp:181-193): sd = n.p - n.point, grad = n.
 // Sphere: sd = |p - c| - r, grad = (p - c)/|p - c| (no reference counterpart).
+// Grid: sdf_model_t(Discregrid::CubicLagrangeDiscreteGrid) (sdf_model.cpp:17, evaluate :70-74).
 struct sdf_t {
-    int kind = 0;  // 0 plane, 1 sphere
+    int kind = 0;  // 0 plane, 1 sphere, 2 grid
     vec3 n;
     double offset = 0;
     vec3 c;
     double r = 0;
+    std::shared_ptr<grid_sdf_t> grid;
     vec3 vmin, vmax;  // englobing volume (bvh_model.cpp:50)
     void evaluate(vec3 p, double* sd, vec3* grad) const;
     bool in_volume(vec3 p) const

@@ -1,0 +1,394 @@
+// Grid SDF construction on gfx950: the signed distance of a triangle mesh at every node of a cubic
+// Lagrange grid (Discregrid::MeshDistance::signedDistanceCached evaluated by
+// CubicLagrangeDiscreteGrid::addFunction, environment_body.cpp:66-73).
+//
+// Distance: the closest point of every triangle (Voronoi-region test of the triangle's vertices,
+// edges and face; Ericson, "Real-Time Collision Detection", 5.1.5), the smallest squared distance
+// winning (the first triangle on ties). Sign: negative where (x - closest point) points against the
+// angle-weighted pseudonormal of the closest feature (Baerentzen & Aanaes 2005): the vertex normal
+// sum of incident face normals weighted by the incident angles, the edge normal sum of the normals
+// of the faces sharing the edge, or the face normal. Zero-area triangles are left out.
+//
+// One thread per grid node; the triangles stream through LDS in tiles shared by the 256 threads of
+// a block. The work is FP64 arithmetic (about 60 operations per node and triangle), bounded by the
+// FP64 vector rate, not by memory.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <map>
+#include <utility>
+#include <vector>
+
+#include "abi_util.h"
+#include "grid_sdf.h"
+#include "sdf.h"
+
+namespace sbs {
+
+namespace {
+
+constexpr int kBlock = 256, kTile = 128;
+
+struct grid_params {
+    uint32_t res[3];
+    double dmin[3], cell[3];
+};
+
+__device__ __forceinline__ void closest_point(const double p[3], const double* t, double q[3], int* feature)
+{
+#pragma clang fp contract(off)
+    double ab[3], ac[3], ap[3], bp[3], cp[3];
+    for (int d = 0; d < 3; ++d) {
+        ab[d] = t[3 + d] - t[d];
+        ac[d] = t[6 + d] - t[d];
+        ap[d] = p[d] - t[d];
+    }
+    double const d1 = ab[0] * ap[0] + ab[1] * ap[1] + ab[2] * ap[2];
+    double const d2 = ac[0] * ap[0] + ac[1] * ap[1] + ac[2] * ap[2];
+    if (d1 <= 0.0 && d2 <= 0.0) {
+        *feature = 0;
+        for (int d = 0; d < 3; ++d) q[d] = t[d];
+        return;
+    }
+    for (int d = 0; d < 3; ++d) bp[d] = p[d] - t[3 + d];
+    double const d3 = ab[0] * bp[0] + ab[1] * bp[1] + ab[2] * bp[2];
+    double const d4 = ac[0] * bp[0] + ac[1] * bp[1] + ac[2] * bp[2];
+    if (d3 >= 0.0 && d4 <= d3) {
+        *feature = 1;
+        for (int d = 0; d < 3; ++d) q[d] = t[3 + d];
+        return;
+    }
+    double const vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+        double const v = d1 / (d1 - d3);
+        *feature = 3;
+        for (int d = 0; d < 3; ++d) q[d] = t[d] + v * ab[d];
+        return;
+    }
+    for (int d = 0; d < 3; ++d) cp[d] = p[d] - t[6 + d];
+    double const d5 = ab[0] * cp[0] + ab[1] * cp[1] + ab[2] * cp[2];
+    double const d6 = ac[0] * cp[0] + ac[1] * cp[1] + ac[2] * cp[2];
+    if (d6 >= 0.0 && d5 <= d6) {
+        *feature = 2;
+        for (int d = 0; d < 3; ++d) q[d] = t[6 + d];
+        return;
+    }
+    double const vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+        double const w = d2 / (d2 - d6);
+        *feature = 5;
+        for (int d = 0; d < 3; ++d) q[d] = t[d] + w * ac[d];
+        return;
+    }
+    double const va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+        double const w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        *feature = 4;
+        for (int d = 0; d < 3; ++d) q[d] = t[3 + d] + w * (t[6 + d] - t[3 + d]);
+        return;
+    }
+    double const den = 1.0 / (va + vb + vc);
+    double const v = vb * den, w = vc * den;
+    *feature = 6;
+    for (int d = 0; d < 3; ++d) q[d] = t[d] + ab[d] * v + ac[d] * w;
+}
+
+__global__ void __launch_bounds__(kBlock) k_grid_nodes(mesh_dev mesh, grid_params gp, uint64_t n,
+                                                       double* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    __shared__ double tc[kTile * 9];
+    uint64_t const l = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    bool const valid = l < n;
+    grid_dims const g(gp.res);
+    double p[3] = {0.0, 0.0, 0.0};
+    if (valid) grid_node_position(g, gp.dmin, gp.cell, l, p);
+    double best = DBL_MAX, bq[3] = {0.0, 0.0, 0.0};
+    uint32_t bf = 0;
+    int bfeat = 6;
+    for (uint32_t t0 = 0; t0 < mesh.m; t0 += kTile) {
+        uint32_t const cnt = min(uint32_t(kTile), mesh.m - t0);
+        __syncthreads();
+        for (uint32_t w = threadIdx.x; w < cnt * 9; w += kBlock) tc[w] = mesh.corners[std::size_t(t0) * 9 + w];
+        __syncthreads();
+        if (valid)
+            for (uint32_t t = 0; t < cnt; ++t) {
+                double q[3];
+                int feat;
+                closest_point(p, tc + 9 * t, q, &feat);
+                double const e0 = p[0] - q[0], e1 = p[1] - q[1], e2 = p[2] - q[2];
+                double const d2 = e0 * e0 + e1 * e1 + e2 * e2;
+                if (d2 < best) {
+                    best = d2;
+                    bf = t0 + t;
+                    bfeat = feat;
+                    bq[0] = q[0];
+                    bq[1] = q[1];
+                    bq[2] = q[2];
+                }
+            }
+    }
+    if (!valid) return;
+    if (mesh.m == 0) {
+        out[l] = DBL_MAX;
+        return;
+    }
+    const double* nrm = bfeat < 3   ? mesh.vnormal + 3 * std::size_t(mesh.tri[3 * std::size_t(bf) + bfeat])
+                        : bfeat < 6 ? mesh.enormal + 9 * std::size_t(bf) + 3 * (bfeat - 3)
+                                    : mesh.fnormal + 3 * std::size_t(bf);
+    double const s = (p[0] - bq[0]) * nrm[0] + (p[1] - bq[1]) * nrm[1] + (p[2] - bq[2]) * nrm[2];
+    double const d = sqrt(best);
+    out[l] = s < 0.0 ? -d : d;
+}
+
+}  // namespace
+
+void launch_grid_nodes(hipStream_t s, mesh_dev mesh, const uint32_t res[3], const double dmin[3],
+                       const double cell[3], double* nodes)
+{
+    grid_params gp{};
+    for (int d = 0; d < 3; ++d) {
+        gp.res[d] = res[d];
+        gp.dmin[d] = dmin[d];
+        gp.cell[d] = cell[d];
+    }
+    uint64_t const n = grid_dims(res).nodes();
+    k_grid_nodes<<<unsigned((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(mesh, gp, n, nodes);
+}
+
+// Host side of the mesh: non-degenerate triangles, corners and pseudonormals (FP64, no FMA
+// contraction), uploaded to the device.
+struct mesh_upload {
+    dev_buf<uint32_t> tri;
+    dev_buf<double> corners, vnormal, enormal, fnormal;
+    uint32_t m = 0;
+
+    mesh_upload(hipStream_t s, const double* pos, uint32_t nv, const uint32_t* tris, uint32_t nt)
+    {
+#pragma clang fp contract(off)
+        auto at = [&](uint32_t v, int d) { return pos[3 * std::size_t(v) + d]; };
+        auto cross = [](const double a[3], const double b[3], double o[3]) {
+            o[0] = a[1] * b[2] - a[2] * b[1];
+            o[1] = a[2] * b[0] - a[0] * b[2];
+            o[2] = a[0] * b[1] - a[1] * b[0];
+        };
+        auto dot = [](const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+        std::vector<uint32_t> t;
+        for (uint32_t f = 0; f < nt; ++f) {
+            double e1[3], e2[3], n[3];
+            for (int d = 0; d < 3; ++d) {
+                e1[d] = at(tris[3 * f + 1], d) - at(tris[3 * f], d);
+                e2[d] = at(tris[3 * f + 2], d) - at(tris[3 * f], d);
+            }
+            cross(e1, e2, n);
+            if (dot(n, n) > 0.0)
+                for (int k = 0; k < 3; ++k) t.push_back(tris[3 * f + k]);
+        }
+        m = uint32_t(t.size() / 3);
+        std::vector<double> c(9 * std::size_t(m)), vn(3 * std::size_t(nv), 0.0), en(9 * std::size_t(m)),
+            fn(3 * std::size_t(m));
+        std::map<std::pair<uint32_t, uint32_t>, std::array<double, 3>> edge_sum;
+        for (uint32_t f = 0; f < m; ++f) {
+            double p[3][3];
+            for (int k = 0; k < 3; ++k)
+                for (int d = 0; d < 3; ++d) {
+                    p[k][d] = at(t[3 * f + k], d);
+                    c[9 * std::size_t(f) + 3 * k + d] = p[k][d];
+                }
+            double e1[3], e2[3], n[3];
+            for (int d = 0; d < 3; ++d) {
+                e1[d] = p[1][d] - p[0][d];
+                e2[d] = p[2][d] - p[0][d];
+            }
+            cross(e1, e2, n);
+            double const ln = std::sqrt(dot(n, n));
+            for (int d = 0; d < 3; ++d) {
+                n[d] = n[d] / ln;
+                fn[3 * std::size_t(f) + d] = n[d];
+            }
+            for (int k = 0; k < 3; ++k) {
+                double a1[3], a2[3];
+                for (int d = 0; d < 3; ++d) {
+                    a1[d] = p[(k + 1) % 3][d] - p[k][d];
+                    a2[d] = p[(k + 2) % 3][d] - p[k][d];
+                }
+                double const cs = dot(a1, a2) / (std::sqrt(dot(a1, a1)) * std::sqrt(dot(a2, a2)));
+                double const alpha = std::acos(std::max(-1.0, std::min(1.0, cs)));
+                uint32_t const v = t[3 * f + k];
+                for (int d = 0; d < 3; ++d) vn[3 * std::size_t(v) + d] = vn[3 * std::size_t(v) + d] + alpha * n[d];
+                uint32_t const w = t[3 * f + (k + 1) % 3];
+                auto& es = edge_sum.try_emplace({std::min(v, w), std::max(v, w)}, std::array<double, 3>{0, 0, 0})
+                               .first->second;
+                for (int d = 0; d < 3; ++d) es[d] = es[d] + n[d];
+            }
+        }
+        for (uint32_t f = 0; f < m; ++f)
+            for (int k = 0; k < 3; ++k) {
+                uint32_t const v = t[3 * f + k], w = t[3 * f + (k + 1) % 3];
+                auto const& es = edge_sum[{std::min(v, w), std::max(v, w)}];
+                for (int d = 0; d < 3; ++d) en[9 * std::size_t(f) + 3 * k + d] = es[d];
+            }
+        tri.upload(t.data(), t.size(), s);
+        corners.upload(c.data(), c.size(), s);
+        vnormal.upload(vn.data(), vn.size(), s);
+        enormal.upload(en.data(), en.size(), s);
+        fnormal.upload(fn.data(), fn.size(), s);
+        hip_check(hipStreamSynchronize(s), "mesh upload");
+    }
+    mesh_dev view() const { return {tri.p, corners.p, vnormal.p, enormal.p, fnormal.p, m}; }
+};
+
+void grid_cell_size(const double dmin[3], const double dmax[3], const uint32_t res[3], double cell[3],
+                    double inv_cell[3])
+{
+    for (int d = 0; d < 3; ++d) {
+        cell[d] = (dmax[d] - dmin[d]) / double(res[d]);
+        inv_cell[d] = 1.0 / cell[d];
+    }
+}
+
+void check_grid(const double dmin[3], const double dmax[3], const uint32_t res[3])
+{
+    require(dmin && dmax && res, "null grid argument");
+    for (int d = 0; d < 3; ++d) {
+        require(res[d] >= 1, "grid resolution must be at least 1 cell per axis");
+        require(std::isfinite(dmin[d]) && std::isfinite(dmax[d]) && dmax[d] > dmin[d],
+                "grid domain must be a finite box with max > min");
+    }
+    require(grid_dims(res).nodes() < (uint64_t(1) << 32), "grid has too many nodes");
+}
+
+void build_mesh_grid(hipStream_t s, const double* pos, uint32_t nv, const uint32_t* tris, uint32_t nt,
+                     const double dmin[3], const double dmax[3], const uint32_t res[3], double* dev_nodes)
+{
+    require((pos || nv == 0) && (tris || nt == 0), "null mesh argument");
+    for (std::size_t i = 0; i < 3 * std::size_t(nt); ++i) require(tris[i] < nv, "triangle vertex index out of range");
+    check_grid(dmin, dmax, res);
+    mesh_upload const mesh(s, pos, nv, tris, nt);
+    double cell[3], inv_cell[3];
+    grid_cell_size(dmin, dmax, res, cell, inv_cell);
+    launch_grid_nodes(s, mesh.view(), res, dmin, cell, dev_nodes);
+    hip_check(hipGetLastError(), "grid node kernel");
+    hip_check(hipStreamSynchronize(s), "grid node kernel");
+}
+
+void environment_domain(const double* pos, uint32_t nv, const double lo_in[3], const double hi_in[3], double lo[3],
+                        double hi[3])
+{
+#pragma clang fp contract(off)
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = lo_in[d];
+        hi[d] = hi_in[d];
+    }
+    for (uint32_t i = 0; i < nv; ++i)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], pos[3 * std::size_t(i) + d]);
+            hi[d] = std::max(hi[d], pos[3 * std::size_t(i) + d]);
+        }
+    // environment_body.cpp:54-64: once per vertex, max grows by 1e-3 |diagonal|, then min by
+    // 1e-3 |diagonal| of the grown box (re-extending by the vertices inside the loop changes nothing
+    // after the first pass)
+    auto diag_norm = [&] {
+        double const a = hi[0] - lo[0], b = hi[1] - lo[1], c = hi[2] - lo[2];
+        return std::sqrt(a * a + b * b + c * c);
+    };
+    for (uint32_t v = 0; v < nv; ++v) {
+        double const g1 = 1.0e-3 * diag_norm();
+        for (int d = 0; d < 3; ++d) hi[d] += g1 * 1.0;
+        double const g2 = 1.0e-3 * diag_norm();
+        for (int d = 0; d < 3; ++d) lo[d] -= g2 * 1.0;
+    }
+}
+
+namespace {
+
+__global__ void k_eval_sdf(sdf_dev f, const double* __restrict__ pts, uint32_t n, double* __restrict__ sd,
+                           double* __restrict__ grad)
+{
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pos4 const p = make_double4(pts[3 * std::size_t(i)], pts[3 * std::size_t(i) + 1], pts[3 * std::size_t(i) + 2], 0.0);
+    double v, g[3];
+    sdf_eval(f, p, v, g);
+    sd[i] = v;
+    grad[3 * std::size_t(i)] = g[0];
+    grad[3 * std::size_t(i) + 1] = g[1];
+    grad[3 * std::size_t(i) + 2] = g[2];
+}
+
+}  // namespace
+
+void evaluate_sdf(hipStream_t s, sdf_dev const& f, const double* pts, uint32_t n, double* sd, double* grad)
+{
+    if (n == 0) return;
+    dev_buf<double> dp, dsd, dg;
+    dp.upload(pts, 3 * std::size_t(n), s);
+    dsd.alloc(n);
+    dg.alloc(3 * std::size_t(n));
+    k_eval_sdf<<<(n + 255) / 256, 256, 0, s>>>(f, dp.p, n, dsd.p, dg.p);
+    hip_check(hipGetLastError(), "sdf evaluation");
+    hip_check(hipMemcpyAsync(sd, dsd.p, n * sizeof(double), hipMemcpyDeviceToHost, s), "read sdf values");
+    hip_check(hipMemcpyAsync(grad, dg.p, 3 * std::size_t(n) * sizeof(double), hipMemcpyDeviceToHost, s),
+              "read sdf gradients");
+    hip_check(hipStreamSynchronize(s), "sdf evaluation");
+}
+
+}  // namespace sbs
+
+using namespace sbs;
+
+extern "C" int sbs_sdf_grid_node_count(const uint32_t resolution[3], uint64_t* count)
+{
+    return guarded([&] {
+        require(resolution && count, "null argument");
+        *count = grid_dims(resolution).nodes();
+    });
+}
+
+extern "C" int sbs_sdf_grid_node_positions(const double domain_min[3], const double domain_max[3],
+                                           const uint32_t resolution[3], double* positions)
+{
+    return guarded([&] {
+        check_grid(domain_min, domain_max, resolution);
+        require(positions != nullptr, "positions is null");
+        double cell[3], inv_cell[3];
+        grid_cell_size(domain_min, domain_max, resolution, cell, inv_cell);
+        grid_dims const g(resolution);
+        for (uint64_t l = 0; l < g.nodes(); ++l) grid_node_position(g, domain_min, cell, l, positions + 3 * l);
+    });
+}
+
+extern "C" int sbs_sdf_environment_domain(const double* positions, uint32_t num_vertices, const double domain_min[3],
+                                          const double domain_max[3], double out_min[3], double out_max[3])
+{
+    return guarded([&] {
+        require((positions || num_vertices == 0) && domain_min && domain_max && out_min && out_max, "null argument");
+        environment_domain(positions, num_vertices, domain_min, domain_max, out_min, out_max);
+    });
+}
+
+extern "C" int sbs_sdf_grid_from_mesh(int device, const double* positions, uint32_t num_vertices,
+                                      const uint32_t* triangles, uint32_t num_triangles, const double domain_min[3],
+                                      const double domain_max[3], const uint32_t resolution[3], double* nodes)
+{
+    return guarded([&] {
+        require(nodes != nullptr, "nodes is null");
+        check_grid(domain_min, domain_max, resolution);
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        hipStream_t s;
+        hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+        try {
+            uint64_t const n = grid_dims(resolution).nodes();
+            dev_buf<double> dn;
+            dn.alloc(n);
+            build_mesh_grid(s, positions, num_vertices, triangles, num_triangles, domain_min, domain_max, resolution,
+                            dn.p);
+            hip_check(hipMemcpyAsync(nodes, dn.p, n * sizeof(double), hipMemcpyDeviceToHost, s), "read grid nodes");
+            hip_check(hipStreamSynchronize(s), "read grid nodes");
+        } catch (...) {
+            (void)hipStreamDestroy(s);
+            throw;
+        }
+        hip_check(hipStreamDestroy(s), "hipStreamDestroy");
+    });
+}

@@ -14,9 +14,13 @@ using pos4 = double4;
 
 // Analytic SDF obstacle in device memory (FP64, like the contact positions).
 struct sdf_dev {
-    int kind;  // 0 plane, 1 sphere
-    double a[4];  // plane: n.xyz, offset | sphere: c.xyz, r
+    int kind;         // 0 plane, 1 sphere, 2 grid
+    uint32_t res[3];  // grid: cells per axis
+    double a[4];      // plane: n.xyz, offset | sphere: c.xyz, r
     double vmin[3], vmax[3];
+    // grid: domain, cell size and its inverse, node values (cubic Lagrange grid layout)
+    double gmin[3], gmax[3], cell[3], inv_cell[3];
+    const double* nodes;
 };
 
 // Arrays of one tet-constraint group in color order. Material layout (3 float4 per tet):

@@ -13,10 +13,12 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "abi_util.h"
 #include "contact_hash.h"
+#include "grid_sdf.h"
 #include "kernels.h"
 #include "scan_sort.h"
 #include "mlp.h"
@@ -98,6 +100,7 @@ struct sbs_gpu {
     std::vector<body_t> bodies;
     std::vector<group_t> groups;
     std::vector<sdf_dev> sdfs;
+    std::vector<std::unique_ptr<dev_buf<double>>> grid_nodes;  // node values of grid SDFs
     bool committed = false;
     uint32_t V = 0;
 
@@ -1184,6 +1187,74 @@ int sbs_gpu_add_sphere(sbs_gpu* sim, const double center[3], double radius)
         }
         f.a[3] = radius;
         sim->sdfs.push_back(f);
+    });
+}
+
+namespace {
+
+sdf_dev grid_record(const double dmin[3], const double dmax[3], const uint32_t res[3], const double* dev_nodes,
+                    const double vol_min[3], const double vol_max[3])
+{
+    sdf_dev f{};
+    f.kind = 2;
+    grid_cell_size(dmin, dmax, res, f.cell, f.inv_cell);
+    for (int d = 0; d < 3; ++d) {
+        f.res[d] = res[d];
+        f.gmin[d] = dmin[d];
+        f.gmax[d] = dmax[d];
+        f.vmin[d] = vol_min[d];
+        f.vmax[d] = vol_max[d];
+    }
+    f.nodes = dev_nodes;
+    return f;
+}
+
+}  // namespace
+
+int sbs_gpu_add_grid_sdf(sbs_gpu* sim, const double domain_min[3], const double domain_max[3],
+                         const uint32_t resolution[3], const double* nodes, const double vol_min[3],
+                         const double vol_max[3])
+{
+    return guarded([&] {
+        require(sim && nodes && vol_min && vol_max, "null argument");
+        require(!sim->committed, "scene already committed", SBS_ERR_STATE);
+        check_grid(domain_min, domain_max, resolution);
+        uint64_t const n = grid_dims(resolution).nodes();
+        auto buf = std::make_unique<dev_buf<double>>();
+        buf->upload(nodes, n, sim->stream);
+        hip_check(hipStreamSynchronize(sim->stream), "upload grid nodes");
+        sim->sdfs.push_back(grid_record(domain_min, domain_max, resolution, buf->p, vol_min, vol_max));
+        sim->grid_nodes.push_back(std::move(buf));
+    });
+}
+
+int sbs_gpu_add_mesh_obstacle(sbs_gpu* sim, const double* positions, uint32_t num_vertices, const uint32_t* triangles,
+                              uint32_t num_triangles, const double domain_min[3], const double domain_max[3],
+                              const uint32_t resolution[3])
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(!sim->committed, "scene already committed", SBS_ERR_STATE);
+        require(domain_min && domain_max, "null domain");
+        require(num_triangles > 0, "the mesh has no triangles");
+        double lo[3], hi[3];
+        environment_domain(positions, num_vertices, domain_min, domain_max, lo, hi);
+        check_grid(lo, hi, resolution);
+        auto buf = std::make_unique<dev_buf<double>>();
+        buf->alloc(grid_dims(resolution).nodes());
+        build_mesh_grid(sim->stream, positions, num_vertices, triangles, num_triangles, lo, hi, resolution, buf->p);
+        sim->sdfs.push_back(grid_record(lo, hi, resolution, buf->p, lo, hi));
+        sim->grid_nodes.push_back(std::move(buf));
+    });
+}
+
+int sbs_gpu_evaluate_sdf(sbs_gpu* sim, uint32_t sdf, const double* points, uint32_t n, double* sd, double* grad)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(sdf < sim->sdfs.size(), "sdf index out of range");
+        require((points && sd && grad) || n == 0, "null argument");
+        evaluate_sdf(sim->stream, sim->sdfs[sdf], points, n, sd, grad);
     });
 }
 

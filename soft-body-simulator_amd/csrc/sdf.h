@@ -1,5 +1,7 @@
-// Analytic SDF evaluation on the device (shared by the contact kernels).
+// SDF evaluation on the device (shared by the contact kernels): analytic planes and spheres, and
+// cubic Lagrange grids.
 #pragma once
+#include "grid_sdf.h"
 #include "kernels.h"
 
 namespace sbs {
@@ -14,6 +16,8 @@ __device__ __forceinline__ void sdf_eval(sdf_dev const& f, pos4 p, double& sd, d
         grad[1] = f.a[1];
         grad[2] = f.a[2];
         sd = grad[0] * p.x + grad[1] * p.y + grad[2] * p.z + f.a[3];
+    } else if (f.kind == 2) {
+        sd = grid_eval(f, p.x, p.y, p.z, grad);
     } else {
         double const d[3] = {p.x - f.a[0], p.y - f.a[1], p.z - f.a[2]};
         double const len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
@@ -41,7 +45,9 @@ __device__ __forceinline__ bool sdf_contact(sdf_dev const& f, pos4 p, double q[3
     if (!(p.x >= f.vmin[0] && p.x <= f.vmax[0] && p.y >= f.vmin[1] && p.y <= f.vmax[1] && p.z >= f.vmin[2] &&
           p.z <= f.vmax[2]))
         return false;
-    double const gl = sqrt(grad[0] * grad[0] + grad[1] * grad[1] + grad[2] * grad[2]);
+    // Eigen's normalized(): a zero gradient stays zero
+    double const gl2 = grad[0] * grad[0] + grad[1] * grad[1] + grad[2] * grad[2];
+    double const gl = gl2 > 0.0 ? sqrt(gl2) : 1.0;
     n[0] = grad[0] / gl;
     n[1] = grad[1] / gl;
     n[2] = grad[2] / gl;

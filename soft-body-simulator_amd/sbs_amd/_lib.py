@@ -79,6 +79,13 @@ _SIGS = {
     "sbs_mesh_ellipsoid6": [_d, _d, _d, _d, _pd, _pu, _pu, _pu],
     "sbs_gpu_add_plane": [_p, _pd, _pd, _pd, _pd],
     "sbs_gpu_add_sphere": [_p, _pd, _d],
+    "sbs_gpu_add_grid_sdf": [_p, _pd, _pd, _pu, _pd, _pd, _pd],
+    "sbs_gpu_add_mesh_obstacle": [_p, _pd, _u32, _pu, _u32, _pd, _pd, _pu],
+    "sbs_gpu_evaluate_sdf": [_p, _u32, _pd, _u32, _pd, _pd],
+    "sbs_sdf_grid_node_count": [_pu, _pu64],
+    "sbs_sdf_grid_node_positions": [_pd, _pd, _pu, _pd],
+    "sbs_sdf_environment_domain": [_pd, _u32, _pd, _pd, _pd, _pd],
+    "sbs_sdf_grid_from_mesh": [_i, _pd, _u32, _pu, _u32, _pd, _pd, _pu, _pd],
     "sbs_gpu_set_group_clusters": [_p, _u32, _pu, _u32],
     "sbs_gpu_set_coloring": [_p, _i],
     "sbs_gpu_set_launch_config": [_p, _i, _i],
@@ -262,6 +269,37 @@ def extract_boundary(nv, tets, device=0):
     check(lib().sbs_gpu_extract_boundary(device, nv, _up(tets), nt, _up(s2t), _up(tris), ctypes.byref(ns),
                                          ctypes.byref(ntr)))
     return s2t[:ns.value].copy(), tris[:3 * ntr.value].reshape(-1, 3).copy()
+
+
+def grid_node_count(res):
+    """Node count of a cubic Lagrange grid of res[0] x res[1] x res[2] cells."""
+    n = ctypes.c_uint64()
+    check(lib().sbs_sdf_grid_node_count(_up(u32(res)), ctypes.byref(n)))
+    return n.value
+
+
+def grid_node_positions(lo, hi, res):
+    """(N, 3) node positions of a cubic Lagrange grid over the box [lo, hi]."""
+    out = np.zeros(3 * grid_node_count(res))
+    check(lib().sbs_sdf_grid_node_positions(_dp(f64(lo)), _dp(f64(hi)), _up(u32(res)), _dp(out)))
+    return out.reshape(-1, 3)
+
+
+def environment_domain(positions, lo, hi):
+    """environment_body_t's extended grid domain for a mesh and a requested domain."""
+    p = f64(positions).reshape(-1)
+    a, b = np.zeros(3), np.zeros(3)
+    check(lib().sbs_sdf_environment_domain(_dp(p), len(p) // 3, _dp(f64(lo)), _dp(f64(hi)), _dp(a), _dp(b)))
+    return a, b
+
+
+def grid_from_mesh(positions, triangles, lo, hi, res, device=0):
+    """Node values of a triangle mesh's signed distance on a cubic Lagrange grid, on the device."""
+    p, t = f64(positions).reshape(-1), u32(triangles).reshape(-1)
+    out = np.zeros(grid_node_count(res))
+    check(lib().sbs_sdf_grid_from_mesh(device, _dp(p), len(p) // 3, _up(t), len(t) // 3, _dp(f64(lo)), _dp(f64(hi)),
+                                       _up(u32(res)), _dp(out)))
+    return out
 
 
 def sort_pairs(keys, vals, bits=32, device=0):
@@ -473,6 +511,29 @@ class Simulation:
 
     def add_sphere(self, center, radius):
         check(self._L.sbs_gpu_add_sphere(self.h, _dp(f64(center)), radius))
+
+    def add_grid_sdf(self, lo, hi, res, nodes, vmin=None, vmax=None):
+        """Grid SDF obstacle from node values (cubic Lagrange grid); volume defaults to the domain."""
+        nodes = f64(nodes)
+        if len(nodes) != grid_node_count(res):
+            raise ValueError(f"{len(nodes)} node values for a grid of {grid_node_count(res)} nodes")
+        check(self._L.sbs_gpu_add_grid_sdf(self.h, _dp(f64(lo)), _dp(f64(hi)), _up(u32(res)), _dp(nodes),
+                                           _dp(f64(lo if vmin is None else vmin)), _dp(f64(hi if vmax is None else vmax))))
+
+    def add_mesh_obstacle(self, positions, triangles, lo, hi, res=(10, 10, 10)):
+        """environment_body_t(geometry, domain, resolution): a triangle-mesh obstacle with its grid
+        SDF built on the device."""
+        p, t = f64(positions).reshape(-1), u32(triangles).reshape(-1)
+        check(self._L.sbs_gpu_add_mesh_obstacle(self.h, _dp(p), len(p) // 3, _up(t), len(t) // 3, _dp(f64(lo)),
+                                                _dp(f64(hi)), _up(u32(res))))
+
+    def evaluate_sdf(self, k, points):
+        """(signed distances (n,), gradients (n, 3)) of obstacle k at points, on the device."""
+        p = f64(points).reshape(-1)
+        n = len(p) // 3
+        sd, g = np.zeros(n), np.zeros(3 * n)
+        check(self._L.sbs_gpu_evaluate_sdf(self.h, k, _dp(p), n, _dp(sd), _dp(g)))
+        return sd, g.reshape(-1, 3)
 
     def set_clusters(self, group, offsets):
         off = u32(offsets)

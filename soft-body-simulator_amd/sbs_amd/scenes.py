@@ -39,6 +39,8 @@ class Scene:
     groups: list = field(default_factory=list)
     planes: list = field(default_factory=list)   # (normal, point, vmin, vmax)
     spheres: list = field(default_factory=list)  # (center, radius)
+    grids: list = field(default_factory=list)    # grid SDFs: (lo, hi, res, node values, vmin, vmax)
+    meshes: list = field(default_factory=list)   # mesh obstacles: (positions, triangles, lo, hi, res)
     gravity: tuple = (0.0, -9.81, 0.0)
     gravity_is_force: bool = True
     collision_compliance: float = 1e-8
@@ -86,6 +88,10 @@ class Scene:
             sim.add_plane(n, p, lo, hi)
         for c, r in self.spheres:
             sim.add_sphere(c, r)
+        for lo, hi, res, nodes, vmin, vmax in self.grids:
+            sim.add_grid_sdf(lo, hi, res, nodes, vmin, vmax)
+        for pos, tris, lo, hi, res in self.meshes:
+            sim.add_mesh_obstacle(pos, tris, lo, hi, res)
         if self.contact_hash is not None and hasattr(sim, "set_contact_hash"):
             sim.set_contact_hash(*self.contact_hash)
 
