@@ -72,9 +72,12 @@ def test_mesh_obstacle_entry_point_matches_oracle():
     so, go = orc.evaluate_sdf(0, P)
     assert np.array_equal(sd, so) and np.array_equal(gr, go)
     inside = sd < 1e300
-    # a horizontal quad: the distance field is the height above it inside its footprint
-    foot = inside & (np.abs(P[:, 0]) < 1.9) & (np.abs(P[:, 2]) < 1.9)
-    assert np.allclose(sd[foot], P[foot, 1], atol=1e-9)
+    # a horizontal quad: the distance field is the height above it inside its footprint, a linear
+    # function the cubic cells reproduce wherever all their nodes lie over the quad (the cells are
+    # about 0.4 wide and the quad's edges at +-2)
+    foot = inside & (np.abs(P[:, 0]) < 1.5) & (np.abs(P[:, 2]) < 1.5)
+    assert foot.sum() > 500
+    assert np.allclose(sd[foot], P[foot, 1], rtol=0, atol=1e-12)
 
 
 def bunny_drop_scene():
