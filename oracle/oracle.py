@@ -205,6 +205,9 @@ class Simulation:
         v = None if v is None else f64(v).reshape(-1)
         self._L.orc_set_body_state(self.h, body, _dp(x), _dp(v))
 
+    def write_body(self, body, x=None, v=None):
+        self.set_body_state(body, x, v)
+
     def set_surface(self, body, s2t):
         s2t = u32(s2t)
         self._L.orc_set_surface(self.h, body, _up(s2t), len(s2t))

@@ -18,6 +18,7 @@ class Body:
     mass: np.ndarray = None  # (n,) or None for unit mass
     surface: np.ndarray = None  # surface vertex -> body vertex
     triangles: np.ndarray = None  # (f, 3) boundary triangles in surface vertex indices
+    v0: np.ndarray = None  # (n, 3) initial velocities, or None for rest
 
 
 @dataclass
@@ -95,6 +96,12 @@ class Scene:
         if self.contact_hash is not None and hasattr(sim, "set_contact_hash"):
             sim.set_contact_hash(*self.contact_hash)
 
+    def set_initial_state(self, sim):
+        """Initial velocities of the bodies that have them (after commit on the device)."""
+        for bi, b in enumerate(self.bodies):
+            if b.v0 is not None:
+                sim.write_body(bi, None, b.v0)
+
     def bbox_diagonal(self):
         x = np.concatenate([b.x0 for b in self.bodies])
         return float(np.linalg.norm(x.max(0) - x.min(0)))
@@ -104,6 +111,7 @@ def build_device(scene, device=0, recolor_passes=2):
     sim = _lib.Simulation(device)
     scene.apply(sim)
     sim.commit(recolor_passes)
+    scene.set_initial_state(sim)
     return sim
 
 

@@ -25,6 +25,7 @@ def build_oracle(scene, colored=True, recolor_passes=2):
     sim = O.Simulation()
     scene.apply(sim)
     sim.set_order(colored, recolor_passes)
+    scene.set_initial_state(sim)
     return sim
 
 
