@@ -91,6 +91,135 @@ def cpu_baseline(scene, sample=None):
             "projections_per_s": run.num_tets * run.iterations * run.substeps / el}
 
 
+def _median_time(fn, reps):
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[len(ts) // 2]
+
+
+def row_boundary(args):
+    """Boundary surface extraction of the 6M-tet beam (400 x 50 x 50 cells, 6-split) on the device,
+    per call including the upload of the tets and the download of the surface; beside it the
+    library's host extraction and the oracle (reference algorithm) on the same mesh."""
+    import numpy as np
+
+    import sbs_amd as S
+
+    pos, tets = S.box6(400, 50, 50, 0.1)
+    s2t, tris = S.extract_boundary(len(pos), tets)
+    dev = _median_time(lambda: S.extract_boundary(len(pos), tets), 5)
+    t0 = time.perf_counter()
+    hs, ht = S.boundary(len(pos), tets)
+    host = time.perf_counter() - t0
+    assert np.array_equal(s2t, hs) and np.array_equal(tris, ht)
+    log(f"boundary: device {1e3 * dev:.1f} ms, host library {1e3 * host:.0f} ms")
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        t0 = time.perf_counter()
+        O.boundary(len(pos), tets)
+        el = time.perf_counter() - t0
+        cpu = {"value": round(len(tets) / el, 1), "unit": "tets/s", "cores": 1, "kind": "port",
+               "sample": f"the whole 6M-tet mesh, oracle boundary extraction (ordered maps), {el:.1f}s on {cpu_model()}"}
+    return {"metric": "boundary surface extraction, 6M-tet beam", "value": round(len(tets) / dev, 1), "unit": "tets/s",
+            "higher_is_better": True, "dtype": "u32", "data": "synthetic",
+            "config": {"workload": "box6 400x50x50", "tets": int(len(tets)), "surface_vertices": int(len(s2t)),
+                       "triangles": int(len(tris))},
+            "ms_per_call": round(1e3 * dev, 3), "host_library_ms": round(1e3 * host, 1), "cpu_baseline": cpu}
+
+
+def icosphere(r, levels):
+    import numpy as np
+
+    t = (1 + 5 ** 0.5) / 2
+    v = [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+         [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]]
+    f = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6],
+         [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10],
+         [8, 6, 7], [9, 8, 1]]
+    v = [np.array(p, float) / np.linalg.norm(p) for p in v]
+    for _ in range(levels):
+        mid, nf = {}, []
+
+        def m(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in mid:
+                p = v[a] + v[b]
+                v.append(p / np.linalg.norm(p))
+                mid[k] = len(v) - 1
+            return mid[k]
+        for a, b, c in f:
+            ab, bc, ca = m(a, b), m(b, c), m(c, a)
+            nf += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        f = nf
+    return r * np.array(v), np.array(f, dtype=np.uint32)
+
+
+def row_grid_sdf(args):
+    """Grid SDF of a triangle mesh: the signed distance of an 81,920-triangle sphere mesh at the
+    245,025 nodes of a 32^3-cell cubic Lagrange grid, on the device (per call, with the host's
+    pseudonormals and the transfers); the oracle times a 4^3-cell grid of the same mesh."""
+    import sbs_amd as S
+
+    V, F = icosphere(0.5, 6)
+    lo, hi, res = [-0.6] * 3, [0.6] * 3, [32, 32, 32]
+    nodes = S.grid_node_count(res)
+    S.grid_from_mesh(V, F, lo, hi, res)
+    dev = _median_time(lambda: S.grid_from_mesh(V, F, lo, hi, res), 3)
+    pairs = nodes * len(F)
+    log(f"grid sdf: {nodes} nodes x {len(F)} triangles in {1e3 * dev:.1f} ms ({pairs / dev:.3g} pairs/s)")
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        small = [4, 4, 4]
+        t0 = time.perf_counter()
+        O.grid_from_mesh(V, F, lo, hi, small)
+        el = time.perf_counter() - t0
+        n_small = O.grid_node_count(small)
+        cpu = {"value": round(n_small * len(F) / el, 1), "unit": "node-triangle pairs/s", "cores": 1, "kind": "port",
+               "sample": f"{n_small} nodes (4^3 cells) x {len(F)} triangles, oracle, {el:.1f}s on {cpu_model()}"}
+    return {"metric": "grid SDF construction from a triangle mesh", "value": round(pairs / dev, 1),
+            "unit": "node-triangle pairs/s", "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "icosphere 81920 triangles, 32^3 cells", "nodes": nodes, "triangles": int(len(F))},
+            "ms_per_call": round(1e3 * dev, 3), "cpu_baseline": cpu}
+
+
+def row_ply(args):
+    """ASCII PLY writing and reading of the 1M-tet beam (179,452 vertices, 997,920 tets) from and
+    to memory; beside it the reference's own ply.cpp (built from its sources as a checker, when
+    present)."""
+    import numpy as np
+
+    import sbs_amd as S
+    from sbs_amd import ply
+
+    pos, tets = S.box6(220, 28, 27, 0.1)
+    g = ply.tet_geometry(pos, tets)
+    data = ply.write_ply_bytes(g)
+    w = _median_time(lambda: ply.write_ply_bytes(g), 3)
+    r = _median_time(lambda: ply.read_ply_bytes(data), 3)
+    mb = len(data) / 1e6
+    log(f"ply: {mb:.1f} MB, write {1e3 * w:.0f} ms, read {1e3 * r:.0f} ms")
+    cpu = None
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ref_ply
+    if ref_ply.available() and not args.no_cpu_baseline:
+        rw = _median_time(lambda: ref_ply.write(g.positions, g.indices, geometry_type=1, fmt=0), 1)
+        rr = _median_time(lambda: ref_ply.read(data), 1)
+        assert ref_ply.write(g.positions, g.indices, geometry_type=1, fmt=0) == data
+        cpu = {"value": round(mb / rr, 1), "unit": "MB/s read", "write_MB_per_s": round(mb / rw, 1), "cores": 1,
+               "kind": "reference", "sample": f"the same {mb:.1f} MB file, the reference's ply.cpp, on {cpu_model()}"}
+    return {"metric": "ASCII PLY read throughput, 1M-tet beam", "value": round(mb / r, 1), "unit": "MB/s",
+            "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "box6 220x28x27 tet mesh", "bytes": len(data)},
+            "write_MB_per_s": round(mb / w, 1), "threads": min(16, os.cpu_count() or 1), "cpu_baseline": cpu}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,11 +232,19 @@ def main():
     ap.add_argument("--no-graphs", action="store_true", help="stream launches instead of hipGraph replays")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run device copies of known size (to scale PMC byte counters)")
+    ap.add_argument("--row", default="step", choices=["step", "boundary", "grid_sdf", "ply"],
+                    help="step: the time step (the headline metric); the others measure the surface, grid SDF "
+                         "and PLY components on their own workloads")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the time step runs on a single MI355X in this build; "
               "multi-GPU sharding is a design (see DESIGN.md), not a measured path.", file=sys.stderr)
         sys.exit(2)
+    if args.row != "step":
+        out = {"boundary": row_boundary, "grid_sdf": row_grid_sdf, "ply": row_ply}[args.row](args)
+        out.update({"n_gpus": 1})
+        print(json.dumps(out), flush=True)
+        return
 
     import sbs_amd as S
     from sbs_amd import scenes
