@@ -401,7 +401,7 @@ def main():
         "residual_l2_last_substep": res,
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_{scene.groups[0].kind}",
+            "kernel": f"k_{scene.groups[0].kind}" + ("_hex" if scene.groups[0].clusters is not None else ""),
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
