@@ -288,7 +288,14 @@ def main():
         found_l, trials_l = autotune.tune_warm_start(sim, scene, net, target, frames=eval_frames, modes=("learned",),
                                                      log=log)
         found.update(found_l)
-        learned = {"found": found, "trials": trials + trials_l, "capture": {"iterations": cap[0], "scale": cap[1]}}
+        # per (mode, iterations): the trial with the smallest mean residual
+        best_trials = {}
+        for t in trials + trials_l:
+            k = (t["mode"], t["iterations"])
+            if k not in best_trials or t["residual"] < best_trials[k]["residual"]:
+                best_trials[k] = t
+        learned = {"found": found, "best_trials": sorted(best_trials.values(), key=lambda t: (t["mode"], t["iterations"])),
+                   "num_trials": len(trials) + len(trials_l), "capture": {"iterations": cap[0], "scale": cap[1]}}
         sim.restore()
 
     log(f"timing {args.steps} frames, plain")
@@ -305,7 +312,7 @@ def main():
                 continue
             sim.restore()
             sim.set_warm_start(mode, net if mode == "learned" else None)
-            sim.set_warm_start_scale(cfg["scale"])
+            sim.set_warm_start_scale(*autotune._scales(cfg["scale"]))
             sim.set_fallback(2.0 * target, scene.iterations)
             sim.step(scene.dt, cfg["iterations"], scene.substeps)  # graph capture
             sim.restore()

@@ -320,6 +320,9 @@ int sbs_mlp_train(sbs_mlp* net, uint32_t steps, uint32_t batch, float lr, uint64
 int sbs_gpu_set_warm_start(sbs_gpu* sim, int mode, sbs_mlp* net);
 /* Factor on the warm-start multipliers (default 1): lambda_init = scale * (the mode's multipliers). */
 int sbs_gpu_set_warm_start_scale(sbs_gpu* sim, double scale);
+/* Separate factors for the neo-Hookean deviatoric and hydrostatic multipliers (no reference
+ * counterpart; sbs_gpu_set_warm_start_scale sets both to one value). */
+int sbs_gpu_set_warm_start_scales(sbs_gpu* sim, double deviatoric, double hydrostatic);
 /* While stride > 0, every substep of sbs_gpu_step appends one training row (the 12 features at the
  * substep start, the change of the multiplier pair over the substep) for every stride-th
  * constraint of each neo-Hookean group to net's dataset, until its capacity is reached. Graphs

@@ -267,10 +267,11 @@ void orc_set_clusters(void* h, uint32_t group, const uint32_t* offsets, uint32_t
 }
 
 void orc_set_coloring(void* h, int order) { static_cast<simulation_t*>(h)->coloring = order; }
-void orc_set_warm_start(void* h, int mode, double scale)
+void orc_set_warm_start(void* h, int mode, double scale, double scale_h)
 {
     static_cast<simulation_t*>(h)->warm_start = mode;
     static_cast<simulation_t*>(h)->warm_scale = scale;
+    static_cast<simulation_t*>(h)->warm_scale_h = scale_h;
 }
 
 // colored != 0 selects the colored visiting order (and computes the coloring of every group).

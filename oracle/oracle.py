@@ -40,7 +40,7 @@ _SIGS = {
     "orc_set_order": (None, [_p, _i, _u32]),
     "orc_set_clusters": (None, [_p, _u32, _pu, _u32]),
     "orc_set_coloring": (None, [_p, _i]),
-    "orc_set_warm_start": (None, [_p, _i, _d]),
+    "orc_set_warm_start": (None, [_p, _i, _d, _d]),
     "orc_group_colors": (_u32, [_p, _u32, _pu]),
     "orc_step": (None, [_p, _d, _u32, _u32]),
     "orc_read_body": (None, [_p, _u32, _pd, _pd]),
@@ -268,10 +268,12 @@ class Simulation:
         """order: "hash" or "index" (sequential greedy in index order)."""
         self._L.orc_set_coloring(self.h, {"hash": 0, "index": 1}[order])
 
-    def set_warm_start(self, mode, scale=1.0):
+    def set_warm_start(self, mode, scale=1.0, hydrostatic=None):
         """mode: "none", "previous" (neo-Hookean multipliers carried over) or "explicit" (set to
-        -C/alpha~ at the predicted positions), times scale; their displacement is applied first."""
-        self._L.orc_set_warm_start(self.h, {"none": 0, "previous": 1, "explicit": 3}[mode], scale)
+        -C/alpha~ at the predicted positions), times scale (the hydrostatic multipliers times
+        `hydrostatic` when given); their displacement is applied first."""
+        self._L.orc_set_warm_start(self.h, {"none": 0, "previous": 1, "explicit": 3}[mode], scale,
+                                   scale if hydrostatic is None else hydrostatic)
 
     def set_order(self, colored, recolor_passes=2):
         self._L.orc_set_order(self.h, int(colored), recolor_passes)

@@ -238,7 +238,7 @@ void neohookean_constraint_t::add_multiplier_displacement(simulation_t const& si
         lagrange_h = -(determinant(F) - (1.0 + mu / lam)) / (1.0 / (lam * V0abs) / dt2);
     }
     lagrange *= sim.warm_scale;
-    lagrange_h *= sim.warm_scale;
+    lagrange_h *= sim.warm_scale_h;
     double const ld = wd < 1e-20 ? 0.0 : lagrange, lh = wh < 1e-20 ? 0.0 : lagrange_h;
     for (int k = 0; k < 4; ++k) d[bi][v[k]] += w[k] * (gd[k] * ld + gh[k] * lh);
 }

@@ -80,7 +80,7 @@ struct neohookean_constraint_t final : constraint_t {
     // Warm start: adds M^-1 (grad C_dev lambda + grad C_hyd lambda_h) at the current positions to
     // d[body][vertex]; multipliers unchanged.
     // With simulation_t::warm_start == 3 the multipliers are first set to -C / alpha~; then they are
-    // multiplied by simulation_t::warm_scale.
+    // multiplied by simulation_t::warm_scale (deviatoric) and warm_scale_h (hydrostatic).
     void add_multiplier_displacement(simulation_t const& sim, std::vector<std::vector<vec3>>& d);
     void residual_sq(simulation_t const& sim, double dt, double* acc) const override;
     double energy(simulation_t const& sim) const override;
@@ -163,7 +163,8 @@ struct simulation_t {
     // displacement M^-1 grad C(x~)^T lambda is added before the iterations.
     int warm_start = 0;
     double warm_dt = 0.0;
-    double warm_scale = 1.0;  // factor on the warm-start multipliers
+    double warm_scale = 1.0;    // factor on the warm-start multipliers (deviatoric)
+    double warm_scale_h = 1.0;  // factor on the hydrostatic warm-start multipliers
 
     // GS visiting order of the elastic constraints (indices into constraints)
     std::vector<uint32_t> order;

@@ -77,15 +77,17 @@ void launch_green(hipStream_t s, tet_group_dev g, pos4* xi, const pos4* xn, uint
 // M^-1 grad C^T lambda of the stored multipliers to xi, kApplyExplicit first sets the multipliers
 // to -C / alpha~ and stores them; both take every gradient at the predicted positions, passed as
 // `xn` (a copy of xi made before the pass), so the pass is a sum independent of visiting order.
-// In those modes beta_dt carries the factor the multipliers are scaled by (and stored).
+// In those modes beta_dt carries the factor the deviatoric multipliers are scaled by (and stored),
+// hyd_scale the hydrostatic multipliers' factor (unused by the solve modes).
 constexpr int kSolveNext = 0, kSolveFirst = 1, kApplyWarmStart = 2, kApplyExplicit = 3;
 void launch_neohookean(hipStream_t s, tet_group_dev g, pos4* xi, const pos4* xn, uint32_t begin,
-                       uint32_t count, float inv_dt2, float beta_dt, int mode, int block);
+                       uint32_t count, float inv_dt2, float beta_dt, int mode, int block, float hyd_scale = 1.0f);
 // One thread per cluster of clusters [begin, begin+count).
 void launch_green_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn,
                            uint32_t begin, uint32_t count, float alpha_tilde, float gamma, int first, int block);
 void launch_neohookean_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn,
-                                uint32_t begin, uint32_t count, float inv_dt2, float beta_dt, int mode, int block);
+                                uint32_t begin, uint32_t count, float inv_dt2, float beta_dt, int mode, int block,
+                                float hyd_scale = 1.0f);
 
 // Clusters that are the 6 tets of one hex cell of the box split, in its slot pattern
 // {0123, 0413, 0253, 0563, 0743, 0673}: particles held in registers. The color's constraints
@@ -94,7 +96,7 @@ void launch_green_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, 
                       uint32_t count, uint32_t tbase, float alpha_tilde, float gamma, int first, int block);
 void launch_neohookean_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn,
                            uint32_t begin, uint32_t count, uint32_t tbase, float inv_dt2, float beta_dt, int mode,
-                           int block);
+                           int block, float hyd_scale = 1.0f);
 // cluster_block value selecting the hex kernel that loads each tet's material just before
 // projecting it (fewer registers, more waves) instead of prefetching all six.
 constexpr int kHexStreamed = SBS_CLUSTER_BLOCK_STREAMED;
@@ -139,7 +141,7 @@ void launch_copy_pos(hipStream_t s, const pos4* a, pos4* b, uint32_t n);
 // pass. ccolor_off / color_off: the group's cluster and constraint offsets per color (ncolors + 1).
 void launch_hex_warm_start(hipStream_t s, tet_group_dev g, cluster_dev cl, const pos4* xt, pos4* xi,
                            const uint32_t* ccolor_off, const uint32_t* color_off, uint32_t ncolors, uint32_t nclusters,
-                           float inv_dt2, int expl, float scale, float4* disp, const uint32_t* csr_off,
+                           float inv_dt2, int expl, float2 scale, float4* disp, const uint32_t* csr_off,
                            const uint32_t* csr, uint32_t n);
 
 // Multiplier warm start of a neo-Hookean group, per constraint t (execution order) at the

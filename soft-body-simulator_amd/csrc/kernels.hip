@@ -287,11 +287,11 @@ __device__ __forceinline__ void neohookean_project(Pos P[4], XnOf&& xn_of, float
 // Warm-start displacement of one tet at positions P (the predicted positions): both gradients at
 // P, d_k = w_k (grad_k C_dev lm.x + grad_k C_hyd lm.y). With explicit the multipliers are first
 // set to -C(P) / alpha~, the pair that zeroes the XPBD residual at P; then they are multiplied by
-// scale (which the caller stores back). A constraint whose
+// scale (x: deviatoric, y: hydrostatic; the caller stores them back). A constraint whose
 // sum w|grad C|^2 is below the projection's epsilon contributes nothing, as it would not project.
 template <class Pos>
 __device__ __forceinline__ void neohookean_displacement(Pos const P[4], float4 a, float4 b, float4 c, float inv_dt2,
-                                                        bool expl, float scale, float2& lm, v3 d[4])
+                                                        bool expl, float2 scale, float2& lm, v3 d[4])
 {
     tet_rows const R = load_rows(a, b, c);
     tet_frame const f = make_frame(P, [](int) { return pos4{}; }, false);
@@ -304,7 +304,7 @@ __device__ __forceinline__ void neohookean_displacement(Pos const P[4], float4 a
         float const Ch = dot(F.c0, Gh.c0) - c.w;
         lm = make_float2(-Cd / (c.y * inv_dt2), -Ch / (c.z * inv_dt2));
     }
-    lm = make_float2(scale * lm.x, scale * lm.y);
+    lm = make_float2(scale.x * lm.x, scale.y * lm.y);
     v3 gd[4], gh[4];
     gd[0] = mul(Gd, R.r0);
     gd[1] = mul(Gd, R.r1);
@@ -336,7 +336,7 @@ __device__ __forceinline__ void add_displacement(pos4* x, v3 d)
 
 __global__ void __launch_bounds__(256) k_neohookean(tet_group_dev g, pos4* xi, const pos4* __restrict__ xn,
                                                     uint32_t begin, uint32_t count, float inv_dt2, float beta_dt,
-                                                    int mode)
+                                                    int mode, float hyd_scale)
 {
     uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
@@ -347,8 +347,9 @@ __global__ void __launch_bounds__(256) k_neohookean(tet_group_dev g, pos4* xi, c
     if (mode >= kApplyWarmStart) {  // gradients at the predicted positions (xn holds a copy of them)
         pos4 const P[4] = {xn[id.x], xn[id.y], xn[id.z], xn[id.w]};
         v3 d[4];
-        // in the warm-start modes beta_dt carries the multipliers' scale (no damping term applies)
-        neohookean_displacement(P, a, b, c, inv_dt2, mode == kApplyExplicit, beta_dt, lm, d);
+        // in the warm-start modes beta_dt carries the deviatoric multiplier's scale (no damping term
+        // applies), hyd_scale the hydrostatic one's
+        neohookean_displacement(P, a, b, c, inv_dt2, mode == kApplyExplicit, make_float2(beta_dt, hyd_scale), lm, d);
         reinterpret_cast<float2*>(g.lambda)[t] = lm;
         add_displacement(&xi[id.x], d[0]);
         add_displacement(&xi[id.y], d[1]);
@@ -532,7 +533,7 @@ __global__ void __launch_bounds__(BLOCK) k_neohookean_clusters(tet_group_dev g, 
 __global__ void __launch_bounds__(128) k_neohookean_clusters_apply(tet_group_dev g, cluster_dev cl, pos4* xi,
                                                                    const pos4* __restrict__ xt, uint32_t begin,
                                                                    uint32_t count, float inv_dt2, int expl,
-                                                                   float scale)
+                                                                   float2 scale)
 {
     uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
@@ -625,7 +626,7 @@ template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_neohookean_hex_apply(tet_group_dev g, cluster_dev cl, pos4* xi,
                                                                 const pos4* __restrict__ xt, uint32_t begin,
                                                                 uint32_t count, uint32_t tbase, float inv_dt2, int expl,
-                                                                float scale)
+                                                                float2 scale)
 {
     uint32_t const i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= count) return;
@@ -663,7 +664,7 @@ __global__ void __launch_bounds__(BLOCK) k_neohookean_hex_apply(tet_group_dev g,
 __global__ void __launch_bounds__(64) k_hex_warm_disp(tet_group_dev g, cluster_dev cl, const pos4* __restrict__ xi,
                                                       const uint32_t* __restrict__ ccolor_off,
                                                       const uint32_t* __restrict__ color_off, uint32_t ncolors,
-                                                      uint32_t nclusters, float inv_dt2, int expl, float scale,
+                                                      uint32_t nclusters, float inv_dt2, int expl, float2 scale,
                                                       float4* __restrict__ disp)
 {
     uint32_t const ci = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1107,10 +1108,11 @@ void launch_green(hipStream_t s, tet_group_dev g, pos4* xi, const pos4* xn, uint
 }
 
 void launch_neohookean(hipStream_t s, tet_group_dev g, pos4* xi, const pos4* xn, uint32_t begin, uint32_t count,
-                       float inv_dt2, float beta_dt, int mode, int block)
+                       float inv_dt2, float beta_dt, int mode, int block, float hyd_scale)
 {
     if (count)
-        k_neohookean<<<grid_for(count, block), block, 0, s>>>(g, xi, xn, begin, count, inv_dt2, beta_dt, mode);
+        k_neohookean<<<grid_for(count, block), block, 0, s>>>(g, xi, xn, begin, count, inv_dt2, beta_dt, mode,
+                                                              hyd_scale);
 }
 
 void launch_distance(hipStream_t s, distance_group_dev g, pos4* xi, const pos4* xn, uint32_t begin, uint32_t count,
@@ -1131,12 +1133,14 @@ void launch_green_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4*
 }
 
 void launch_neohookean_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn,
-                                uint32_t begin, uint32_t count, float inv_dt2, float beta_dt, int mode, int block)
+                                uint32_t begin, uint32_t count, float inv_dt2, float beta_dt, int mode, int block,
+                                float hyd_scale)
 {
     if (!count) return;
     if (mode >= kApplyWarmStart) {
         k_neohookean_clusters_apply<<<grid_for(count, 128), 128, 0, s>>>(g, cl, xi, xn, begin, count, inv_dt2,
-                                                                         mode == kApplyExplicit, beta_dt);
+                                                                         mode == kApplyExplicit,
+                                                                         make_float2(beta_dt, hyd_scale));
         return;
     }
     if (block <= 64)
@@ -1147,12 +1151,14 @@ void launch_neohookean_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, 
 }
 
 void launch_neohookean_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn, uint32_t begin,
-                           uint32_t count, uint32_t tbase, float inv_dt2, float beta_dt, int mode, int block)
+                           uint32_t count, uint32_t tbase, float inv_dt2, float beta_dt, int mode, int block,
+                           float hyd_scale)
 {
     if (!count) return;
     if (mode >= kApplyWarmStart) {
         k_neohookean_hex_apply<64><<<grid_for(count, 64), 64, 0, s>>>(g, cl, xi, xn, begin, count, tbase, inv_dt2,
-                                                                       mode == kApplyExplicit, beta_dt);
+                                                                       mode == kApplyExplicit,
+                                                                       make_float2(beta_dt, hyd_scale));
         return;
     }
     // block 64 / 128: material prefetched; kHexStreamed: 64 threads, material loaded per tet, at
@@ -1250,7 +1256,7 @@ void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t value, uint32_t n)
 
 void launch_hex_warm_start(hipStream_t s, tet_group_dev g, cluster_dev cl, const pos4* xt, pos4* xi,
                            const uint32_t* ccolor_off, const uint32_t* color_off, uint32_t ncolors, uint32_t nclusters,
-                           float inv_dt2, int expl, float scale, float4* disp, const uint32_t* csr_off,
+                           float inv_dt2, int expl, float2 scale, float4* disp, const uint32_t* csr_off,
                            const uint32_t* csr, uint32_t n)
 {
     if (!nclusters) return;

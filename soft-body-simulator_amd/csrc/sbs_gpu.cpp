@@ -151,7 +151,8 @@ struct sbs_gpu {
     std::array<double, 16> graph_key{};
 
     int ws_mode = SBS_WARM_START_NONE;
-    double ws_scale = 1.0;
+    double ws_scale = 1.0;      // deviatoric multipliers' warm-start factor
+    double ws_scale_hyd = 1.0;  // hydrostatic multipliers' factor
     sbs_mlp* ws_net = nullptr;
     sbs_mlp* cap_net = nullptr;
     uint32_t cap_stride = 0;
@@ -675,17 +676,18 @@ void sbs_gpu::launch_group_colors(group_t const& g, double dt, int mode, const p
                              cluster_block);
         else if (g.hex && g.kind == 1)
             launch_neohookean_hex(stream, g.tet(), g.clusters(), xi.p, xref, cb, cn, b, float(1.0 / dt2),
-                                  bdt, mode, cluster_block);
+                                  bdt, mode, cluster_block, float(ws_scale_hyd));
         else if (g.clustered() && g.kind == 0)
             launch_green_clusters(stream, g.tet(), g.clusters(), xi.p, xref, cb, cn, float(at), float(gamma), first,
                                   cluster_block);
         else if (g.clustered() && g.kind == 1)
             launch_neohookean_clusters(stream, g.tet(), g.clusters(), xi.p, xref, cb, cn, float(1.0 / dt2),
-                                       bdt, mode, cluster_block);
+                                       bdt, mode, cluster_block, float(ws_scale_hyd));
         else if (g.kind == 0)
             launch_green(stream, g.tet(), xi.p, xref, b, n, float(at), float(gamma), first, block);
         else if (g.kind == 1)
-            launch_neohookean(stream, g.tet(), xi.p, xref, b, n, float(1.0 / dt2), bdt, mode, block);
+            launch_neohookean(stream, g.tet(), xi.p, xref, b, n, float(1.0 / dt2), bdt, mode, block,
+                              float(ws_scale_hyd));
         else
             launch_distance(stream, g.dist(), xi.p, xref, b, n, float(at), float(gamma), first, block);
     }
@@ -741,7 +743,8 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
                     timed(SBS_KERNEL_WARM_START, g.count, [&] {
                         launch_hex_warm_start(stream, g.tet(), g.clusters(), xtilde, xi.p, g.ws_ccolor_off.p, g.ws_color_off.p,
                                               g.ncolors, g.num_elements(), float(1.0 / dt2), mode == kApplyExplicit,
-                                              float(ws_scale), g.ws_disp.p, g.ws_csr_off.p, g.ws_csr.p, V);
+                                              make_float2(float(ws_scale), float(ws_scale_hyd)), g.ws_disp.p,
+                                              g.ws_csr_off.p, g.ws_csr.p, V);
                     }, 2);
                 else
                     timed(SBS_KERNEL_WARM_START, g.count, [&] { launch_group_colors(g, dt, mode, xt.p); }, g.ncolors);
@@ -907,6 +910,7 @@ void sbs_gpu::run_frame(double dt, uint32_t iterations, uint32_t substeps, int w
         key[11] = double(reinterpret_cast<uintptr_t>(warm == SBS_WARM_START_LEARNED ? ws_net : nullptr));
         key[12] = ws_scale;
         key[13] = surface_refresh;
+        key[14] = ws_scale_hyd;
         if (!graph_exec || key != graph_key) {
             if (graph_exec) {
                 // the previous graph may still be running on the stream: let it finish first
@@ -1810,6 +1814,18 @@ int sbs_gpu_set_warm_start_scale(sbs_gpu* sim, double scale)
         require(sim != nullptr, "sim is null");
         require(std::isfinite(scale) && scale >= 0.0, "scale must be finite and >= 0");
         sim->ws_scale = scale;
+        sim->ws_scale_hyd = scale;
+    });
+}
+
+int sbs_gpu_set_warm_start_scales(sbs_gpu* sim, double deviatoric, double hydrostatic)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(std::isfinite(deviatoric) && deviatoric >= 0.0 && std::isfinite(hydrostatic) && hydrostatic >= 0.0,
+                "scales must be finite and >= 0");
+        sim->ws_scale = deviatoric;
+        sim->ws_scale_hyd = hydrostatic;
     });
 }
 

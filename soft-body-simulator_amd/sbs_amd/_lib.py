@@ -131,6 +131,7 @@ _SIGS = {
     "sbs_gpu_extract_boundary": [_i, _u32, _pu, _u32, _pu, _pu, _pu, _pu],
     "sbs_gpu_capture": [_p, _p, _u32],
     "sbs_gpu_set_warm_start_scale": [_p, _d],
+    "sbs_gpu_set_warm_start_scales": [_p, _d, _d],
     "sbs_gpu_snapshot": [_p],
     "sbs_gpu_restore": [_p],
     "sbs_gpu_set_fallback": [_p, _d, _u32],
@@ -600,8 +601,13 @@ class Simulation:
         """mode: "none", "previous", "explicit" or "learned" (needs net, an Mlp(12, 2))."""
         check(self._L.sbs_gpu_set_warm_start(self.h, WARM_START[mode], None if net is None else net.h))
 
-    def set_warm_start_scale(self, scale):
-        check(self._L.sbs_gpu_set_warm_start_scale(self.h, scale))
+    def set_warm_start_scale(self, scale, hydrostatic=None):
+        """Warm-start factor of the multipliers; with `hydrostatic`, scale applies to the deviatoric
+        multipliers and `hydrostatic` to the hydrostatic ones."""
+        if hydrostatic is None:
+            check(self._L.sbs_gpu_set_warm_start_scale(self.h, scale))
+        else:
+            check(self._L.sbs_gpu_set_warm_start_scales(self.h, scale, hydrostatic))
 
     def capture(self, net, stride):
         check(self._L.sbs_gpu_capture(self.h, None if net is None else net.h, stride))

@@ -62,18 +62,30 @@ def tune_launch(sim, scene, frames=5, blocks=(128, 256), cluster_blocks=(64, 128
     return {"block": best[1], "cluster_block": best[2], "ms_per_frame": 1e3 * best[0] / frames}
 
 
+def _scales(s):
+    """A warm-start scale: one factor, or a (deviatoric, hydrostatic) pair."""
+    return tuple(s) if isinstance(s, (tuple, list)) else (s,)
+
+
+# Warm-start scales the tuner tries: pairs (deviatoric, hydrostatic) on a 0.05 grid. The stable
+# neo-Hookean pair responds differently to carried-over multipliers: on the 1M beam a deviatoric
+# factor near 0.7 with a hydrostatic one near 0.65 reaches the plain residual with fewer iterations
+# than any single factor.
+SCALE_GRID = tuple((d, h) for d in (0.5, 0.55, 0.6, 0.65, 0.7, 0.75) for h in (0.5, 0.55, 0.6, 0.65, 0.7, 0.75))
+
+
 def train_warm_start(sim, scene, frames=6, iterations=24, scale=0.5, max_rows=4_000_000, steps=1500, batch=16384,
                      lr=1e-3, seed=0, log=print):
-    """Captures training rows from a previous-multiplier warm-started trajectory (`scale`,
-    `iterations` per substep) starting at the current state, trains a 12-input, 2-output network on
-    the device and restores the state. Returns the network."""
+    """Captures training rows from a previous-multiplier warm-started trajectory (`scale`, one factor
+    or a (deviatoric, hydrostatic) pair, `iterations` per substep) starting at the current state,
+    trains a 12-input, 2-output network on the device and restores the state. Returns the network."""
     rows_per_substep = sum(len(g.conn) for g in scene.groups if g.kind == "neohookean")
     stride = max(1, -(-rows_per_substep * scene.substeps * frames // max_rows))
     net = Mlp(12, 2, seed=seed)
     net.reserve(max_rows)
     sim.snapshot()
     sim.set_warm_start("previous")
-    sim.set_warm_start_scale(scale)
+    sim.set_warm_start_scale(*_scales(scale))
     sim.capture(net, stride)
     for _ in range(frames):
         sim.step(scene.dt, iterations, scene.substeps)
@@ -89,11 +101,12 @@ def train_warm_start(sim, scene, frames=6, iterations=24, scale=0.5, max_rows=4_
     return net
 
 
-def tune_warm_start(sim, scene, net, target, frames=8, scales=(0.3, 0.4, 0.5, 0.6, 0.75), iterations=(2, 3, 4, 5, 6, 7),
+def tune_warm_start(sim, scene, net, target, frames=8, scales=SCALE_GRID, iterations=(2, 3, 4, 5, 6, 7),
                     modes=("learned", "previous"), log=print):
     """For each mode, the cheapest (iterations, scale) whose mean frame residual from the current
-    state is <= target. Returns {mode: {"iterations", "scale", "residual"} or None} and leaves the
-    simulation at the state it started from, warm start off."""
+    state is <= target; a scale is one factor or a (deviatoric, hydrostatic) pair. Returns
+    {mode: {"iterations", "scale", "residual"} or None} and leaves the simulation at the state it
+    started from, warm start off."""
     sim.snapshot()
     found = {}
     trials = []
@@ -103,7 +116,7 @@ def tune_warm_start(sim, scene, net, target, frames=8, scales=(0.3, 0.4, 0.5, 0.
             for s in scales:
                 sim.restore()
                 sim.set_warm_start(mode, net if mode == "learned" else None)
-                sim.set_warm_start_scale(s)
+                sim.set_warm_start_scale(*_scales(s))
                 r = mean_residual(sim, scene, frames, k)
                 trials.append({"mode": mode, "iterations": k, "scale": s, "residual": r})
                 log(f"  {mode} x{s} {k} iterations: mean residual {r:.4g} (target {target:.4g})")

@@ -83,16 +83,19 @@ def _beam(cluster, iterations=8):
 # and the device both blow up within a few frames). The previous-multiplier start at scale 0.5 is;
 # the explicit start (an explicit elastic step) blows up within 3 frames even at 0.5, so it is
 # compared over the first frame only.
-@pytest.mark.parametrize("mode,scale,frames", [("previous", 1.0, 2), ("previous", 0.5, 20), ("explicit", 0.5, 1)])
+# A pair of scales applies the first to the deviatoric, the second to the hydrostatic multipliers.
+@pytest.mark.parametrize("mode,scale,frames", [("previous", 1.0, 2), ("previous", 0.5, 20), ("explicit", 0.5, 1),
+                                               ("previous", (0.7, 0.65), 20), ("explicit", (0.5, 0.2), 1)])
 @pytest.mark.parametrize("cluster", [True, False])
 def test_warm_start_matches_oracle(mode, scale, frames, cluster):
     sc = _beam(cluster)
     say(f"warm start {mode} x{scale} cluster={cluster}")
+    scales = scale if isinstance(scale, tuple) else (scale,)
     dev = scenes.build_device(sc)
     dev.set_warm_start(mode)
-    dev.set_warm_start_scale(scale)
+    dev.set_warm_start_scale(*scales)
     orc = build_oracle(sc)
-    orc.set_warm_start(mode, scale)
+    orc.set_warm_start(mode, *scales)
     diag = sc.bbox_diagonal()
     for f in range(1, frames + 1):
         dev.step(sc.dt, sc.iterations, sc.substeps)
