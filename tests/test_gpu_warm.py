@@ -83,9 +83,11 @@ def _beam(cluster, iterations=8):
 # and the device both blow up within a few frames). The previous-multiplier start at scale 0.5 is;
 # the explicit start (an explicit elastic step) blows up within 3 frames even at 0.5, so it is
 # compared over the first frame only.
-# A pair of scales applies the first to the deviatoric, the second to the hydrostatic multipliers.
+# A pair of scales applies the first to the deviatoric, the second to the hydrostatic multipliers
+# ((0.7, 0.65), the pair the tuner finds on the 1M beam, blows up on this small beam in the oracle
+# as on the device; (0.4, 0.55) is stable here).
 @pytest.mark.parametrize("mode,scale,frames", [("previous", 1.0, 2), ("previous", 0.5, 20), ("explicit", 0.5, 1),
-                                               ("previous", (0.7, 0.65), 20), ("explicit", (0.5, 0.2), 1)])
+                                               ("previous", (0.4, 0.55), 20), ("explicit", (0.5, 0.2), 1)])
 @pytest.mark.parametrize("cluster", [True, False])
 def test_warm_start_matches_oracle(mode, scale, frames, cluster):
     sc = _beam(cluster)
