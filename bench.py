@@ -281,11 +281,13 @@ def main():
         # trajectory (or a 24-iteration one when it never reaches the target), the multipliers the
         # iterations end with, and is tuned the same way
         found, trials = autotune.tune_warm_start(sim, scene, None, target, frames=eval_frames, modes=("previous",),
+                                                 verify_frames=args.steps,
                                                  log=log)
         prev = found["previous"]
         cap = (prev["iterations"], prev["scale"]) if prev else (24, 0.5)
         net = autotune.train_warm_start(sim, scene, iterations=cap[0], scale=cap[1], log=log)
         found_l, trials_l = autotune.tune_warm_start(sim, scene, net, target, frames=eval_frames, modes=("learned",),
+                                                     verify_frames=args.steps,
                                                      log=log)
         found.update(found_l)
         # per (mode, iterations): the trial with the smallest mean residual
