@@ -162,16 +162,20 @@ __device__ inline double grid_eval(sdf_dev const& f, double px, double py, doubl
     return phi;
 }
 
-// Mesh data of the node-distance kernel: non-degenerate triangles (3 vertex ids), their corner
-// positions (9 doubles each), and the pseudonormals of vertices, edges (3 per triangle: (0,1),
-// (1,2), (2,0)) and faces.
+// Mesh data of the node-distance kernel: non-degenerate triangles (3 vertex ids) and the
+// pseudonormals of vertices, edges (3 per triangle: (0,1), (1,2), (2,0)) and faces, indexed by
+// triangle; the triangles' corner positions (9 doubles each) in a spatial order with the original
+// index of each, and the bounding boxes (lo xyz, hi xyz) of tiles of consecutive triangles in
+// that order.
 struct mesh_dev {
     const uint32_t* tri;
     const double* corners;
     const double* vnormal;
     const double* enormal;
     const double* fnormal;
-    uint32_t m;
+    const uint32_t* orig;
+    const double* tile_box;
+    uint32_t m, ntiles;
 };
 
 void launch_grid_nodes(hipStream_t s, mesh_dev mesh, const uint32_t res[3], const double dmin[3],

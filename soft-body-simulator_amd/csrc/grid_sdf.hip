@@ -10,8 +10,9 @@
 // of the faces sharing the edge, or the face normal. Zero-area triangles are left out.
 //
 // One thread per grid node; the triangles stream through LDS in tiles shared by the 256 threads of
-// a block. The work is FP64 arithmetic (about 60 operations per node and triangle), bounded by the
-// FP64 vector rate, not by memory.
+// a block, tiles that cannot hold a nearer triangle for any node of the block skipped (below). The
+// work is FP64 arithmetic (about 60 operations per node and triangle), bounded by the FP64 vector
+// rate, not by memory.
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -93,40 +94,94 @@ __device__ __forceinline__ void closest_point(const double p[3], const double* t
     for (int d = 0; d < 3; ++d) q[d] = t[d] + ab[d] * v + ac[d] * w;
 }
 
+// Squared distance from p to the box lo..hi (0 inside).
+__device__ __forceinline__ double box_dist2(const double p[3], const double* box)
+{
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double const e = p[d] < box[d] ? box[d] - p[d] : (p[d] > box[3 + d] ? p[d] - box[3 + d] : 0.0);
+        s += e * e;
+    }
+    return s;
+}
+
+// One thread per node. The triangles are in Morton order of their centroids, in tiles of kTile
+// with bounding boxes. A block starts at the tile nearest its first node and visits the others
+// cyclically; a tile is loaded into LDS only when some node of the block may have a nearer
+// triangle in it (box distance^2 <= best * (1 + 1e-9), a margin well above the rounding of both
+// sides), and a node skips the tile otherwise. The nearest triangle is the smallest (squared
+// distance, original index) pair, so the visiting order does not change the result: it equals the
+// scan of every triangle in index order keeping the first nearest.
 __global__ void __launch_bounds__(kBlock) k_grid_nodes(mesh_dev mesh, grid_params gp, uint64_t n,
                                                        double* __restrict__ out)
 {
 #pragma clang fp contract(off)
     __shared__ double tc[kTile * 9];
+    __shared__ uint32_t to[kTile];
+    __shared__ double red_d[kBlock];
+    __shared__ uint32_t red_i[kBlock];
     uint64_t const l = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
     bool const valid = l < n;
     grid_dims const g(gp.res);
     double p[3] = {0.0, 0.0, 0.0};
     if (valid) grid_node_position(g, gp.dmin, gp.cell, l, p);
+    // the tile nearest the block's first node
+    double p0[3];
+    grid_node_position(g, gp.dmin, gp.cell, uint64_t(blockIdx.x) * kBlock, p0);
+    double md = DBL_MAX;
+    uint32_t mi = 0;
+    for (uint32_t i = threadIdx.x; i < mesh.ntiles; i += kBlock) {
+        double const dd = box_dist2(p0, mesh.tile_box + 6 * std::size_t(i));
+        if (dd < md) {
+            md = dd;
+            mi = i;
+        }
+    }
+    red_d[threadIdx.x] = md;
+    red_i[threadIdx.x] = mi;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            double const od = red_d[threadIdx.x + w];
+            uint32_t const oi = red_i[threadIdx.x + w];
+            if (od < red_d[threadIdx.x] || (od == red_d[threadIdx.x] && oi < red_i[threadIdx.x])) {
+                red_d[threadIdx.x] = od;
+                red_i[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    uint32_t const start = red_i[0];
     double best = DBL_MAX, bq[3] = {0.0, 0.0, 0.0};
-    uint32_t bf = 0;
+    uint32_t bf = 0xffffffffu;
     int bfeat = 6;
-    for (uint32_t t0 = 0; t0 < mesh.m; t0 += kTile) {
-        uint32_t const cnt = min(uint32_t(kTile), mesh.m - t0);
-        __syncthreads();
+    for (uint32_t u = 0; u < mesh.ntiles; ++u) {
+        uint32_t const tile = start + u < mesh.ntiles ? start + u : start + u - mesh.ntiles;
+        bool const need = valid && box_dist2(p, mesh.tile_box + 6 * std::size_t(tile)) <= best * (1.0 + 1e-9);
+        if (!__syncthreads_or(need)) continue;
+        uint32_t const t0 = tile * kTile, cnt = min(uint32_t(kTile), mesh.m - t0);
         for (uint32_t w = threadIdx.x; w < cnt * 9; w += kBlock) tc[w] = mesh.corners[std::size_t(t0) * 9 + w];
+        for (uint32_t w = threadIdx.x; w < cnt; w += kBlock) to[w] = mesh.orig[t0 + w];
         __syncthreads();
-        if (valid)
+        if (need)
             for (uint32_t t = 0; t < cnt; ++t) {
                 double q[3];
                 int feat;
                 closest_point(p, tc + 9 * t, q, &feat);
                 double const e0 = p[0] - q[0], e1 = p[1] - q[1], e2 = p[2] - q[2];
                 double const d2 = e0 * e0 + e1 * e1 + e2 * e2;
-                if (d2 < best) {
+                uint32_t const o = to[t];
+                if (d2 < best || (d2 == best && o < bf)) {
                     best = d2;
-                    bf = t0 + t;
+                    bf = o;
                     bfeat = feat;
                     bq[0] = q[0];
                     bq[1] = q[1];
                     bq[2] = q[2];
                 }
             }
+        __syncthreads();
     }
     if (!valid) return;
     if (mesh.m == 0) {
@@ -159,9 +214,9 @@ void launch_grid_nodes(hipStream_t s, mesh_dev mesh, const uint32_t res[3], cons
 // Host side of the mesh: non-degenerate triangles, corners and pseudonormals (FP64, no FMA
 // contraction), uploaded to the device.
 struct mesh_upload {
-    dev_buf<uint32_t> tri;
-    dev_buf<double> corners, vnormal, enormal, fnormal;
-    uint32_t m = 0;
+    dev_buf<uint32_t> tri, orig;
+    dev_buf<double> corners, vnormal, enormal, fnormal, tile_box;
+    uint32_t m = 0, ntiles = 0;
 
     mesh_upload(hipStream_t s, const double* pos, uint32_t nv, const uint32_t* tris, uint32_t nt)
     {
@@ -228,14 +283,65 @@ struct mesh_upload {
                 auto const& es = edge_sum[{std::min(v, w), std::max(v, w)}];
                 for (int d = 0; d < 3; ++d) en[9 * std::size_t(f) + 3 * k + d] = es[d];
             }
+        // triangles in Morton order of their centroids (10 bits per axis over the mesh's box),
+        // corners in that order, each with its original index; bounding boxes of tiles of kTile
+        std::vector<uint32_t> ord(m);
+        {
+            double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+            for (std::size_t i = 0; i < 9 * std::size_t(m); ++i) {
+                lo[i % 3] = std::min(lo[i % 3], c[i]);
+                hi[i % 3] = std::max(hi[i % 3], c[i]);
+            }
+            auto spread = [](uint32_t v) {
+                uint64_t x = v & 0x3ffu;
+                x = (x | (x << 16)) & 0x30000ffull;
+                x = (x | (x << 8)) & 0x300f00full;
+                x = (x | (x << 4)) & 0x30c30c3ull;
+                x = (x | (x << 2)) & 0x9249249ull;
+                return x;
+            };
+            std::vector<std::pair<uint64_t, uint32_t>> key(m);
+            for (uint32_t f = 0; f < m; ++f) {
+                uint64_t code = 0;
+                for (int d = 0; d < 3; ++d) {
+                    double const cen = (c[9 * std::size_t(f) + d] + c[9 * std::size_t(f) + 3 + d] +
+                                        c[9 * std::size_t(f) + 6 + d]) / 3.0;
+                    double const ext = hi[d] - lo[d];
+                    double const u = ext > 0.0 ? (cen - lo[d]) / ext : 0.0;
+                    uint32_t const q = uint32_t(std::min(1023.0, std::max(0.0, u * 1024.0)));
+                    code |= spread(q) << d;
+                }
+                key[f] = {code, f};
+            }
+            std::sort(key.begin(), key.end());
+            for (uint32_t i = 0; i < m; ++i) ord[i] = key[i].second;
+        }
+        std::vector<double> cs(9 * std::size_t(m));
+        for (uint32_t i = 0; i < m; ++i)
+            std::copy(c.begin() + 9 * std::size_t(ord[i]), c.begin() + 9 * std::size_t(ord[i]) + 9,
+                      cs.begin() + 9 * std::size_t(i));
+        ntiles = (m + kTile - 1) / kTile;
+        std::vector<double> box(6 * std::size_t(ntiles));
+        for (uint32_t tl = 0; tl < ntiles; ++tl) {
+            double* b = box.data() + 6 * std::size_t(tl);
+            b[0] = b[1] = b[2] = DBL_MAX;
+            b[3] = b[4] = b[5] = -DBL_MAX;
+            for (uint32_t i = tl * kTile; i < std::min(m, (tl + 1) * kTile); ++i)
+                for (int k = 0; k < 9; ++k) {
+                    b[k % 3] = std::min(b[k % 3], cs[9 * std::size_t(i) + k]);
+                    b[3 + k % 3] = std::max(b[3 + k % 3], cs[9 * std::size_t(i) + k]);
+                }
+        }
         tri.upload(t.data(), t.size(), s);
-        corners.upload(c.data(), c.size(), s);
+        corners.upload(cs.data(), cs.size(), s);
+        orig.upload(ord.data(), ord.size(), s);
+        tile_box.upload(box.data(), box.size(), s);
         vnormal.upload(vn.data(), vn.size(), s);
         enormal.upload(en.data(), en.size(), s);
         fnormal.upload(fn.data(), fn.size(), s);
         hip_check(hipStreamSynchronize(s), "mesh upload");
     }
-    mesh_dev view() const { return {tri.p, corners.p, vnormal.p, enormal.p, fnormal.p, m}; }
+    mesh_dev view() const { return {tri.p, corners.p, vnormal.p, enormal.p, fnormal.p, orig.p, tile_box.p, m, ntiles}; }
 };
 
 void grid_cell_size(const double dmin[3], const double dmax[3], const uint32_t res[3], double cell[3],
