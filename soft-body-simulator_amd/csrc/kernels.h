@@ -47,6 +47,16 @@ struct cluster_dev {
 
 constexpr int kClusterSlots = 8;
 
+// The clusters of one color split over the 8 XCDs. Workgroups are dispatched to the XCDs round-robin
+// (workgroup b runs on XCD b % 8), so the hex kernels run the color's clusters [off[x], off[x+1])
+// in workgroups b with b % 8 == x. The host orders every color's clusters by spatial region (8
+// regions of the group's clusters in Morton order), so an XCD keeps working on the same particles
+// from color to color and from launch to launch, and finds them in its own L2.
+struct xcd_split {
+    uint32_t off[9];
+};
+constexpr int kXcds = 8;
+
 struct distance_group_dev {
     const uint2* idx;
     const float* rest;
@@ -95,8 +105,8 @@ void launch_neohookean_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, 
 void launch_green_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn, uint32_t begin,
                       uint32_t count, uint32_t tbase, float alpha_tilde, float gamma, int first, int block);
 void launch_neohookean_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn,
-                           uint32_t begin, uint32_t count, uint32_t tbase, float inv_dt2, float beta_dt, int mode,
-                           int block, float hyd_scale = 1.0f);
+                           uint32_t begin, uint32_t count, uint32_t tbase, xcd_split const& sp, float inv_dt2,
+                           float beta_dt, int mode, int block, float hyd_scale = 1.0f);
 // cluster_block value selecting the hex kernel that loads each tet's material just before
 // projecting it (fewer registers, more waves) instead of prefetching all six.
 constexpr int kHexStreamed = SBS_CLUSTER_BLOCK_STREAMED;

@@ -8,6 +8,8 @@
 // the body is: every constraint forms its edge differences in FP64, does all of its arithmetic in
 // FP32 on those differences, and adds its FP32 correction back in FP64. Constraint data of a group
 // is stored in execution (color) order so that one color is one contiguous, coalesced range.
+#include <algorithm>
+
 #include "device_math.h"
 #include "kernels.h"
 #include "sdf.h"
@@ -571,11 +573,12 @@ constexpr int kHexSlots[kHexTets][4] = {{0, 1, 2, 3}, {0, 4, 1, 3}, {0, 2, 5, 3}
 template <int BLOCK, bool PREFETCH, int MIN_WAVES, bool DAMPED>
 __global__ void __launch_bounds__(BLOCK, MIN_WAVES) k_neohookean_hex(tet_group_dev g, cluster_dev cl, pos4* xi,
                                                                      const pos4* __restrict__ xn, uint32_t begin,
-                                                                     uint32_t count, uint32_t tbase, float inv_dt2,
-                                                                     float beta_dt, int mode)
+                                                                     uint32_t count, uint32_t tbase, xcd_split sp,
+                                                                     float inv_dt2, float beta_dt, int mode)
 {
-    uint32_t const i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= count) return;
+    uint32_t const x = blockIdx.x % kXcds, local = (blockIdx.x / kXcds) * BLOCK + threadIdx.x;
+    if (local >= sp.off[x + 1] - sp.off[x]) return;
+    uint32_t const i = sp.off[x] + local;
     uint32_t const c = begin + i;
     int4 const va = cl.v0[c], vb = cl.v1[c];
     int const ids[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
@@ -1151,8 +1154,8 @@ void launch_neohookean_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, 
 }
 
 void launch_neohookean_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn, uint32_t begin,
-                           uint32_t count, uint32_t tbase, float inv_dt2, float beta_dt, int mode, int block,
-                           float hyd_scale)
+                           uint32_t count, uint32_t tbase, xcd_split const& sp, float inv_dt2, float beta_dt, int mode,
+                           int block, float hyd_scale)
 {
     if (!count) return;
     if (mode >= kApplyWarmStart) {
@@ -1166,7 +1169,9 @@ void launch_neohookean_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4*
     // variants that carry the xn differences.
     bool const damped = beta_dt != 0.0f;
     auto go = [&](auto kernel, int b) {
-        kernel<<<grid_for(count, b), b, 0, s>>>(g, cl, xi, xn, begin, count, tbase, inv_dt2, beta_dt, mode);
+        uint32_t most = 0;  // workgroups per XCD
+        for (int x = 0; x < kXcds; ++x) most = std::max(most, uint32_t(grid_for(sp.off[x + 1] - sp.off[x], b)));
+        if (most) kernel<<<most * kXcds, b, 0, s>>>(g, cl, xi, xn, begin, count, tbase, sp, inv_dt2, beta_dt, mode);
     };
     if (block == kHexStreamed)
         damped ? go(k_neohookean_hex<64, false, 2, true>, 64) : go(k_neohookean_hex<64, false, 3, false>, 64);

@@ -53,6 +53,7 @@ struct group_t {
     std::vector<uint32_t> color;          // per constraint, insertion order
     std::vector<uint32_t> color_offsets;  // ncolors + 1, constraint positions in execution order
     std::vector<uint32_t> cluster_color_offsets;  // ncolors + 1, cluster positions in execution order
+    std::vector<xcd_split> xsplit;  // per color: its clusters by XCD (offsets inside the color)
     dev_buf<uint32_t> cl_off, cl_loc;
     dev_buf<int4> cl_v0, cl_v1;
     bool hex = false;  // every cluster is a hex cell in the fixed slot pattern
@@ -382,6 +383,62 @@ void sbs_gpu::upload_group(group_t& g)
         std::vector<uint32_t> fill(g.cluster_color_offsets.begin(), g.cluster_color_offsets.end() - 1);
         for (uint32_t e = 0; e < ne; ++e) eorder[fill[g.color[e_begin(e)]]++] = e;
     }
+    if (g.clustered()) {
+        // XCD affinity: the clusters in Morton order of their first constraint's rest centroid,
+        // split into 8 regions of equal count; inside each color, clusters by region (then Morton
+        // rank), with the region boundaries as the color's XCD split. Elements of one color are
+        // independent, so their order does not change the result.
+        std::vector<std::array<double, 3>> cen(ne);
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (uint32_t e = 0; e < ne; ++e) {
+            uint32_t const c = e_begin(e);
+            body_t const& b = bodies[g.body_of(c)];
+            for (int d = 0; d < 3; ++d) {
+                double m = 0.0;
+                for (int k = 0; k < 4; ++k) m += b.x0[3 * std::size_t(g.conn[4 * std::size_t(c) + k]) + d];
+                cen[e][d] = 0.25 * m;
+                lo[d] = std::min(lo[d], cen[e][d]);
+                hi[d] = std::max(hi[d], cen[e][d]);
+            }
+        }
+        auto spread = [](uint64_t v) {
+            v &= 0x1fffffull;
+            v = (v | (v << 32)) & 0x1f00000000ffffull;
+            v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+            v = (v | (v << 8)) & 0x100f00f00f00f00full;
+            v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+            v = (v | (v << 2)) & 0x1249249249249249ull;
+            return v;
+        };
+        std::vector<std::pair<uint64_t, uint32_t>> key(ne);
+        for (uint32_t e = 0; e < ne; ++e) {
+            uint64_t code = 0;
+            for (int d = 0; d < 3; ++d) {
+                double const ext = hi[d] - lo[d];
+                double const u = ext > 0.0 ? (cen[e][d] - lo[d]) / ext : 0.0;
+                code |= spread(uint64_t(std::min(2097151.0, std::max(0.0, u * 2097152.0)))) << d;
+            }
+            key[e] = {code, e};
+        }
+        std::sort(key.begin(), key.end());
+        std::vector<uint32_t> rank(ne), region(ne);
+        for (uint32_t r = 0; r < ne; ++r) rank[key[r].second] = r;
+        for (uint32_t e = 0; e < ne; ++e) region[e] = uint32_t(uint64_t(rank[e]) * kXcds / ne);
+        g.xsplit.assign(g.ncolors, xcd_split{});
+        for (uint32_t c = 0; c < g.ncolors; ++c) {
+            uint32_t const cb = g.cluster_color_offsets[c], ce = g.cluster_color_offsets[c + 1];
+            std::sort(eorder.begin() + cb, eorder.begin() + ce, [&](uint32_t a, uint32_t b) {
+                return region[a] != region[b] ? region[a] < region[b] : rank[a] < rank[b];
+            });
+            xcd_split& sp = g.xsplit[c];
+            uint32_t j = cb;
+            for (int x = 0; x <= kXcds; ++x) {
+                while (j < ce && int(region[eorder[j]]) < x) ++j;
+                sp.off[x] = j - cb;
+            }
+            sp.off[kXcds] = ce - cb;
+        }
+    }
     std::vector<uint32_t> order;
     order.reserve(n);
     for (uint32_t e : eorder)
@@ -675,8 +732,8 @@ void sbs_gpu::launch_group_colors(group_t const& g, double dt, int mode, const p
             launch_green_hex(stream, g.tet(), g.clusters(), xi.p, xref, cb, cn, b, float(at), float(gamma), first,
                              cluster_block);
         else if (g.hex && g.kind == 1)
-            launch_neohookean_hex(stream, g.tet(), g.clusters(), xi.p, xref, cb, cn, b, float(1.0 / dt2),
-                                  bdt, mode, cluster_block, float(ws_scale_hyd));
+            launch_neohookean_hex(stream, g.tet(), g.clusters(), xi.p, xref, cb, cn, b, g.xsplit[col],
+                                  float(1.0 / dt2), bdt, mode, cluster_block, float(ws_scale_hyd));
         else if (g.clustered() && g.kind == 0)
             launch_green_clusters(stream, g.tet(), g.clusters(), xi.p, xref, cb, cn, float(at), float(gamma), first,
                                   cluster_block);
