@@ -103,7 +103,8 @@ void launch_neohookean_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, 
 // {0123, 0413, 0253, 0563, 0743, 0673}: particles held in registers. The color's constraints
 // start at tbase, slot-major: tet k of the color's i-th cluster is tbase + k * count + i.
 void launch_green_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn, uint32_t begin,
-                      uint32_t count, uint32_t tbase, float alpha_tilde, float gamma, int first, int block);
+                      uint32_t count, uint32_t tbase, xcd_split const& sp, float alpha_tilde, float gamma, int first,
+                      int block);
 void launch_neohookean_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn,
                            uint32_t begin, uint32_t count, uint32_t tbase, xcd_split const& sp, float inv_dt2,
                            float beta_dt, int mode, int block, float hyd_scale = 1.0f);

@@ -729,10 +729,12 @@ __global__ void k_gather_disp(const uint32_t* __restrict__ off, const uint32_t* 
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_green_hex(tet_group_dev g, cluster_dev cl, pos4* xi,
                                                      const pos4* __restrict__ xn, uint32_t begin, uint32_t count,
-                                                     uint32_t tbase, float alpha_tilde, float gamma, int first)
+                                                     uint32_t tbase, xcd_split sp, float alpha_tilde, float gamma,
+                                                     int first)
 {
-    uint32_t const i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= count) return;
+    uint32_t const x = blockIdx.x % kXcds, local = (blockIdx.x / kXcds) * BLOCK + threadIdx.x;
+    if (local >= sp.off[x + 1] - sp.off[x]) return;
+    uint32_t const i = sp.off[x] + local;
     uint32_t const c = begin + i;
     int4 const va = cl.v0[c], vb = cl.v1[c];
     int const ids[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
@@ -1182,15 +1184,19 @@ void launch_neohookean_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4*
 }
 
 void launch_green_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn, uint32_t begin,
-                      uint32_t count, uint32_t tbase, float alpha_tilde, float gamma, int first, int block)
+                      uint32_t count, uint32_t tbase, xcd_split const& sp, float alpha_tilde, float gamma, int first,
+                      int block)
 {
     if (!count) return;
-    if (block <= 64)
-        k_green_hex<64><<<grid_for(count, 64), 64, 0, s>>>(g, cl, xi, xn, begin, count, tbase, alpha_tilde, gamma,
-                                                           first);
+    int const b = block <= 64 ? 64 : 128;
+    uint32_t most = 0;  // workgroups per XCD
+    for (int x = 0; x < kXcds; ++x) most = std::max(most, uint32_t(grid_for(sp.off[x + 1] - sp.off[x], b)));
+    if (!most) return;
+    if (b == 64)
+        k_green_hex<64><<<most * kXcds, 64, 0, s>>>(g, cl, xi, xn, begin, count, tbase, sp, alpha_tilde, gamma, first);
     else
-        k_green_hex<128><<<grid_for(count, 128), 128, 0, s>>>(g, cl, xi, xn, begin, count, tbase, alpha_tilde, gamma,
-                                                              first);
+        k_green_hex<128><<<most * kXcds, 128, 0, s>>>(g, cl, xi, xn, begin, count, tbase, sp, alpha_tilde, gamma,
+                                                      first);
 }
 
 void launch_detect(hipStream_t s, contact_dev c, const pos4* x, const sdf_dev* sdfs, uint32_t nsdf, int block)

@@ -729,8 +729,8 @@ void sbs_gpu::launch_group_colors(group_t const& g, double dt, int mode, const p
         uint32_t const cb = g.cluster_color_offsets[col];
         uint32_t const cn = g.cluster_color_offsets[col + 1] - cb;
         if (g.hex && g.kind == 0)
-            launch_green_hex(stream, g.tet(), g.clusters(), xi.p, xref, cb, cn, b, float(at), float(gamma), first,
-                             cluster_block);
+            launch_green_hex(stream, g.tet(), g.clusters(), xi.p, xref, cb, cn, b, g.xsplit[col], float(at),
+                             float(gamma), first, cluster_block);
         else if (g.hex && g.kind == 1)
             launch_neohookean_hex(stream, g.tet(), g.clusters(), xi.p, xref, cb, cn, b, g.xsplit[col],
                                   float(1.0 / dt2), bdt, mode, cluster_block, float(ws_scale_hyd));
