@@ -385,9 +385,10 @@ void sbs_gpu::upload_group(group_t& g)
     }
     if (g.clustered()) {
         // XCD affinity: the clusters in Morton order of their first constraint's rest centroid,
-        // split into 8 regions of equal count; inside each color, clusters by region (then Morton
-        // rank), with the region boundaries as the color's XCD split. Elements of one color are
-        // independent, so their order does not change the result.
+        // split into 8 regions of equal count; inside each color, clusters by region, then by
+        // index (the caller's order, which keeps neighbouring clusters' particle gathers
+        // coalesced), with the region boundaries as the color's XCD split. Elements of one color
+        // are independent, so their order does not change the result.
         std::vector<std::array<double, 3>> cen(ne);
         double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
         for (uint32_t e = 0; e < ne; ++e) {
@@ -427,9 +428,8 @@ void sbs_gpu::upload_group(group_t& g)
         g.xsplit.assign(g.ncolors, xcd_split{});
         for (uint32_t c = 0; c < g.ncolors; ++c) {
             uint32_t const cb = g.cluster_color_offsets[c], ce = g.cluster_color_offsets[c + 1];
-            std::sort(eorder.begin() + cb, eorder.begin() + ce, [&](uint32_t a, uint32_t b) {
-                return region[a] != region[b] ? region[a] < region[b] : rank[a] < rank[b];
-            });
+            std::stable_sort(eorder.begin() + cb, eorder.begin() + ce,
+                             [&](uint32_t a, uint32_t b) { return region[a] < region[b]; });
             xcd_split& sp = g.xsplit[c];
             uint32_t j = cb;
             for (int x = 0; x <= kXcds; ++x) {
@@ -923,22 +923,18 @@ void sbs_gpu::restore()
 // sqrt of the summed squared FP64 residuals of every neo-Hookean group (state after the last step).
 double sbs_gpu::residual_norm(double dt)
 {
+    // one launch per group over all its colors (the total does not need the per-color sums)
     double const dt2 = dt * dt;
     std::size_t total = 0;
     for (auto const& g : groups)
-        if (g.kind == 1)
-            for (uint32_t col = 0; col < g.ncolors; ++col)
-                total += (g.color_offsets[col + 1] - g.color_offsets[col] + 255) / 256;
+        if (g.kind == 1) total += (g.count + 255) / 256;
     if (total == 0) return 0.0;
     if (partial.n < total) partial.alloc(std::max<std::size_t>(total, 1024));
     std::size_t at = 0;
     for (auto const& g : groups) {
         if (g.kind != 1) continue;
-        for (uint32_t col = 0; col < g.ncolors; ++col) {
-            uint32_t const b = g.color_offsets[col], n = g.color_offsets[col + 1] - b;
-            launch_neohookean_residual(stream, g.tet(), xn.p, b, n, float(1.0 / dt2), partial.p + at, 256);
-            at += (n + 255) / 256;
-        }
+        launch_neohookean_residual(stream, g.tet(), xn.p, 0, g.count, float(1.0 / dt2), partial.p + at, 256);
+        at += (g.count + 255) / 256;
     }
     std::vector<double> h(total);
     hip_check(hipMemcpyAsync(h.data(), partial.p, total * sizeof(double), hipMemcpyDeviceToHost, stream), "residual");
