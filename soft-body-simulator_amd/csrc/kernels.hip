@@ -570,11 +570,14 @@ constexpr int kHexSlots[kHexTets][4] = {{0, 1, 2, 3}, {0, 4, 1, 3}, {0, 2, 5, 3}
 // memory round trip, ~240 VGPRs, 2 waves per SIMD: for launches too small to fill the chip, where
 // latency rules); otherwise each tet's are loaded just before it is projected (fewer registers,
 // more waves in flight: for launches that fill the chip, where bandwidth rules).
-template <int BLOCK, bool PREFETCH, int MIN_WAVES, bool DAMPED>
+// FIRST: the first iteration of a substep (multipliers start at zero). A compile-time switch: a
+// multiplier load under a run-time condition sat in a branch, after which the compiler drained the
+// memory queue, serializing the prefetch of the six tets' data into several round trips.
+template <int BLOCK, bool PREFETCH, int MIN_WAVES, bool DAMPED, bool FIRST>
 __global__ void __launch_bounds__(BLOCK, MIN_WAVES) k_neohookean_hex(tet_group_dev g, cluster_dev cl, pos4* xi,
                                                                      const pos4* __restrict__ xn, uint32_t begin,
                                                                      uint32_t count, uint32_t tbase, xcd_split sp,
-                                                                     float inv_dt2, float beta_dt, int mode)
+                                                                     float inv_dt2, float beta_dt)
 {
     uint32_t const x = blockIdx.x % kXcds, local = (blockIdx.x / kXcds) * BLOCK + threadIdx.x;
     if (local >= sp.off[x + 1] - sp.off[x]) return;
@@ -590,7 +593,7 @@ __global__ void __launch_bounds__(BLOCK, MIN_WAVES) k_neohookean_hex(tet_group_d
         A[slot] = g.m0[t];
         B[slot] = g.m1[t];
         Cm[slot] = g.m2[t];
-        L[slot] = mode == kSolveFirst ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(g.lambda)[t];
+        L[slot] = FIRST ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(g.lambda)[t];
     };
     auto store = [&](int k, int slot) { reinterpret_cast<float2*>(g.lambda)[tbase + uint32_t(k) * count + i] = L[slot]; };
     if (PREFETCH) {
@@ -726,11 +729,12 @@ __global__ void k_gather_disp(const uint32_t* __restrict__ off, const uint32_t* 
     xi[p] = x;
 }
 
-template <int BLOCK>
+// FIRST: the first iteration of a substep, a compile-time switch for the reason given at
+// k_neohookean_hex.
+template <int BLOCK, bool FIRST>
 __global__ void __launch_bounds__(BLOCK) k_green_hex(tet_group_dev g, cluster_dev cl, pos4* xi,
                                                      const pos4* __restrict__ xn, uint32_t begin, uint32_t count,
-                                                     uint32_t tbase, xcd_split sp, float alpha_tilde, float gamma,
-                                                     int first)
+                                                     uint32_t tbase, xcd_split sp, float alpha_tilde, float gamma)
 {
     uint32_t const x = blockIdx.x % kXcds, local = (blockIdx.x / kXcds) * BLOCK + threadIdx.x;
     if (local >= sp.off[x + 1] - sp.off[x]) return;
@@ -746,7 +750,7 @@ __global__ void __launch_bounds__(BLOCK) k_green_hex(tet_group_dev g, cluster_de
         A[k] = g.m0[t];
         B[k] = g.m1[t];
         Cm[k] = g.m2[t];
-        L[k] = first ? 0.0f : g.lambda[t];
+        L[k] = FIRST ? 0.0f : g.lambda[t];
     }
     pos4 P[8];
 #pragma unroll
@@ -1169,18 +1173,31 @@ void launch_neohookean_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4*
     // block 64 / 128: material prefetched; kHexStreamed: 64 threads, material loaded per tet, at
     // least 3 waves per SIMD (2 with damping, which would spill at 3). Damping (beta != 0) takes the
     // variants that carry the xn differences.
-    bool const damped = beta_dt != 0.0f;
+    bool const damped = beta_dt != 0.0f, first = mode == kSolveFirst;
     auto go = [&](auto kernel, int b) {
         uint32_t most = 0;  // workgroups per XCD
         for (int x = 0; x < kXcds; ++x) most = std::max(most, uint32_t(grid_for(sp.off[x + 1] - sp.off[x], b)));
-        if (most) kernel<<<most * kXcds, b, 0, s>>>(g, cl, xi, xn, begin, count, tbase, sp, inv_dt2, beta_dt, mode);
+        if (most) kernel<<<most * kXcds, b, 0, s>>>(g, cl, xi, xn, begin, count, tbase, sp, inv_dt2, beta_dt);
     };
-    if (block == kHexStreamed)
-        damped ? go(k_neohookean_hex<64, false, 2, true>, 64) : go(k_neohookean_hex<64, false, 3, false>, 64);
-    else if (block <= 64)
-        damped ? go(k_neohookean_hex<64, true, 1, true>, 64) : go(k_neohookean_hex<64, true, 1, false>, 64);
-    else
-        damped ? go(k_neohookean_hex<128, true, 1, true>, 128) : go(k_neohookean_hex<128, true, 1, false>, 128);
+    if (block == kHexStreamed) {
+        if (damped)
+            first ? go(k_neohookean_hex<64, false, 2, true, true>, 64) : go(k_neohookean_hex<64, false, 2, true, false>, 64);
+        else
+            first ? go(k_neohookean_hex<64, false, 3, false, true>, 64)
+                  : go(k_neohookean_hex<64, false, 3, false, false>, 64);
+    } else if (block <= 64) {
+        if (damped)
+            first ? go(k_neohookean_hex<64, true, 1, true, true>, 64) : go(k_neohookean_hex<64, true, 1, true, false>, 64);
+        else
+            first ? go(k_neohookean_hex<64, true, 1, false, true>, 64) : go(k_neohookean_hex<64, true, 1, false, false>, 64);
+    } else {
+        if (damped)
+            first ? go(k_neohookean_hex<128, true, 1, true, true>, 128)
+                  : go(k_neohookean_hex<128, true, 1, true, false>, 128);
+        else
+            first ? go(k_neohookean_hex<128, true, 1, false, true>, 128)
+                  : go(k_neohookean_hex<128, true, 1, false, false>, 128);
+    }
 }
 
 void launch_green_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn, uint32_t begin,
@@ -1192,11 +1209,11 @@ void launch_green_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, 
     uint32_t most = 0;  // workgroups per XCD
     for (int x = 0; x < kXcds; ++x) most = std::max(most, uint32_t(grid_for(sp.off[x + 1] - sp.off[x], b)));
     if (!most) return;
+    auto go = [&](auto kernel) { kernel<<<most * kXcds, b, 0, s>>>(g, cl, xi, xn, begin, count, tbase, sp, alpha_tilde, gamma); };
     if (b == 64)
-        k_green_hex<64><<<most * kXcds, 64, 0, s>>>(g, cl, xi, xn, begin, count, tbase, sp, alpha_tilde, gamma, first);
+        first ? go(k_green_hex<64, true>) : go(k_green_hex<64, false>);
     else
-        k_green_hex<128><<<most * kXcds, 128, 0, s>>>(g, cl, xi, xn, begin, count, tbase, sp, alpha_tilde, gamma,
-                                                      first);
+        first ? go(k_green_hex<128, true>) : go(k_green_hex<128, false>);
 }
 
 void launch_detect(hipStream_t s, contact_dev c, const pos4* x, const sdf_dev* sdfs, uint32_t nsdf, int block)
