@@ -225,62 +225,127 @@ __device__ __forceinline__ tet_frame make_frame(Pos const P[4], XnOf&& xn_of, bo
     return f;
 }
 
-// One XPBD update of a tet constraint with gradients g_k = G r_k (k < 3), g_3 = -sum.
-__device__ __forceinline__ void tet_update(tet_frame& f, m3 const& G, tet_rows const& R, float C, float alpha_tilde,
+// Rest data of a neo-Hookean tet: with R = DmInv (rows r_k), Ds = [e0 e1 e2] and F = Ds R,
+//   ||F||_F^2 = sum_mn (e_m . e_n) Q_mn and F r_k = sum_m Q_mk e_m, Q = R R^T (symmetric);
+//   cof(F) r_k = cof(Ds) cof(R) r_k = det(R) (e_{k+1} x e_{k+2}), det F = det(R) e0 . (e1 x e2)
+// (cof(R) = det(R) R^-T and R^-T r_k = Dm^T r_k is the unit vector k). So the deviatoric gradient
+// dC/dx_k = F r_k / ||F|| and the hydrostatic one cof(F) r_k need no F and no cofactor matrix.
+// Material layout m0 = (Q00, Q11, Q22, Q01), m1 = (Q02, Q12, det R, -),
+// m2 = (-, 1/(mu|V0|), 1/(lambda|V0|), 1 + mu/lambda).
+struct nh_rest {
+    float q00, q11, q22, q01, q02, q12, detr;
+};
+
+__device__ __forceinline__ nh_rest load_nh(float4 a, float4 b) { return {a.x, a.y, a.z, a.w, b.x, b.y, b.z}; }
+
+// The neo-Hookean updates below spell out every fused multiply-add and contract nothing else, so
+// their rounding is fixed by the source and not by the kernel they are inlined into: the hex
+// kernel's launch variants (material prefetched or streamed) give bit-identical positions.
+#pragma clang fp contract(on)
+
+__device__ __forceinline__ float fdot(v3 a, v3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
+
+__device__ __forceinline__ v3 fcross(v3 a, v3 b)
+{
+    return {fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)), fmaf(a.x, b.y, -(a.y * b.x))};
+}
+
+// a e0 + b e1 + c e2
+__device__ __forceinline__ v3 fcomb(float a, v3 e0, float b, v3 e1, float c, v3 e2)
+{
+    return {fmaf(c, e2.x, fmaf(b, e1.x, a * e0.x)), fmaf(c, e2.y, fmaf(b, e1.y, a * e0.y)),
+            fmaf(c, e2.z, fmaf(b, e1.z, a * e0.z))};
+}
+
+// x + s a
+__device__ __forceinline__ v3 faxpy(float s, v3 a, v3 x) { return {fmaf(s, a.x, x.x), fmaf(s, a.y, x.y), fmaf(s, a.z, x.z)}; }
+
+// -(a + b + c)
+__device__ __forceinline__ v3 fnegsum(v3 a, v3 b, v3 c)
+{
+    return {-((a.x + b.x) + c.x), -((a.y + b.y) + c.y), -((a.z + b.z) + c.z)};
+}
+
+// Deviatoric constraint C = ||F||_F at edges e; gradients s h_k with s = 1/C (0 when C = 0).
+__device__ __forceinline__ float nh_deviatoric(v3 const e[3], nh_rest const& q, v3 h[4], float& s)
+{
+    float const g00 = fdot(e[0], e[0]), g11 = fdot(e[1], e[1]), g22 = fdot(e[2], e[2]);
+    float const g01 = fdot(e[0], e[1]), g02 = fdot(e[0], e[2]), g12 = fdot(e[1], e[2]);
+    float const off = fmaf(q.q01, g01, fmaf(q.q02, g02, q.q12 * g12));
+    float const c2 = fmaf(q.q00, g00, fmaf(q.q11, g11, fmaf(q.q22, g22, 2.0f * off)));
+    float const C = hw_sqrt(fmaxf(c2, 0.0f));
+    s = C > 0.0f ? hw_rcp(C) : 0.0f;
+    h[0] = fcomb(q.q00, e[0], q.q01, e[1], q.q02, e[2]);
+    h[1] = fcomb(q.q01, e[0], q.q11, e[1], q.q12, e[2]);
+    h[2] = fcomb(q.q02, e[0], q.q12, e[1], q.q22, e[2]);
+    h[3] = fnegsum(h[0], h[1], h[2]);
+    return C;
+}
+
+// Hydrostatic constraint C = det F - gamma0 at edges e; gradients s h_k with s = det R.
+__device__ __forceinline__ float nh_hydrostatic(v3 const e[3], nh_rest const& q, float gamma0, v3 h[4], float& s)
+{
+    h[0] = fcross(e[1], e[2]);
+    h[1] = fcross(e[2], e[0]);
+    h[2] = fcross(e[0], e[1]);
+    h[3] = fnegsum(h[0], h[1], h[2]);
+    s = q.detr;
+    return fmaf(q.detr, fdot(e[0], h[0]), -gamma0);
+}
+
+// Sum over the tet's vertices of w_k |s h_k|^2.
+__device__ __forceinline__ float grad_norm2(float const w[4], v3 const h[4], float s)
+{
+    float const n = fmaf(w[3], fdot(h[3], h[3]), fmaf(w[2], fdot(h[2], h[2]), fmaf(w[1], fdot(h[1], h[1]), w[0] * fdot(h[0], h[0]))));
+    return (s * s) * n;
+}
+
+// One XPBD update of a tet constraint with gradients g_k = s h_k: dl from the multiplier and the
+// damping term, then d_k += w_k s dl h_k (and dx_k), e_k += (corr_k - corr_3).
+__device__ __forceinline__ void tet_update(tet_frame& f, v3 const h[4], float s, float C, float alpha_tilde,
                                            float gamma, float& lm)
 {
-    v3 gk[4];
-    gk[0] = mul(G, R.r0);
-    gk[1] = mul(G, R.r1);
-    gk[2] = mul(G, R.r2);
-    gk[3] = -(gk[0] + gk[1] + gk[2]);
-    float const wsum = f.w[0] * dot(gk[0], gk[0]) + f.w[1] * dot(gk[1], gk[1]) + f.w[2] * dot(gk[2], gk[2]) +
-                       f.w[3] * dot(gk[3], gk[3]);
+    float const wsum = grad_norm2(f.w, h, s);
     if (wsum < 1e-20f) return;
     float gdd = 0.0f;
     if (gamma != 0.0f)
-        gdd = dot(gk[0], f.dx[0]) + dot(gk[1], f.dx[1]) + dot(gk[2], f.dx[2]) + dot(gk[3], f.dx[3]);
-    float const dl = hw_div(-(C + alpha_tilde * lm) + gamma * gdd, (1.0f + gamma) * wsum + alpha_tilde);
+        gdd = s * (((fdot(h[0], f.dx[0]) + fdot(h[1], f.dx[1])) + fdot(h[2], f.dx[2])) + fdot(h[3], f.dx[3]));
+    float const dl = hw_div(fmaf(gamma, gdd, -fmaf(alpha_tilde, lm, C)), fmaf(1.0f + gamma, wsum, alpha_tilde));
     lm += dl;
-    v3 c[4];
+    float const sd = s * dl;
+    float ws[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        c[k] = (f.w[k] * dl) * gk[k];
-        f.d[k] = f.d[k] + c[k];
-        f.dx[k] = f.dx[k] + c[k];
+        ws[k] = f.w[k] * sd;
+        f.d[k] = faxpy(ws[k], h[k], f.d[k]);
+        f.dx[k] = faxpy(ws[k], h[k], f.dx[k]);
     }
 #pragma unroll
-    for (int k = 0; k < 3; ++k) f.e[k] = f.e[k] + (c[k] - c[3]);
-}
-
-__device__ __forceinline__ m3 cofactor(m3 const& F)
-{
-    return {cross(F.c1, F.c2), cross(F.c2, F.c0), cross(F.c0, F.c1)};
+    for (int k = 0; k < 3; ++k) f.e[k] = faxpy(ws[k], h[k], faxpy(-ws[3], h[3], f.e[k]));
 }
 
 // Deviatoric then hydrostatic projection of one tet (P0..P3), multipliers lm = (deviatoric,
-// hydrostatic); c = (r2.z, 1/(mu|V0|), 1/(lambda|V0|), 1 + mu/lambda). xn_of(k) is the substep
-// start position of vertex k, read only with damping (beta_dt != 0).
+// hydrostatic), material (a, b, c) as laid out at nh_rest. xn_of(k) is the substep start position
+// of vertex k, read only with damping (beta_dt != 0).
 template <class Pos, class XnOf>
 __device__ __forceinline__ void neohookean_project(Pos P[4], XnOf&& xn_of, float4 a, float4 b, float4 c, float inv_dt2,
                                                    float beta_dt, float2& lm)
 {
-    tet_rows const R = load_rows(a, b, c);
+    nh_rest const q = load_nh(a, b);
     tet_frame f = make_frame(P, xn_of, beta_dt != 0.0f);
-    {  // deviatoric: C = ||F||_F, dC/dF = F / C, compliance 1/(mu |V0|)
-        m3 const F = deformation_gradient(f.e[0], f.e[1], f.e[2], R.r0, R.r1, R.r2);
-        float const C = hw_sqrt(dot(F.c0, F.c0) + dot(F.c1, F.c1) + dot(F.c2, F.c2));
-        float const inv = C > 0.0f ? hw_rcp(C) : 0.0f;
-        m3 const G = {inv * F.c0, inv * F.c1, inv * F.c2};
+    {  // deviatoric: C = ||F||_F, compliance 1/(mu |V0|)
+        v3 h[4];
+        float s;
+        float const C = nh_deviatoric(f.e, q, h, s);
         float const at = c.y * inv_dt2;
-        tet_update(f, G, R, C, at, at * beta_dt, lm.x);
+        tet_update(f, h, s, C, at, at * beta_dt, lm.x);
     }
-    {  // hydrostatic: C = det F - (1 + mu/lambda), dC/dF = cof F, compliance 1/(lambda |V0|)
-        m3 const F = deformation_gradient(f.e[0], f.e[1], f.e[2], R.r0, R.r1, R.r2);
-        m3 const G = cofactor(F);
-        float const C = dot(F.c0, G.c0) - c.w;
+    {  // hydrostatic: C = det F - (1 + mu/lambda), compliance 1/(lambda |V0|)
+        v3 h[4];
+        float s;
+        float const C = nh_hydrostatic(f.e, q, c.w, h, s);
         float const at = c.z * inv_dt2;
-        tet_update(f, G, R, C, at, at * beta_dt, lm.y);
+        tet_update(f, h, s, C, at, at * beta_dt, lm.y);
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) add_to(P[k], f.d[k]);
@@ -295,37 +360,24 @@ template <class Pos>
 __device__ __forceinline__ void neohookean_displacement(Pos const P[4], float4 a, float4 b, float4 c, float inv_dt2,
                                                         bool expl, float2 scale, float2& lm, v3 d[4])
 {
-    tet_rows const R = load_rows(a, b, c);
+    nh_rest const q = load_nh(a, b);
     tet_frame const f = make_frame(P, [](int) { return pos4{}; }, false);
-    m3 const F = deformation_gradient(f.e[0], f.e[1], f.e[2], R.r0, R.r1, R.r2);
-    float const Cd = hw_sqrt(dot(F.c0, F.c0) + dot(F.c1, F.c1) + dot(F.c2, F.c2));
-    float const inv = Cd > 0.0f ? hw_rcp(Cd) : 0.0f;
-    m3 const Gd = {inv * F.c0, inv * F.c1, inv * F.c2};
-    m3 const Gh = cofactor(F);
-    if (expl) {
-        float const Ch = dot(F.c0, Gh.c0) - c.w;
-        lm = make_float2(-Cd / (c.y * inv_dt2), -Ch / (c.z * inv_dt2));
-    }
+    v3 hd[4], hh[4];
+    float sd, sh;
+    float const Cd = nh_deviatoric(f.e, q, hd, sd);
+    float const Ch = nh_hydrostatic(f.e, q, c.w, hh, sh);
+    if (expl) lm = make_float2(-Cd / (c.y * inv_dt2), -Ch / (c.z * inv_dt2));
     lm = make_float2(scale.x * lm.x, scale.y * lm.y);
-    v3 gd[4], gh[4];
-    gd[0] = mul(Gd, R.r0);
-    gd[1] = mul(Gd, R.r1);
-    gd[2] = mul(Gd, R.r2);
-    gd[3] = -(gd[0] + gd[1] + gd[2]);
-    gh[0] = mul(Gh, R.r0);
-    gh[1] = mul(Gh, R.r1);
-    gh[2] = mul(Gh, R.r2);
-    gh[3] = -(gh[0] + gh[1] + gh[2]);
-    float wd = 0.0f, wh = 0.0f;
+    float const wd = grad_norm2(f.w, hd, sd), wh = grad_norm2(f.w, hh, sh);
+    float const ld = wd < 1e-20f ? 0.0f : lm.x * sd, lh = wh < 1e-20f ? 0.0f : lm.y * sh;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        wd += f.w[k] * dot(gd[k], gd[k]);
-        wh += f.w[k] * dot(gh[k], gh[k]);
+        float const a = f.w[k] * ld, b = f.w[k] * lh;
+        d[k] = faxpy(b, hh[k], mk(a * hd[k].x, a * hd[k].y, a * hd[k].z));
     }
-    float const ld = wd < 1e-20f ? 0.0f : lm.x, lh = wh < 1e-20f ? 0.0f : lm.y;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) d[k] = f.w[k] * (ld * gd[k] + lh * gh[k]);
 }
+
+#pragma clang fp contract(fast)
 
 // x += d in FP64 (read, add, write) for the x, y, z of one particle.
 __device__ __forceinline__ void add_displacement(pos4* x, v3 d)
@@ -377,14 +429,6 @@ __global__ void k_copy_pos(const pos4* __restrict__ a, pos4* __restrict__ b, uin
     if (i < n) b[i] = a[i];
 }
 
-// Sum over the tet's vertices of w_k |grad_k C|^2 for dC/dF = G.
-__device__ __forceinline__ float grad_norm2(tet_frame const& f, m3 const& G, tet_rows const& R)
-{
-    v3 const g0 = mul(G, R.r0), g1 = mul(G, R.r1), g2 = mul(G, R.r2);
-    v3 const g3 = -(g0 + g1 + g2);
-    return f.w[0] * dot(g0, g0) + f.w[1] * dot(g1, g1) + f.w[2] * dot(g2, g2) + f.w[3] * dot(g3, g3);
-}
-
 // Features of neo-Hookean constraint t at the predicted positions (see launch_ws_features).
 __global__ void __launch_bounds__(256) k_ws_features(tet_group_dev g, const pos4* __restrict__ xi,
                                                      float2* __restrict__ hist, uint32_t count, float inv_dt2,
@@ -394,20 +438,18 @@ __global__ void __launch_bounds__(256) k_ws_features(tet_group_dev g, const pos4
     uint32_t const t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
     int4 const id = g.idx[t];
-    float4 const a = g.m0[t], b = g.m1[t], c = g.m2[t];
-    tet_rows const R = load_rows(a, b, c);
+    float4 const c = g.m2[t];
+    nh_rest const q = load_nh(g.m0[t], g.m1[t]);
     pos4 const P[4] = {xi[id.x], xi[id.y], xi[id.z], xi[id.w]};
     tet_frame const f = make_frame(P, [](int) { return pos4{}; }, false);
     float2 const lp = reinterpret_cast<const float2*>(g.lambda)[t];
     float2 const lp2 = hist[t];
     hist[t] = lp;
-    m3 const F = deformation_gradient(f.e[0], f.e[1], f.e[2], R.r0, R.r1, R.r2);
-    float const Cd = hw_sqrt(dot(F.c0, F.c0) + dot(F.c1, F.c1) + dot(F.c2, F.c2));
-    float const inv = Cd > 0.0f ? hw_rcp(Cd) : 0.0f;
-    m3 const Gd = {inv * F.c0, inv * F.c1, inv * F.c2};
-    m3 const Gh = cofactor(F);
-    float const Ch = dot(F.c0, Gh.c0) - c.w;
-    float const wd = grad_norm2(f, Gd, R), wh = grad_norm2(f, Gh, R);
+    v3 hd[4], hh[4];
+    float sd, sh;
+    float const Cd = nh_deviatoric(f.e, q, hd, sd);
+    float const Ch = nh_hydrostatic(f.e, q, c.w, hh, sh);
+    float const wd = grad_norm2(f.w, hd, sd), wh = grad_norm2(f.w, hh, sh);
     float const atd = c.y * inv_dt2, ath = c.z * inv_dt2;
     float const v[mlp::kMaxInputs] = {lp.x,
                                       lp.y,
@@ -895,6 +937,25 @@ __global__ void __launch_bounds__(kRedBlock) k_green_residual(tet_group_dev g, c
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// ||F||_F^2 and det F - gamma0 of neo-Hookean tet id at positions x, in FP64 from the FP64 edge
+// vectors and the FP32 rest data (the forms at nh_rest).
+__device__ __forceinline__ void nh_constraints_f64(const pos4* __restrict__ x, int4 id, nh_rest const& q, float gamma0,
+                                                   double& f2, double& det_minus)
+{
+    pos4 const p3 = x[id.w], p0 = x[id.x], p1 = x[id.y], p2 = x[id.z];
+    double const e[3][3] = {{p0.x - p3.x, p0.y - p3.y, p0.z - p3.z},
+                            {p1.x - p3.x, p1.y - p3.y, p1.z - p3.z},
+                            {p2.x - p3.x, p2.y - p3.y, p2.z - p3.z}};
+    auto d = [&](int m, int n) { return e[m][0] * e[n][0] + e[m][1] * e[n][1] + e[m][2] * e[n][2]; };
+    f2 = double(q.q00) * d(0, 0) + double(q.q11) * d(1, 1) + double(q.q22) * d(2, 2) +
+         2.0 * (double(q.q01) * d(0, 1) + double(q.q02) * d(0, 2) + double(q.q12) * d(1, 2));
+    f2 = f2 > 0.0 ? f2 : 0.0;
+    double const vol = e[0][0] * (e[1][1] * e[2][2] - e[1][2] * e[2][1]) +
+                       e[0][1] * (e[1][2] * e[2][0] - e[1][0] * e[2][2]) +
+                       e[0][2] * (e[1][0] * e[2][1] - e[1][1] * e[2][0]);
+    det_minus = double(q.detr) * vol - double(gamma0);
+}
+
 __global__ void __launch_bounds__(kRedBlock) k_neohookean_residual(tet_group_dev g, const pos4* __restrict__ x,
                                                                    uint32_t begin, uint32_t count, float inv_dt2,
                                                                    double* partial)
@@ -905,12 +966,10 @@ __global__ void __launch_bounds__(kRedBlock) k_neohookean_residual(tet_group_dev
         uint32_t const t = begin + i;
         int4 const id = g.idx[t];
         float4 const c = g.m2[t];
-        tet_rows const R = load_rows(g.m0[t], g.m1[t], c);
-        pos4 const p3 = x[id.w];
-        m3 const F = deformation_gradient(sub(x[id.x], p3), sub(x[id.y], p3), sub(x[id.z], p3), R.r0, R.r1, R.r2);
+        double cd, ch;
+        nh_constraints_f64(x, id, load_nh(g.m0[t], g.m1[t]), c.w, cd, ch);
         float2 const lm = reinterpret_cast<const float2*>(g.lambda)[t];
-        double const cd = sqrt(double(dot(F.c0, F.c0)) + double(dot(F.c1, F.c1)) + double(dot(F.c2, F.c2)));
-        double const ch = double(dot(F.c0, cross(F.c1, F.c2))) - double(c.w);
+        cd = sqrt(cd);
         double const hd = cd + double(c.y) * double(inv_dt2) * double(lm.x);
         double const hh = ch + double(c.z) * double(inv_dt2) * double(lm.y);
         h2 = hd * hd + hh * hh;
@@ -996,11 +1055,8 @@ __global__ void __launch_bounds__(kRedBlock) k_neohookean_energy(tet_group_dev g
     if (t < count) {
         int4 const id = g.idx[t];
         float4 const c = g.m2[t];
-        tet_rows const R = load_rows(g.m0[t], g.m1[t], c);
-        pos4 const p3 = x[id.w];
-        m3 const F = deformation_gradient(sub(x[id.x], p3), sub(x[id.y], p3), sub(x[id.z], p3), R.r0, R.r1, R.r2);
-        double const ic = double(dot(F.c0, F.c0)) + double(dot(F.c1, F.c1)) + double(dot(F.c2, F.c2));
-        double const j = double(dot(F.c0, cross(F.c1, F.c2))) - double(c.w);
+        double ic, j;
+        nh_constraints_f64(x, id, load_nh(g.m0[t], g.m1[t]), c.w, ic, j);
         // mu |V0| = 1 / m2.y, lambda |V0| = 1 / m2.z
         e = 0.5 / double(c.y) * (ic - 3.0) + 0.5 / double(c.z) * j * j;
     }

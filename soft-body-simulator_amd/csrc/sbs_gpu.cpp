@@ -531,17 +531,24 @@ void sbs_gpu::upload_group(group_t& g)
             A[2][2] = D[0][0] * D[1][1] - D[0][1] * D[1][0];
             double const det = D[0][0] * A[0][0] + D[0][1] * A[1][0] + D[0][2] * A[2][0];
             require(det != 0.0, "degenerate tet in rest shape");
-            float r[9];
-            for (int i = 0; i < 9; ++i) r[i] = float(A[i / 3][i % 3] / det);
+            double R[3][3];  // DmInv, rows r_k
+            for (int i = 0; i < 9; ++i) R[i / 3][i % 3] = A[i / 3][i % 3] / det;
             double const V0 = det / 6.0;
             idx[j] = make_int4(int(b.offset + t[0]), int(b.offset + t[1]), int(b.offset + t[2]), int(b.offset + t[3]));
-            m0[j] = make_float4(r[0], r[1], r[2], r[3]);
-            m1[j] = make_float4(r[4], r[5], r[6], r[7]);
-            if (g.kind == 0) {
+            if (g.kind == 0) {  // green: the rows of DmInv, V0, mu, lambda
+                float r[9];
+                for (int i = 0; i < 9; ++i) r[i] = float(R[i / 3][i % 3]);
+                m0[j] = make_float4(r[0], r[1], r[2], r[3]);
+                m1[j] = make_float4(r[4], r[5], r[6], r[7]);
                 m2[j] = make_float4(r[8], float(V0), float(mu), float(lam));
-            } else {
+            } else {  // neo-Hookean: Q = R R^T, det R and the compliances (the kernels' nh_rest)
+                auto q = [&](int a, int c) {
+                    return float(R[a][0] * R[c][0] + R[a][1] * R[c][1] + R[a][2] * R[c][2]);
+                };
                 double const av = std::fabs(V0);
-                m2[j] = make_float4(r[8], float(1.0 / (mu * av)), float(1.0 / (lam * av)), float(1.0 + mu / lam));
+                m0[j] = make_float4(q(0, 0), q(1, 1), q(2, 2), q(0, 1));
+                m1[j] = make_float4(q(0, 2), q(1, 2), float(1.0 / det), 0.0f);
+                m2[j] = make_float4(0.0f, float(1.0 / (mu * av)), float(1.0 / (lam * av)), float(1.0 + mu / lam));
             }
         }
         g.idx.upload(idx.data(), n, stream);
