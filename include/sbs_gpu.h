@@ -332,7 +332,8 @@ int sbs_gpu_capture(sbs_gpu* sim, sbs_mlp* net, uint32_t stride);
 int sbs_gpu_snapshot(sbs_gpu* sim);
 int sbs_gpu_restore(sbs_gpu* sim);
 /* With a warm start and limit > 0, each sbs_gpu_step snapshots the state and, when the FP64
- * residual norm sqrt(sum over neo-Hookean groups of sum h^2) after the frame exceeds limit, restores
+ * residual norm sqrt(sum over neo-Hookean groups of sum h^2) after the frame exceeds limit or is
+ * not a number (a frame that diverged), restores
  * it and repeats the frame without warm start (the plain solver as fallback) with `iterations`
  * iterations per substep (0: the step's own count). *fallbacks counts the repeated frames. */
 int sbs_gpu_set_fallback(sbs_gpu* sim, double limit, uint32_t iterations);

@@ -1386,7 +1386,8 @@ int sbs_gpu_step(sbs_gpu* sim, double dt, uint32_t iterations, uint32_t substeps
         bool const check = sim->fb_limit > 0.0 && sim->ws_mode != SBS_WARM_START_NONE;
         if (check) sim->snapshot();
         sim->run_frame(dt, iterations, substeps, sim->ws_mode);
-        if (check && sim->residual_norm(dt / double(substeps)) > sim->fb_limit) {
+        // !(r <= limit): a frame whose residual is NaN (one that diverged) is repeated as well
+        if (check && !(sim->residual_norm(dt / double(substeps)) <= sim->fb_limit)) {
             sim->restore();
             sim->run_frame(dt, sim->fb_iterations ? sim->fb_iterations : iterations, substeps, SBS_WARM_START_NONE);
             ++sim->fb_count;

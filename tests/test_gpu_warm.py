@@ -204,6 +204,26 @@ def test_fallback_with_zero_limit_is_the_plain_solver():
     assert np.array_equal(a.read_body(0)[0], b.read_body(0)[0]) and np.array_equal(a.read_body(0)[1], b.read_body(0)[1])
 
 
+def test_fallback_repeats_a_diverged_frame():
+    # (0.7, 0.65) diverges on this beam (the oracle's positions are NaN after 7 frames); with a limit no
+    # finite residual exceeds, only the diverged frames are repeated by the plain solver
+    sc = _beam(True)
+    say("fallback on divergence")
+    a = scenes.build_device(sc)
+    b = scenes.build_device(sc)
+    for d in (a, b):
+        d.set_warm_start("previous")
+        d.set_warm_start_scale(0.7, 0.65)
+    b.set_fallback(1e300, sc.iterations)
+    for _ in range(12):
+        a.step(sc.dt, sc.iterations, sc.substeps)
+        b.step(sc.dt, sc.iterations, sc.substeps)
+    say(f"fallback frames {b.fallback_count()}")
+    assert not np.all(np.isfinite(a.read_body(0)[0]))
+    assert b.fallback_count() >= 1
+    assert np.all(np.isfinite(b.read_body(0)[0])) and np.all(np.isfinite(b.read_body(0)[1]))
+
+
 def test_learned_warm_start_trains_and_runs():
     from sbs_amd import autotune
     sc = _beam(True, iterations=8)
