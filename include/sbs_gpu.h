@@ -273,7 +273,7 @@ int sbs_gpu_reset_stats(sbs_gpu* sim);
 
 typedef struct sbs_mlp sbs_mlp;
 
-/* Network 16 -> 128 -> 128 -> 128 (ReLU) -> 32 of which num_inputs (<= 12) inputs and num_outputs
+/* Network 16 -> 128 -> 128 -> 128 (ReLU) -> 32 of which num_inputs (<= 13) inputs and num_outputs
  * (<= 4) outputs are used; BF16 matrix-core arithmetic with FP32 accumulation, FP32 master
  * parameters (He initialization from seed, output layer scaled by 0.1). */
 int sbs_mlp_create(int device, uint32_t num_inputs, uint32_t num_outputs, uint64_t seed, sbs_mlp** out);
@@ -309,13 +309,14 @@ int sbs_mlp_train(sbs_mlp* net, uint32_t steps, uint32_t batch, float lr, uint64
 
 #define SBS_WARM_START_NONE 0     /* reference behaviour: lambda_init = 0 */
 #define SBS_WARM_START_PREVIOUS 1 /* lambda_init = the previous substep's final multipliers */
-#define SBS_WARM_START_LEARNED 2  /* lambda_init = previous + net(features), net with 12 inputs, 2 outputs */
+#define SBS_WARM_START_LEARNED 2  /* lambda_init = previous + net(features), net with 13 inputs, 2 outputs */
 #define SBS_WARM_START_EXPLICIT 3 /* lambda_init = -C(x~) / alpha~ (zero XPBD residual at x~) */
 /* Warm start of the neo-Hookean groups' multipliers (other groups start from zero). The network's
- * 12 inputs per constraint are, at the predicted positions: lambda_prev (deviatoric, hydrostatic),
+ * 13 inputs per constraint are, at the predicted positions: lambda_prev (deviatoric, hydrostatic),
  * the pair before it, C_dev, C_hyd, log sum_k w_k |grad_k C_dev|^2, the same for C_hyd,
- * log alpha~_dev, log alpha~_hyd, and the multipliers -C / alpha~ of both constraints; its 2
- * outputs are the change of the pair over the substep. The handle keeps
+ * log alpha~_dev, log alpha~_hyd, the multipliers -C / alpha~ of both constraints, and the log of
+ * the number of tets of the constraint's body (one network serves bodies of every size in a
+ * batched group); its 2 outputs are the change of the pair over the substep. The handle keeps
  * `net` (not owned) until the mode changes. */
 int sbs_gpu_set_warm_start(sbs_gpu* sim, int mode, sbs_mlp* net);
 /* Factor on the warm-start multipliers (default 1): lambda_init = scale * (the mode's multipliers). */
@@ -323,7 +324,7 @@ int sbs_gpu_set_warm_start_scale(sbs_gpu* sim, double scale);
 /* Separate factors for the neo-Hookean deviatoric and hydrostatic multipliers (no reference
  * counterpart; sbs_gpu_set_warm_start_scale sets both to one value). */
 int sbs_gpu_set_warm_start_scales(sbs_gpu* sim, double deviatoric, double hydrostatic);
-/* While stride > 0, every substep of sbs_gpu_step appends one training row (the 12 features at the
+/* While stride > 0, every substep of sbs_gpu_step appends one training row (the 13 features at the
  * substep start, the change of the multiplier pair over the substep) for every stride-th
  * constraint of each neo-Hookean group to net's dataset, until its capacity is reached. Graphs
  * are not used while capturing. stride 0 stops. */

@@ -11,7 +11,7 @@ FP32 on the device.
 """
 import numpy as np
 
-IN, H, OUT, MAX_IN, MAX_OUT = 16, 128, 32, 12, 4
+IN, H, OUT, MAX_IN, MAX_OUT = 16, 128, 32, 13, 4
 W1, B1 = 0, 16 * 128
 W2 = B1 + 128
 B2 = W2 + 128 * 128

@@ -156,14 +156,15 @@ void launch_hex_warm_start(hipStream_t s, tet_group_dev g, cluster_dev cl, const
                            const uint32_t* csr, uint32_t n);
 
 // Multiplier warm start of a neo-Hookean group, per constraint t (execution order) at the
-// predicted positions xi: the 12 features
+// predicted positions xi: the 13 features
 //   lambda_prev (dev, hyd), lambda_prev2 (dev, hyd), C_dev, C_hyd, log sum w|grad C_dev|^2,
-//   log sum w|grad C_hyd|^2, log alpha~_dev, log alpha~_hyd, -C_dev / alpha~_dev, -C_hyd / alpha~_hyd
+//   log sum w|grad C_hyd|^2, log alpha~_dev, log alpha~_hyd, -C_dev / alpha~_dev, -C_hyd / alpha~_hyd,
+//   log of the number of tets of the constraint's body (body_size[t])
 // where lambda_prev is the multiplier pair stored in g.lambda (the previous substep's final) and
-// lambda_prev2 the pair before it (hist, which receives lambda_prev). raw gets the 12 FP32 values
-// ([count][12]) and/or X the normalized BF16 network input ([count][16]); either may be null.
-void launch_ws_features(hipStream_t s, tet_group_dev g, const pos4* xi, float2* hist, uint32_t count, float inv_dt2,
-                        float* raw, mlp::bf16* X, const mlp::norm_dev* norm, int block);
+// lambda_prev2 the pair before it (hist, which receives lambda_prev). raw gets the 13 FP32 values
+// ([count][13]) and/or X the normalized BF16 network input ([count][16]); either may be null.
+void launch_ws_features(hipStream_t s, tet_group_dev g, const pos4* xi, float2* hist, const float* body_size,
+                        uint32_t count, float inv_dt2, float* raw, mlp::bf16* X, const mlp::norm_dev* norm, int block);
 // Training rows of constraints t = offset + i * stride, i < rows: ds_x[base + i] = raw[t],
 // ds_t[base + i] = lambda[t] - lambda_prev[t] (the multipliers' change over the substep); a row
 // holding a non-finite value is stored as zeros.

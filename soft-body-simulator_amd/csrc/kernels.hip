@@ -431,7 +431,8 @@ __global__ void k_copy_pos(const pos4* __restrict__ a, pos4* __restrict__ b, uin
 
 // Features of neo-Hookean constraint t at the predicted positions (see launch_ws_features).
 __global__ void __launch_bounds__(256) k_ws_features(tet_group_dev g, const pos4* __restrict__ xi,
-                                                     float2* __restrict__ hist, uint32_t count, float inv_dt2,
+                                                     float2* __restrict__ hist, const float* __restrict__ body_size,
+                                                     uint32_t count, float inv_dt2,
                                                      float* __restrict__ raw, mlp::bf16* __restrict__ X,
                                                      const mlp::norm_dev* __restrict__ norm)
 {
@@ -462,7 +463,8 @@ __global__ void __launch_bounds__(256) k_ws_features(tet_group_dev g, const pos4
                                       logf(atd),
                                       logf(ath),
                                       -Cd / atd,
-                                      -Ch / ath};
+                                      -Ch / ath,
+                                      body_size[t]};
     if (raw) {
 #pragma unroll
         for (int k = 0; k < mlp::kMaxInputs; ++k) raw[std::size_t(t) * mlp::kMaxInputs + k] = v[k];
@@ -1354,11 +1356,11 @@ void launch_copy_pos(hipStream_t s, const pos4* a, pos4* b, uint32_t n)
     if (n) k_copy_pos<<<grid_for(n, 256), 256, 0, s>>>(a, b, n);
 }
 
-void launch_ws_features(hipStream_t s, tet_group_dev g, const pos4* xi, float2* hist, uint32_t count, float inv_dt2,
-                        float* raw, mlp::bf16* X, const mlp::norm_dev* norm, int block)
+void launch_ws_features(hipStream_t s, tet_group_dev g, const pos4* xi, float2* hist, const float* body_size,
+                        uint32_t count, float inv_dt2, float* raw, mlp::bf16* X, const mlp::norm_dev* norm, int block)
 {
     if (!count) return;
-    k_ws_features<<<grid_for(count, 256), 256, 0, s>>>(g, xi, hist, count, inv_dt2, raw, X, norm);
+    k_ws_features<<<grid_for(count, 256), 256, 0, s>>>(g, xi, hist, body_size, count, inv_dt2, raw, X, norm);
 }
 
 void launch_ws_capture(hipStream_t s, const float* raw, const float* lambda, uint32_t stride, uint32_t offset,

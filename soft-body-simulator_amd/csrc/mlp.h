@@ -12,7 +12,7 @@ namespace sbs {
 namespace mlp {
 
 constexpr int kIn = 16, kHidden = 128, kOut = 32;
-constexpr int kMaxInputs = 12, kMaxOutputs = 4;
+constexpr int kMaxInputs = 13, kMaxOutputs = 4;
 
 // Flat FP32 parameter vector: W1[16][128] b1[128] W2[128][128] b2[128] W3[128][128] b3[128]
 // W4[128][32] b4[32]; W_l[i][o] multiplies input unit i into output unit o.

@@ -20,7 +20,7 @@ struct sbs_mlp {
     sbs::mlp::norm_dev hnorm{};
     int adam_t = 0;
 
-    // training rows: features [cap][12], targets [cap][4] (FP32, raw units)
+    // training rows: features [cap][13], targets [cap][4] (FP32, raw units)
     sbs::dev_buf<float> ds_x, ds_t;
     uint64_t ds_size = 0, ds_cap = 0;
 

@@ -64,7 +64,8 @@ struct group_t {
     dev_buf<float> lambda;
     // multiplier warm start (neo-Hookean groups)
     dev_buf<float2> ws_hist;     // the multiplier pairs one substep before the stored ones
-    dev_buf<float> ws_raw;       // [count][12] features at the substep start (capture)
+    dev_buf<float> ws_raw;       // [count][13] features at the substep start (capture)
+    dev_buf<float> ws_bsize;     // per constraint: log of the number of tets of its body
     dev_buf<mlp::bf16> ws_X;     // [count][16] network inputs (learned warm start)
     dev_buf<float4> ws_disp;     // hex groups: slot displacements of the warm-start pass
     dev_buf<uint32_t> ws_csr_off, ws_csr, ws_ccolor_off, ws_color_off;  // particle -> slot list
@@ -556,6 +557,13 @@ void sbs_gpu::upload_group(group_t& g)
         g.m1.upload(m1.data(), n, stream);
         g.m2.upload(m2.data(), n, stream);
         g.lambda.alloc(std::size_t(n) * (g.kind == 1 ? 2 : 1));
+        if (g.kind == 1) {  // warm-start feature: log of the number of the group's tets in each tet's body
+            std::vector<uint32_t> per_body(bodies.size(), 0);
+            for (uint32_t c = 0; c < n; ++c) ++per_body[g.body_of(c)];
+            std::vector<float> bs(n);
+            for (uint32_t j = 0; j < n; ++j) bs[j] = float(std::log(double(per_body[g.body_of(order[j])])));
+            g.ws_bsize.upload(bs.data(), n, stream);
+        }
     }
     // multipliers start at zero (the "previous" multipliers of a warm-started first substep)
     if (g.lambda.n) hip_check(hipMemsetAsync(g.lambda.p, 0, g.lambda.n * sizeof(float), stream), "clear multipliers");
@@ -793,7 +801,7 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
             bool const learned = warm == SBS_WARM_START_LEARNED;
             if (capture || learned) {
                 timed(SBS_KERNEL_WARM_START, g.count, [&] {
-                    launch_ws_features(stream, g.tet(), xi.p, g.ws_hist.p, g.count, float(1.0 / dt2),
+                    launch_ws_features(stream, g.tet(), xi.p, g.ws_hist.p, g.ws_bsize.p, g.count, float(1.0 / dt2),
                                        capture ? g.ws_raw.p : nullptr, learned ? g.ws_X.p : nullptr,
                                        learned ? ws_net->norm.p : nullptr, block);
                     if (learned)
@@ -1835,7 +1843,7 @@ int sbs_gpu_set_warm_start(sbs_gpu* sim, int mode, sbs_mlp* net)
         if (mode == SBS_WARM_START_LEARNED) {
             require(net != nullptr, "the learned warm start needs a network");
             require(net->nin == uint32_t(mlp::kMaxInputs) && net->nout == 2,
-                    "the warm-start network must have 12 inputs and 2 outputs");
+                    "the warm-start network must have 13 inputs and 2 outputs");
             require(net->device == sim->device, "network and simulation are on different devices");
         }
         sim->set_current();
@@ -1860,7 +1868,7 @@ int sbs_gpu_capture(sbs_gpu* sim, sbs_mlp* net, uint32_t stride)
         }
         require(net != nullptr, "net is null");
         require(net->nin == uint32_t(mlp::kMaxInputs) && net->nout == 2,
-                "the warm-start network must have 12 inputs and 2 outputs");
+                "the warm-start network must have 13 inputs and 2 outputs");
         require(net->device == sim->device, "network and simulation are on different devices");
         sim->ensure_warm_buffers(true, false);
         hip_check(hipStreamSynchronize(net->stream), "capture");

@@ -598,7 +598,7 @@ class Simulation:
         return out
 
     def set_warm_start(self, mode, net=None):
-        """mode: "none", "previous", "explicit" or "learned" (needs net, an Mlp(12, 2))."""
+        """mode: "none", "previous", "explicit" or "learned" (needs net, an Mlp(13, 2))."""
         check(self._L.sbs_gpu_set_warm_start(self.h, WARM_START[mode], None if net is None else net.h))
 
     def set_warm_start_scale(self, scale, hydrostatic=None):

@@ -83,10 +83,10 @@ def train_warm_start(sim, scene, frames=6, iterations=24, scale=0.5, max_rows=4_
                      lr=1e-3, seed=0, log=print):
     """Captures training rows from a previous-multiplier warm-started trajectory (`scale`, one factor
     or a (deviatoric, hydrostatic) pair, `iterations` per substep) starting at the current state,
-    trains a 12-input, 2-output network on the device and restores the state. Returns the network."""
+    trains a 13-input, 2-output network on the device and restores the state. Returns the network."""
     rows_per_substep = sum(len(g.conn) for g in scene.groups if g.kind == "neohookean")
     stride = max(1, -(-rows_per_substep * scene.substeps * frames // max_rows))
-    net = Mlp(12, 2, seed=seed)
+    net = Mlp(13, 2, seed=seed)
     net.reserve(max_rows)
     sim.snapshot()
     sim.set_warm_start("previous")

@@ -217,12 +217,14 @@ def _random_rotation(rng):
 
 
 def batch(num_bodies=512, tet_range=(2e3, 2e5), edge=0.05, seed=0xB0D1E5, density=1000.0, iterations=8,
-          substeps=4, name=None):
+          substeps=4, name=None, large=None):
     """Batched bodies: `num_bodies` boxes and ellipsoids (alternating) of 6-tet hex cells with tet
     counts log-uniform in tet_range, random rotations, dropped from heights U[0.5, 3] m onto a
     ground plane at y = 0, Young's modulus log-uniform in [1e4, 1e6], Poisson ratio U[0.3, 0.48],
     stable neo-Hookean constraints in one batched group with hex-cell clusters (index-order
-    coloring), 4 substeps x 8 iterations."""
+    coloring), 4 substeps x 8 iterations. large = (count, tets): that many more bodies of about
+    `tets` tets each (same shapes and materials), in a row beside the grid of the others, in the
+    same group."""
     rng = np.random.default_rng(seed)
     sc = Scene(name or f"batch_{num_bodies}_bodies", gravity_is_force=False, iterations=iterations,
                substeps=substeps, coloring="index")
@@ -244,11 +246,26 @@ def batch(num_bodies=512, tet_range=(2e3, 2e5), edge=0.05, seed=0xB0D1E5, densit
         meshes.append((pos @ _random_rotation(rng).T, tets))
     extent = max(float(np.ptp(p, axis=0).max()) for p, _ in meshes) + 0.2
     side = int(np.ceil(np.sqrt(num_bodies)))
+    if large:
+        for k in range(int(large[0])):
+            cells = max(float(large[1]) / 6.0, 1.0)
+            if k % 2 == 0:
+                n = max(int(round(np.cbrt(cells))), 1)
+                pos, tets = _lib.box6(n, n, max(int(round(cells / (n * n))), 1), edge)
+                pos = pos - pos.mean(0)
+            else:
+                r = np.full(3, np.cbrt(cells * edge ** 3 * 3.0 / (4.0 * np.pi)))
+                pos, tets = _lib.ellipsoid6(r[0], r[1], r[2], edge)
+            meshes.append((pos @ _random_rotation(rng).T, tets))
+    big = max((float(np.ptp(p, axis=0).max()) for p, _ in meshes[num_bodies:]), default=0.0) + 0.2
     bodies, bconn, young, poisson, clusters = [], [], [], [], [0]
     for b, (pos, tets) in enumerate(meshes):
         gx, gz = b % side, b // side
         drop = rng.uniform(0.5, 3.0)
-        x0 = pos + np.array([gx * extent, drop - pos[:, 1].min(), gz * extent])
+        corner = np.array([gx * extent, 0.0, gz * extent])
+        if b >= num_bodies:  # the large bodies: one row beyond the grid
+            corner = np.array([(b - num_bodies) * big, 0.0, side * extent + 0.5 * big])
+        x0 = pos + corner + np.array([0.0, drop - pos[:, 1].min(), 0.0])
         mass = _lib.lumped_mass(x0, tets, density)
         s2t, tris = _lib.boundary(len(x0), tets)
         sc.bodies.append(Body(x0=x0, tets=tets, mass=mass, surface=s2t, triangles=tris))
@@ -269,3 +286,9 @@ def batch(num_bodies=512, tet_range=(2e3, 2e5), edge=0.05, seed=0xB0D1E5, densit
 def cfg2(num_bodies=512):
     """Config 2: 512 batched bodies (about 22M tets)."""
     return batch(num_bodies, name=f"cfg2_batch_{num_bodies}_bodies")
+
+
+def cfg4(num_bodies=512, large=(8, 5e5)):
+    """Config 4: the 512-body batch plus 8 bodies of about 500k tets in the same batched group
+    (about 26M tets); the learned warm start serves all of them with one network."""
+    return batch(num_bodies, large=large, name=f"cfg4_batch_{num_bodies}_plus_{int(large[0])}_large")

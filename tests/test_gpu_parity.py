@@ -304,6 +304,22 @@ def test_batched_bodies_parity():
             assert err / diag <= TOL
 
 
+def test_batch_with_large_bodies_parity():
+    # the shape of the combined configuration (a batch plus large bodies in one batched group), small
+    sc = scenes.batch(3, tet_range=(2e3, 4e3), large=(1, 1.2e4), name="batch_3_plus_1_large")
+    say(f"{sc.name} ({sc.num_tets} tets) 40 frames")
+    dev = scenes.build_device(sc)
+    orc = build_oracle(sc)
+    diag = max(float(np.linalg.norm(np.ptp(b.x0, axis=0))) for b in sc.bodies)
+    for f in range(1, 41):
+        dev.step(sc.dt, sc.iterations, sc.substeps)
+        orc.step(sc.dt, sc.iterations, sc.substeps)
+        if f in (1, 40):
+            err = max(float(np.abs(dev.read_body(b)[0] - orc.read_body(b)[0]).max()) for b in range(len(sc.bodies)))
+            print("batch + large error frame", f, err / diag, flush=True)
+            assert err / diag <= TOL
+
+
 def test_graph_replay_is_bit_identical_to_stream_launches():
     sc = scenes.cantilever(10, 3, 3, 0.1, "neohookean", 5e5, 0.45, num_spheres=3, iterations=6, substeps=3,
                            cluster=True)

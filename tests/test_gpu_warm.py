@@ -140,7 +140,7 @@ def test_capture_rows_and_features():
     sc = _beam(True, iterations=4)
     say("capture")
     dev = scenes.build_device(sc)
-    net = S.Mlp(12, 2)
+    net = S.Mlp(13, 2)
     T = sc.num_tets
     net.reserve(10 * T)
     dev.capture(net, 1)
@@ -174,6 +174,24 @@ def test_capture_rows_and_features():
     assert np.allclose(f0[:, 5], Ch, rtol=1e-3, atol=1e-5)
     assert np.allclose(f0[:, 8], np.log(ad), rtol=1e-5) and np.allclose(f0[:, 9], np.log(ah), rtol=1e-5)
     assert np.allclose(f0[:, 10], -Cd / ad, rtol=1e-5) and np.allclose(f0[:, 11], -Ch / ah, rtol=1e-3, atol=1e-9)
+    assert np.all(f0[:, 12] == np.float32(np.log(T)))  # one body: log of its tet count
+
+
+def test_body_size_feature_on_a_mixed_batch():
+    # one network serves bodies of every size: feature 12 is the log of the tet's body's tet count
+    sc = scenes.batch(3, tet_range=(2e3, 4e3), large=(1, 1.2e4), name="batch_3_plus_1_large")
+    say("body-size feature")
+    dev = scenes.build_device(sc)
+    net = S.Mlp(13, 2)
+    T = sc.num_tets
+    net.reserve(T)
+    dev.capture(net, 1)
+    dev.step(sc.dt, sc.iterations, 1)
+    dev.capture(None, 0)
+    x, _ = net.dataset()
+    expected = np.concatenate([np.full(len(b.tets), np.float32(np.log(len(b.tets)))) for b in sc.bodies])
+    assert len(np.unique(expected)) == len(sc.bodies)
+    assert np.array_equal(np.sort(x[:T, 12]), np.sort(expected))
 
 
 def _execution_order(dev, sc):
