@@ -17,8 +17,8 @@ none qualifies).
 
 After the timed regions an instrumented region (HIP events around every launch on the library's
 stream) measures the average duration of the tet projection kernel for the roofline figure. The
-CPU baseline times the FP64 oracle in the reference's insertion order on a bounded sample (one
-frame) of the same scene.
+CPU baseline times the FP64 oracle in the reference's insertion order on a bounded sample (frames of
+the same scene, or of a 16-body batch for the batches, until about 10 s of CPU work).
 """
 import argparse
 import json
@@ -69,9 +69,10 @@ def copy_calibration(nbytes=256 << 20, reps=4):
     log(f"copy calibration: {reps} device copies of {nbytes} bytes")
 
 
-def cpu_baseline(scene, sample=None):
-    """FP64 oracle, reference (insertion) order, one thread, one frame of the same scene (or of a
-    sample scene, scaled to the full scene by tet count)."""
+def cpu_baseline(scene, sample=None, min_seconds=10.0, max_frames=8):
+    """FP64 oracle, reference (insertion) order, one thread: frames of the same scene (or of a
+    sample scene, scaled to the full scene by tet count) until min_seconds of CPU work or
+    max_frames frames."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
@@ -81,16 +82,20 @@ def cpu_baseline(scene, sample=None):
     run.apply(sim)
     sim.set_order(False)
     setup = time.perf_counter() - t0
-    log(f"cpu baseline: {run.name} built in {setup:.1f}s, timing one frame ({run.substeps} substeps)")
-    t0 = time.perf_counter()
-    sim.step(run.dt, run.iterations, run.substeps)
-    el = time.perf_counter() - t0
+    log(f"cpu baseline: {run.name} built in {setup:.1f}s, timing frames of {run.substeps} substeps "
+        f"for at least {min_seconds:.0f}s")
+    frames, el = 0, 0.0
+    while frames < max_frames and (frames == 0 or el < min_seconds):
+        t0 = time.perf_counter()
+        sim.step(run.dt, run.iterations, run.substeps)
+        el += time.perf_counter() - t0
+        frames += 1
     scale = scene.num_tets / run.num_tets
-    what = (f"1 frame = {run.substeps} substeps x {run.iterations} iterations of {run.name} ({run.num_tets} tets)"
-            + (f", scaled by tet count to {scene.num_tets} tets" if sample else ""))
-    return {"value": scene.substeps / (el * scale), "unit": "substeps/s", "cores": 1, "kind": "port",
+    what = (f"{frames} frames = {frames * run.substeps} substeps x {run.iterations} iterations of {run.name} "
+            f"({run.num_tets} tets)" + (f", scaled by tet count to {scene.num_tets} tets" if sample else ""))
+    return {"value": frames * scene.substeps / (el * scale), "unit": "substeps/s", "cores": 1, "kind": "port",
             "sample": f"{what}; FP64 oracle in reference insertion order, single thread, {el:.1f}s on {cpu_model()}",
-            "projections_per_s": run.num_tets * run.iterations * run.substeps / el}
+            "projections_per_s": run.num_tets * run.iterations * run.substeps * frames / el}
 
 
 def _median_time(fn, reps):
