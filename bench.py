@@ -244,8 +244,8 @@ def main():
                          "and PLY components on their own workloads")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the time step runs on a single MI355X in this build; "
-              "multi-GPU sharding is a design (see DESIGN.md), not a measured path.", file=sys.stderr)
+        print("bench.py measures one GPU: the time step runs on a single MI355X; sharding the bodies or the "
+              "beam across GPUs is not implemented, so --gpus must be 1.", file=sys.stderr)
         sys.exit(2)
     if args.row != "step":
         out = {"boundary": row_boundary, "grid_sdf": row_grid_sdf, "ply": row_ply}[args.row](args)
