@@ -105,6 +105,19 @@ void launch_neohookean_clusters(hipStream_t s, tet_group_dev g, cluster_dev cl, 
 void launch_green_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn, uint32_t begin,
                       uint32_t count, uint32_t tbase, xcd_split const& sp, float alpha_tilde, float gamma, int first,
                       int block);
+// Device copies of up to kMaxSpans arrays in one launch (sizes in bytes, multiples of 4; 16-byte
+// aligned starts).
+constexpr int kMaxSpans = 8;
+struct copy_spans {
+    void* dst[kMaxSpans];
+    const void* src[kMaxSpans];
+    uint64_t bytes[kMaxSpans];
+    int n;
+};
+void launch_copy_spans(hipStream_t s, copy_spans const& spans);
+// out[0] = the sum of partial[0, n) by one block in a fixed order (out may be pinned host memory).
+void launch_sum_partials(hipStream_t s, const double* partial, uint32_t n, double* out);
+
 void launch_neohookean_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn,
                            uint32_t begin, uint32_t count, uint32_t tbase, xcd_split const& sp, float inv_dt2,
                            float beta_dt, int mode, int block, float hyd_scale = 1.0f);

@@ -1297,6 +1297,38 @@ void launch_neohookean_residual(hipStream_t s, tet_group_dev g, const pos4* x, u
         k_neohookean_residual<<<grid_for(count, kRedBlock), kRedBlock, 0, s>>>(g, x, begin, count, inv_dt2, partial);
 }
 
+__global__ void __launch_bounds__(256) k_copy_spans(copy_spans sp)
+{
+    uint64_t const tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x, stride = uint64_t(gridDim.x) * blockDim.x;
+    for (int k = 0; k < sp.n; ++k) {
+        uint64_t const words = sp.bytes[k] / 16;
+        uint4* d = static_cast<uint4*>(sp.dst[k]);
+        const uint4* s = static_cast<const uint4*>(sp.src[k]);
+        for (uint64_t i = tid; i < words; i += stride) d[i] = s[i];
+        uint64_t const tail = (sp.bytes[k] - words * 16) / 4;  // the last < 16 bytes, 4 at a time
+        if (tid < tail)
+            static_cast<uint32_t*>(sp.dst[k])[words * 4 + tid] = static_cast<const uint32_t*>(sp.src[k])[words * 4 + tid];
+    }
+}
+
+void launch_copy_spans(hipStream_t s, copy_spans const& spans)
+{
+    if (spans.n > 0) k_copy_spans<<<1024, 256, 0, s>>>(spans);
+}
+
+__global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__ partial, uint32_t n, double* out)
+{
+    double v = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += 256) v += partial[i];
+    double const t = block_sum<256>(v);
+    if (threadIdx.x == 0) out[0] = t;
+}
+
+void launch_sum_partials(hipStream_t s, const double* partial, uint32_t n, double* out)
+{
+    k_sum_partials<<<1, 256, 0, s>>>(partial, n, out);
+}
+
 void launch_distance_residual(hipStream_t s, distance_group_dev g, const pos4* x, uint32_t begin, uint32_t count,
                               float alpha_tilde, double* partial, int)
 {

@@ -162,6 +162,8 @@ struct sbs_gpu {
     dev_buf<float4> snap_vel;
     dev_buf<double4> snap_normal;
     bool have_snapshot = false;
+    double* resid_host = nullptr;  // pinned: the last frame's squared residual norm (fallback check)
+    dev_buf<double> check_partial;  // its per-block sums (its own buffer: frame graphs hold the pointer)
     double fb_limit = 0.0;
     uint32_t fb_iterations = 0;  // 0: the step's own iteration count
     uint64_t fb_count = 0;
@@ -175,6 +177,7 @@ struct sbs_gpu {
             (void)hipEventDestroy(r.b);
         }
         for (auto e : event_pool) (void)hipEventDestroy(e);
+        if (resid_host) (void)hipHostFree(resid_host);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -234,14 +237,18 @@ struct sbs_gpu {
     void commit(uint32_t recolor_passes);
     void color_group(group_t& g, uint32_t recolor_passes);
     void upload_group(group_t& g);
-    void enqueue_frame(double dt, uint32_t iterations, uint32_t substeps, int warm);
-    void run_frame(double dt, uint32_t iterations, uint32_t substeps, int warm);
+    void enqueue_frame(double dt, uint32_t iterations, uint32_t substeps, int warm, bool check = false);
+    void run_frame(double dt, uint32_t iterations, uint32_t substeps, int warm, bool check = false);
     void detect();
     void launch_group_colors(group_t const& g, double dt, int mode, const pos4* xref);
     void ensure_warm_buffers(bool raw, bool X);
     void snapshot();
+    void prepare_snapshot();
+    void enqueue_snapshot();
     void restore();
-    double residual_norm(double dt);
+    std::size_t residual_blocks() const;
+    void prepare_check();
+    void enqueue_residual(double dt);
 };
 
 namespace {
@@ -765,8 +772,11 @@ void sbs_gpu::launch_group_colors(group_t const& g, double dt, int mode, const p
     }
 }
 
-void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t substeps, int warm)
+// check: the frame starts with a snapshot of the state and ends with the residual for the
+// fallback decision (the caller prepared both).
+void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t substeps, int warm, bool check)
 {
+    if (check) enqueue_snapshot();
     double const dt = dt_frame / double(substeps);
     double const dt2 = dt * dt;
     float const g0 = float(params.gravity[0]), g1 = float(params.gravity[1]), g2 = float(params.gravity[2]);
@@ -849,6 +859,7 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
     timed(SBS_KERNEL_COMMIT, V, [&] { launch_commit(stream, xi.p, xn.p, vel.p, V, float(dt), block); });
     if (surface_refresh && ntris)  // timestep.cpp:61-66
         launch_surface_refresh(stream, surf.p, xn.p, stris.p, stri_off.p, stri_inc.p, snormal.p, S);
+    if (check) enqueue_residual(dt);
 }
 
 void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
@@ -889,31 +900,55 @@ void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
     }
 }
 
-void sbs_gpu::snapshot()
+// Buffers of the snapshot (allocated outside any graph capture).
+void sbs_gpu::prepare_snapshot()
 {
     if (!snap_xn.p) {
         snap_xn.alloc(V);
         snap_vel.alloc(V);
     }
-    hip_check(hipMemcpyAsync(snap_xn.p, xn.p, V * sizeof(pos4), hipMemcpyDeviceToDevice, stream), "snapshot");
-    hip_check(hipMemcpyAsync(snap_vel.p, vel.p, V * sizeof(float4), hipMemcpyDeviceToDevice, stream), "snapshot");
-    if (snormal.p) {
-        if (!snap_normal.p) snap_normal.alloc(S);
-        hip_check(hipMemcpyAsync(snap_normal.p, snormal.p, S * sizeof(double4), hipMemcpyDeviceToDevice, stream),
-                  "snapshot");
-    }
+    if (snormal.p && !snap_normal.p) snap_normal.alloc(S);
     for (auto& g : groups) {
-        if (!g.lambda.n) continue;
-        if (!g.snap_lambda.p) g.snap_lambda.alloc(g.lambda.n);
-        hip_check(hipMemcpyAsync(g.snap_lambda.p, g.lambda.p, g.lambda.n * sizeof(float), hipMemcpyDeviceToDevice, stream),
-                  "snapshot");
-        if (g.ws_hist.p) {
-            if (!g.snap_hist.p) g.snap_hist.alloc(g.ws_hist.n);
-            hip_check(hipMemcpyAsync(g.snap_hist.p, g.ws_hist.p, g.ws_hist.n * sizeof(float2), hipMemcpyDeviceToDevice,
-                                     stream), "snapshot");
-        }
+        if (g.lambda.n && !g.snap_lambda.p) g.snap_lambda.alloc(g.lambda.n);
+        if (g.ws_hist.p && !g.snap_hist.p) g.snap_hist.alloc(g.ws_hist.n);
     }
+}
+
+// Copies of the dynamic state (committed positions, velocities, normals, multipliers and their
+// history) in one launch.
+void sbs_gpu::enqueue_snapshot()
+{
+    copy_spans sp{};
+    auto add = [&](void* d, const void* src, std::size_t bytes) {
+        if (!bytes) return;
+        if (sp.n == kMaxSpans) {
+            launch_copy_spans(stream, sp);
+            sp = copy_spans{};
+        }
+        sp.dst[sp.n] = d;
+        sp.src[sp.n] = src;
+        sp.bytes[sp.n] = bytes;
+        ++sp.n;
+    };
+    require(snap_xn.p && snap_vel.p && (!snormal.p || snap_normal.p), "snapshot buffers not prepared", SBS_ERR_STATE);
+    for (auto const& g : groups)
+        require((!g.lambda.n || g.snap_lambda.p) && (!g.ws_hist.p || g.snap_hist.p), "snapshot buffers not prepared",
+                SBS_ERR_STATE);
+    add(snap_xn.p, xn.p, V * sizeof(pos4));
+    add(snap_vel.p, vel.p, V * sizeof(float4));
+    if (snormal.p) add(snap_normal.p, snormal.p, S * sizeof(double4));
+    for (auto& g : groups) {
+        if (g.lambda.n) add(g.snap_lambda.p, g.lambda.p, g.lambda.n * sizeof(float));
+        if (g.ws_hist.p) add(g.snap_hist.p, g.ws_hist.p, g.ws_hist.n * sizeof(float2));
+    }
+    launch_copy_spans(stream, sp);
     have_snapshot = true;
+}
+
+void sbs_gpu::snapshot()
+{
+    prepare_snapshot();
+    enqueue_snapshot();
 }
 
 void sbs_gpu::restore()
@@ -936,30 +971,49 @@ void sbs_gpu::restore()
 }
 
 // sqrt of the summed squared FP64 residuals of every neo-Hookean group (state after the last step).
-double sbs_gpu::residual_norm(double dt)
+std::size_t sbs_gpu::residual_blocks() const
 {
-    // one launch per group over all its colors (the total does not need the per-color sums)
-    double const dt2 = dt * dt;
     std::size_t total = 0;
     for (auto const& g : groups)
         if (g.kind == 1) total += (g.count + 255) / 256;
-    if (total == 0) return 0.0;
-    if (partial.n < total) partial.alloc(std::max<std::size_t>(total, 1024));
+    return total;
+}
+
+// Buffers of the per-frame fallback check (allocated outside any graph capture).
+void sbs_gpu::prepare_check()
+{
+    prepare_snapshot();
+    std::size_t const total = std::max<std::size_t>(residual_blocks(), 1);
+    if (check_partial.n < total) {
+        if (graph_exec) {  // a cached frame graph may hold the old buffer
+            hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            hip_check(hipGraphExecDestroy(graph_exec), "hipGraphExecDestroy");
+            graph_exec = nullptr;
+        }
+        check_partial.alloc(total);
+    }
+    if (!resid_host) {
+        void* p = nullptr;
+        hip_check(hipHostMalloc(&p, sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+        resid_host = static_cast<double*>(p);
+    }
+}
+
+// resid_host[0] = sum over the neo-Hookean groups of sum h^2 at the committed positions: one
+// launch per group over all its colors, then one block sums the partial sums in order.
+void sbs_gpu::enqueue_residual(double dt)
+{
+    double const dt2 = dt * dt;
     std::size_t at = 0;
     for (auto const& g : groups) {
         if (g.kind != 1) continue;
-        launch_neohookean_residual(stream, g.tet(), xn.p, 0, g.count, float(1.0 / dt2), partial.p + at, 256);
+        launch_neohookean_residual(stream, g.tet(), xn.p, 0, g.count, float(1.0 / dt2), check_partial.p + at, 256);
         at += (g.count + 255) / 256;
     }
-    std::vector<double> h(total);
-    hip_check(hipMemcpyAsync(h.data(), partial.p, total * sizeof(double), hipMemcpyDeviceToHost, stream), "residual");
-    hip_check(hipStreamSynchronize(stream), "residual");
-    double acc = 0.0;
-    for (double x : h) acc += x;
-    return std::sqrt(acc);
+    launch_sum_partials(stream, check_partial.p, uint32_t(at), resid_host);
 }
 
-void sbs_gpu::run_frame(double dt, uint32_t iterations, uint32_t substeps, int warm)
+void sbs_gpu::run_frame(double dt, uint32_t iterations, uint32_t substeps, int warm, bool check)
 {
     bool const capture = cap_net != nullptr && cap_stride > 0;
     if (use_graphs && !timing && !capture) {
@@ -979,6 +1033,7 @@ void sbs_gpu::run_frame(double dt, uint32_t iterations, uint32_t substeps, int w
         key[12] = ws_scale;
         key[13] = surface_refresh;
         key[14] = ws_scale_hyd;
+        key[15] = check;
         if (!graph_exec || key != graph_key) {
             if (graph_exec) {
                 // the previous graph may still be running on the stream: let it finish first
@@ -989,7 +1044,7 @@ void sbs_gpu::run_frame(double dt, uint32_t iterations, uint32_t substeps, int w
             hipGraph_t graph;
             hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
             try {
-                enqueue_frame(dt, iterations, substeps, warm);
+                enqueue_frame(dt, iterations, substeps, warm, check);
             } catch (...) {
                 hipGraph_t g2 = nullptr;
                 (void)hipStreamEndCapture(stream, &g2);
@@ -1004,7 +1059,7 @@ void sbs_gpu::run_frame(double dt, uint32_t iterations, uint32_t substeps, int w
         }
         hip_check(hipGraphLaunch(graph_exec, stream), "hipGraphLaunch");
     } else {
-        enqueue_frame(dt, iterations, substeps, warm);
+        enqueue_frame(dt, iterations, substeps, warm, check);
         hip_check(hipGetLastError(), "kernel launch");
     }
 }
@@ -1392,13 +1447,18 @@ int sbs_gpu_step(sbs_gpu* sim, double dt, uint32_t iterations, uint32_t substeps
         require(substeps >= 1, "substeps must be >= 1");
         sim->set_current();
         bool const check = sim->fb_limit > 0.0 && sim->ws_mode != SBS_WARM_START_NONE;
-        if (check) sim->snapshot();
-        sim->run_frame(dt, iterations, substeps, sim->ws_mode);
-        // !(r <= limit): a frame whose residual is NaN (one that diverged) is repeated as well
-        if (check && !(sim->residual_norm(dt / double(substeps)) <= sim->fb_limit)) {
-            sim->restore();
-            sim->run_frame(dt, sim->fb_iterations ? sim->fb_iterations : iterations, substeps, SBS_WARM_START_NONE);
-            ++sim->fb_count;
+        // with the check the frame itself snapshots the state and ends with its residual
+        if (check) sim->prepare_check();
+        sim->run_frame(dt, iterations, substeps, sim->ws_mode, check);
+        if (check) {
+            hip_check(hipStreamSynchronize(sim->stream), "fallback check");
+            double const r = std::sqrt(*sim->resid_host);
+            // !(r <= limit): a frame whose residual is NaN (one that diverged) is repeated as well
+            if (!(r <= sim->fb_limit)) {
+                sim->restore();
+                sim->run_frame(dt, sim->fb_iterations ? sim->fb_iterations : iterations, substeps, SBS_WARM_START_NONE);
+                ++sim->fb_count;
+            }
         }
         if (sim->cap_net && sim->cap_stride) hip_check(hipStreamSynchronize(sim->stream), "capture");
     });
