@@ -115,6 +115,12 @@ struct copy_spans {
     int n;
 };
 void launch_copy_spans(hipStream_t s, copy_spans const& spans);
+// Residual partial sums (as launch_neohookean_residual's) of a hex-cluster neo-Hookean group, one
+// thread per cluster of every color (block sums of its six tets' h^2; partial gets
+// ceil(nclusters / 256) values). ccolor_off / color_off: the colors' first cluster / constraint.
+void launch_neohookean_hex_residual(hipStream_t s, tet_group_dev g, cluster_dev cl, const pos4* x,
+                                    const uint32_t* ccolor_off, const uint32_t* color_off, uint32_t ncolors,
+                                    uint32_t nclusters, float inv_dt2, double* partial);
 // out[0] = the sum of partial[0, n) by one block in a fixed order (out may be pinned host memory).
 void launch_sum_partials(hipStream_t s, const double* partial, uint32_t n, double* out);
 

@@ -941,10 +941,9 @@ __global__ void __launch_bounds__(kRedBlock) k_green_residual(tet_group_dev g, c
 
 // ||F||_F^2 and det F - gamma0 of neo-Hookean tet id at positions x, in FP64 from the FP64 edge
 // vectors and the FP32 rest data (the forms at nh_rest).
-__device__ __forceinline__ void nh_constraints_f64(const pos4* __restrict__ x, int4 id, nh_rest const& q, float gamma0,
+__device__ __forceinline__ void nh_constraints_f64(pos4 p0, pos4 p1, pos4 p2, pos4 p3, nh_rest const& q, float gamma0,
                                                    double& f2, double& det_minus)
 {
-    pos4 const p3 = x[id.w], p0 = x[id.x], p1 = x[id.y], p2 = x[id.z];
     double const e[3][3] = {{p0.x - p3.x, p0.y - p3.y, p0.z - p3.z},
                             {p1.x - p3.x, p1.y - p3.y, p1.z - p3.z},
                             {p2.x - p3.x, p2.y - p3.y, p2.z - p3.z}};
@@ -958,6 +957,58 @@ __device__ __forceinline__ void nh_constraints_f64(const pos4* __restrict__ x, i
     det_minus = double(q.detr) * vol - double(gamma0);
 }
 
+__device__ __forceinline__ void nh_constraints_f64(const pos4* __restrict__ x, int4 id, nh_rest const& q, float gamma0,
+                                                   double& f2, double& det_minus)
+{
+    nh_constraints_f64(x[id.x], x[id.y], x[id.z], x[id.w], q, gamma0, f2, det_minus);
+}
+
+// h^2 of both constraints of neo-Hookean tet t at positions P0..P3 (FP64).
+__device__ __forceinline__ double nh_residual2(tet_group_dev const& g, uint32_t t, pos4 p0, pos4 p1, pos4 p2, pos4 p3,
+                                               float inv_dt2)
+{
+    float4 const c = g.m2[t];
+    double cd, ch;
+    nh_constraints_f64(p0, p1, p2, p3, load_nh(g.m0[t], g.m1[t]), c.w, cd, ch);
+    float2 const lm = reinterpret_cast<const float2*>(g.lambda)[t];
+    cd = sqrt(cd);
+    double const hd = cd + double(c.y) * double(inv_dt2) * double(lm.x);
+    double const hh = ch + double(c.z) * double(inv_dt2) * double(lm.y);
+    return hd * hd + hh * hh;
+}
+
+// The same over a hex-cluster group, one thread per cluster of every color: the cluster's eight
+// particles are read once for its six tets.
+__global__ void __launch_bounds__(kRedBlock) k_neohookean_hex_residual(tet_group_dev g, cluster_dev cl,
+                                                                       const pos4* __restrict__ x,
+                                                                       const uint32_t* __restrict__ ccolor_off,
+                                                                       const uint32_t* __restrict__ color_off,
+                                                                       uint32_t ncolors, uint32_t nclusters,
+                                                                       float inv_dt2, double* partial)
+{
+    uint32_t const ci = blockIdx.x * blockDim.x + threadIdx.x;
+    double h2 = 0.0;
+    if (ci < nclusters) {
+        uint32_t lo = 0, hi = ncolors;  // the color: ccolor_off[col] <= ci < ccolor_off[col + 1]
+        while (hi - lo > 1) {
+            uint32_t const mid = (lo + hi) / 2;
+            if (ccolor_off[mid] <= ci)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        uint32_t const i = ci - ccolor_off[lo], count = ccolor_off[lo + 1] - ccolor_off[lo], tbase = color_off[lo];
+        int4 const va = cl.v0[ci], vb = cl.v1[ci];
+        pos4 const P[8] = {x[va.x], x[va.y], x[va.z], x[va.w], x[vb.x], x[vb.y], x[vb.z], x[vb.w]};
+#pragma unroll
+        for (int k = 0; k < kHexTets; ++k)
+            h2 += nh_residual2(g, tbase + uint32_t(k) * count + i, P[kHexSlots[k][0]], P[kHexSlots[k][1]],
+                               P[kHexSlots[k][2]], P[kHexSlots[k][3]], inv_dt2);
+    }
+    double const s = block_sum<kRedBlock>(h2);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
 __global__ void __launch_bounds__(kRedBlock) k_neohookean_residual(tet_group_dev g, const pos4* __restrict__ x,
                                                                    uint32_t begin, uint32_t count, float inv_dt2,
                                                                    double* partial)
@@ -967,14 +1018,7 @@ __global__ void __launch_bounds__(kRedBlock) k_neohookean_residual(tet_group_dev
     if (i < count) {
         uint32_t const t = begin + i;
         int4 const id = g.idx[t];
-        float4 const c = g.m2[t];
-        double cd, ch;
-        nh_constraints_f64(x, id, load_nh(g.m0[t], g.m1[t]), c.w, cd, ch);
-        float2 const lm = reinterpret_cast<const float2*>(g.lambda)[t];
-        cd = sqrt(cd);
-        double const hd = cd + double(c.y) * double(inv_dt2) * double(lm.x);
-        double const hh = ch + double(c.z) * double(inv_dt2) * double(lm.y);
-        h2 = hd * hd + hh * hh;
+        h2 = nh_residual2(g, t, x[id.x], x[id.y], x[id.z], x[id.w], inv_dt2);
     }
     double const s = block_sum<kRedBlock>(h2);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -1309,6 +1353,16 @@ __global__ void __launch_bounds__(256) k_copy_spans(copy_spans sp)
         if (tid < tail)
             static_cast<uint32_t*>(sp.dst[k])[words * 4 + tid] = static_cast<const uint32_t*>(sp.src[k])[words * 4 + tid];
     }
+}
+
+void launch_neohookean_hex_residual(hipStream_t s, tet_group_dev g, cluster_dev cl, const pos4* x,
+                                    const uint32_t* ccolor_off, const uint32_t* color_off, uint32_t ncolors,
+                                    uint32_t nclusters, float inv_dt2, double* partial)
+{
+    if (nclusters)
+        k_neohookean_hex_residual<<<grid_for(nclusters, kRedBlock), kRedBlock, 0, s>>>(g, cl, x, ccolor_off, color_off,
+                                                                                     ncolors, nclusters, inv_dt2,
+                                                                                     partial);
 }
 
 void launch_copy_spans(hipStream_t s, copy_spans const& spans)

@@ -162,6 +162,13 @@ struct sbs_gpu {
     dev_buf<float4> snap_vel;
     dev_buf<double4> snap_normal;
     bool have_snapshot = false;
+    // the per-frame fallback check's own state copy (the public snapshot is left alone): committed
+    // positions, velocities, normals, and in learned mode the multiplier history. The plain frame
+    // that replaces a rejected one writes every multiplier before reading it, so they are not kept.
+    dev_buf<pos4> chk_xn;
+    dev_buf<float4> chk_vel;
+    dev_buf<double4> chk_normal;
+    std::vector<dev_buf<float2>> chk_hist;  // per group
     double* resid_host = nullptr;  // pinned: the last frame's squared residual norm (fallback check)
     dev_buf<double> check_partial;  // its per-block sums (its own buffer: frame graphs hold the pointer)
     double fb_limit = 0.0;
@@ -245,7 +252,9 @@ struct sbs_gpu {
     void snapshot();
     void prepare_snapshot();
     void enqueue_snapshot();
+    void enqueue_check_snapshot();
     void restore();
+    void restore_check();
     std::size_t residual_blocks() const;
     void prepare_check();
     void enqueue_residual(double dt);
@@ -776,7 +785,7 @@ void sbs_gpu::launch_group_colors(group_t const& g, double dt, int mode, const p
 // fallback decision (the caller prepared both).
 void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t substeps, int warm, bool check)
 {
-    if (check) enqueue_snapshot();
+    if (check) enqueue_check_snapshot();
     double const dt = dt_frame / double(substeps);
     double const dt2 = dt * dt;
     float const g0 = float(params.gravity[0]), g1 = float(params.gravity[1]), g2 = float(params.gravity[2]);
@@ -887,8 +896,10 @@ void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
                 for (int k = 0; k < 8; ++k) ent[fill[slot(c, k)]++] = c * 8 + uint32_t(k);
             g.ws_csr_off.upload(off.data(), off.size(), stream);
             g.ws_csr.upload(ent.data(), ent.size(), stream);
-            g.ws_ccolor_off.upload(g.cluster_color_offsets.data(), g.cluster_color_offsets.size(), stream);
-            g.ws_color_off.upload(g.color_offsets.data(), g.color_offsets.size(), stream);
+            if (!g.ws_ccolor_off.p) {  // (may be there already: the fallback check's residual uses them)
+                g.ws_ccolor_off.upload(g.cluster_color_offsets.data(), g.cluster_color_offsets.size(), stream);
+                g.ws_color_off.upload(g.color_offsets.data(), g.color_offsets.size(), stream);
+            }
             g.ws_disp.alloc(std::size_t(nc) * 8);
         }
         if (!g.ws_hist.p && g.count) {
@@ -951,6 +962,46 @@ void sbs_gpu::snapshot()
     enqueue_snapshot();
 }
 
+void sbs_gpu::enqueue_check_snapshot()
+{
+    copy_spans sp{};
+    auto add = [&](void* d, const void* src, std::size_t bytes) {
+        require(d != nullptr && sp.n < kMaxSpans, "check buffers not prepared", SBS_ERR_STATE);
+        sp.dst[sp.n] = d;
+        sp.src[sp.n] = src;
+        sp.bytes[sp.n] = bytes;
+        ++sp.n;
+    };
+    add(chk_xn.p, xn.p, V * sizeof(pos4));
+    add(chk_vel.p, vel.p, V * sizeof(float4));
+    if (snormal.p) add(chk_normal.p, snormal.p, S * sizeof(double4));
+    if (ws_mode == SBS_WARM_START_LEARNED)
+        for (std::size_t i = 0; i < groups.size(); ++i)
+            if (groups[i].ws_hist.p) {
+                if (sp.n == kMaxSpans) {
+                    launch_copy_spans(stream, sp);
+                    sp = copy_spans{};
+                }
+                add(chk_hist[i].p, groups[i].ws_hist.p, groups[i].ws_hist.n * sizeof(float2));
+            }
+    launch_copy_spans(stream, sp);
+}
+
+void sbs_gpu::restore_check()
+{
+    hip_check(hipMemcpyAsync(xn.p, chk_xn.p, V * sizeof(pos4), hipMemcpyDeviceToDevice, stream), "restore");
+    hip_check(hipMemcpyAsync(xi.p, chk_xn.p, V * sizeof(pos4), hipMemcpyDeviceToDevice, stream), "restore");
+    hip_check(hipMemcpyAsync(vel.p, chk_vel.p, V * sizeof(float4), hipMemcpyDeviceToDevice, stream), "restore");
+    if (snormal.p)
+        hip_check(hipMemcpyAsync(snormal.p, chk_normal.p, S * sizeof(double4), hipMemcpyDeviceToDevice, stream),
+                  "restore");
+    if (ws_mode == SBS_WARM_START_LEARNED)
+        for (std::size_t i = 0; i < groups.size(); ++i)
+            if (groups[i].ws_hist.p && chk_hist[i].p)
+                hip_check(hipMemcpyAsync(groups[i].ws_hist.p, chk_hist[i].p, groups[i].ws_hist.n * sizeof(float2),
+                                         hipMemcpyDeviceToDevice, stream), "restore");
+}
+
 void sbs_gpu::restore()
 {
     require(have_snapshot, "no snapshot has been taken", SBS_ERR_STATE);
@@ -971,18 +1022,33 @@ void sbs_gpu::restore()
 }
 
 // sqrt of the summed squared FP64 residuals of every neo-Hookean group (state after the last step).
+// Blocks of the check's residual: per hex-cluster neo-Hookean group one per 256 clusters, per other
+// neo-Hookean group one per 256 tets.
 std::size_t sbs_gpu::residual_blocks() const
 {
     std::size_t total = 0;
     for (auto const& g : groups)
-        if (g.kind == 1) total += (g.count + 255) / 256;
+        if (g.kind == 1) total += ((g.hex ? g.num_elements() : g.count) + 255) / 256;
     return total;
 }
 
 // Buffers of the per-frame fallback check (allocated outside any graph capture).
 void sbs_gpu::prepare_check()
 {
-    prepare_snapshot();
+    if (!chk_xn.p) {
+        chk_xn.alloc(V);
+        chk_vel.alloc(V);
+    }
+    if (snormal.p && !chk_normal.p) chk_normal.alloc(S);
+    chk_hist.resize(groups.size());
+    for (auto& g : groups)
+        if (g.kind == 1 && g.hex && !g.ws_ccolor_off.p) {
+            g.ws_ccolor_off.upload(g.cluster_color_offsets.data(), g.cluster_color_offsets.size(), stream);
+            g.ws_color_off.upload(g.color_offsets.data(), g.color_offsets.size(), stream);
+        }
+    if (ws_mode == SBS_WARM_START_LEARNED)
+        for (std::size_t i = 0; i < groups.size(); ++i)
+            if (groups[i].ws_hist.p && !chk_hist[i].p) chk_hist[i].alloc(groups[i].ws_hist.n);
     std::size_t const total = std::max<std::size_t>(residual_blocks(), 1);
     if (check_partial.n < total) {
         if (graph_exec) {  // a cached frame graph may hold the old buffer
@@ -1000,15 +1066,23 @@ void sbs_gpu::prepare_check()
 }
 
 // resid_host[0] = sum over the neo-Hookean groups of sum h^2 at the committed positions: one
-// launch per group over all its colors, then one block sums the partial sums in order.
+// launch per group over all its colors (per cluster for hex groups), then one block sums the
+// partial sums in order.
 void sbs_gpu::enqueue_residual(double dt)
 {
     double const dt2 = dt * dt;
     std::size_t at = 0;
     for (auto const& g : groups) {
         if (g.kind != 1) continue;
-        launch_neohookean_residual(stream, g.tet(), xn.p, 0, g.count, float(1.0 / dt2), check_partial.p + at, 256);
-        at += (g.count + 255) / 256;
+        if (g.hex) {
+            uint32_t const nc = g.num_elements();
+            launch_neohookean_hex_residual(stream, g.tet(), g.clusters(), xn.p, g.ws_ccolor_off.p, g.ws_color_off.p,
+                                           g.ncolors, nc, float(1.0 / dt2), check_partial.p + at);
+            at += (nc + 255) / 256;
+        } else {
+            launch_neohookean_residual(stream, g.tet(), xn.p, 0, g.count, float(1.0 / dt2), check_partial.p + at, 256);
+            at += (g.count + 255) / 256;
+        }
     }
     launch_sum_partials(stream, check_partial.p, uint32_t(at), resid_host);
 }
@@ -1455,7 +1529,7 @@ int sbs_gpu_step(sbs_gpu* sim, double dt, uint32_t iterations, uint32_t substeps
             double const r = std::sqrt(*sim->resid_host);
             // !(r <= limit): a frame whose residual is NaN (one that diverged) is repeated as well
             if (!(r <= sim->fb_limit)) {
-                sim->restore();
+                sim->restore_check();
                 sim->run_frame(dt, sim->fb_iterations ? sim->fb_iterations : iterations, substeps, SBS_WARM_START_NONE);
                 ++sim->fb_count;
             }
