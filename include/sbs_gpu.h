@@ -332,9 +332,10 @@ int sbs_gpu_capture(sbs_gpu* sim, sbs_mlp* net, uint32_t stride);
 /* Device copy of the dynamic state (positions, velocities, multipliers) and its restoration. */
 int sbs_gpu_snapshot(sbs_gpu* sim);
 int sbs_gpu_restore(sbs_gpu* sim);
-/* With a warm start and limit > 0, each sbs_gpu_step snapshots the state and, when the FP64
- * residual norm sqrt(sum over neo-Hookean groups of sum h^2) after the frame exceeds limit or is
- * not a number (a frame that diverged), restores
+/* With a warm start and limit > 0, each sbs_gpu_step copies the state it needs to repeat the frame
+ * (its own copy: a snapshot taken with sbs_gpu_snapshot is kept) and, when the FP64 residual norm
+ * sqrt(sum over neo-Hookean groups of sum h^2) after the frame exceeds limit or is not a number (a
+ * frame that diverged), restores
  * it and repeats the frame without warm start (the plain solver as fallback) with `iterations`
  * iterations per substep (0: the step's own count). *fallbacks counts the repeated frames. */
 int sbs_gpu_set_fallback(sbs_gpu* sim, double limit, uint32_t iterations);
