@@ -262,6 +262,26 @@ def test_fallback_repeats_a_diverged_frame():
     assert np.all(np.isfinite(b.read_body(0)[0])) and np.all(np.isfinite(b.read_body(0)[1]))
 
 
+def test_checked_frames_keep_the_public_snapshot():
+    # the fallback check copies the state into buffers of its own
+    sc = _beam(True)
+    say("snapshot kept across checked frames")
+    dev = scenes.build_device(sc)
+    for _ in range(2):
+        dev.step(sc.dt, sc.iterations, sc.substeps)
+    x0, v0 = (a.copy() for a in dev.read_body(0))
+    dev.snapshot()
+    dev.set_warm_start("previous")
+    dev.set_warm_start_scale(0.5)
+    dev.set_fallback(1e300, sc.iterations)
+    for _ in range(3):
+        dev.step(sc.dt, sc.iterations, sc.substeps)
+    assert not np.array_equal(dev.read_body(0)[0], x0)
+    dev.restore()
+    x1, v1 = dev.read_body(0)
+    assert np.array_equal(x1, x0) and np.array_equal(v1, v0)
+
+
 def test_learned_warm_start_trains_and_runs():
     from sbs_amd import autotune
     sc = _beam(True, iterations=8)
