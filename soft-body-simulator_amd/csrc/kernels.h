@@ -168,7 +168,8 @@ void launch_copy_pos(hipStream_t s, const pos4* a, pos4* b, uint32_t n);
 // every cluster's slot displacements at the predicted positions xt into disp [8 per cluster], then
 // for each particle p < n, xi[p] += the FP64 sum over csr[csr_off[p] .. csr_off[p+1]) (indices into
 // disp, in cluster execution order) -- the same additions in the same order as the per-color
-// pass. ccolor_off / color_off: the group's cluster and constraint offsets per color (ncolors + 1).
+// pass; with csr_off null, csr is a dense [n][8] table of those indices (unused: 0xffffffff).
+// ccolor_off / color_off: the group's cluster and constraint offsets per color (ncolors + 1).
 void launch_hex_warm_start(hipStream_t s, tet_group_dev g, cluster_dev cl, const pos4* xt, pos4* xi,
                            const uint32_t* ccolor_off, const uint32_t* color_off, uint32_t ncolors, uint32_t nclusters,
                            float inv_dt2, int expl, float2 scale, float4* disp, const uint32_t* csr_off,

@@ -773,6 +773,31 @@ __global__ void k_gather_disp(const uint32_t* __restrict__ off, const uint32_t* 
     xi[p] = x;
 }
 
+// The same with the list as a dense row of 8 slot indices per particle (unused: 0xffffffff), when
+// no particle is in more than 8 clusters (hex lattices): the 8 loads of a row go out together and
+// the unused ones add an exact 0.
+__global__ void k_gather_disp8(const uint4* __restrict__ ent, const float4* __restrict__ disp, pos4* __restrict__ xi,
+                               uint32_t n)
+{
+    uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    uint4 const a = ent[2 * std::size_t(p)], b = ent[2 * std::size_t(p) + 1];
+    uint32_t const e[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    if (e[0] == 0xffffffffu) return;
+    float4 d[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) d[m] = disp[e[m] != 0xffffffffu ? e[m] : 0u];
+    pos4 x = xi[p];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        bool const ok = e[m] != 0xffffffffu;
+        x.x += ok ? double(d[m].x) : 0.0;
+        x.y += ok ? double(d[m].y) : 0.0;
+        x.z += ok ? double(d[m].z) : 0.0;
+    }
+    xi[p] = x;
+}
+
 // FIRST: the first iteration of a substep, a compile-time switch for the reason given at
 // k_neohookean_hex.
 template <int BLOCK, bool FIRST>
@@ -1434,7 +1459,10 @@ void launch_hex_warm_start(hipStream_t s, tet_group_dev g, cluster_dev cl, const
     if (!nclusters) return;
     k_hex_warm_disp<<<grid_for(nclusters, 64), 64, 0, s>>>(g, cl, xt, ccolor_off, color_off, ncolors, nclusters,
                                                           inv_dt2, expl, scale, disp);
-    k_gather_disp<<<grid_for(n, 256), 256, 0, s>>>(csr_off, csr, disp, xi, n);
+    if (csr_off)
+        k_gather_disp<<<grid_for(n, 256), 256, 0, s>>>(csr_off, csr, disp, xi, n);
+    else
+        k_gather_disp8<<<grid_for(n, 256), 256, 0, s>>>(reinterpret_cast<const uint4*>(csr), disp, xi, n);
 }
 
 void launch_copy_pos(hipStream_t s, const pos4* a, pos4* b, uint32_t n)

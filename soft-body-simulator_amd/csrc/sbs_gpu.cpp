@@ -890,12 +890,21 @@ void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
             };
             for (uint32_t c = 0; c < nc; ++c)
                 for (int k = 0; k < 8; ++k) off[std::size_t(slot(c, k)) + 1]++;
+            uint32_t widest = 0;
+            for (std::size_t p = 0; p < V; ++p) widest = std::max(widest, off[p + 1]);
             for (std::size_t p = 0; p < V; ++p) off[p + 1] += off[p];
             std::vector<uint32_t> fill(off.begin(), off.end() - 1);
             for (uint32_t c = 0; c < nc; ++c)
                 for (int k = 0; k < 8; ++k) ent[fill[slot(c, k)]++] = c * 8 + uint32_t(k);
-            g.ws_csr_off.upload(off.data(), off.size(), stream);
-            g.ws_csr.upload(ent.data(), ent.size(), stream);
+            if (widest <= 8) {  // a dense row of 8 per particle, same order (ws_csr_off stays empty)
+                std::vector<uint32_t> dense(std::size_t(V) * 8, 0xffffffffu);
+                for (std::size_t p = 0; p < V; ++p)
+                    for (uint32_t j = off[p]; j < off[p + 1]; ++j) dense[8 * p + (j - off[p])] = ent[j];
+                g.ws_csr.upload(dense.data(), dense.size(), stream);
+            } else {
+                g.ws_csr_off.upload(off.data(), off.size(), stream);
+                g.ws_csr.upload(ent.data(), ent.size(), stream);
+            }
             if (!g.ws_ccolor_off.p) {  // (may be there already: the fallback check's residual uses them)
                 g.ws_ccolor_off.upload(g.cluster_color_offsets.data(), g.cluster_color_offsets.size(), stream);
                 g.ws_color_off.upload(g.color_offsets.data(), g.color_offsets.size(), stream);
