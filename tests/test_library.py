@@ -128,3 +128,28 @@ def test_batch_scene_clusters_take_eight_balanced_colors():
     assert nc == 8
     per_color = np.bincount(col, minlength=8)
     assert per_color.min() > 0.5 * per_color.max()  # odd box dimensions make parity classes uneven
+
+
+def test_null_handles_are_status_codes():
+    # every sbs_gpu_* entry point taking the handle first rejects a null one with a status code and a
+    # message (no device work: the check comes before any)
+    from sbs_amd import _lib
+    L = S.lib()
+    skip = {"sbs_gpu_destroy", "sbs_gpu_last_error", "sbs_gpu_create", "sbs_gpu_extract_boundary"}
+    checked = 0
+    for name, argtypes in _lib._SIGS.items():
+        if not name.startswith("sbs_gpu_") or name in skip or not argtypes or argtypes[0] is not _lib._p:
+            continue
+        args = []
+        for t in argtypes:
+            if t in (_lib._d, _lib._f):
+                args.append(t(0.0))
+            elif t in (_lib._i, _lib._u32, _lib._u64):
+                args.append(t(0))
+            else:
+                args.append(None)
+        rc = getattr(L, name)(*args)
+        assert rc != 0, name
+        assert L.sbs_gpu_last_error().decode(), name
+        checked += 1
+    assert checked >= 30
