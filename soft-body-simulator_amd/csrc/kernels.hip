@@ -230,13 +230,18 @@ __device__ __forceinline__ tet_frame make_frame(Pos const P[4], XnOf&& xn_of, bo
 //   cof(F) r_k = cof(Ds) cof(R) r_k = det(R) (e_{k+1} x e_{k+2}), det F = det(R) e0 . (e1 x e2)
 // (cof(R) = det(R) R^-T and R^-T r_k = Dm^T r_k is the unit vector k). So the deviatoric gradient
 // dC/dx_k = F r_k / ||F|| and the hydrostatic one cof(F) r_k need no F and no cofactor matrix.
-// Material layout m0 = (Q00, Q11, Q22, Q01), m1 = (Q02, Q12, det R, -),
-// m2 = (-, 1/(mu|V0|), 1/(lambda|V0|), 1 + mu/lambda).
+// Material layout (40 B per tet): m0 = (Q00, Q11, Q22, Q01), m1 = (Q02, Q12, det R, 1/(mu|V0|)),
+// and m2 read as float2 per tet = (1/(lambda|V0|), 1 + mu/lambda).
 struct nh_rest {
     float q00, q11, q22, q01, q02, q12, detr;
 };
 
 __device__ __forceinline__ nh_rest load_nh(float4 a, float4 b) { return {a.x, a.y, a.z, a.w, b.x, b.y, b.z}; }
+
+__device__ __forceinline__ float2 nh_m2(tet_group_dev const& g, uint32_t t)
+{
+    return reinterpret_cast<const float2*>(g.m2)[t];
+}
 
 // The neo-Hookean updates below spell out every fused multiply-add and contract nothing else, so
 // their rounding is fixed by the source and not by the kernel they are inlined into: the hex
@@ -328,7 +333,7 @@ __device__ __forceinline__ void tet_update(tet_frame& f, v3 const h[4], float s,
 // hydrostatic), material (a, b, c) as laid out at nh_rest. xn_of(k) is the substep start position
 // of vertex k, read only with damping (beta_dt != 0).
 template <class Pos, class XnOf>
-__device__ __forceinline__ void neohookean_project(Pos P[4], XnOf&& xn_of, float4 a, float4 b, float4 c, float inv_dt2,
+__device__ __forceinline__ void neohookean_project(Pos P[4], XnOf&& xn_of, float4 a, float4 b, float2 c, float inv_dt2,
                                                    float beta_dt, float2& lm)
 {
     nh_rest const q = load_nh(a, b);
@@ -337,14 +342,14 @@ __device__ __forceinline__ void neohookean_project(Pos P[4], XnOf&& xn_of, float
         v3 h[4];
         float s;
         float const C = nh_deviatoric(f.e, q, h, s);
-        float const at = c.y * inv_dt2;
+        float const at = b.w * inv_dt2;
         tet_update(f, h, s, C, at, at * beta_dt, lm.x);
     }
     {  // hydrostatic: C = det F - (1 + mu/lambda), compliance 1/(lambda |V0|)
         v3 h[4];
         float s;
-        float const C = nh_hydrostatic(f.e, q, c.w, h, s);
-        float const at = c.z * inv_dt2;
+        float const C = nh_hydrostatic(f.e, q, c.y, h, s);
+        float const at = c.x * inv_dt2;
         tet_update(f, h, s, C, at, at * beta_dt, lm.y);
     }
 #pragma unroll
@@ -357,7 +362,7 @@ __device__ __forceinline__ void neohookean_project(Pos P[4], XnOf&& xn_of, float
 // scale (x: deviatoric, y: hydrostatic; the caller stores them back). A constraint whose
 // sum w|grad C|^2 is below the projection's epsilon contributes nothing, as it would not project.
 template <class Pos>
-__device__ __forceinline__ void neohookean_displacement(Pos const P[4], float4 a, float4 b, float4 c, float inv_dt2,
+__device__ __forceinline__ void neohookean_displacement(Pos const P[4], float4 a, float4 b, float2 c, float inv_dt2,
                                                         bool expl, float2 scale, float2& lm, v3 d[4])
 {
     nh_rest const q = load_nh(a, b);
@@ -365,8 +370,8 @@ __device__ __forceinline__ void neohookean_displacement(Pos const P[4], float4 a
     v3 hd[4], hh[4];
     float sd, sh;
     float const Cd = nh_deviatoric(f.e, q, hd, sd);
-    float const Ch = nh_hydrostatic(f.e, q, c.w, hh, sh);
-    if (expl) lm = make_float2(-Cd / (c.y * inv_dt2), -Ch / (c.z * inv_dt2));
+    float const Ch = nh_hydrostatic(f.e, q, c.y, hh, sh);
+    if (expl) lm = make_float2(-Cd / (b.w * inv_dt2), -Ch / (c.x * inv_dt2));
     lm = make_float2(scale.x * lm.x, scale.y * lm.y);
     float const wd = grad_norm2(f.w, hd, sd), wh = grad_norm2(f.w, hh, sh);
     float const ld = wd < 1e-20f ? 0.0f : lm.x * sd, lh = wh < 1e-20f ? 0.0f : lm.y * sh;
@@ -396,7 +401,8 @@ __global__ void __launch_bounds__(256) k_neohookean(tet_group_dev g, pos4* xi, c
     if (i >= count) return;
     uint32_t const t = begin + i;
     int4 const id = g.idx[t];
-    float4 const a = g.m0[t], b = g.m1[t], c = g.m2[t];
+    float4 const a = g.m0[t], b = g.m1[t];
+    float2 const c = nh_m2(g, t);
     float2 lm = mode == kSolveFirst ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(g.lambda)[t];
     if (mode >= kApplyWarmStart) {  // gradients at the predicted positions (xn holds a copy of them)
         pos4 const P[4] = {xn[id.x], xn[id.y], xn[id.z], xn[id.w]};
@@ -439,8 +445,9 @@ __global__ void __launch_bounds__(256) k_ws_features(tet_group_dev g, const pos4
     uint32_t const t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
     int4 const id = g.idx[t];
-    float4 const c = g.m2[t];
-    nh_rest const q = load_nh(g.m0[t], g.m1[t]);
+    float4 const b = g.m1[t];
+    float2 const c = nh_m2(g, t);
+    nh_rest const q = load_nh(g.m0[t], b);
     pos4 const P[4] = {xi[id.x], xi[id.y], xi[id.z], xi[id.w]};
     tet_frame const f = make_frame(P, [](int) { return pos4{}; }, false);
     float2 const lp = reinterpret_cast<const float2*>(g.lambda)[t];
@@ -449,9 +456,9 @@ __global__ void __launch_bounds__(256) k_ws_features(tet_group_dev g, const pos4
     v3 hd[4], hh[4];
     float sd, sh;
     float const Cd = nh_deviatoric(f.e, q, hd, sd);
-    float const Ch = nh_hydrostatic(f.e, q, c.w, hh, sh);
+    float const Ch = nh_hydrostatic(f.e, q, c.y, hh, sh);
     float const wd = grad_norm2(f.w, hd, sd), wh = grad_norm2(f.w, hh, sh);
-    float const atd = c.y * inv_dt2, ath = c.z * inv_dt2;
+    float const atd = b.w * inv_dt2, ath = c.x * inv_dt2;
     float const v[mlp::kMaxInputs] = {lp.x,
                                       lp.y,
                                       lp2.x,
@@ -568,7 +575,7 @@ __global__ void __launch_bounds__(BLOCK) k_neohookean_clusters(tet_group_dev g, 
 {
     run_cluster<BLOCK>(cl, xi, xn, begin, count, [&](uint32_t t, pos4 P[4], auto&& xn_of) {
         float2 lm = mode == kSolveFirst ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(g.lambda)[t];
-        neohookean_project(P, xn_of, g.m0[t], g.m1[t], g.m2[t], inv_dt2, beta_dt, lm);
+        neohookean_project(P, xn_of, g.m0[t], g.m1[t], nh_m2(g, t), inv_dt2, beta_dt, lm);
         reinterpret_cast<float2*>(g.lambda)[t] = lm;
     });
 }
@@ -592,7 +599,7 @@ __global__ void __launch_bounds__(128) k_neohookean_clusters_apply(tet_group_dev
         pos4 const P[4] = {xt[id[0]], xt[id[1]], xt[id[2]], xt[id[3]]};
         float2 lm = reinterpret_cast<const float2*>(g.lambda)[t];
         v3 d[4];
-        neohookean_displacement(P, g.m0[t], g.m1[t], g.m2[t], inv_dt2, expl != 0, scale, lm, d);
+        neohookean_displacement(P, g.m0[t], g.m1[t], nh_m2(g, t), inv_dt2, expl != 0, scale, lm, d);
         reinterpret_cast<float2*>(g.lambda)[t] = lm;
 #pragma unroll
         for (int k = 0; k < 4; ++k) add_displacement(&xi[id[k]], d[k]);
@@ -630,13 +637,14 @@ __global__ void __launch_bounds__(BLOCK, MIN_WAVES) k_neohookean_hex(tet_group_d
     int4 const va = cl.v0[c], vb = cl.v1[c];
     int const ids[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
     constexpr int NP = PREFETCH ? kHexTets : 1;
-    float4 A[NP], B[NP], Cm[NP];
+    float4 A[NP], B[NP];
+    float2 Cm[NP];
     float2 L[NP];
     auto load = [&](int k, int slot) {
         uint32_t const t = tbase + uint32_t(k) * count + i;
         A[slot] = g.m0[t];
         B[slot] = g.m1[t];
-        Cm[slot] = g.m2[t];
+        Cm[slot] = nh_m2(g, t);
         L[slot] = FIRST ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(g.lambda)[t];
     };
     auto store = [&](int k, int slot) { reinterpret_cast<float2*>(g.lambda)[tbase + uint32_t(k) * count + i] = L[slot]; };
@@ -696,7 +704,7 @@ __global__ void __launch_bounds__(BLOCK) k_neohookean_hex_apply(tet_group_dev g,
         pos4 const Q[4] = {P[s0], P[s1], P[s2], P[s3]};
         float2 lm = reinterpret_cast<const float2*>(g.lambda)[t];
         v3 d[4];
-        neohookean_displacement(Q, g.m0[t], g.m1[t], g.m2[t], inv_dt2, expl != 0, scale, lm, d);
+        neohookean_displacement(Q, g.m0[t], g.m1[t], nh_m2(g, t), inv_dt2, expl != 0, scale, lm, d);
         reinterpret_cast<float2*>(g.lambda)[t] = lm;
         D[s0] = D[s0] + d[0];
         D[s1] = D[s1] + d[1];
@@ -743,7 +751,7 @@ __global__ void __launch_bounds__(64) k_hex_warm_disp(tet_group_dev g, cluster_d
         pos4 const Q[4] = {P[s0], P[s1], P[s2], P[s3]};
         float2 lm = reinterpret_cast<const float2*>(g.lambda)[t];
         v3 d[4];
-        neohookean_displacement(Q, g.m0[t], g.m1[t], g.m2[t], inv_dt2, expl != 0, scale, lm, d);
+        neohookean_displacement(Q, g.m0[t], g.m1[t], nh_m2(g, t), inv_dt2, expl != 0, scale, lm, d);
         reinterpret_cast<float2*>(g.lambda)[t] = lm;
         D[s0] = D[s0] + d[0];
         D[s1] = D[s1] + d[1];
@@ -992,13 +1000,14 @@ __device__ __forceinline__ void nh_constraints_f64(const pos4* __restrict__ x, i
 __device__ __forceinline__ double nh_residual2(tet_group_dev const& g, uint32_t t, pos4 p0, pos4 p1, pos4 p2, pos4 p3,
                                                float inv_dt2)
 {
-    float4 const c = g.m2[t];
+    float4 const b = g.m1[t];
+    float2 const c = nh_m2(g, t);
     double cd, ch;
-    nh_constraints_f64(p0, p1, p2, p3, load_nh(g.m0[t], g.m1[t]), c.w, cd, ch);
+    nh_constraints_f64(p0, p1, p2, p3, load_nh(g.m0[t], b), c.y, cd, ch);
     float2 const lm = reinterpret_cast<const float2*>(g.lambda)[t];
     cd = sqrt(cd);
-    double const hd = cd + double(c.y) * double(inv_dt2) * double(lm.x);
-    double const hh = ch + double(c.z) * double(inv_dt2) * double(lm.y);
+    double const hd = cd + double(b.w) * double(inv_dt2) * double(lm.x);
+    double const hh = ch + double(c.x) * double(inv_dt2) * double(lm.y);
     return hd * hd + hh * hh;
 }
 
@@ -1125,11 +1134,12 @@ __global__ void __launch_bounds__(kRedBlock) k_neohookean_energy(tet_group_dev g
     double e = 0.0;
     if (t < count) {
         int4 const id = g.idx[t];
-        float4 const c = g.m2[t];
+        float4 const b = g.m1[t];
+        float2 const c = nh_m2(g, t);
         double ic, j;
-        nh_constraints_f64(x, id, load_nh(g.m0[t], g.m1[t]), c.w, ic, j);
-        // mu |V0| = 1 / m2.y, lambda |V0| = 1 / m2.z
-        e = 0.5 / double(c.y) * (ic - 3.0) + 0.5 / double(c.z) * j * j;
+        nh_constraints_f64(x, id, load_nh(g.m0[t], b), c.y, ic, j);
+        // mu |V0| = 1 / m1.w, lambda |V0| = 1 / m2.x
+        e = 0.5 / double(b.w) * (ic - 3.0) + 0.5 / double(c.x) * j * j;
     }
     double const s = block_sum<kRedBlock>(e);
     if (threadIdx.x == 0) {

@@ -527,7 +527,8 @@ void sbs_gpu::upload_group(group_t& g)
         g.lambda.alloc(n);
     } else {
         std::vector<int4> idx(n);
-        std::vector<float4> m0(n), m1(n), m2(n);
+        std::vector<float4> m0(n), m1(n), m2(g.kind == 1 ? 0 : n);
+        std::vector<float2> m2h(g.kind == 1 ? n + (n & 1u) : 0, make_float2(0.0f, 0.0f));
         for (uint32_t j = 0; j < n; ++j) {
             uint32_t const* t = &g.conn[4 * std::size_t(order[j])];
             body_t const& b = bodies[g.body_of(order[j])];
@@ -558,20 +559,23 @@ void sbs_gpu::upload_group(group_t& g)
                 m0[j] = make_float4(r[0], r[1], r[2], r[3]);
                 m1[j] = make_float4(r[4], r[5], r[6], r[7]);
                 m2[j] = make_float4(r[8], float(V0), float(mu), float(lam));
-            } else {  // neo-Hookean: Q = R R^T, det R and the compliances (the kernels' nh_rest)
+            } else {  // neo-Hookean: Q = R R^T, det R and the compliances (the kernels' nh_rest), 40 B
                 auto q = [&](int a, int c) {
                     return float(R[a][0] * R[c][0] + R[a][1] * R[c][1] + R[a][2] * R[c][2]);
                 };
                 double const av = std::fabs(V0);
                 m0[j] = make_float4(q(0, 0), q(1, 1), q(2, 2), q(0, 1));
-                m1[j] = make_float4(q(0, 2), q(1, 2), float(1.0 / det), 0.0f);
-                m2[j] = make_float4(0.0f, float(1.0 / (mu * av)), float(1.0 / (lam * av)), float(1.0 + mu / lam));
+                m1[j] = make_float4(q(0, 2), q(1, 2), float(1.0 / det), float(1.0 / (mu * av)));
+                m2h[j] = make_float2(float(1.0 / (lam * av)), float(1.0 + mu / lam));
             }
         }
         g.idx.upload(idx.data(), n, stream);
         g.m0.upload(m0.data(), n, stream);
         g.m1.upload(m1.data(), n, stream);
-        g.m2.upload(m2.data(), n, stream);
+        if (g.kind == 1)  // float2 per tet, in float4 storage
+            g.m2.upload(reinterpret_cast<const float4*>(m2h.data()), m2h.size() / 2, stream);
+        else
+            g.m2.upload(m2.data(), n, stream);
         g.lambda.alloc(std::size_t(n) * (g.kind == 1 ? 2 : 1));
         if (g.kind == 1) {  // warm-start feature: log of the number of the group's tets in each tet's body
             std::vector<uint32_t> per_body(bodies.size(), 0);
