@@ -117,10 +117,9 @@ struct copy_spans {
 void launch_copy_spans(hipStream_t s, copy_spans const& spans);
 // Residual partial sums (as launch_neohookean_residual's) of a hex-cluster neo-Hookean group, one
 // thread per cluster of every color (block sums of its six tets' h^2; partial gets
-// ceil(nclusters / 256) values). ccolor_off / color_off: the colors' first cluster / constraint.
+// ceil(nclusters / 256) values). cinfo as at launch_hex_warm_start.
 void launch_neohookean_hex_residual(hipStream_t s, tet_group_dev g, cluster_dev cl, const pos4* x,
-                                    const uint32_t* ccolor_off, const uint32_t* color_off, uint32_t ncolors,
-                                    uint32_t nclusters, float inv_dt2, double* partial);
+                                    const uint4* cinfo, uint32_t nclusters, float inv_dt2, double* partial);
 // out[0] = the sum of partial[0, n) by one block in a fixed order (out may be pinned host memory).
 void launch_sum_partials(hipStream_t s, const double* partial, uint32_t n, double* out);
 
@@ -169,11 +168,10 @@ void launch_copy_pos(hipStream_t s, const pos4* a, pos4* b, uint32_t n);
 // for each particle p < n, xi[p] += the FP64 sum over csr[csr_off[p] .. csr_off[p+1]) (indices into
 // disp, in cluster execution order) -- the same additions in the same order as the per-color
 // pass; with csr_off null, csr is a dense [n][8] table of those indices (unused: 0xffffffff).
-// ccolor_off / color_off: the group's cluster and constraint offsets per color (ncolors + 1).
+// cinfo[c]: (c's index in its color, the color's cluster count, the color's first constraint).
 void launch_hex_warm_start(hipStream_t s, tet_group_dev g, cluster_dev cl, const pos4* xt, pos4* xi,
-                           const uint32_t* ccolor_off, const uint32_t* color_off, uint32_t ncolors, uint32_t nclusters,
-                           float inv_dt2, int expl, float2 scale, float4* disp, const uint32_t* csr_off,
-                           const uint32_t* csr, uint32_t n);
+                           const uint4* cinfo, uint32_t nclusters, float inv_dt2, int expl, float2 scale,
+                           float4* disp, const uint32_t* csr_off, const uint32_t* csr, uint32_t n);
 
 // Multiplier warm start of a neo-Hookean group, per constraint t (execution order) at the
 // predicted positions xi: the 13 features

@@ -720,24 +720,26 @@ __global__ void __launch_bounds__(BLOCK) k_neohookean_hex_apply(tet_group_dev g,
 // them, or the working positions themselves before any displacement is added); cluster ci
 // (execution order) writes the summed displacement of its slot k to disp[8 ci + k].
 __global__ void __launch_bounds__(64) k_hex_warm_disp(tet_group_dev g, cluster_dev cl, const pos4* __restrict__ xi,
-                                                      const uint32_t* __restrict__ ccolor_off,
-                                                      const uint32_t* __restrict__ color_off, uint32_t ncolors,
-                                                      uint32_t nclusters, float inv_dt2, int expl, float2 scale,
-                                                      float4* __restrict__ disp)
+                                                      const uint4* __restrict__ cinfo, uint32_t nclusters,
+                                                      float inv_dt2, int expl, float2 scale, float4* __restrict__ disp)
 {
     uint32_t const ci = blockIdx.x * blockDim.x + threadIdx.x;
     if (ci >= nclusters) return;
-    uint32_t lo = 0, hi = ncolors;  // the color: ccolor_off[col] <= ci < ccolor_off[col + 1]
-    while (hi - lo > 1) {
-        uint32_t const mid = (lo + hi) / 2;
-        if (ccolor_off[mid] <= ci)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    uint32_t const i = ci - ccolor_off[lo], count = ccolor_off[lo + 1] - ccolor_off[lo], tbase = color_off[lo];
+    uint4 const info = cinfo[ci];  // (index in its color, clusters in the color, the color's first constraint)
+    uint32_t const i = info.x, count = info.y, tbase = info.z;
     int4 const va = cl.v0[ci], vb = cl.v1[ci];
     int const ids[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
+    // every tet's material and multipliers are loaded before the first multiplier is stored back
+    float4 A[kHexTets], B[kHexTets];
+    float2 C[kHexTets], L[kHexTets];
+#pragma unroll
+    for (int k = 0; k < kHexTets; ++k) {
+        uint32_t const t = tbase + uint32_t(k) * count + i;
+        A[k] = g.m0[t];
+        B[k] = g.m1[t];
+        C[k] = nh_m2(g, t);
+        L[k] = reinterpret_cast<const float2*>(g.lambda)[t];
+    }
     pos4 P[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) P[k] = xi[ids[k]];
@@ -746,18 +748,17 @@ __global__ void __launch_bounds__(64) k_hex_warm_disp(tet_group_dev g, cluster_d
     for (int k = 0; k < 8; ++k) D[k] = mk(0.0f, 0.0f, 0.0f);
 #pragma unroll
     for (int k = 0; k < kHexTets; ++k) {
-        uint32_t const t = tbase + uint32_t(k) * count + i;
         int const s0 = kHexSlots[k][0], s1 = kHexSlots[k][1], s2 = kHexSlots[k][2], s3 = kHexSlots[k][3];
         pos4 const Q[4] = {P[s0], P[s1], P[s2], P[s3]};
-        float2 lm = reinterpret_cast<const float2*>(g.lambda)[t];
         v3 d[4];
-        neohookean_displacement(Q, g.m0[t], g.m1[t], nh_m2(g, t), inv_dt2, expl != 0, scale, lm, d);
-        reinterpret_cast<float2*>(g.lambda)[t] = lm;
+        neohookean_displacement(Q, A[k], B[k], C[k], inv_dt2, expl != 0, scale, L[k], d);
         D[s0] = D[s0] + d[0];
         D[s1] = D[s1] + d[1];
         D[s2] = D[s2] + d[2];
         D[s3] = D[s3] + d[3];
     }
+#pragma unroll
+    for (int k = 0; k < kHexTets; ++k) reinterpret_cast<float2*>(g.lambda)[tbase + uint32_t(k) * count + i] = L[k];
 #pragma unroll
     for (int k = 0; k < 8; ++k) disp[std::size_t(ci) * 8 + k] = make_float4(D[k].x, D[k].y, D[k].z, 0.0f);
 }
@@ -1015,28 +1016,19 @@ __device__ __forceinline__ double nh_residual2(tet_group_dev const& g, uint32_t 
 // particles are read once for its six tets.
 __global__ void __launch_bounds__(kRedBlock) k_neohookean_hex_residual(tet_group_dev g, cluster_dev cl,
                                                                        const pos4* __restrict__ x,
-                                                                       const uint32_t* __restrict__ ccolor_off,
-                                                                       const uint32_t* __restrict__ color_off,
-                                                                       uint32_t ncolors, uint32_t nclusters,
-                                                                       float inv_dt2, double* partial)
+                                                                       const uint4* __restrict__ cinfo,
+                                                                       uint32_t nclusters, float inv_dt2,
+                                                                       double* partial)
 {
     uint32_t const ci = blockIdx.x * blockDim.x + threadIdx.x;
     double h2 = 0.0;
     if (ci < nclusters) {
-        uint32_t lo = 0, hi = ncolors;  // the color: ccolor_off[col] <= ci < ccolor_off[col + 1]
-        while (hi - lo > 1) {
-            uint32_t const mid = (lo + hi) / 2;
-            if (ccolor_off[mid] <= ci)
-                lo = mid;
-            else
-                hi = mid;
-        }
-        uint32_t const i = ci - ccolor_off[lo], count = ccolor_off[lo + 1] - ccolor_off[lo], tbase = color_off[lo];
+        uint4 const info = cinfo[ci];  // (index in its color, clusters in the color, the color's first constraint)
         int4 const va = cl.v0[ci], vb = cl.v1[ci];
         pos4 const P[8] = {x[va.x], x[va.y], x[va.z], x[va.w], x[vb.x], x[vb.y], x[vb.z], x[vb.w]};
 #pragma unroll
         for (int k = 0; k < kHexTets; ++k)
-            h2 += nh_residual2(g, tbase + uint32_t(k) * count + i, P[kHexSlots[k][0]], P[kHexSlots[k][1]],
+            h2 += nh_residual2(g, info.z + uint32_t(k) * info.y + info.x, P[kHexSlots[k][0]], P[kHexSlots[k][1]],
                                P[kHexSlots[k][2]], P[kHexSlots[k][3]], inv_dt2);
     }
     double const s = block_sum<kRedBlock>(h2);
@@ -1391,13 +1383,11 @@ __global__ void __launch_bounds__(256) k_copy_spans(copy_spans sp)
 }
 
 void launch_neohookean_hex_residual(hipStream_t s, tet_group_dev g, cluster_dev cl, const pos4* x,
-                                    const uint32_t* ccolor_off, const uint32_t* color_off, uint32_t ncolors,
-                                    uint32_t nclusters, float inv_dt2, double* partial)
+                                    const uint4* cinfo, uint32_t nclusters, float inv_dt2, double* partial)
 {
     if (nclusters)
-        k_neohookean_hex_residual<<<grid_for(nclusters, kRedBlock), kRedBlock, 0, s>>>(g, cl, x, ccolor_off, color_off,
-                                                                                     ncolors, nclusters, inv_dt2,
-                                                                                     partial);
+        k_neohookean_hex_residual<<<grid_for(nclusters, kRedBlock), kRedBlock, 0, s>>>(g, cl, x, cinfo, nclusters,
+                                                                                     inv_dt2, partial);
 }
 
 void launch_copy_spans(hipStream_t s, copy_spans const& spans)
@@ -1462,13 +1452,11 @@ void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t value, uint32_t n)
 
 
 void launch_hex_warm_start(hipStream_t s, tet_group_dev g, cluster_dev cl, const pos4* xt, pos4* xi,
-                           const uint32_t* ccolor_off, const uint32_t* color_off, uint32_t ncolors, uint32_t nclusters,
-                           float inv_dt2, int expl, float2 scale, float4* disp, const uint32_t* csr_off,
-                           const uint32_t* csr, uint32_t n)
+                           const uint4* cinfo, uint32_t nclusters, float inv_dt2, int expl, float2 scale,
+                           float4* disp, const uint32_t* csr_off, const uint32_t* csr, uint32_t n)
 {
     if (!nclusters) return;
-    k_hex_warm_disp<<<grid_for(nclusters, 64), 64, 0, s>>>(g, cl, xt, ccolor_off, color_off, ncolors, nclusters,
-                                                          inv_dt2, expl, scale, disp);
+    k_hex_warm_disp<<<grid_for(nclusters, 64), 64, 0, s>>>(g, cl, xt, cinfo, nclusters, inv_dt2, expl, scale, disp);
     if (csr_off)
         k_gather_disp<<<grid_for(n, 256), 256, 0, s>>>(csr_off, csr, disp, xi, n);
     else

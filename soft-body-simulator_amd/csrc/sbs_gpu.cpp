@@ -68,7 +68,8 @@ struct group_t {
     dev_buf<float> ws_bsize;     // per constraint: log of the number of tets of its body
     dev_buf<mlp::bf16> ws_X;     // [count][16] network inputs (learned warm start)
     dev_buf<float4> ws_disp;     // hex groups: slot displacements of the warm-start pass
-    dev_buf<uint32_t> ws_csr_off, ws_csr, ws_ccolor_off, ws_color_off;  // particle -> slot list
+    dev_buf<uint32_t> ws_csr_off, ws_csr;  // particle -> slot list
+    dev_buf<uint4> ws_cinfo;  // hex groups, per cluster: (index in its color, clusters in the color, first constraint)
     dev_buf<float> snap_lambda;  // snapshot copies
     dev_buf<float2> snap_hist;
     bool clustered() const { return !cluster_off.empty(); }
@@ -249,6 +250,7 @@ struct sbs_gpu {
     void detect();
     void launch_group_colors(group_t const& g, double dt, int mode, const pos4* xref);
     void ensure_warm_buffers(bool raw, bool X);
+    void ensure_cinfo(group_t& g);
     void snapshot();
     void prepare_snapshot();
     void enqueue_snapshot();
@@ -836,8 +838,8 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
                 int const mode = warm == SBS_WARM_START_EXPLICIT ? kApplyExplicit : kApplyWarmStart;
                 if (g.hex)
                     timed(SBS_KERNEL_WARM_START, g.count, [&] {
-                        launch_hex_warm_start(stream, g.tet(), g.clusters(), xtilde, xi.p, g.ws_ccolor_off.p, g.ws_color_off.p,
-                                              g.ncolors, g.num_elements(), float(1.0 / dt2), mode == kApplyExplicit,
+                        launch_hex_warm_start(stream, g.tet(), g.clusters(), xtilde, xi.p, g.ws_cinfo.p,
+                                              g.num_elements(), float(1.0 / dt2), mode == kApplyExplicit,
                                               make_float2(float(ws_scale), float(ws_scale_hyd)), g.ws_disp.p,
                                               g.ws_csr_off.p, g.ws_csr.p, V);
                     }, 2);
@@ -875,6 +877,20 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
     if (check) enqueue_residual(dt);
 }
 
+// Per cluster of a hex group: its index in its color, the color's cluster count and first constraint
+// (uploaded once; frame graphs hold the pointer).
+void sbs_gpu::ensure_cinfo(group_t& g)
+{
+    if (g.ws_cinfo.p || !g.hex) return;
+    uint32_t const nc = g.num_elements();
+    std::vector<uint4> info(nc);
+    for (uint32_t c = 0; c < g.ncolors; ++c) {
+        uint32_t const cb = g.cluster_color_offsets[c], cn = g.cluster_color_offsets[c + 1] - cb;
+        for (uint32_t j = 0; j < cn; ++j) info[cb + j] = make_uint4(j, cn, g.color_offsets[c], 0u);
+    }
+    g.ws_cinfo.upload(info.data(), nc, stream);
+}
+
 void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
 {
     if (!xt.p) xt.alloc(V);
@@ -909,10 +925,7 @@ void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
                 g.ws_csr_off.upload(off.data(), off.size(), stream);
                 g.ws_csr.upload(ent.data(), ent.size(), stream);
             }
-            if (!g.ws_ccolor_off.p) {  // (may be there already: the fallback check's residual uses them)
-                g.ws_ccolor_off.upload(g.cluster_color_offsets.data(), g.cluster_color_offsets.size(), stream);
-                g.ws_color_off.upload(g.color_offsets.data(), g.color_offsets.size(), stream);
-            }
+            ensure_cinfo(g);
             g.ws_disp.alloc(std::size_t(nc) * 8);
         }
         if (!g.ws_hist.p && g.count) {
@@ -1055,10 +1068,7 @@ void sbs_gpu::prepare_check()
     if (snormal.p && !chk_normal.p) chk_normal.alloc(S);
     chk_hist.resize(groups.size());
     for (auto& g : groups)
-        if (g.kind == 1 && g.hex && !g.ws_ccolor_off.p) {
-            g.ws_ccolor_off.upload(g.cluster_color_offsets.data(), g.cluster_color_offsets.size(), stream);
-            g.ws_color_off.upload(g.color_offsets.data(), g.color_offsets.size(), stream);
-        }
+        if (g.kind == 1) ensure_cinfo(g);
     if (ws_mode == SBS_WARM_START_LEARNED)
         for (std::size_t i = 0; i < groups.size(); ++i)
             if (groups[i].ws_hist.p && !chk_hist[i].p) chk_hist[i].alloc(groups[i].ws_hist.n);
@@ -1089,8 +1099,8 @@ void sbs_gpu::enqueue_residual(double dt)
         if (g.kind != 1) continue;
         if (g.hex) {
             uint32_t const nc = g.num_elements();
-            launch_neohookean_hex_residual(stream, g.tet(), g.clusters(), xn.p, g.ws_ccolor_off.p, g.ws_color_off.p,
-                                           g.ncolors, nc, float(1.0 / dt2), check_partial.p + at);
+            launch_neohookean_hex_residual(stream, g.tet(), g.clusters(), xn.p, g.ws_cinfo.p, nc, float(1.0 / dt2),
+                                           check_partial.p + at);
             at += (nc + 255) / 256;
         } else {
             launch_neohookean_residual(stream, g.tet(), xn.p, 0, g.count, float(1.0 / dt2), check_partial.p + at, 256);
