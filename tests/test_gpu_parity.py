@@ -62,6 +62,25 @@ def test_single_tet_projection(kind, case):
             assert np.abs(xd - x).max() <= 1e-7
 
 
+@pytest.mark.parametrize("case", ["flat", "line", "point"])
+def test_single_tet_degenerate_neohookean(case):
+    # collapsed tets: det F = 0 (flat), rank-1 F (no hydrostatic gradient), F = 0 (no deviatoric
+    # gradient either: both constraints take the early-out)
+    say(f"single tet neohookean {case}")
+    x0 = np.array([[0.0, 0.0, 0.0], [0.1, 0.0, 0.0], [0.0, 0.1, 0.0], [0.0, 0.0, 0.1]])
+    F = {"flat": np.diag([1.0, 1.0, 0.0]), "line": np.diag([1.0, 0.0, 0.0]), "point": np.zeros((3, 3))}[case]
+    x = x0 @ F.T + np.array([0.2, 0.1, -0.3])
+    dev, orc = one_tet("neohookean", x, w=(1.0, 0.5, 2.0, 1.0), v=np.zeros((4, 3)), beta=0.0, young=1e5, poisson=0.3)
+    for it in (1, 4):
+        dev.step(0.01, it, 1)
+        orc.step(0.01, it, 1)
+        xd, xo = dev.read_body(0)[0], orc.read_body(0)[0]
+        assert np.all(np.isfinite(xd))
+        assert np.abs(xd - xo).max() <= 1e-5 * 0.1 + 1e-6 * np.abs(xo).max(), (xd, xo)
+        if case == "point":
+            assert np.array_equal(xd, x)
+
+
 def test_free_fall_closed_form():
     say("free fall")
     sc = scenes.Scene("free_fall", gravity_is_force=True)
