@@ -237,6 +237,8 @@ def main():
     ap.add_argument("--workload", default="cfg1_1m", choices=["cfg1_1m", "cfg1_6m", "cfg0", "cfg2", "cfg4"])
     ap.add_argument("--no-learned", action="store_true", help="plain solver only")
     ap.add_argument("--no-graphs", action="store_true", help="stream launches instead of hipGraph replays")
+    ap.add_argument("--no-contact-fusion", action="store_true",
+                    help="the first iteration's contacts in their own launch, not in the warm-start gather")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run device copies of known size (to scale PMC byte counters)")
     ap.add_argument("--row", default="step", choices=["step", "boundary", "grid_sdf", "ply"],
@@ -271,6 +273,7 @@ def main():
     from sbs_amd import autotune
 
     sim.use_graphs(not args.no_graphs)
+    sim.set_contact_fusion(not args.no_contact_fusion)
     launch = autotune.tune_launch(sim, scene)
     log(f"launch config {launch}")
     for _ in range(args.warmup):

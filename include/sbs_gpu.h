@@ -204,6 +204,13 @@ int sbs_gpu_set_coloring(sbs_gpu* sim, int order);
  * more waves in flight, for launches large enough to be bandwidth bound). Default 256 and 64. */
 #define SBS_CLUSTER_BLOCK_STREAMED 65
 int sbs_gpu_set_launch_config(sbs_gpu* sim, int block, int cluster_block);
+/* Default on (no reference counterpart): when the scene's only constraint group is neo-Hookean
+ * with hex-cell clusters and warm-started, the first iteration's contact projections run in the
+ * warm-start pass's last launch (the per-particle gather of the displacements), each surface
+ * vertex's contacts right after its displacement, in place of their own launch. They run on the
+ * same positions in the same order as that launch would (gauss_seidel_solver.cpp:28-31), so the
+ * results are bit-identical. Off: the contacts have their own launch. */
+int sbs_gpu_set_contact_fusion(sbs_gpu* sim, int enable);
 
 /* Colors every constraint group and uploads the scene. recolor_passes: extra Jones-Plassmann
  * passes of SBS_COLORING_HASH (0 = hash order only). After this, no more scene additions. */

@@ -169,9 +169,13 @@ void launch_copy_pos(hipStream_t s, const pos4* a, pos4* b, uint32_t n);
 // disp, in cluster execution order) -- the same additions in the same order as the per-color
 // pass; with csr_off null, csr is a dense [n][8] table of those indices (unused: 0xffffffff).
 // cinfo[c]: (c's index in its color, the color's cluster count, the color's first constraint).
+// With surface_of (dense list only; particle -> surface vertex or -1) the gather also runs the first
+// iteration's contact projections (compliance c_at) of the surface vertices, in place of that
+// iteration's contact launch.
 void launch_hex_warm_start(hipStream_t s, tet_group_dev g, cluster_dev cl, const pos4* xt, pos4* xi,
                            const uint4* cinfo, uint32_t nclusters, float inv_dt2, int expl, float2 scale,
-                           float4* disp, const uint32_t* csr_off, const uint32_t* csr, uint32_t n);
+                           float4* disp, const uint32_t* csr_off, const uint32_t* csr, uint32_t n,
+                           const int* surface_of = nullptr, contact_dev const& c = {}, float c_at = 0.0f);
 
 // Multiplier warm start of a neo-Hookean group, per constraint t (execution order) at the
 // predicted positions xi: the 13 features

@@ -382,6 +382,34 @@ __device__ __forceinline__ void neohookean_displacement(Pos const P[4], float4 a
     }
 }
 
+// The contacts of surface vertex s (all its slots in SDF order) projected onto P, the vertex's
+// position (collision_constraint.cpp:21-41): C = (p - q).n; when C < 0, dl = -(C + alpha~ lm) /
+// (w + alpha~), lm += dl, p += w n dl. first: the first iteration (multipliers start at zero, and
+// are stored as zero for slots without penetration).
+__device__ __forceinline__ void contact_vertex_at(contact_dev const& c, pos4& P, uint32_t s, float alpha_tilde,
+                                                  bool first)
+{
+    uint32_t const cnt = c.count[s];
+    float const w = wof(P);
+    for (uint32_t k = 0; k < cnt; ++k) {
+        uint32_t const slot = k * c.S + s;
+        pos4 const q = c.q[slot];
+        v3 const n = xyz(c.n[slot]);
+        float const lm = first ? 0.0f : c.lambda[slot];
+        float const C = fdot(sub(P, q), n);
+        if (C >= 0.0f) {
+            if (first) c.lambda[slot] = 0.0f;
+            continue;
+        }
+        float const dl = -fmaf(alpha_tilde, lm, C) / (w + alpha_tilde);
+        c.lambda[slot] = lm + dl;
+        float const sd = w * dl;
+        P.x += double(sd * n.x);
+        P.y += double(sd * n.y);
+        P.z += double(sd * n.z);
+    }
+}
+
 #pragma clang fp contract(fast)
 
 // x += d in FP64 (read, add, write) for the x, y, z of one particle.
@@ -785,14 +813,17 @@ __global__ void k_gather_disp(const uint32_t* __restrict__ off, const uint32_t* 
 // The same with the list as a dense row of 8 slot indices per particle (unused: 0xffffffff), when
 // no particle is in more than 8 clusters (hex lattices): the 8 loads of a row go out together and
 // the unused ones add an exact 0.
+// With surface_of (particle -> surface vertex or -1), the first iteration's contact projections of
+// the surface vertices follow in the same thread, as k_contact(first) would run them next.
 __global__ void k_gather_disp8(const uint4* __restrict__ ent, const float4* __restrict__ disp, pos4* __restrict__ xi,
-                               uint32_t n)
+                               uint32_t n, const int* __restrict__ surface_of, contact_dev c, float c_at)
 {
     uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     uint4 const a = ent[2 * std::size_t(p)], b = ent[2 * std::size_t(p) + 1];
     uint32_t const e[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-    if (e[0] == 0xffffffffu) return;
+    int const sv = surface_of ? surface_of[p] : -1;
+    if (e[0] == 0xffffffffu && sv < 0) return;
     float4 d[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) d[m] = disp[e[m] != 0xffffffffu ? e[m] : 0u];
@@ -804,6 +835,7 @@ __global__ void k_gather_disp8(const uint4* __restrict__ ent, const float4* __re
         x.y += ok ? double(d[m].y) : 0.0;
         x.z += ok ? double(d[m].z) : 0.0;
     }
+    if (sv >= 0) contact_vertex_at(c, x, uint32_t(sv), c_at, true);
     xi[p] = x;
 }
 
@@ -909,25 +941,10 @@ __global__ void k_contact(contact_dev c, pos4* xi, float alpha_tilde, int first)
 {
     uint32_t const s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= c.S) return;
-    uint32_t const cnt = c.count[s];
-    if (cnt == 0) return;
+    if (c.count[s] == 0) return;
     uint32_t const vi = c.surf[s];
     pos4 P = xi[vi];
-    float const w = wof(P);
-    for (uint32_t k = 0; k < cnt; ++k) {
-        uint32_t const slot = k * c.S + s;
-        pos4 const q = c.q[slot];
-        v3 const n = xyz(c.n[slot]);
-        float const lm = first ? 0.0f : c.lambda[slot];
-        float const C = dot(sub(P, q), n);
-        if (C >= 0.0f) {
-            if (first) c.lambda[slot] = 0.0f;
-            continue;
-        }
-        float const dl = -(C + alpha_tilde * lm) / (w + alpha_tilde);
-        c.lambda[slot] = lm + dl;
-        add_to(P, (w * dl) * n);
-    }
+    contact_vertex_at(c, P, s, alpha_tilde, first != 0);
     xi[vi] = P;
 }
 
@@ -1453,14 +1470,16 @@ void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t value, uint32_t n)
 
 void launch_hex_warm_start(hipStream_t s, tet_group_dev g, cluster_dev cl, const pos4* xt, pos4* xi,
                            const uint4* cinfo, uint32_t nclusters, float inv_dt2, int expl, float2 scale,
-                           float4* disp, const uint32_t* csr_off, const uint32_t* csr, uint32_t n)
+                           float4* disp, const uint32_t* csr_off, const uint32_t* csr, uint32_t n,
+                           const int* surface_of, contact_dev const& c, float c_at)
 {
     if (!nclusters) return;
     k_hex_warm_disp<<<grid_for(nclusters, 64), 64, 0, s>>>(g, cl, xt, cinfo, nclusters, inv_dt2, expl, scale, disp);
     if (csr_off)
         k_gather_disp<<<grid_for(n, 256), 256, 0, s>>>(csr_off, csr, disp, xi, n);
     else
-        k_gather_disp8<<<grid_for(n, 256), 256, 0, s>>>(reinterpret_cast<const uint4*>(csr), disp, xi, n);
+        k_gather_disp8<<<grid_for(n, 256), 256, 0, s>>>(reinterpret_cast<const uint4*>(csr), disp, xi, n, surface_of,
+                                                        c, c_at);
 }
 
 void launch_copy_pos(hipStream_t s, const pos4* a, pos4* b, uint32_t n)

@@ -140,6 +140,8 @@ struct sbs_gpu {
 
     int block = 256;
     int cluster_block = 64;
+    bool fuse_first_contacts = true;  // the first iteration's contacts in the warm-start gather
+    dev_buf<int> surface_of_dev;      // particle -> surface vertex, or -1
     int coloring_order = SBS_COLORING_HASH;
 
     bool timing = false;
@@ -662,6 +664,11 @@ void sbs_gpu::commit(uint32_t recolor_passes)
     }
     slots = uint32_t(std::min<std::size_t>(sdfs.size(), kMaxContactSlots));
     if (S) surf.upload(sv.data(), S, stream);
+    {
+        std::vector<int> so(V, -1);
+        for (uint32_t s = 0; s < S; ++s) so[sv[s]] = int(s);
+        surface_of_dev.upload(so.data(), so.size(), stream);
+    }
     if (S && slots) {
         cq.alloc(std::size_t(S) * slots);
         cn.alloc(std::size_t(S) * slots);
@@ -819,6 +826,11 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
             colored_pass = colored_pass || (g.kind == 1 && !g.hex);
         }
         bool const copy_xt = warm != SBS_WARM_START_NONE && (colored_pass || nh > 1);
+        // the first iteration's contacts ride in the warm-start pass's gather when it is the last
+        // launch before them: one group, hex clusters, the dense gather
+        bool const fused0 = have_contacts && fuse_first_contacts && iterations > 0 && warm != SBS_WARM_START_NONE &&
+                            groups.size() == 1 &&
+                            groups[0].kind == 1 && groups[0].hex && !groups[0].ws_csr_off.p;
         if (copy_xt) timed(SBS_KERNEL_WARM_START, V, [&] { launch_copy_pos(stream, xi.p, xt.p, V); });
         const pos4* const xtilde = copy_xt ? xt.p : xi.p;
         for (auto& g : groups) {
@@ -841,7 +853,8 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
                         launch_hex_warm_start(stream, g.tet(), g.clusters(), xtilde, xi.p, g.ws_cinfo.p,
                                               g.num_elements(), float(1.0 / dt2), mode == kApplyExplicit,
                                               make_float2(float(ws_scale), float(ws_scale_hyd)), g.ws_disp.p,
-                                              g.ws_csr_off.p, g.ws_csr.p, V);
+                                              g.ws_csr_off.p, g.ws_csr.p, V, fused0 ? surface_of_dev.p : nullptr, c,
+                                              c_at);
                     }, 2);
                 else
                     timed(SBS_KERNEL_WARM_START, g.count, [&] { launch_group_colors(g, dt, mode, xt.p); }, g.ncolors);
@@ -850,7 +863,7 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
         }
         for (uint32_t it = 0; it < iterations; ++it) {
             int const first = it == 0;
-            if (have_contacts)
+            if (have_contacts && !(first && fused0 && warmed))
                 timed(SBS_KERNEL_CONTACT, S, [&] { launch_contact(stream, c, xi.p, c_at, first, block); });
             for (auto const& g : groups) {
                 int const kind = g.kind == 0 ? SBS_KERNEL_GREEN : (g.kind == 1 ? SBS_KERNEL_NEOHOOKEAN : SBS_KERNEL_DISTANCE);
@@ -1130,7 +1143,7 @@ void sbs_gpu::run_frame(double dt, uint32_t iterations, uint32_t substeps, int w
         key[12] = ws_scale;
         key[13] = surface_refresh;
         key[14] = ws_scale_hyd;
-        key[15] = check;
+        key[15] = check + 2 * fuse_first_contacts;
         if (!graph_exec || key != graph_key) {
             if (graph_exec) {
                 // the previous graph may still be running on the stream: let it finish first
@@ -1511,6 +1524,14 @@ int sbs_gpu_set_coloring(sbs_gpu* sim, int order)
         require(!sim->committed, "scene already committed", SBS_ERR_STATE);
         require(order == SBS_COLORING_HASH || order == SBS_COLORING_INDEX, "unknown coloring order");
         sim->coloring_order = order;
+    });
+}
+
+int sbs_gpu_set_contact_fusion(sbs_gpu* sim, int enable)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        sim->fuse_first_contacts = enable != 0;
     });
 }
 

@@ -89,6 +89,7 @@ _SIGS = {
     "sbs_gpu_set_group_clusters": [_p, _u32, _pu, _u32],
     "sbs_gpu_set_coloring": [_p, _i],
     "sbs_gpu_set_launch_config": [_p, _i, _i],
+    "sbs_gpu_set_contact_fusion": [_p, _i],
     "sbs_gpu_commit": [_p, _u32],
     "sbs_gpu_step": [_p, _d, _u32, _u32],
     "sbs_gpu_synchronize": [_p],
@@ -546,6 +547,9 @@ class Simulation:
 
     def set_launch_config(self, block=256, cluster_block=64):
         check(self._L.sbs_gpu_set_launch_config(self.h, block, cluster_block))
+
+    def set_contact_fusion(self, enable=True):
+        check(self._L.sbs_gpu_set_contact_fusion(self.h, int(enable)))
 
     def commit(self, recolor_passes=2):
         check(self._L.sbs_gpu_commit(self.h, recolor_passes))

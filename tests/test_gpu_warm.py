@@ -330,3 +330,23 @@ def test_hex_kernel_variants_are_bit_identical(beta):
             dev.step(sc.dt, sc.iterations, sc.substeps)
         out.append(dev.read_body(0)[0])
     assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+
+
+def test_fused_first_contacts_are_bit_identical():
+    # the first iteration's contact projections run inside the warm-start gather or in their own
+    # launch: the same projections on the same positions, so the same bits
+    sc = scenes.obstacle_field(num_spheres=300, nx=10, ny=4, nz=4)
+    say("fused first-iteration contacts")
+    out = []
+    for fuse in (True, False):
+        dev = scenes.build_device(sc)
+        dev.set_contact_fusion(fuse)
+        dev.set_warm_start("previous")
+        dev.set_warm_start_scale(0.5)
+        ncontacts = 0
+        for _ in range(4):
+            dev.step(sc.dt, sc.iterations, sc.substeps)
+            ncontacts += len(dev.read_contacts()[0])
+        assert ncontacts > 0
+        out.append(dev.read_body(0))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
