@@ -116,10 +116,10 @@ struct copy_spans {
 };
 void launch_copy_spans(hipStream_t s, copy_spans const& spans);
 // Residual partial sums (as launch_neohookean_residual's) of a hex-cluster neo-Hookean group, one
-// thread per cluster of every color (block sums of its six tets' h^2; partial gets
-// ceil(nclusters / 256) values). cinfo as at launch_hex_warm_start.
+// thread per row of cinfo (block sums of its cluster's six tets' h^2; partial gets
+// ceil(nrows / 256) values). cinfo as at launch_hex_warm_start.
 void launch_neohookean_hex_residual(hipStream_t s, tet_group_dev g, cluster_dev cl, const pos4* x,
-                                    const uint4* cinfo, uint32_t nclusters, float inv_dt2, double* partial);
+                                    const uint4* cinfo, uint32_t nrows, float inv_dt2, double* partial);
 // out[0] = the sum of partial[0, n) by one block in a fixed order (out may be pinned host memory).
 void launch_sum_partials(hipStream_t s, const double* partial, uint32_t n, double* out);
 
@@ -164,16 +164,18 @@ void launch_jp_keys(hipStream_t s, uint32_t n, const uint32_t* prev, uint64_t* k
 void launch_copy_pos(hipStream_t s, const pos4* a, pos4* b, uint32_t n);
 
 // Warm-start pass of a hex-cluster neo-Hookean group in two launches instead of one per color:
-// every cluster's slot displacements at the predicted positions xt into disp [8 per cluster], then
-// for each particle p < n, xi[p] += the FP64 sum over csr[csr_off[p] .. csr_off[p+1]) (indices into
-// disp, in cluster execution order) -- the same additions in the same order as the per-color
-// pass; with csr_off null, csr is a dense [n][8] table of those indices (unused: 0xffffffff).
-// cinfo[c]: (c's index in its color, the color's cluster count, the color's first constraint).
+// every cluster's slot displacements at the predicted positions xt into disp ([8][nrows]: slot k of
+// the cluster in row r at k * nrows + r), then for each particle p < n, xi[p] += the FP64 sum over
+// csr[csr_off[p] .. csr_off[p+1]) (indices into disp, in cluster execution order) -- the same
+// additions in the same order as the per-color pass; with csr_off null, csr is a dense [n][8] table
+// of those indices (unused: 0xffffffff). cinfo[r] (nrows rows, a multiple of 64): (the cluster's
+// index in its color, the color's cluster count, the color's first constraint, the cluster), or
+// padding with cluster 0xffffffff.
 // With surface_of (dense list only; particle -> surface vertex or -1) the gather also runs the first
 // iteration's contact projections (compliance c_at) of the surface vertices, in place of that
 // iteration's contact launch.
 void launch_hex_warm_start(hipStream_t s, tet_group_dev g, cluster_dev cl, const pos4* xt, pos4* xi,
-                           const uint4* cinfo, uint32_t nclusters, float inv_dt2, int expl, float2 scale,
+                           const uint4* cinfo, uint32_t nrows, float inv_dt2, int expl, float2 scale,
                            float4* disp, const uint32_t* csr_off, const uint32_t* csr, uint32_t n,
                            const int* surface_of = nullptr, contact_dev const& c = {}, float c_at = 0.0f);
 

@@ -745,16 +745,18 @@ __global__ void __launch_bounds__(BLOCK) k_neohookean_hex_apply(tet_group_dev g,
 
 // Warm-start displacements of every hex cluster of a group in one launch (all colors): nothing
 // moves during the launch, so every gradient is taken at the predicted positions xi (a copy of
-// them, or the working positions themselves before any displacement is added); cluster ci
-// (execution order) writes the summed displacement of its slot k to disp[8 ci + k].
+// them, or the working positions themselves before any displacement is added). Row r (64-row
+// workgroups) holds cluster cinfo[r].w (execution order) or padding (0xffffffff); the summed
+// displacement of the cluster's slot k goes to disp[k * nrows + r] (one coalesced store per slot).
 __global__ void __launch_bounds__(64) k_hex_warm_disp(tet_group_dev g, cluster_dev cl, const pos4* __restrict__ xi,
-                                                      const uint4* __restrict__ cinfo, uint32_t nclusters,
+                                                      const uint4* __restrict__ cinfo, uint32_t nrows,
                                                       float inv_dt2, int expl, float2 scale, float4* __restrict__ disp)
 {
-    uint32_t const ci = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ci >= nclusters) return;
-    uint4 const info = cinfo[ci];  // (index in its color, clusters in the color, the color's first constraint)
-    uint32_t const i = info.x, count = info.y, tbase = info.z;
+    uint32_t const r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    uint4 const info = cinfo[r];  // (index in its color, clusters in the color, the color's first constraint, cluster)
+    if (info.w == 0xffffffffu) return;
+    uint32_t const i = info.x, count = info.y, tbase = info.z, ci = info.w;
     int4 const va = cl.v0[ci], vb = cl.v1[ci];
     int const ids[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
     // every tet's material and multipliers are loaded before the first multiplier is stored back
@@ -788,7 +790,7 @@ __global__ void __launch_bounds__(64) k_hex_warm_disp(tet_group_dev g, cluster_d
 #pragma unroll
     for (int k = 0; k < kHexTets; ++k) reinterpret_cast<float2*>(g.lambda)[tbase + uint32_t(k) * count + i] = L[k];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) disp[std::size_t(ci) * 8 + k] = make_float4(D[k].x, D[k].y, D[k].z, 0.0f);
+    for (int k = 0; k < 8; ++k) disp[std::size_t(k) * nrows + r] = make_float4(D[k].x, D[k].y, D[k].z, 0.0f);
 }
 
 // x_p += the displacements of p's cluster slots, in FP64, in the order of the list (execution
@@ -1029,18 +1031,20 @@ __device__ __forceinline__ double nh_residual2(tet_group_dev const& g, uint32_t 
     return hd * hd + hh * hh;
 }
 
-// The same over a hex-cluster group, one thread per cluster of every color: the cluster's eight
-// particles are read once for its six tets.
+// The same over a hex-cluster group, one thread per row of the all-color rows (a cluster of any
+// color, or padding): the cluster's eight particles are read once for its six tets.
 __global__ void __launch_bounds__(kRedBlock) k_neohookean_hex_residual(tet_group_dev g, cluster_dev cl,
                                                                        const pos4* __restrict__ x,
                                                                        const uint4* __restrict__ cinfo,
-                                                                       uint32_t nclusters, float inv_dt2,
+                                                                       uint32_t nrows, float inv_dt2,
                                                                        double* partial)
 {
-    uint32_t const ci = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t const r = blockIdx.x * blockDim.x + threadIdx.x;
     double h2 = 0.0;
-    if (ci < nclusters) {
-        uint4 const info = cinfo[ci];  // (index in its color, clusters in the color, the color's first constraint)
+    uint4 const info = r < nrows ? cinfo[r] : make_uint4(0u, 0u, 0u, 0xffffffffu);
+    if (info.w != 0xffffffffu) {
+        // (index in its color, clusters in the color, the color's first constraint, cluster)
+        uint32_t const ci = info.w;
         int4 const va = cl.v0[ci], vb = cl.v1[ci];
         pos4 const P[8] = {x[va.x], x[va.y], x[va.z], x[va.w], x[vb.x], x[vb.y], x[vb.z], x[vb.w]};
 #pragma unroll
@@ -1400,10 +1404,10 @@ __global__ void __launch_bounds__(256) k_copy_spans(copy_spans sp)
 }
 
 void launch_neohookean_hex_residual(hipStream_t s, tet_group_dev g, cluster_dev cl, const pos4* x,
-                                    const uint4* cinfo, uint32_t nclusters, float inv_dt2, double* partial)
+                                    const uint4* cinfo, uint32_t nrows, float inv_dt2, double* partial)
 {
-    if (nclusters)
-        k_neohookean_hex_residual<<<grid_for(nclusters, kRedBlock), kRedBlock, 0, s>>>(g, cl, x, cinfo, nclusters,
+    if (nrows)
+        k_neohookean_hex_residual<<<grid_for(nrows, kRedBlock), kRedBlock, 0, s>>>(g, cl, x, cinfo, nrows,
                                                                                      inv_dt2, partial);
 }
 
@@ -1469,12 +1473,12 @@ void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t value, uint32_t n)
 
 
 void launch_hex_warm_start(hipStream_t s, tet_group_dev g, cluster_dev cl, const pos4* xt, pos4* xi,
-                           const uint4* cinfo, uint32_t nclusters, float inv_dt2, int expl, float2 scale,
+                           const uint4* cinfo, uint32_t nrows, float inv_dt2, int expl, float2 scale,
                            float4* disp, const uint32_t* csr_off, const uint32_t* csr, uint32_t n,
                            const int* surface_of, contact_dev const& c, float c_at)
 {
-    if (!nclusters) return;
-    k_hex_warm_disp<<<grid_for(nclusters, 64), 64, 0, s>>>(g, cl, xt, cinfo, nclusters, inv_dt2, expl, scale, disp);
+    if (!nrows) return;
+    k_hex_warm_disp<<<grid_for(nrows, 64), 64, 0, s>>>(g, cl, xt, cinfo, nrows, inv_dt2, expl, scale, disp);
     if (csr_off)
         k_gather_disp<<<grid_for(n, 256), 256, 0, s>>>(csr_off, csr, disp, xi, n);
     else

@@ -67,9 +67,13 @@ struct group_t {
     dev_buf<float> ws_raw;       // [count][13] features at the substep start (capture)
     dev_buf<float> ws_bsize;     // per constraint: log of the number of tets of its body
     dev_buf<mlp::bf16> ws_X;     // [count][16] network inputs (learned warm start)
-    dev_buf<float4> ws_disp;     // hex groups: slot displacements of the warm-start pass
+    dev_buf<float4> ws_disp;     // hex groups: slot displacements of the warm-start pass, [8][rows]
     dev_buf<uint32_t> ws_csr_off, ws_csr;  // particle -> slot list
-    dev_buf<uint4> ws_cinfo;  // hex groups, per cluster: (index in its color, clusters in the color, first constraint)
+    // hex groups, per row of the all-color launches: (index in its color, clusters in the color,
+    // first constraint, cluster) or a padding row (cluster 0xffffffff); rows in 64-row blocks, block
+    // b holding clusters of XCD region b mod 8
+    dev_buf<uint4> ws_cinfo;
+    std::vector<uint32_t> ws_rowof;  // cluster -> its row
     dev_buf<float> snap_lambda;  // snapshot copies
     dev_buf<float2> snap_hist;
     bool clustered() const { return !cluster_off.empty(); }
@@ -851,7 +855,7 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
                 if (g.hex)
                     timed(SBS_KERNEL_WARM_START, g.count, [&] {
                         launch_hex_warm_start(stream, g.tet(), g.clusters(), xtilde, xi.p, g.ws_cinfo.p,
-                                              g.num_elements(), float(1.0 / dt2), mode == kApplyExplicit,
+                                              uint32_t(g.ws_cinfo.n), float(1.0 / dt2), mode == kApplyExplicit,
                                               make_float2(float(ws_scale), float(ws_scale_hyd)), g.ws_disp.p,
                                               g.ws_csr_off.p, g.ws_csr.p, V, fused0 ? surface_of_dev.p : nullptr, c,
                                               c_at);
@@ -890,18 +894,40 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
     if (check) enqueue_residual(dt);
 }
 
-// Per cluster of a hex group: its index in its color, the color's cluster count and first constraint
-// (uploaded once; frame graphs hold the pointer).
+// The rows of a hex group's all-color launches (warm-start displacements, residual): per cluster its
+// index in its color, the color's cluster count and first constraint, and the cluster itself. The
+// clusters of XCD region x (every color's share, color by color) fill the 64-row blocks b with
+// b mod 8 = x, so that a workgroup runs on the XCD whose L2 holds its region's particles, as in the
+// per-color launches; short regions end in padding rows. Uploaded once; frame graphs hold the pointer.
 void sbs_gpu::ensure_cinfo(group_t& g)
 {
     if (g.ws_cinfo.p || !g.hex) return;
     uint32_t const nc = g.num_elements();
-    std::vector<uint4> info(nc);
+    std::vector<std::vector<uint4>> region(kXcds);
     for (uint32_t c = 0; c < g.ncolors; ++c) {
         uint32_t const cb = g.cluster_color_offsets[c], cn = g.cluster_color_offsets[c + 1] - cb;
-        for (uint32_t j = 0; j < cn; ++j) info[cb + j] = make_uint4(j, cn, g.color_offsets[c], 0u);
+        xcd_split sp{};
+        if (c < g.xsplit.size())
+            sp = g.xsplit[c];
+        else
+            for (int x = 0; x <= kXcds; ++x) sp.off[x] = uint32_t(uint64_t(cn) * uint32_t(x) / kXcds);
+        for (int x = 0; x < kXcds; ++x)
+            for (uint32_t j = sp.off[x]; j < sp.off[x + 1]; ++j)
+                region[x].push_back(make_uint4(j, cn, g.color_offsets[c], cb + j));
     }
-    g.ws_cinfo.upload(info.data(), nc, stream);
+    std::size_t blocks = 0;
+    for (auto const& r : region) blocks = std::max(blocks, (r.size() + 63) / 64);
+    std::size_t const rows = std::max<std::size_t>(blocks * kXcds * 64, 64);
+    require(rows < 0xffffffffull / 8, "too many clusters for the warm-start rows");
+    std::vector<uint4> info(rows, make_uint4(0u, 0u, 0u, 0xffffffffu));
+    g.ws_rowof.assign(nc, 0u);
+    for (int x = 0; x < kXcds; ++x)
+        for (std::size_t q = 0; q < region[x].size(); ++q) {
+            std::size_t const r = ((q / 64) * kXcds + std::size_t(x)) * 64 + q % 64;
+            info[r] = region[x][q];
+            g.ws_rowof[region[x][q].w] = uint32_t(r);
+        }
+    g.ws_cinfo.upload(info.data(), rows, stream);
 }
 
 void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
@@ -915,6 +941,8 @@ void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
             std::vector<int4> v0(nc), v1(nc);
             hip_check(hipMemcpy(v0.data(), g.cl_v0.p, nc * sizeof(int4), hipMemcpyDeviceToHost), "read clusters");
             hip_check(hipMemcpy(v1.data(), g.cl_v1.p, nc * sizeof(int4), hipMemcpyDeviceToHost), "read clusters");
+            ensure_cinfo(g);
+            std::size_t const rows = g.ws_cinfo.n;  // slot k of the cluster in row r: disp[k * rows + r]
             std::vector<uint32_t> off(std::size_t(V) + 1, 0), ent(std::size_t(nc) * 8);
             auto slot = [&](uint32_t c, int k) {
                 int4 const& a = k < 4 ? v0[c] : v1[c];
@@ -928,7 +956,7 @@ void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
             for (std::size_t p = 0; p < V; ++p) off[p + 1] += off[p];
             std::vector<uint32_t> fill(off.begin(), off.end() - 1);
             for (uint32_t c = 0; c < nc; ++c)
-                for (int k = 0; k < 8; ++k) ent[fill[slot(c, k)]++] = c * 8 + uint32_t(k);
+                for (int k = 0; k < 8; ++k) ent[fill[slot(c, k)]++] = uint32_t(k * rows + g.ws_rowof[c]);
             if (widest <= 8) {  // a dense row of 8 per particle, same order (ws_csr_off stays empty)
                 std::vector<uint32_t> dense(std::size_t(V) * 8, 0xffffffffu);
                 for (std::size_t p = 0; p < V; ++p)
@@ -938,8 +966,7 @@ void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
                 g.ws_csr_off.upload(off.data(), off.size(), stream);
                 g.ws_csr.upload(ent.data(), ent.size(), stream);
             }
-            ensure_cinfo(g);
-            g.ws_disp.alloc(std::size_t(nc) * 8);
+            g.ws_disp.alloc(rows * 8);
         }
         if (!g.ws_hist.p && g.count) {
             g.ws_hist.alloc(g.count);
@@ -1067,7 +1094,7 @@ std::size_t sbs_gpu::residual_blocks() const
 {
     std::size_t total = 0;
     for (auto const& g : groups)
-        if (g.kind == 1) total += ((g.hex ? g.num_elements() : g.count) + 255) / 256;
+        if (g.kind == 1) total += ((g.hex ? g.ws_cinfo.n : g.count) + 255) / 256;
     return total;
 }
 
@@ -1111,10 +1138,10 @@ void sbs_gpu::enqueue_residual(double dt)
     for (auto const& g : groups) {
         if (g.kind != 1) continue;
         if (g.hex) {
-            uint32_t const nc = g.num_elements();
-            launch_neohookean_hex_residual(stream, g.tet(), g.clusters(), xn.p, g.ws_cinfo.p, nc, float(1.0 / dt2),
+            uint32_t const rows = uint32_t(g.ws_cinfo.n);
+            launch_neohookean_hex_residual(stream, g.tet(), g.clusters(), xn.p, g.ws_cinfo.p, rows, float(1.0 / dt2),
                                            check_partial.p + at);
-            at += (nc + 255) / 256;
+            at += (rows + 255) / 256;
         } else {
             launch_neohookean_residual(stream, g.tet(), xn.p, 0, g.count, float(1.0 / dt2), check_partial.p + at, 256);
             at += (g.count + 255) / 256;
