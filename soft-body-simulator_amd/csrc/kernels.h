@@ -117,7 +117,7 @@ struct copy_spans {
 void launch_copy_spans(hipStream_t s, copy_spans const& spans);
 // Residual partial sums (as launch_neohookean_residual's) of a hex-cluster neo-Hookean group, one
 // thread per row of cinfo (block sums of its cluster's six tets' h^2; partial gets
-// ceil(nrows / 256) values). cinfo as at launch_hex_warm_start.
+// ceil(nrows / 64) values). cinfo as at launch_hex_warm_start.
 void launch_neohookean_hex_residual(hipStream_t s, tet_group_dev g, cluster_dev cl, const pos4* x,
                                     const uint4* cinfo, uint32_t nrows, float inv_dt2, double* partial);
 // out[0] = the sum of partial[0, n) by one block in a fixed order (out may be pinned host memory).

@@ -1032,8 +1032,10 @@ __device__ __forceinline__ double nh_residual2(tet_group_dev const& g, uint32_t 
 }
 
 // The same over a hex-cluster group, one thread per row of the all-color rows (a cluster of any
-// color, or padding): the cluster's eight particles are read once for its six tets.
-__global__ void __launch_bounds__(kRedBlock) k_neohookean_hex_residual(tet_group_dev g, cluster_dev cl,
+// color, or padding): the cluster's eight particles are read once for its six tets. One 64-row
+// block of rows per workgroup, so that workgroup b runs on the XCD of its rows' region (b mod 8).
+constexpr int kHexRedBlock = 64;
+__global__ void __launch_bounds__(kHexRedBlock) k_neohookean_hex_residual(tet_group_dev g, cluster_dev cl,
                                                                        const pos4* __restrict__ x,
                                                                        const uint4* __restrict__ cinfo,
                                                                        uint32_t nrows, float inv_dt2,
@@ -1052,7 +1054,7 @@ __global__ void __launch_bounds__(kRedBlock) k_neohookean_hex_residual(tet_group
             h2 += nh_residual2(g, info.z + uint32_t(k) * info.y + info.x, P[kHexSlots[k][0]], P[kHexSlots[k][1]],
                                P[kHexSlots[k][2]], P[kHexSlots[k][3]], inv_dt2);
     }
-    double const s = block_sum<kRedBlock>(h2);
+    double const s = block_sum<kHexRedBlock>(h2);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
@@ -1407,8 +1409,8 @@ void launch_neohookean_hex_residual(hipStream_t s, tet_group_dev g, cluster_dev 
                                     const uint4* cinfo, uint32_t nrows, float inv_dt2, double* partial)
 {
     if (nrows)
-        k_neohookean_hex_residual<<<grid_for(nrows, kRedBlock), kRedBlock, 0, s>>>(g, cl, x, cinfo, nrows,
-                                                                                     inv_dt2, partial);
+        k_neohookean_hex_residual<<<grid_for(nrows, kHexRedBlock), kHexRedBlock, 0, s>>>(g, cl, x, cinfo, nrows,
+                                                                                           inv_dt2, partial);
 }
 
 void launch_copy_spans(hipStream_t s, copy_spans const& spans)

@@ -1094,7 +1094,7 @@ std::size_t sbs_gpu::residual_blocks() const
 {
     std::size_t total = 0;
     for (auto const& g : groups)
-        if (g.kind == 1) total += ((g.hex ? g.ws_cinfo.n : g.count) + 255) / 256;
+        if (g.kind == 1) total += g.hex ? (g.ws_cinfo.n + 63) / 64 : (g.count + 255) / 256;
     return total;
 }
 
@@ -1141,7 +1141,7 @@ void sbs_gpu::enqueue_residual(double dt)
             uint32_t const rows = uint32_t(g.ws_cinfo.n);
             launch_neohookean_hex_residual(stream, g.tet(), g.clusters(), xn.p, g.ws_cinfo.p, rows, float(1.0 / dt2),
                                            check_partial.p + at);
-            at += (rows + 255) / 256;
+            at += (rows + 63) / 64;
         } else {
             launch_neohookean_residual(stream, g.tet(), xn.p, 0, g.count, float(1.0 / dt2), check_partial.p + at, 256);
             at += (g.count + 255) / 256;
