@@ -234,7 +234,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5, help="untimed frames")
     ap.add_argument("--profile-steps", type=int, default=3, help="frames of the instrumented region")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="cfg1_1m", choices=["cfg1_1m", "cfg1_6m", "cfg0", "cfg2", "cfg4"])
+    ap.add_argument("--workload", default="beam_1m", choices=["beam_1m", "beam_6m", "beam_540", "batch_512", "batch_512_large"])
     ap.add_argument("--no-learned", action="store_true", help="plain solver only")
     ap.add_argument("--no-graphs", action="store_true", help="stream launches instead of hipGraph replays")
     ap.add_argument("--no-contact-fusion", action="store_true",
@@ -261,8 +261,8 @@ def main():
     if args.copy_calibration:
         copy_calibration()
 
-    scene = {"cfg1_1m": scenes.cfg1_1m, "cfg1_6m": scenes.cfg1_6m, "cfg0": scenes.cfg0,
-             "cfg2": scenes.cfg2, "cfg4": scenes.cfg4}[args.workload]()
+    scene = {"beam_1m": scenes.beam_1m, "beam_6m": scenes.beam_6m, "beam_540": scenes.beam_540,
+             "batch_512": scenes.batch_512, "batch_512_large": scenes.batch_512_large}[args.workload]()
     log(f"workload {scene.name}: {scene.num_vertices} vertices, {scene.num_tets} tets")
     t0 = time.perf_counter()
     sim = scenes.build_device(scene)
@@ -387,7 +387,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(scene, scenes.batch(16) if args.workload in ("cfg2", "cfg4") else None)
+        cpu = cpu_baseline(scene, scenes.batch(16) if args.workload in ("batch_512", "batch_512_large") else None)
         log(f"cpu baseline {cpu['value']:.3f} substeps/s")
 
     out = {

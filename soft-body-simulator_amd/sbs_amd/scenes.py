@@ -185,26 +185,26 @@ def cantilever(nx, ny, nz, edge=0.1, model="green", young=1e6, poisson=0.45, den
     return sc
 
 
-def cfg0():
+def beam_540():
     """Plain XPBD cantilever: 10x3x3 cells of 0.1 m, 6 tets each (540 tets, 176 vertices), left
     face pinned, green constraints E=1e6, nu=0.45, density 1000, ground at y=-1, dt 1/60,
     1 substep x 10 iterations."""
-    return cantilever(10, 3, 3, 0.1, "green", 1e6, 0.45, 1000.0, iterations=10, substeps=1, name="cfg0_beam_540")
+    return cantilever(10, 3, 3, 0.1, "green", 1e6, 0.45, 1000.0, iterations=10, substeps=1, name="green_beam_540")
 
 
-def cfg1_1m(cluster=True, coloring="index"):
+def beam_1m(cluster=True, coloring="index"):
     """Large beam, 1M-tet variant: 220x28x27 cells (997,920 tets, 179,452 vertices), stable
     neo-Hookean E=5e5, nu=0.45, 4 substeps x 8 iterations, ground plane and 16 static spheres.
     By default the 6 tets of a cell form one cluster and clusters are colored in index order
     (8 colors on the lattice)."""
     return cantilever(220, 28, 27, 0.1, "neohookean", 5e5, 0.45, 1000.0, num_spheres=16, iterations=8, substeps=4,
-                      name="cfg1_beam_1M_neohookean", cluster=cluster, coloring=coloring)
+                      name="beam_1M_neohookean", cluster=cluster, coloring=coloring)
 
 
-def cfg1_6m(cluster=True, coloring="index"):
+def beam_6m(cluster=True, coloring="index"):
     """Large beam: 400x50x50 cells (6.0M tets, 1,043,001 vertices), same material and solver."""
     return cantilever(400, 50, 50, 0.1, "neohookean", 5e5, 0.45, 1000.0, num_spheres=16, iterations=8, substeps=4,
-                      name="cfg1_beam_6M_neohookean", cluster=cluster, coloring=coloring)
+                      name="beam_6M_neohookean", cluster=cluster, coloring=coloring)
 
 
 def _random_rotation(rng):
@@ -283,12 +283,12 @@ def batch(num_bodies=512, tet_range=(2e3, 2e5), edge=0.05, seed=0xB0D1E5, densit
     return sc
 
 
-def cfg2(num_bodies=512):
-    """Config 2: 512 batched bodies (about 22M tets)."""
-    return batch(num_bodies, name=f"cfg2_batch_{num_bodies}_bodies")
+def batch_512(num_bodies=512):
+    """512 batched bodies (about 22M tets)."""
+    return batch(num_bodies, name=f"batch_{num_bodies}_bodies")
 
 
-def cfg4(num_bodies=512, large=(8, 5e5)):
-    """Config 4: the 512-body batch plus 8 bodies of about 500k tets in the same batched group
+def batch_512_large(num_bodies=512, large=(8, 5e5)):
+    """The 512-body batch plus 8 bodies of about 500k tets in the same batched group
     (about 26M tets); the learned warm start serves all of them with one network."""
-    return batch(num_bodies, large=large, name=f"cfg4_batch_{num_bodies}_plus_{int(large[0])}_large")
+    return batch(num_bodies, large=large, name=f"batch_{num_bodies}_plus_{int(large[0])}_large")

@@ -106,7 +106,7 @@ def _coloring_scenes():
     bar.groups.append(scenes.Group("distance", 0, np.array([[0, a, 0, b] for a, b in edges], dtype=np.uint32),
                                    alpha=1e-6))
     big = scenes.cantilever(16, 6, 5, model="neohookean")
-    return [scenes.cfg0(), scenes.reference_demo(), bar, big]
+    return [scenes.beam_540(), scenes.reference_demo(), bar, big]
 
 
 @pytest.mark.parametrize("idx", [0, 1, 2, 3])
@@ -168,16 +168,16 @@ def _oracle_self_spread(scene, frames, rel=2.0 ** -24, seed=2):
     return out
 
 
-def test_cfg0_beam_within_reference_conditioning():
+def test_beam_540_within_reference_conditioning():
     # With nu = 0.45 the reference's green constraint makes Gauss-Seidel chaotic on this beam: the
     # FP64 oracle disagrees with itself by ~2e-4 (frame 1) and ~5e-2 (frame 60) when its initial
     # state is perturbed at FP32 rounding level, so no implementation that is not bit-identical
     # can meet 1e-4. The device must stay inside that envelope.
-    sc = scenes.cfg0()
+    sc = scenes.beam_540()
     say(f"{sc.name} 60 frames vs oracle self-spread")
     spread = _oracle_self_spread(sc, (1, 60))
     errs, dev, orc = run_pair(sc, 60, check_frames=(1, 60))
-    print("cfg0 device errors", errs, "oracle self-spread", spread, flush=True)
+    print("beam_540 device errors", errs, "oracle self-spread", spread, flush=True)
     assert spread[1] > TOL  # the documented ill-conditioning
     for f in (1, 60):
         assert errs[f] <= max(TOL, 3.0 * spread[f])
@@ -187,7 +187,7 @@ def test_cfg0_beam_within_reference_conditioning():
 
 
 def test_green_beam_parity_1_and_60_frames():
-    # cfg0's beam, masses, solver and contacts with nu = 0.3, where the reference method is well
+    # beam_540's beam, masses, solver and contacts with nu = 0.3, where the reference method is well
     # conditioned: strict 1e-4 bound
     sc = scenes.cantilever(10, 3, 3, 0.1, "green", 1e6, 0.30, 1000.0, iterations=10, substeps=1, name="beam_nu03")
     say(f"{sc.name} 60 frames")
@@ -406,7 +406,7 @@ def test_errors_are_reported_not_thrown():
 
 
 def test_million_tet_beam_properties():
-    sc = scenes.cfg1_1m()
+    sc = scenes.beam_1m()
     say(f"{sc.name} commit + 2 frames")
     dev = scenes.build_device(sc)
     colors, nc = dev.group_colors(0)

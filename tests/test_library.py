@@ -61,7 +61,7 @@ def test_mesh_errors_are_status_codes():
 
 
 def test_benchmark_scene_sizes():
-    sc = scenes.cfg0()
+    sc = scenes.beam_540()
     assert (sc.num_vertices, sc.num_tets, len(sc.bodies[0].surface)) == (176, 540, 140)
     pinned = sc.bodies[0].mass == 0
     assert pinned.sum() == 4 * 4 and np.all(sc.bodies[0].x0[pinned, 0] == 0.0)
