@@ -77,8 +77,15 @@ struct contact_dev {
 void launch_predict(hipStream_t s, pos4* xi, const pos4* xn, float4* v, uint32_t n, float gx, float gy,
                     float gz, int gravity_is_force, float dt, int block);
 void launch_commit(hipStream_t s, const pos4* xi, pos4* xn, float4* v, uint32_t n, float dt, int block);
+// Particle rows: a permutation of the particles in workgroups of kParticleRowBlock rows, workgroup
+// b holding particles of XCD region b mod 8 (padding rows 0xffffffff), so that a per-particle pass
+// runs on the XCD whose L2 holds the region's particles. With prow (nprows rows), the commit and
+// prediction run over the rows (kParticleRowBlock threads per workgroup), else over [0, n) with
+// `block` threads.
+constexpr int kParticleRowBlock = 256;
 void launch_commit_predict(hipStream_t s, pos4* xi, pos4* xn, float4* v, uint32_t n, float gx, float gy,
-                           float gz, int gravity_is_force, float dt, int block);
+                           float gz, int gravity_is_force, float dt, int block, const uint32_t* prow = nullptr,
+                           uint32_t nprows = 0);
 
 void launch_green(hipStream_t s, tet_group_dev g, pos4* xi, const pos4* xn, uint32_t begin, uint32_t count,
                   float alpha_tilde, float gamma, int first, int block);
@@ -173,11 +180,12 @@ void launch_copy_pos(hipStream_t s, const pos4* a, pos4* b, uint32_t n);
 // padding with cluster 0xffffffff.
 // With surface_of (dense list only; particle -> surface vertex or -1) the gather also runs the first
 // iteration's contact projections (compliance c_at) of the surface vertices, in place of that
-// iteration's contact launch.
+// iteration's contact launch. With prow (dense list only), the gather runs over the particle rows.
 void launch_hex_warm_start(hipStream_t s, tet_group_dev g, cluster_dev cl, const pos4* xt, pos4* xi,
                            const uint4* cinfo, uint32_t nrows, float inv_dt2, int expl, float2 scale,
                            float4* disp, const uint32_t* csr_off, const uint32_t* csr, uint32_t n,
-                           const int* surface_of = nullptr, contact_dev const& c = {}, float c_at = 0.0f);
+                           const int* surface_of = nullptr, contact_dev const& c = {}, float c_at = 0.0f,
+                           const uint32_t* prow = nullptr, uint32_t nprows = 0);
 
 // Multiplier warm start of a neo-Hookean group, per constraint t (execution order) at the
 // predicted positions xi: the 13 features

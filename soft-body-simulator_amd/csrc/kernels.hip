@@ -91,12 +91,17 @@ __global__ void k_commit(const pos4* __restrict__ xi, pos4* __restrict__ xn, flo
     xn[i] = p;
 }
 
-// Commit of one substep fused with the prediction of the next (same per-particle order).
+// Commit of one substep fused with the prediction of the next (same per-particle order). With prow,
+// thread r handles particle prow[r] (rows [0, n); padding rows hold 0xffffffff), else particle r.
 __global__ void k_commit_predict(pos4* __restrict__ xi, pos4* __restrict__ xn, float4* __restrict__ v, uint32_t n,
-                                 float gx, float gy, float gz, int force, float dt)
+                                 float gx, float gy, float gz, int force, float dt, const uint32_t* __restrict__ prow)
 {
-    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (prow) {
+        i = prow[i];
+        if (i == 0xffffffffu) return;
+    }
     pos4 const p = xi[i], q = xn[i];
     float4 vel = velocity(p, q, dt);
     pos4 np;
@@ -817,11 +822,17 @@ __global__ void k_gather_disp(const uint32_t* __restrict__ off, const uint32_t* 
 // the unused ones add an exact 0.
 // With surface_of (particle -> surface vertex or -1), the first iteration's contact projections of
 // the surface vertices follow in the same thread, as k_contact(first) would run them next.
+// With prow, thread r handles particle prow[r] (rows [0, n); padding rows hold 0xffffffff).
 __global__ void k_gather_disp8(const uint4* __restrict__ ent, const float4* __restrict__ disp, pos4* __restrict__ xi,
-                               uint32_t n, const int* __restrict__ surface_of, contact_dev c, float c_at)
+                               uint32_t n, const int* __restrict__ surface_of, contact_dev c, float c_at,
+                               const uint32_t* __restrict__ prow)
 {
-    uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
+    if (prow) {
+        p = prow[p];
+        if (p == 0xffffffffu) return;
+    }
     uint4 const a = ent[2 * std::size_t(p)], b = ent[2 * std::size_t(p) + 1];
     uint32_t const e[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
     int const sv = surface_of ? surface_of[p] : -1;
@@ -1257,9 +1268,13 @@ void launch_commit(hipStream_t s, const pos4* xi, pos4* xn, float4* v, uint32_t 
 }
 
 void launch_commit_predict(hipStream_t s, pos4* xi, pos4* xn, float4* v, uint32_t n, float gx, float gy, float gz,
-                           int force, float dt, int block)
+                           int force, float dt, int block, const uint32_t* prow, uint32_t nprows)
 {
-    if (n) k_commit_predict<<<grid_for(n, block), block, 0, s>>>(xi, xn, v, n, gx, gy, gz, force, dt);
+    if (prow && nprows)
+        k_commit_predict<<<grid_for(nprows, kParticleRowBlock), kParticleRowBlock, 0, s>>>(xi, xn, v, nprows, gx, gy,
+                                                                                          gz, force, dt, prow);
+    else if (n)
+        k_commit_predict<<<grid_for(n, block), block, 0, s>>>(xi, xn, v, n, gx, gy, gz, force, dt, nullptr);
 }
 
 void launch_green(hipStream_t s, tet_group_dev g, pos4* xi, const pos4* xn, uint32_t begin, uint32_t count,
@@ -1477,15 +1492,19 @@ void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t value, uint32_t n)
 void launch_hex_warm_start(hipStream_t s, tet_group_dev g, cluster_dev cl, const pos4* xt, pos4* xi,
                            const uint4* cinfo, uint32_t nrows, float inv_dt2, int expl, float2 scale,
                            float4* disp, const uint32_t* csr_off, const uint32_t* csr, uint32_t n,
-                           const int* surface_of, contact_dev const& c, float c_at)
+                           const int* surface_of, contact_dev const& c, float c_at, const uint32_t* prow,
+                           uint32_t nprows)
 {
     if (!nrows) return;
     k_hex_warm_disp<<<grid_for(nrows, 64), 64, 0, s>>>(g, cl, xt, cinfo, nrows, inv_dt2, expl, scale, disp);
     if (csr_off)
         k_gather_disp<<<grid_for(n, 256), 256, 0, s>>>(csr_off, csr, disp, xi, n);
+    else if (prow && nprows)
+        k_gather_disp8<<<grid_for(nprows, kParticleRowBlock), kParticleRowBlock, 0, s>>>(
+            reinterpret_cast<const uint4*>(csr), disp, xi, nprows, surface_of, c, c_at, prow);
     else
         k_gather_disp8<<<grid_for(n, 256), 256, 0, s>>>(reinterpret_cast<const uint4*>(csr), disp, xi, n, surface_of,
-                                                        c, c_at);
+                                                        c, c_at, nullptr);
 }
 
 void launch_copy_pos(hipStream_t s, const pos4* a, pos4* b, uint32_t n)

@@ -146,6 +146,7 @@ struct sbs_gpu {
     int cluster_block = 64;
     bool fuse_first_contacts = true;  // the first iteration's contacts in the warm-start gather
     dev_buf<int> surface_of_dev;      // particle -> surface vertex, or -1
+    dev_buf<uint32_t> prow;           // particle rows by XCD region (built with the hex warm-start tables)
     int coloring_order = SBS_COLORING_HASH;
 
     bool timing = false;
@@ -817,7 +818,10 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
             timed(SBS_KERNEL_PREDICT, V, [&] { launch_predict(stream, xi.p, xn.p, vel.p, V, g0, g1, g2, force, float(dt), block); });
         else
             timed(SBS_KERNEL_PREDICT, V,
-                  [&] { launch_commit_predict(stream, xi.p, xn.p, vel.p, V, g0, g1, g2, force, float(dt), block); });
+                  [&] {
+                      launch_commit_predict(stream, xi.p, xn.p, vel.p, V, g0, g1, g2, force, float(dt), block, prow.p,
+                                            uint32_t(prow.n));
+                  });
         // multiplier warm start of the neo-Hookean groups: lambda_init (previous, plus the network's
         // change, or -C(x~)/alpha~), then x = x~ + M^-1 grad C(x~)^T lambda_init
         bool warmed = false;
@@ -858,7 +862,7 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
                                               uint32_t(g.ws_cinfo.n), float(1.0 / dt2), mode == kApplyExplicit,
                                               make_float2(float(ws_scale), float(ws_scale_hyd)), g.ws_disp.p,
                                               g.ws_csr_off.p, g.ws_csr.p, V, fused0 ? surface_of_dev.p : nullptr, c,
-                                              c_at);
+                                              c_at, prow.p, uint32_t(prow.n));
                     }, 2);
                 else
                     timed(SBS_KERNEL_WARM_START, g.count, [&] { launch_group_colors(g, dt, mode, xt.p); }, g.ncolors);
@@ -933,6 +937,7 @@ void sbs_gpu::ensure_cinfo(group_t& g)
 void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
 {
     if (!xt.p) xt.alloc(V);
+    std::vector<int8_t> preg;  // particle -> XCD region of the first hex cluster (execution order) holding it
     for (auto& g : groups) {
         if (g.kind != 1) continue;
         if (g.hex && !g.ws_csr.p) {
@@ -957,6 +962,14 @@ void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
             std::vector<uint32_t> fill(off.begin(), off.end() - 1);
             for (uint32_t c = 0; c < nc; ++c)
                 for (int k = 0; k < 8; ++k) ent[fill[slot(c, k)]++] = uint32_t(k * rows + g.ws_rowof[c]);
+            if (!prow.p) {
+                preg.resize(V, -1);
+                for (uint32_t c = 0; c < nc; ++c)
+                    for (int k = 0; k < 8; ++k) {
+                        int8_t& r = preg[std::size_t(slot(c, k))];
+                        if (r < 0) r = int8_t((g.ws_rowof[c] / 64) % kXcds);
+                    }
+            }
             if (widest <= 8) {  // a dense row of 8 per particle, same order (ws_csr_off stays empty)
                 std::vector<uint32_t> dense(std::size_t(V) * 8, 0xffffffffu);
                 for (std::size_t p = 0; p < V; ++p)
@@ -974,6 +987,21 @@ void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
         }
         if (raw && !g.ws_raw.p) g.ws_raw.alloc(std::size_t(g.count) * mlp::kMaxInputs);
         if (X && !g.ws_X.p) g.ws_X.alloc(std::size_t(g.count) * mlp::kIn);
+    }
+    if (!prow.p && !preg.empty()) {
+        // per region its particles in index order (particles of no hex cluster by index eighths),
+        // in workgroups of kParticleRowBlock rows with workgroup b on region b mod 8
+        std::vector<std::vector<uint32_t>> region(kXcds);
+        for (std::size_t p = 0; p < V; ++p)
+            region[preg[p] >= 0 ? preg[p] : int(p * kXcds / V)].push_back(uint32_t(p));
+        std::size_t blocks = 0;
+        for (auto const& r : region) blocks = std::max(blocks, (r.size() + kParticleRowBlock - 1) / kParticleRowBlock);
+        std::vector<uint32_t> rows(blocks * kXcds * kParticleRowBlock, 0xffffffffu);
+        for (int x = 0; x < kXcds; ++x)
+            for (std::size_t q = 0; q < region[x].size(); ++q)
+                rows[((q / kParticleRowBlock) * kXcds + std::size_t(x)) * kParticleRowBlock + q % kParticleRowBlock] =
+                    region[x][q];
+        prow.upload(rows.data(), rows.size(), stream);
     }
 }
 

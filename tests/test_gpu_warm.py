@@ -350,3 +350,21 @@ def test_fused_first_contacts_are_bit_identical():
         assert ncontacts > 0
         out.append(dev.read_body(0))
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_particle_rows_leave_plain_frames_unchanged():
+    # the warm-start tables bring the particle rows (every particle once, grouped by XCD region)
+    # that the commit + prediction then runs over: a plain solve with them equals one without
+    sc = scenes.cantilever(8, 3, 3, 0.1, "neohookean", 5e5, 0.45, 1000.0, num_spheres=2, iterations=6, substeps=3,
+                           cluster=True, coloring="index")
+    say("particle rows, plain frames")
+    out = []
+    for rows in (False, True):
+        dev = scenes.build_device(sc)
+        if rows:
+            dev.set_warm_start("previous")
+            dev.set_warm_start("none")
+        for _ in range(5):
+            dev.step(sc.dt, sc.iterations, sc.substeps)
+        out.append(dev.read_body(0))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
