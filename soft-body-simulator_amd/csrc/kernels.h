@@ -175,9 +175,10 @@ void launch_copy_pos(hipStream_t s, const pos4* a, pos4* b, uint32_t n);
 // the cluster in row r at k * nrows + r), then for each particle p < n, xi[p] += the FP64 sum over
 // csr[csr_off[p] .. csr_off[p+1]) (indices into disp, in cluster execution order) -- the same
 // additions in the same order as the per-color pass; with csr_off null, csr is a dense [n][8] table
-// of those indices (unused: 0xffffffff). cinfo[r] (nrows rows, a multiple of 64): (the cluster's
-// index in its color, the color's cluster count, the color's first constraint, the cluster), or
-// padding with cluster 0xffffffff.
+// of those indices (unused: 0xffffffff). nrows is a multiple of 64; per 64-row workgroup b,
+// cinfo[2b] = (its first cluster's index in its color, the color's cluster count, the color's first
+// constraint, the first cluster) and cinfo[2b + 1].x = its rows in use, t < that running cluster
+// cinfo[2b].w + t.
 // With surface_of (dense list only; particle -> surface vertex or -1) the gather also runs the first
 // iteration's contact projections (compliance c_at) of the surface vertices, in place of that
 // iteration's contact launch. With prow (dense list only), the gather runs over the particle rows.

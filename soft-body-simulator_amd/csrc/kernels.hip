@@ -750,18 +750,20 @@ __global__ void __launch_bounds__(BLOCK) k_neohookean_hex_apply(tet_group_dev g,
 
 // Warm-start displacements of every hex cluster of a group in one launch (all colors): nothing
 // moves during the launch, so every gradient is taken at the predicted positions xi (a copy of
-// them, or the working positions themselves before any displacement is added). Row r (64-row
-// workgroups) holds cluster cinfo[r].w (execution order) or padding (0xffffffff); the summed
-// displacement of the cluster's slot k goes to disp[k * nrows + r] (one coalesced store per slot).
+// them, or the working positions themselves before any displacement is added). Workgroup b (64
+// rows) runs the clusters cinfo[2b].w + t, t < cinfo[2b + 1].x, of one color (cinfo[2b]: the first
+// one's index in its color, the color's cluster count and first constraint); the summed
+// displacement of the cluster's slot k goes to disp[k * nrows + 64 b + t] (one coalesced store per
+// slot).
 __global__ void __launch_bounds__(64) k_hex_warm_disp(tet_group_dev g, cluster_dev cl, const pos4* __restrict__ xi,
                                                       const uint4* __restrict__ cinfo, uint32_t nrows,
                                                       float inv_dt2, int expl, float2 scale, float4* __restrict__ disp)
 {
-    uint32_t const r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrows) return;
-    uint4 const info = cinfo[r];  // (index in its color, clusters in the color, the color's first constraint, cluster)
-    if (info.w == 0xffffffffu) return;
-    uint32_t const i = info.x, count = info.y, tbase = info.z, ci = info.w;
+    uint4 const info = cinfo[2 * blockIdx.x];
+    uint32_t const used = cinfo[2 * blockIdx.x + 1].x;
+    if (threadIdx.x >= used) return;
+    uint32_t const r = blockIdx.x * 64u + threadIdx.x;
+    uint32_t const i = info.x + threadIdx.x, count = info.y, tbase = info.z, ci = info.w + threadIdx.x;
     int4 const va = cl.v0[ci], vb = cl.v1[ci];
     int const ids[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
     // every tet's material and multipliers are loaded before the first multiplier is stored back
@@ -1044,7 +1046,7 @@ __device__ __forceinline__ double nh_residual2(tet_group_dev const& g, uint32_t 
 
 // The same over a hex-cluster group, one thread per row of the all-color rows (a cluster of any
 // color, or padding): the cluster's eight particles are read once for its six tets. One 64-row
-// block of rows per workgroup, so that workgroup b runs on the XCD of its rows' region (b mod 8).
+// workgroup per record pair, so that workgroup b runs on the XCD of its rows' region (b mod 8).
 constexpr int kHexRedBlock = 64;
 __global__ void __launch_bounds__(kHexRedBlock) k_neohookean_hex_residual(tet_group_dev g, cluster_dev cl,
                                                                        const pos4* __restrict__ x,
@@ -1052,17 +1054,17 @@ __global__ void __launch_bounds__(kHexRedBlock) k_neohookean_hex_residual(tet_gr
                                                                        uint32_t nrows, float inv_dt2,
                                                                        double* partial)
 {
-    uint32_t const r = blockIdx.x * blockDim.x + threadIdx.x;
+    // the workgroup's run of clusters, as at k_hex_warm_disp
+    uint4 const info = cinfo[2 * blockIdx.x];
+    uint32_t const used = cinfo[2 * blockIdx.x + 1].x;
     double h2 = 0.0;
-    uint4 const info = r < nrows ? cinfo[r] : make_uint4(0u, 0u, 0u, 0xffffffffu);
-    if (info.w != 0xffffffffu) {
-        // (index in its color, clusters in the color, the color's first constraint, cluster)
-        uint32_t const ci = info.w;
+    if (threadIdx.x < used) {
+        uint32_t const ci = info.w + threadIdx.x, i = info.x + threadIdx.x;
         int4 const va = cl.v0[ci], vb = cl.v1[ci];
         pos4 const P[8] = {x[va.x], x[va.y], x[va.z], x[va.w], x[vb.x], x[vb.y], x[vb.z], x[vb.w]};
 #pragma unroll
         for (int k = 0; k < kHexTets; ++k)
-            h2 += nh_residual2(g, info.z + uint32_t(k) * info.y + info.x, P[kHexSlots[k][0]], P[kHexSlots[k][1]],
+            h2 += nh_residual2(g, info.z + uint32_t(k) * info.y + i, P[kHexSlots[k][0]], P[kHexSlots[k][1]],
                                P[kHexSlots[k][2]], P[kHexSlots[k][3]], inv_dt2);
     }
     double const s = block_sum<kHexRedBlock>(h2);

@@ -69,10 +69,12 @@ struct group_t {
     dev_buf<mlp::bf16> ws_X;     // [count][16] network inputs (learned warm start)
     dev_buf<float4> ws_disp;     // hex groups: slot displacements of the warm-start pass, [8][rows]
     dev_buf<uint32_t> ws_csr_off, ws_csr;  // particle -> slot list
-    // hex groups, per row of the all-color launches: (index in its color, clusters in the color,
-    // first constraint, cluster) or a padding row (cluster 0xffffffff); rows in 64-row blocks, block
-    // b holding clusters of XCD region b mod 8
+    // hex groups, the all-color launches: rows in 64-row workgroups, workgroup b holding clusters
+    // of XCD region b mod 8, all of one color; per workgroup two records (first cluster's index in
+    // its color, clusters in the color, the color's first constraint, first cluster) and (rows in
+    // use, 0, 0, 0)
     dev_buf<uint4> ws_cinfo;
+    uint32_t ws_rows = 0;
     std::vector<uint32_t> ws_rowof;  // cluster -> its row
     dev_buf<float> snap_lambda;  // snapshot copies
     dev_buf<float2> snap_hist;
@@ -859,7 +861,7 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
                 if (g.hex)
                     timed(SBS_KERNEL_WARM_START, g.count, [&] {
                         launch_hex_warm_start(stream, g.tet(), g.clusters(), xtilde, xi.p, g.ws_cinfo.p,
-                                              uint32_t(g.ws_cinfo.n), float(1.0 / dt2), mode == kApplyExplicit,
+                                              g.ws_rows, float(1.0 / dt2), mode == kApplyExplicit,
                                               make_float2(float(ws_scale), float(ws_scale_hyd)), g.ws_disp.p,
                                               g.ws_csr_off.p, g.ws_csr.p, V, fused0 ? surface_of_dev.p : nullptr, c,
                                               c_at, prow.p, uint32_t(prow.n));
@@ -898,16 +900,18 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
     if (check) enqueue_residual(dt);
 }
 
-// The rows of a hex group's all-color launches (warm-start displacements, residual): per cluster its
-// index in its color, the color's cluster count and first constraint, and the cluster itself. The
-// clusters of XCD region x (every color's share, color by color) fill the 64-row blocks b with
-// b mod 8 = x, so that a workgroup runs on the XCD whose L2 holds its region's particles, as in the
-// per-color launches; short regions end in padding rows. Uploaded once; frame graphs hold the pointer.
+// The rows of a hex group's all-color launches (warm-start displacements, residual). The clusters of
+// XCD region x (every color's share, color by color, in runs of at most 64 consecutive clusters of
+// one color) fill the 64-row workgroups b with b mod 8 = x, so that a workgroup runs on the XCD
+// whose L2 holds its region's particles, as in the per-color launches; a workgroup's record gives
+// its run (the kernels read it once per workgroup), unused rows and workgroups are padding. Uploaded
+// once; frame graphs hold the pointer.
 void sbs_gpu::ensure_cinfo(group_t& g)
 {
     if (g.ws_cinfo.p || !g.hex) return;
     uint32_t const nc = g.num_elements();
-    std::vector<std::vector<uint4>> region(kXcds);
+    // per region, its workgroups: each a run of at most 64 consecutive clusters of one color
+    std::vector<std::vector<std::array<uint32_t, 5>>> region(kXcds);
     for (uint32_t c = 0; c < g.ncolors; ++c) {
         uint32_t const cb = g.cluster_color_offsets[c], cn = g.cluster_color_offsets[c + 1] - cb;
         xcd_split sp{};
@@ -916,22 +920,25 @@ void sbs_gpu::ensure_cinfo(group_t& g)
         else
             for (int x = 0; x <= kXcds; ++x) sp.off[x] = uint32_t(uint64_t(cn) * uint32_t(x) / kXcds);
         for (int x = 0; x < kXcds; ++x)
-            for (uint32_t j = sp.off[x]; j < sp.off[x + 1]; ++j)
-                region[x].push_back(make_uint4(j, cn, g.color_offsets[c], cb + j));
+            for (uint32_t j = sp.off[x]; j < sp.off[x + 1]; j += 64)
+                region[x].push_back({j, cn, g.color_offsets[c], cb + j, std::min<uint32_t>(64u, sp.off[x + 1] - j)});
     }
-    std::size_t blocks = 0;
-    for (auto const& r : region) blocks = std::max(blocks, (r.size() + 63) / 64);
-    std::size_t const rows = std::max<std::size_t>(blocks * kXcds * 64, 64);
-    require(rows < 0xffffffffull / 8, "too many clusters for the warm-start rows");
-    std::vector<uint4> info(rows, make_uint4(0u, 0u, 0u, 0xffffffffu));
+    std::size_t per = 1;
+    for (auto const& r : region) per = std::max(per, r.size());
+    std::size_t const blocks = per * kXcds;
+    require(blocks * 64 < 0xffffffffull / 8, "too many clusters for the warm-start rows");
+    std::vector<uint4> rec(2 * blocks, make_uint4(0u, 0u, 0u, 0u));
     g.ws_rowof.assign(nc, 0u);
     for (int x = 0; x < kXcds; ++x)
         for (std::size_t q = 0; q < region[x].size(); ++q) {
-            std::size_t const r = ((q / 64) * kXcds + std::size_t(x)) * 64 + q % 64;
-            info[r] = region[x][q];
-            g.ws_rowof[region[x][q].w] = uint32_t(r);
+            auto const& w = region[x][q];
+            std::size_t const b = q * kXcds + std::size_t(x);
+            rec[2 * b] = make_uint4(w[0], w[1], w[2], w[3]);
+            rec[2 * b + 1] = make_uint4(w[4], 0u, 0u, 0u);
+            for (uint32_t t = 0; t < w[4]; ++t) g.ws_rowof[w[3] + t] = uint32_t(b * 64 + t);
         }
-    g.ws_cinfo.upload(info.data(), rows, stream);
+    g.ws_rows = uint32_t(blocks * 64);
+    g.ws_cinfo.upload(rec.data(), rec.size(), stream);
 }
 
 void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
@@ -947,7 +954,7 @@ void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
             hip_check(hipMemcpy(v0.data(), g.cl_v0.p, nc * sizeof(int4), hipMemcpyDeviceToHost), "read clusters");
             hip_check(hipMemcpy(v1.data(), g.cl_v1.p, nc * sizeof(int4), hipMemcpyDeviceToHost), "read clusters");
             ensure_cinfo(g);
-            std::size_t const rows = g.ws_cinfo.n;  // slot k of the cluster in row r: disp[k * rows + r]
+            std::size_t const rows = g.ws_rows;  // slot k of the cluster in row r: disp[k * rows + r]
             std::vector<uint32_t> off(std::size_t(V) + 1, 0), ent(std::size_t(nc) * 8);
             auto slot = [&](uint32_t c, int k) {
                 int4 const& a = k < 4 ? v0[c] : v1[c];
@@ -1122,7 +1129,7 @@ std::size_t sbs_gpu::residual_blocks() const
 {
     std::size_t total = 0;
     for (auto const& g : groups)
-        if (g.kind == 1) total += g.hex ? (g.ws_cinfo.n + 63) / 64 : (g.count + 255) / 256;
+        if (g.kind == 1) total += g.hex ? (g.ws_rows + 63) / 64 : (g.count + 255) / 256;
     return total;
 }
 
@@ -1166,7 +1173,7 @@ void sbs_gpu::enqueue_residual(double dt)
     for (auto const& g : groups) {
         if (g.kind != 1) continue;
         if (g.hex) {
-            uint32_t const rows = uint32_t(g.ws_cinfo.n);
+            uint32_t const rows = g.ws_rows;
             launch_neohookean_hex_residual(stream, g.tet(), g.clusters(), xn.p, g.ws_cinfo.p, rows, float(1.0 / dt2),
                                            check_partial.p + at);
             at += (rows + 63) / 64;
