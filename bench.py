@@ -69,33 +69,122 @@ def copy_calibration(nbytes=256 << 20, reps=4):
     log(f"copy calibration: {reps} device copies of {nbytes} bytes")
 
 
-def cpu_baseline(scene, sample=None, min_seconds=10.0, max_frames=8):
-    """FP64 oracle, reference (insertion) order, one thread: frames of the same scene (or of a
-    sample scene, scaled to the full scene by tet count) until min_seconds of CPU work or
-    max_frames frames."""
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
+    return O
 
+
+def cpu_baseline(scene, sample=None, warmup_frames=5, frames=5):
+    """The FP64 oracle in the reference's insertion order, one thread pinned to one host core
+    (os.sched_setaffinity of the calling thread, restored afterwards): `warmup_frames` untimed
+    frames, then `frames` frames timed one by one; the median frame gives substeps/s (of the same
+    scene, or of a sample scene scaled to the full scene by tet count)."""
+    O = _oracle()
     run = sample or scene
     t0 = time.perf_counter()
     sim = O.Simulation()
     run.apply(sim)
     sim.set_order(False)
     setup = time.perf_counter() - t0
-    log(f"cpu baseline: {run.name} built in {setup:.1f}s, timing frames of {run.substeps} substeps "
-        f"for at least {min_seconds:.0f}s")
-    frames, el = 0, 0.0
-    while frames < max_frames and (frames == 0 or el < min_seconds):
-        t0 = time.perf_counter()
-        sim.step(run.dt, run.iterations, run.substeps)
-        el += time.perf_counter() - t0
-        frames += 1
+    mask = os.sched_getaffinity(0)
+    core = min(mask)
+    os.sched_setaffinity(0, {core})
+    try:
+        log(f"cpu baseline: {run.name} built in {setup:.1f}s; {warmup_frames} warm-up and {frames} timed frames "
+            f"of {run.substeps} substeps on core {core}")
+        for _ in range(warmup_frames):
+            sim.step(run.dt, run.iterations, run.substeps)
+        ts = []
+        for _ in range(frames):
+            t0 = time.perf_counter()
+            sim.step(run.dt, run.iterations, run.substeps)
+            ts.append(time.perf_counter() - t0)
+    finally:
+        os.sched_setaffinity(0, mask)
+    med = sorted(ts)[len(ts) // 2]
     scale = scene.num_tets / run.num_tets
-    what = (f"{frames} frames = {frames * run.substeps} substeps x {run.iterations} iterations of {run.name} "
-            f"({run.num_tets} tets)" + (f", scaled by tet count to {scene.num_tets} tets" if sample else ""))
-    return {"value": frames * scene.substeps / (el * scale), "unit": "substeps/s", "cores": 1, "kind": "port",
-            "sample": f"{what}; FP64 oracle in reference insertion order, single thread, {el:.1f}s on {cpu_model()}",
-            "projections_per_s": run.num_tets * run.iterations * run.substeps * frames / el}
+    what = (f"median of {frames} frames (after {warmup_frames} warm-up frames) of {run.substeps} substeps x "
+            f"{run.iterations} iterations of {run.name} ({run.num_tets} tets)"
+            + (f", scaled by tet count to {scene.num_tets} tets" if sample else ""))
+    return {"value": run.substeps / (med * scale), "unit": "substeps/s", "cores": 1, "kind": "port",
+            "sample": f"{what}; FP64 oracle in reference insertion order, one thread pinned to core {core}",
+            "frame_seconds": [round(t, 4) for t in ts], "host_cpu": cpu_model(), "nproc": os.cpu_count(),
+            "affinity_cores": len(mask),
+            "projections_per_s": run.num_tets * run.iterations * run.substeps / med}
+
+
+def oracle_threads():
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def accuracy_legs(scene, legs, launch, frames=60, check=(1, 60)):
+    """Position error of the device against the FP64 oracle, relative to the bounding-box diagonal,
+    after the frames in `check`, from rest. legs: a list of {"name", "iterations", "mode", "scale",
+    "net"} (mode "none" for the plain solver; no fallback, so the warm start's own trajectory is
+    measured). Each device leg is compared with the oracle running the same solver (colored order,
+    the same warm start), and each warm-started device leg also with the plain oracle leg, the judge
+    of the warm starts ("<name>_vs_plain"). The oracle runs the colored order's stages on
+    oracle_threads() host threads (bit-identical to one thread)."""
+    import numpy as np
+
+    from sbs_amd import autotune, scenes
+
+    O = _oracle()
+    diag = scene.bbox_diagonal()
+    dev_x, orc_x = {}, {}
+
+    def positions(sim):
+        return np.concatenate([sim.read_body(b)[0] for b in range(len(scene.bodies))])
+
+    for leg in legs:
+        name, k, mode = leg["name"], leg["iterations"], leg["mode"]
+        dev = scenes.build_device(scene)
+        dev.set_launch_config(launch["block"], launch["cluster_block"])
+        dev.use_graphs(True)
+        if mode != "none":
+            dev.set_warm_start(mode, leg.get("net"))
+            dev.set_warm_start_scale(*autotune._scales(leg["scale"]))
+        xs = {}
+        for f in range(1, frames + 1):
+            dev.step(scene.dt, k, scene.substeps)
+            if f in check:
+                xs[f] = positions(dev)
+        dev_x[name] = xs
+        dev.close()
+        if mode == "learned" and not hasattr(O.Simulation, "set_learned_warm_start"):
+            continue
+        t0 = time.perf_counter()
+        orc = O.Simulation()
+        scene.apply(orc)
+        orc.set_order(True, 2)
+        scene.set_initial_state(orc)
+        orc.set_threads(oracle_threads())
+        if mode == "learned":
+            orc.set_learned_warm_start(leg["net"], *autotune._scales(leg["scale"]))
+        elif mode != "none":
+            orc.set_warm_start(mode, *autotune._scales(leg["scale"]))
+        xs = {}
+        for f in range(1, frames + 1):
+            orc.step(scene.dt, k, scene.substeps)
+            if f in check:
+                xs[f] = positions(orc)
+        orc_x[name] = xs
+        del orc
+        log(f"accuracy: {name} ({k} iterations) on the device and in the oracle, {frames} frames; oracle "
+            f"{time.perf_counter() - t0:.1f}s on {oracle_threads()} threads")
+
+    def err(a, b):
+        e = float(np.abs(a - b).max()) / diag
+        return e if np.isfinite(e) else None
+
+    out = {}
+    for name in dev_x:
+        if name in orc_x:
+            out[name] = {f"frame_{f}": err(dev_x[name][f], orc_x[name][f]) for f in check}
+        if name != "plain" and "plain" in orc_x:
+            out[f"{name}_vs_plain"] = {f"frame_{f}": err(dev_x[name][f], orc_x["plain"][f]) for f in check}
+    return out
 
 
 def _median_time(fn, reps):
@@ -236,6 +325,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="beam_1m", choices=["beam_1m", "beam_6m", "beam_540", "batch_512", "batch_512_large"])
     ap.add_argument("--no-learned", action="store_true", help="plain solver only")
+    ap.add_argument("--tune-frames", type=int, default=8, help="frames after warm-up the warm starts are tuned on")
+    ap.add_argument("--reps", type=int, default=5, help="repeats of the timed frames (median)")
+    ap.add_argument("--accuracy-frames", type=int, default=60,
+                    help="frames of the accuracy legs from rest (0: none); errors at frame 1 and the last")
+    ap.add_argument("--train-frames", type=int, default=None, help="captured frames of the network's training data")
+    ap.add_argument("--also", nargs="*", default=["beam_6m", "batch_512"],
+                    choices=["beam_1m", "beam_6m", "beam_540", "batch_512", "batch_512_large"],
+                    help="further workloads measured with the plain solver (no tuning)")
     ap.add_argument("--no-graphs", action="store_true", help="stream launches instead of hipGraph replays")
     ap.add_argument("--no-contact-fusion", action="store_true",
                     help="the first iteration's contacts in their own launch, not in the warm-start gather")
@@ -255,22 +352,24 @@ def main():
         print(json.dumps(out), flush=True)
         return
 
+    import numpy as np
+
     import sbs_amd as S
-    from sbs_amd import scenes
+    from sbs_amd import autotune, scenes
 
     if args.copy_calibration:
         copy_calibration()
 
-    scene = {"beam_1m": scenes.beam_1m, "beam_6m": scenes.beam_6m, "beam_540": scenes.beam_540,
-             "batch_512": scenes.batch_512, "batch_512_large": scenes.batch_512_large}[args.workload]()
+    make_scene = {"beam_1m": scenes.beam_1m, "beam_6m": scenes.beam_6m, "beam_540": scenes.beam_540,
+                  "batch_512": scenes.batch_512, "batch_512_large": scenes.batch_512_large}
+    scene = make_scene[args.workload]()
     log(f"workload {scene.name}: {scene.num_vertices} vertices, {scene.num_tets} tets")
     t0 = time.perf_counter()
     sim = scenes.build_device(scene)
     _, ncolors = sim.group_colors(0)
     log(f"committed in {time.perf_counter() - t0:.1f}s, {ncolors} colors")
     kind = {"green": S._lib.KERNEL_GREEN, "neohookean": S._lib.KERNEL_NEOHOOKEAN}[scene.groups[0].kind]
-
-    from sbs_amd import autotune
+    warmable = scene.groups[0].kind == "neohookean" and not args.no_learned
 
     sim.use_graphs(not args.no_graphs)
     sim.set_contact_fusion(not args.no_contact_fusion)
@@ -279,78 +378,90 @@ def main():
     for _ in range(args.warmup):
         sim.step(scene.dt, scene.iterations, scene.substeps)
     sim.synchronize()
-    sim.snapshot()
-    # the plain solver's frame residuals over the timed horizon from the snapshot: their mean is the
-    # residual a warm start must reach over the same frames, twice their largest is the fallback
-    # limit of a timed warm-started frame; the first eval_frames of them screen the tuner's trials
-    eval_frames = 8
-    plain_frames = autotune.frame_residuals(sim, scene, args.steps, scene.iterations)
-    sim.restore()
-    target = float(plain_frames[:eval_frames].mean())
-    horizon_target = float(plain_frames.mean())
-    limit = 2.0 * float(plain_frames.max())
-    log(f"warmup done ({args.warmup} frames); plain {scene.iterations}-iteration frame residual: mean {horizon_target:.4g} "
-        f"over {args.steps} frames (first {eval_frames}: {target:.4g}), largest {plain_frames.max():.4g}")
 
-    learned = None
-    if not args.no_learned:
-        # the previous-multiplier start first; the network then learns, from that configuration's own
-        # trajectory (or a 24-iteration one when it never reaches the target), the multipliers the
-        # iterations end with, and is tuned the same way
-        found, trials = autotune.tune_warm_start(sim, scene, None, target, frames=eval_frames, modes=("previous",),
-                                                 verify_frames=args.steps, verify_target=horizon_target, limit=limit,
-                                                 log=log)
-        prev = found["previous"]
-        cap = (prev["iterations"], prev["scale"]) if prev else (24, 0.5)
-        net = autotune.train_warm_start(sim, scene, iterations=cap[0], scale=cap[1], log=log)
-        found_l, trials_l = autotune.tune_warm_start(sim, scene, net, target, frames=eval_frames, modes=("learned",),
-                                                     verify_frames=args.steps, verify_target=horizon_target,
-                                                     limit=limit, log=log)
-        found.update(found_l)
-        # per (mode, iterations): the trial with the smallest mean residual
-        best_trials = {}
-        for t in trials + trials_l:
-            k = (t["mode"], t["iterations"])
-            if k not in best_trials or t["residual"] < best_trials[k]["residual"]:
-                best_trials[k] = t
-        learned = {"found": found, "best_trials": sorted(best_trials.values(), key=lambda t: (t["mode"], t["iterations"])),
-                   "num_trials": len(trials) + len(trials_l), "capture": {"iterations": cap[0], "scale": cap[1]}}
-        sim.restore()
+    # Tuning horizon: the tune_frames frames after warm-up (state S0). The plain solver's frame
+    # residuals over them screen the warm-start candidates, which then qualify per state on them.
+    sim.snapshot(0)
+    tune = args.tune_frames
+    plain_tune = autotune.frame_residuals(sim, scene, tune, scene.iterations)
+    sim.restore(0)
+    limit = 2.0 * float(plain_tune.max())
+    log(f"warm-up done ({args.warmup} frames); plain {scene.iterations}-iteration frame residual over the {tune} "
+        f"tuning frames: mean {plain_tune.mean():.4g}, largest {plain_tune.max():.4g}")
+    found, trials, net, train_info = {}, [], None, None
+    if warmable:
+        f_prev, t_prev = autotune.tune_warm_start(sim, scene, None, float(plain_tune.mean()), frames=tune,
+                                                  modes=("previous",), log=log)
+        found.update(f_prev)
+        trials += t_prev
+        net, train_info = autotune.train_warm_start(sim, scene, log=log, **autotune.training_config(args))
+        f_l, t_l = autotune.tune_warm_start(sim, scene, net, float(plain_tune.mean()), frames=tune,
+                                            modes=("learned",), log=log)
+        found.update(f_l)
+        trials += t_l
+    # the timed horizon starts where the plain solver leaves the tuning horizon (state S1)
+    sim.restore(0)
+    sim.set_warm_start("none")
+    for _ in range(tune):
+        sim.step(scene.dt, scene.iterations, scene.substeps)
+    sim.synchronize()
+    sim.snapshot(0)
 
-    log(f"timing {args.steps} frames, plain")
-    elapsed_plain = autotune.time_frames(sim, scene, args.steps, scene.iterations)
-    plain_sps = args.steps * scene.substeps / elapsed_plain
-    log(f"plain: {elapsed_plain:.3f}s -> {plain_sps:.1f} substeps/s")
-    # every warm-start configuration that reached the plain residual is timed with its per-frame
-    # residual check and plain fallback on; the fastest configuration overall is the result
-    best = {"solver": "plain", "substeps_per_s": plain_sps, "elapsed": elapsed_plain, "iterations": scene.iterations}
-    if learned:
-        learned["timed"] = {}
-        for mode, cfg in learned["found"].items():
-            if not cfg:
-                continue
-            sim.restore()
-            sim.set_warm_start(mode, net if mode == "learned" else None)
+    def configure(mode, cfg):
+        sim.restore(0)
+        sim.set_warm_start(mode, net if mode == "learned" else None)
+        if mode != "none":
             sim.set_warm_start_scale(*autotune._scales(cfg["scale"]))
             sim.set_fallback(limit, scene.iterations)
-            sim.step(scene.dt, cfg["iterations"], scene.substeps)  # graph capture
-            sim.restore()
-            f0 = sim.fallback_count()
-            el = autotune.time_frames(sim, scene, args.steps, cfg["iterations"])
-            sps = args.steps * scene.substeps / el
-            nfb = sim.fallback_count() - f0
-            learned["timed"][mode] = {"substeps_per_s": round(sps, 2), "iterations": cfg["iterations"],
-                                      "scale": cfg["scale"], "fallback_frames": nfb, "fallback_limit": limit,
-                                      "speedup_vs_plain": round(sps / plain_sps, 3)}
-            log(f"{mode} warm start: {el:.3f}s -> {sps:.1f} substeps/s ({sps / plain_sps:.2f}x plain), "
-                f"{nfb} fallback frames")
-            if sps > best["substeps_per_s"]:
-                best = {"solver": mode, "substeps_per_s": sps, "elapsed": el, "iterations": cfg["iterations"]}
+        else:
             sim.set_fallback(0.0)
-            sim.set_warm_start("none")
-    sim.restore()
-    substeps_per_s, elapsed, solver, iterations = best["substeps_per_s"], best["elapsed"], best["solver"], best["iterations"]
-    log(f"fastest at the plain residual: {solver} ({substeps_per_s:.1f} substeps/s)")
+
+    legs = {"plain": {"iterations": scene.iterations}}
+    for mode in ("previous", "learned"):
+        if found.get(mode):
+            legs[mode] = dict(found[mode])
+    # timing: every leg from S1, `reps` repeats of the --steps timed frames, median
+    for mode, cfg in legs.items():
+        configure(mode, cfg)
+        sim.step(scene.dt, cfg["iterations"], scene.substeps)  # graph capture
+        f0 = sim.fallback_count()
+        times = []
+        for _ in range(args.reps):
+            configure(mode, cfg)
+            times.append(autotune.time_frames(sim, scene, args.steps, cfg["iterations"]))
+        med = sorted(times)[len(times) // 2]
+        cfg.update({"substeps_per_s": round(args.steps * scene.substeps / med, 2), "seconds": times,
+                    "fallback_frames": sim.fallback_count() - f0})
+        log(f"{mode}: median of {args.reps} x {args.steps} frames {1e3 * med:.2f} ms -> {cfg['substeps_per_s']:.1f} "
+            f"substeps/s, {cfg['fallback_frames']} fallback frames")
+    plain_sps = legs["plain"]["substeps_per_s"]
+    # equal residual on the timed frames, per state: along each warm-started trajectory from S1,
+    # every frame's residual against the plain solver's from the same state
+    for mode, cfg in legs.items():
+        if mode == "plain":
+            continue
+        configure(mode, cfg)
+        sim.set_fallback(0.0)
+        rw, rp = autotune.per_state_ratios(sim, scene, args.steps, cfg["iterations"])
+        ratio = rw / rp
+        cfg["timed_frames_per_state"] = {
+            "ratio_max": float(np.nanmax(ratio)), "ratio_median": float(np.nanmedian(ratio)),
+            "ratio_min": float(np.nanmin(ratio)), "frames_above_1": int(np.sum(~(ratio <= 1.0))),
+            "frames": int(len(ratio))}
+        cfg["speedup_vs_plain"] = round(cfg["substeps_per_s"] / plain_sps, 3)
+        log(f"{mode} per state over the {args.steps} timed frames: ratio max {np.nanmax(ratio):.4g}, median "
+            f"{np.nanmedian(ratio):.4g}, {cfg['timed_frames_per_state']['frames_above_1']} frames above 1")
+    sim.set_warm_start("none")
+    sim.set_fallback(0.0)
+    sim.restore(0)
+    if "learned" in legs:
+        solver, head = "learned", legs["learned"]
+    else:
+        solver, head = "plain", legs["plain"]
+    substeps_per_s, iterations = head["substeps_per_s"], head["iterations"]
+    log(f"value: {solver} ({substeps_per_s:.1f} substeps/s, {substeps_per_s / plain_sps:.2f}x plain)")
+    overflow = sim.contact_overflow()
+    assert overflow == 0, f"{overflow} contacts dropped"
 
     # instrumented region: events around every launch, stream launches (no graph)
     sim.use_graphs(False)
@@ -374,8 +485,8 @@ def main():
     achieved = bytes_per_launch / avg_launch_s / 1e9
     log(f"projection kernel: {launches} launches, avg {1e6 * avg_launch_s:.2f} us, {units / launches:.0f} tets/launch, "
         f"{achieved:.1f} GB/s algorithmic")
-    # residual after the last substep (FP64), a correctness anchor for later comparisons
     res = float(sum(sim.residual(0, scene.dt / scene.substeps)) ** 0.5)
+    sim.close()
 
     # HBM bytes per launch measured by the rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs recorded
     # under profiles/ (scaled by a device copy of known size); null for other workloads
@@ -385,10 +496,27 @@ def main():
     if os.path.exists(tpath):
         traffic = json.load(open(tpath))["bytes_per_launch"]
 
-    cpu = None
+    cpu, accuracy = None, None
     if not args.no_cpu_baseline:
+        # the oracle: accuracy of the measured solvers from rest (the checker), then the baseline
+        if args.accuracy_frames:
+            acc_legs = [{"name": "plain", "iterations": scene.iterations, "mode": "none"}]
+            for mode in ("previous", "learned"):
+                if mode in legs:
+                    acc_legs.append({"name": mode, "iterations": legs[mode]["iterations"], "mode": mode,
+                                     "scale": legs[mode]["scale"], "net": net})
+            accuracy = accuracy_legs(scene, acc_legs, launch, frames=args.accuracy_frames,
+                                     check=(1, args.accuracy_frames))
+            accuracy["definition"] = ("max over vertices of |x_device - x_oracle|, divided by the rest bounding-box "
+                                      "diagonal, from rest; <leg>: the oracle runs the same solver (colored order, "
+                                      "same warm start); <leg>_vs_plain: against the plain FP64 solver; no fallback")
+            log(f"accuracy {accuracy}")
         cpu = cpu_baseline(scene, scenes.batch(16) if args.workload in ("batch_512", "batch_512_large") else None)
         log(f"cpu baseline {cpu['value']:.3f} substeps/s")
+
+    others = {}
+    for w in args.also:
+        others[w] = plain_throughput(make_scene[w](), kind, args)
 
     out = {
         "metric": METRIC,
@@ -397,7 +525,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "ms_per_step": round(1e3 * scene.substeps / substeps_per_s, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -406,9 +534,8 @@ def main():
         "config": {
             "workload": scene.name,
             "solver": {"learned": "learned multiplier warm start + colored Gauss-Seidel XPBD",
-                       "previous": "previous-multiplier warm start + colored Gauss-Seidel XPBD",
-                       "plain": "plain GPU XPBD (colored Gauss-Seidel, no warm start)"}[solver]
-                      + ", fastest configuration reaching the plain solver's mean frame residual",
+                       "plain": "plain GPU XPBD (colored Gauss-Seidel, no warm start): no learned configuration "
+                                "reached the plain residual per state"}[solver],
             "tets": scene.num_tets,
             "vertices": scene.num_vertices,
             "substeps_per_step": scene.substeps,
@@ -419,16 +546,23 @@ def main():
             "obstacles": f"ground plane + {len(scene.spheres)} spheres",
             "launch": "hipGraph per frame",
             "parallelism": "single GPU",
+            "positions": "fp64 storage, fp32 constraint arithmetic, fp64 residual norms",
         },
         "constraint_projections_per_s": round(substeps_per_s * iterations * scene.num_tets, 1),
-        "plain_substeps_per_s": round(plain_sps, 2),
-        "residual_target": horizon_target,
-        "residual_definition": (f"mean FP64 frame residual sqrt(sum h^2), h = C + alpha~ lambda after each frame's last "
-                                f"substep, over the {args.steps} timed frames from the same state; a warm-started "
-                                f"configuration qualifies when its mean is <= the plain solver's and no frame exceeds "
-                                f"twice the plain solver's largest (its per-frame fallback limit)"),
-        "plain_frame_residual_max": float(plain_frames.max()),
-        "learned": learned,
+        "plain_substeps_per_s": plain_sps,
+        "speedup_vs_plain": round(substeps_per_s / plain_sps, 3),
+        "legs": legs if warmable else None,
+        "residual_definition": (
+            f"FP64 frame residual sqrt(sum h^2), h = C + alpha~ lambda, after a frame's last substep. Tuning on the "
+            f"{args.tune_frames} frames after the {args.warmup} warm-up frames: a candidate qualifies when on each of "
+            f"them its residual is at most the plain {scene.iterations}-iteration solver's from the same state. "
+            f"Timing on the next {args.steps} frames (disjoint), median of {args.reps} repeats, fallback check on "
+            f"(limit {limit:.4g} = twice the largest plain tuning frame residual); timed_frames_per_state: the same "
+            f"per-state comparison over the timed frames"),
+        "training": train_info,
+        "num_trials": len(trials),
+        "contact_overflow": overflow,
+        "accuracy": accuracy,
         "residual_l2_last_substep": res,
         "roofline": {
             "bound": "hbm",
@@ -443,9 +577,48 @@ def main():
             "avg_launch_us": round(1e6 * avg_launch_s, 3),
         },
         "kernels": per_kernel,
+        "other_workloads": others or None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def plain_throughput(scene, kind, args, warmup=3, frames=5, reps=3):
+    """The plain solver on another workload (no tuning): median of `reps` x `frames` graph-replayed
+    frames after `warmup`, and the projection kernel's average launch from an instrumented frame."""
+    import sbs_amd as S
+    from sbs_amd import autotune, scenes
+
+    t0 = time.perf_counter()
+    sim = scenes.build_device(scene)
+    build = time.perf_counter() - t0
+    sim.use_graphs(True)
+    launch = autotune.tune_launch(sim, scene, frames=2)
+    for _ in range(warmup):
+        sim.step(scene.dt, scene.iterations, scene.substeps)
+    sim.snapshot(0)
+    times = []
+    for _ in range(reps):
+        sim.restore(0)
+        times.append(autotune.time_frames(sim, scene, frames, scene.iterations))
+    med = sorted(times)[len(times) // 2]
+    sim.use_graphs(False)
+    sim.set_timing(True)
+    sim.reset_stats()
+    sim.step(scene.dt, scene.iterations, scene.substeps)
+    launches, ms, units = sim.kernel_stats(kind)
+    sim.set_timing(False)
+    overflow = sim.contact_overflow()
+    sim.close()
+    avg = ms / 1e3 / launches
+    gbs = BYTES_PER_TET * units / launches / avg / 1e9
+    sps = frames * scene.substeps / med
+    log(f"{scene.name}: plain {sps:.1f} substeps/s, projection {1e6 * avg:.2f} us per launch ({gbs:.0f} GB/s), "
+        f"built in {build:.1f}s")
+    return {"workload": scene.name, "tets": scene.num_tets, "plain_substeps_per_s": round(sps, 2),
+            "launch_config": launch, "projection_avg_launch_us": round(1e6 * avg, 3),
+            "roofline_frac": round(gbs / HBM_PEAK_GBS, 4), "achieved_GBps": round(gbs, 1),
+            "contact_overflow": overflow, "frames": f"median of {reps} x {frames} frames after {warmup}"}
 
 
 if __name__ == "__main__":

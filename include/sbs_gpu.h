@@ -37,6 +37,11 @@ extern "C" {
 #define SBS_ERR_INTERNAL 6
 #define SBS_ERR_IO 7     /* a file could not be opened, read or written */
 #define SBS_ERR_FORMAT 8 /* a file is not a well-formed PLY file */
+/* A frame dropped contacts because a surface vertex had more than 8: the state no longer follows
+ * the reference's contact set (which keeps every one). Reported by sbs_gpu_step (for frames the
+ * device finished before the call) and by sbs_gpu_synchronize; sticky until the handle is
+ * destroyed. */
+#define SBS_ERR_CONTACT_OVERFLOW 9
 
 typedef struct sbs_gpu sbs_gpu;
 
@@ -116,12 +121,13 @@ int sbs_gpu_add_neohookean_constraints_batched(sbs_gpu* sim, const uint32_t* bod
 int sbs_gpu_add_distance_constraints(sbs_gpu* sim, const uint32_t* b1_v1_b2_v2, uint32_t n, double alpha,
                                      double beta, uint32_t* group_id);
 
-/* sdf_model_t::from_plane(Hyperplane(normal, point), volume) (src/physics/collision/sdf_model.cpp:181-193).
+/* sdf_model_t::from_plane(Hyperplane(normal, point), volume) (src/physics/collision/sdf_model.cpp:52-64).
  * vol_min/vol_max is the SDF's englobing volume; a penetrating vertex outside it gets no contact. */
 int sbs_gpu_add_plane(sbs_gpu* sim, const double normal[3], const double point[3], const double vol_min[3],
                       const double vol_max[3]);
 
-/* Analytic sphere obstacle, sd = |p - c| - r (an analytic sdf_model_t, sdf_model.cpp:149-153). */
+/* Analytic sphere obstacle, sd = |p - c| - r (an analytic sdf_model_t: the constructor from an analytic
+ * function and a volume, sdf_model.cpp:20-24). */
 int sbs_gpu_add_sphere(sbs_gpu* sim, const double center[3], double radius);
 
 /* Grid SDF obstacles: sdf_model_t over a Discregrid::CubicLagrangeDiscreteGrid (sdf_model.cpp:17,
@@ -166,7 +172,8 @@ int sbs_gpu_write_surface_ply(sbs_gpu* sim, uint32_t body, const char* path, int
  * either every SDF (brute force) or, with the spatial hash, the SDFs whose volumes overlap its grid
  * cell plus every SDF without a finite volume. Both give the same contacts in the same order (SDF
  * order per vertex). At most 8 contacts are kept per surface vertex; more are counted
- * (sbs_gpu_contact_overflow). */
+ * (sbs_gpu_contact_overflow), and once any has been dropped sbs_gpu_step and sbs_gpu_synchronize
+ * fail with SBS_ERR_CONTACT_OVERFLOW. */
 #define SBS_CONTACT_HASH_AUTO 0 /* hash from 32 obstacles with finite volume on (default) */
 #define SBS_CONTACT_HASH_ON 1
 #define SBS_CONTACT_HASH_OFF 2
@@ -336,9 +343,16 @@ int sbs_gpu_set_warm_start_scales(sbs_gpu* sim, double deviatoric, double hydros
  * constraint of each neo-Hookean group to net's dataset, until its capacity is reached. Graphs
  * are not used while capturing. stride 0 stops. */
 int sbs_gpu_capture(sbs_gpu* sim, sbs_mlp* net, uint32_t stride);
-/* Device copy of the dynamic state (positions, velocities, multipliers) and its restoration. */
+/* Device copy of the dynamic state (positions, velocities, normals, multipliers and their warm-start
+ * history) and its restoration, in slot 0. No reference counterpart (the reference keeps no copies
+ * of its state); the tuner and the bench compare solvers from identical states with them. History
+ * buffers created after a snapshot was taken are restored as zeros, their contents at that time. */
 int sbs_gpu_snapshot(sbs_gpu* sim);
 int sbs_gpu_restore(sbs_gpu* sim);
+/* The same in slot `slot` (0 <= slot < SBS_SNAPSHOT_SLOTS); slots are independent. */
+#define SBS_SNAPSHOT_SLOTS 2
+int sbs_gpu_snapshot_slot(sbs_gpu* sim, int slot);
+int sbs_gpu_restore_slot(sbs_gpu* sim, int slot);
 /* With a warm start and limit > 0, each sbs_gpu_step copies the state it needs to repeat the frame
  * (its own copy: a snapshot taken with sbs_gpu_snapshot is kept) and, when the FP64 residual norm
  * sqrt(sum over neo-Hookean groups of sum h^2) after the frame exceeds limit or is not a number (a

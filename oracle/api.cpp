@@ -267,6 +267,8 @@ void orc_set_clusters(void* h, uint32_t group, const uint32_t* offsets, uint32_t
 }
 
 void orc_set_coloring(void* h, int order) { static_cast<simulation_t*>(h)->coloring = order; }
+// Threads of the colored order's stages (1: sequential); the result is the same bit for bit.
+void orc_set_threads(void* h, int threads) { static_cast<simulation_t*>(h)->threads = threads < 1 ? 1 : threads; }
 void orc_set_warm_start(void* h, int mode, double scale, double scale_h)
 {
     static_cast<simulation_t*>(h)->warm_start = mode;

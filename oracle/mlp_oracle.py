@@ -4,7 +4,7 @@ pass, Adam) that the GPU kernels implement.
 TEST INFRASTRUCTURE ONLY: the checker of the network kernels in tests/. The reference simulator
 has no learned component, so there is no reference file to follow; the restatement is pinned by
 the mathematics instead: its gradients are checked against central finite differences of its own
-FP64 loss (tests/test_oracle_mlp.py), and its BF16 rounding points are the ones the kernels use:
+FP64 loss (tests/test_oracle_warm.py), and its BF16 rounding points are the ones the kernels use:
 inputs after normalization, weights, hidden activations after bias + ReLU, and the backward
 signals dY and dZ are rounded to BF16 (round to nearest even); every sum is exact (FP64) here and
 FP32 on the device.

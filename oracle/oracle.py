@@ -38,6 +38,7 @@ _SIGS = {
     "orc_add_plane": (None, [_p, _pd, _pd, _pd, _pd]),
     "orc_add_sphere": (None, [_p, _pd, _d]),
     "orc_set_order": (None, [_p, _i, _u32]),
+    "orc_set_threads": (None, [_p, _i]),
     "orc_set_clusters": (None, [_p, _u32, _pu, _u32]),
     "orc_set_coloring": (None, [_p, _i]),
     "orc_set_warm_start": (None, [_p, _i, _d, _d]),
@@ -277,6 +278,10 @@ class Simulation:
 
     def set_order(self, colored, recolor_passes=2):
         self._L.orc_set_order(self.h, int(colored), recolor_passes)
+
+    def set_threads(self, threads):
+        """Threads for the colored order's color stages (bit-identical to one thread)."""
+        self._L.orc_set_threads(self.h, int(threads))
 
     def group_colors(self, group, n):
         c = np.zeros(n, dtype=np.uint32)

@@ -216,7 +216,7 @@ void neohookean_constraint_t::project_positions(simulation_t& sim, double dt)
     }
 }
 
-void neohookean_constraint_t::add_multiplier_displacement(simulation_t const& sim, std::vector<std::vector<vec3>>& d)
+void neohookean_constraint_t::multiplier_displacement(simulation_t const& sim, vec3 d[4])
 {
     auto const& P = sim.particles[bi];
     particle_t const* pp[4] = {&P[v[0]], &P[v[1]], &P[v[2]], &P[v[3]]};
@@ -240,7 +240,7 @@ void neohookean_constraint_t::add_multiplier_displacement(simulation_t const& si
     lagrange *= sim.warm_scale;
     lagrange_h *= sim.warm_scale_h;
     double const ld = wd < 1e-20 ? 0.0 : lagrange, lh = wh < 1e-20 ? 0.0 : lagrange_h;
-    for (int k = 0; k < 4; ++k) d[bi][v[k]] += w[k] * (gd[k] * ld + gh[k] * lh);
+    for (int k = 0; k < 4; ++k) d[k] = w[k] * (gd[k] * ld + gh[k] * lh);
 }
 
 void neohookean_constraint_t::residual_sq(simulation_t const& sim, double dt, double* acc) const
@@ -368,18 +368,29 @@ void simulation_t::detect_contacts()
 void simulation_t::build_order()
 {
     order.clear();
+    stage_off.assign(1, 0);
+    stage_elems.clear();
     if (!colored_order) {
         for (uint32_t i = 0; i < constraints.size(); ++i) order.push_back(i);
     } else {
         for (auto const& g : groups) {
             std::size_t const ne = g.cluster_off.empty() ? g.count : g.cluster_off.size() - 1;
-            for (uint32_t c = 0; c < g.num_colors; ++c)
-                for (std::size_t e = 0; e < ne; ++e) {
+            // elements of each color in index order (one pass over the elements)
+            std::vector<std::vector<uint32_t>> by_color(g.num_colors);
+            for (std::size_t e = 0; e < ne; ++e) {
+                std::size_t const b = g.cluster_off.empty() ? e : g.cluster_off[e];
+                by_color[g.color[b]].push_back(uint32_t(e));
+            }
+            for (uint32_t c = 0; c < g.num_colors; ++c) {
+                for (uint32_t e : by_color[c]) {
                     std::size_t const b = g.cluster_off.empty() ? e : g.cluster_off[e];
                     std::size_t const en = g.cluster_off.empty() ? e + 1 : g.cluster_off[e + 1];
-                    if (g.color[b] != c) continue;
+                    uint32_t const at = uint32_t(order.size());
                     for (std::size_t i = b; i < en; ++i) order.push_back(uint32_t(g.first + i));
+                    stage_elems.emplace_back(at, uint32_t(order.size()));
                 }
+                stage_off.push_back(stage_elems.size());
+            }
         }
     }
     order_dirty = false;
@@ -392,8 +403,16 @@ void simulation_t::apply_warm_start()
 {
     std::vector<std::vector<vec3>> d(particles.size());
     for (std::size_t b = 0; b < particles.size(); ++b) d[b].assign(particles[b].size(), vec3{});
+    // every constraint's displacement (independent: gradients at the predicted positions, each its
+    // own multipliers), then the sums per particle in constraint order
+    std::vector<neohookean_constraint_t*> nh;
     for (auto const& c : constraints)
-        if (auto* nh = dynamic_cast<neohookean_constraint_t*>(c.get())) nh->add_multiplier_displacement(*this, d);
+        if (auto* p = dynamic_cast<neohookean_constraint_t*>(c.get())) nh.push_back(p);
+    std::vector<vec3> dd(nh.size() * 4);
+#pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1)
+    for (long i = 0; i < long(nh.size()); ++i) nh[i]->multiplier_displacement(*this, &dd[4 * std::size_t(i)]);
+    for (std::size_t i = 0; i < nh.size(); ++i)
+        for (int k = 0; k < 4; ++k) d[nh[i]->bi][nh[i]->v[k]] += dd[4 * i + k];
     for (std::size_t b = 0; b < particles.size(); ++b)
         for (std::size_t i = 0; i < particles[b].size(); ++i) particles[b][i].xi += d[b][i];
 }
@@ -406,9 +425,20 @@ void simulation_t::solve(double dt, std::size_t iterations)
         if (!(warm_start && dynamic_cast<neohookean_constraint_t*>(c.get()))) c->prepare_for_projection();
     warm_dt = dt;
     if (warm_start) apply_warm_start();
+    bool const parallel = threads > 1 && colored_order;
     for (std::size_t k = 0; k < iterations; ++k) {
         for (auto& c : collision_constraints) c->project_positions(*this, dt);
-        for (uint32_t i : order) constraints[i]->project_positions(*this, dt);
+        if (!parallel) {
+            for (uint32_t i : order) constraints[i]->project_positions(*this, dt);
+            continue;
+        }
+        for (std::size_t st = 0; st + 1 < stage_off.size(); ++st) {
+            long const a = long(stage_off[st]), b = long(stage_off[st + 1]);
+#pragma omp parallel for schedule(static) num_threads(threads)
+            for (long e = a; e < b; ++e)
+                for (uint32_t j = stage_elems[e].first; j < stage_elems[e].second; ++j)
+                    constraints[order[j]]->project_positions(*this, dt);
+        }
     }
 }
 

@@ -13,7 +13,7 @@
 //   collision_constraint_t      src/physics/xpbd/collision_constraint.cpp:9-48
 //   gauss_seidel_solver_t       src/physics/gauss_seidel_solver.cpp:8-37
 //   timestep_t::step            src/physics/timestep.cpp:20-70
-//   contact generation          src/physics/collision/bvh_model.cpp:66-95, sdf_model.cpp:181-204,
+//   contact generation          src/physics/collision/bvh_model.cpp:66-95, sdf_model.cpp:52-75,
 //                               src/physics/xpbd/contact_handler.cpp:14-54
 // and adds the components the reference lacks (stable neo-Hookean constraint, colored visiting
 // order, sphere SDFs, residual norms, energies), each documented where it is defined.
@@ -77,11 +77,11 @@ struct neohookean_constraint_t final : constraint_t {
                             double young, double poisson);
     void prepare_impl() override { lagrange_h = 0.0; }
     void project_positions(simulation_t& sim, double dt) override;
-    // Warm start: adds M^-1 (grad C_dev lambda + grad C_hyd lambda_h) at the current positions to
-    // d[body][vertex]; multipliers unchanged.
+    // Warm start: d[k] = M^-1 (grad_k C_dev lambda + grad_k C_hyd lambda_h) at the current positions
+    // for the tet's vertices k.
     // With simulation_t::warm_start == 3 the multipliers are first set to -C / alpha~; then they are
     // multiplied by simulation_t::warm_scale (deviatoric) and warm_scale_h (hydrostatic).
-    void add_multiplier_displacement(simulation_t const& sim, std::vector<std::vector<vec3>>& d);
+    void multiplier_displacement(simulation_t const& sim, vec3 d[4]);
     void residual_sq(simulation_t const& sim, double dt, double* acc) const override;
     double energy(simulation_t const& sim) const override;
 };
@@ -106,7 +106,7 @@ struct collision_constraint_t final : constraint_t {
 };
 
 // Analytic SDF obstacles. Plane: Eigen::Hyperplane(n, point) semantics used by
-// sdf_model_t::from_plane (sdf_model.cpp:181-193): sd = n.p - n.point, grad = n.
+// sdf_model_t::from_plane (sdf_model.cpp:52-64): sd = n.p - n.point, grad = n.
 // Sphere: sd = |p - c| - r, grad = (p - c)/|p - c| (no reference counterpart).
 // Grid: sdf_model_t(Discregrid::CubicLagrangeDiscreteGrid) (sdf_model.cpp:17, evaluate :70-74).
 struct sdf_t {
@@ -168,6 +168,12 @@ struct simulation_t {
 
     // GS visiting order of the elastic constraints (indices into constraints)
     std::vector<uint32_t> order;
+    // colored order only: the order split into stages (one per group color); a stage's elements
+    // (stage_elems[stage_off[k] .. stage_off[k+1]), each a range [first, last) of `order`) share no
+    // particle, so with threads > 1 they run in parallel and give the sequential order's bits
+    std::vector<std::size_t> stage_off;
+    std::vector<std::pair<uint32_t, uint32_t>> stage_elems;
+    int threads = 1;
     bool order_dirty = true;
     void build_order();
 

@@ -129,6 +129,9 @@ void launch_neohookean_hex_residual(hipStream_t s, tet_group_dev g, cluster_dev 
                                     const uint4* cinfo, uint32_t nrows, float inv_dt2, double* partial);
 // out[0] = the sum of partial[0, n) by one block in a fixed order (out may be pinned host memory).
 void launch_sum_partials(hipStream_t s, const double* partial, uint32_t n, double* out);
+// *dst = *src by one thread (dst may be pinned host memory): publishes a device counter to the host
+// without a copy node in a frame graph.
+void launch_publish_u32(hipStream_t s, const uint32_t* src, uint32_t* dst);
 
 void launch_neohookean_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, const pos4* xn,
                            uint32_t begin, uint32_t count, uint32_t tbase, xcd_split const& sp, float inv_dt2,

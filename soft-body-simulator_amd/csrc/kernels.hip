@@ -1448,6 +1448,16 @@ void launch_sum_partials(hipStream_t s, const double* partial, uint32_t n, doubl
     k_sum_partials<<<1, 256, 0, s>>>(partial, n, out);
 }
 
+__global__ void k_publish_u32(const uint32_t* __restrict__ src, uint32_t* dst)
+{
+    if (threadIdx.x == 0) *reinterpret_cast<volatile uint32_t*>(dst) = *src;
+}
+
+void launch_publish_u32(hipStream_t s, const uint32_t* src, uint32_t* dst)
+{
+    k_publish_u32<<<1, 64, 0, s>>>(src, dst);
+}
+
 void launch_distance_residual(hipStream_t s, distance_group_dev g, const pos4* x, uint32_t begin, uint32_t count,
                               float alpha_tilde, double* partial, int)
 {

@@ -199,6 +199,21 @@ int vertex_slot(std::string const& n)
     return -1;
 }
 
+// The arrays are sized from the last vertex element that names all of an array's properties
+// (size_arrays); a header may declare several vertex elements, so every element that writes into
+// an array must fit it: an element with more items than the array holds is a format error.
+void check_vertex_fit(ply_element const& el, std::vector<int> const& slot, geometry const& g)
+{
+    for (int s : slot) {
+        if (s < 0) continue;
+        std::size_t const have = s < 3 ? g.positions.size() / 3
+                                 : s < 6 ? g.normals.size() / 3
+                                 : s < 9 ? g.colors.size() / 3
+                                         : g.uvs.size() / 2;
+        if (el.count > have) format_error("vertex element '" + el.name + "' has more items than its arrays");
+    }
+}
+
 // std::stof on a token: the longest convertible prefix; an error when nothing converts or the
 // value overflows or underflows. Fast path: a plain decimal whose mantissa and power of ten make
 // the double quotient or product correctly rounded (mantissa < 2^53, |exponent| <= 22), converted
@@ -327,6 +342,7 @@ void read_ascii(const char* data, std::size_t n, ply_header const& h, geometry& 
                                              : !g.uvs.empty();
                 slot[j] = present ? s : -1;
             }
+        if (kind == 0) check_vertex_fit(el, slot, g);
         int const nl = kind == 1 ? 3 : 4;
         bool const has_list = kind > 0 && std::any_of(el.props.begin(), el.props.end(), [](ply_property const& pr) {
                                   return pr.name == "indices" && pr.is_list;
@@ -480,6 +496,7 @@ void read_binary(const char* data, std::size_t n, ply_header const& h, geometry&
                                              : !g.uvs.empty();
                 slot[j] = present ? s : -1;
             }
+        if (kind == 0) check_vertex_fit(el, slot, g);
         int const nl = kind == 1 ? 3 : 4;
         for (uint64_t i = 0; i < el.count; ++i) {
             for (std::size_t j = 0; j < el.props.size(); ++j) {

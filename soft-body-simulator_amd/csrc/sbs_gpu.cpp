@@ -76,8 +76,6 @@ struct group_t {
     dev_buf<uint4> ws_cinfo;
     uint32_t ws_rows = 0;
     std::vector<uint32_t> ws_rowof;  // cluster -> its row
-    dev_buf<float> snap_lambda;  // snapshot copies
-    dev_buf<float2> snap_hist;
     bool clustered() const { return !cluster_off.empty(); }
     uint32_t body_of(uint32_t c) const { return cbody.empty() ? body : cbody[c]; }
     double young_of(uint32_t c) const { return cyoung.empty() ? young : cyoung[c]; }
@@ -132,6 +130,7 @@ struct sbs_gpu {
     dev_buf<uint32_t> ccount;
     dev_buf<sdf_dev> dsdf;
     dev_buf<uint32_t> coverflow;  // contacts dropped for want of slots (cumulative)
+    uint32_t* overflow_host = nullptr;  // pinned: coverflow as of the last frame the device finished
     // spatial hash of the obstacles (contact broad phase)
     int hash_mode = SBS_CONTACT_HASH_AUTO;
     double hash_cell_size = 0.0;  // requested (0: automatic)
@@ -159,8 +158,18 @@ struct sbs_gpu {
     uint64_t st_units[SBS_KERNEL_COUNT] = {};
 
     bool use_graphs = false;
-    hipGraphExec_t graph_exec = nullptr;
-    std::array<double, 16> graph_key{};
+    // instantiated frame graphs by launch key, most recently used last: a checked warm-started frame
+    // and the plain frame that replaces a rejected one keep a graph each
+    static constexpr std::size_t kGraphCache = 4;
+    std::vector<std::pair<std::array<double, 16>, hipGraphExec_t>> graphs;
+    void drop_graphs()
+    {
+        if (graphs.empty()) return;
+        // a cached graph may still be running on the stream: let it finish first
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+        graphs.clear();
+    }
 
     int ws_mode = SBS_WARM_START_NONE;
     double ws_scale = 1.0;      // deviatoric multipliers' warm-start factor
@@ -168,10 +177,17 @@ struct sbs_gpu {
     sbs_mlp* ws_net = nullptr;
     sbs_mlp* cap_net = nullptr;
     uint32_t cap_stride = 0;
-    dev_buf<pos4> snap_xn;
-    dev_buf<float4> snap_vel;
-    dev_buf<double4> snap_normal;
-    bool have_snapshot = false;
+    // snapshot slots: copies of the dynamic state (committed positions, velocities, normals, and per
+    // group the multipliers and their warm-start history)
+    struct snap_slot {
+        dev_buf<pos4> xn;
+        dev_buf<float4> vel;
+        dev_buf<double4> normal;
+        std::vector<dev_buf<float>> lambda;
+        std::vector<dev_buf<float2>> hist;
+        bool taken = false;
+    };
+    std::array<snap_slot, SBS_SNAPSHOT_SLOTS> snaps;
     // the per-frame fallback check's own state copy (the public snapshot is left alone): committed
     // positions, velocities, normals, and in learned mode the multiplier history. The plain frame
     // that replaces a rejected one writes every multiplier before reading it, so they are not kept.
@@ -188,13 +204,14 @@ struct sbs_gpu {
     ~sbs_gpu()
     {
         if (stream) (void)hipStreamSynchronize(stream);
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
         for (auto& r : pending) {
             (void)hipEventDestroy(r.a);
             (void)hipEventDestroy(r.b);
         }
         for (auto e : event_pool) (void)hipEventDestroy(e);
         if (resid_host) (void)hipHostFree(resid_host);
+        if (overflow_host) (void)hipHostFree(overflow_host);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -250,6 +267,17 @@ struct sbs_gpu {
     void build_hash();
 
     void set_current() { hip_check(hipSetDevice(device), "hipSetDevice"); }
+    // SBS_ERR_CONTACT_OVERFLOW once a finished frame has dropped a contact (the pinned word is
+    // written by the frame after its detection; read without waiting)
+    void check_overflow() const
+    {
+        uint32_t const n = overflow_host ? *reinterpret_cast<volatile const uint32_t*>(overflow_host) : 0u;
+        if (n)
+            throw api_error(SBS_ERR_CONTACT_OVERFLOW,
+                            std::to_string(n) + " contacts were dropped because a surface vertex had more than " +
+                                std::to_string(kMaxContactSlots) +
+                                " (the state no longer follows the reference's contact set)");
+    }
 
     void commit(uint32_t recolor_passes);
     void color_group(group_t& g, uint32_t recolor_passes);
@@ -260,11 +288,11 @@ struct sbs_gpu {
     void launch_group_colors(group_t const& g, double dt, int mode, const pos4* xref);
     void ensure_warm_buffers(bool raw, bool X);
     void ensure_cinfo(group_t& g);
-    void snapshot();
-    void prepare_snapshot();
-    void enqueue_snapshot();
+    void snapshot(int slot);
+    void prepare_snapshot(int slot);
+    void enqueue_snapshot(int slot);
     void enqueue_check_snapshot();
-    void restore();
+    void restore(int slot);
     void restore_check();
     std::size_t residual_blocks() const;
     void prepare_check();
@@ -683,6 +711,10 @@ void sbs_gpu::commit(uint32_t recolor_passes)
         ccount.alloc(S);
         coverflow.alloc(1);
         hip_check(hipMemsetAsync(coverflow.p, 0, sizeof(uint32_t), stream), "clear overflow");
+        void* p = nullptr;
+        hip_check(hipHostMalloc(&p, sizeof(uint32_t), hipHostMallocDefault), "hipHostMalloc");
+        overflow_host = static_cast<uint32_t*>(p);
+        *overflow_host = 0;
         dsdf.upload(sdfs.data(), sdfs.size(), stream);
         build_hash();
     }
@@ -812,6 +844,7 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
     int const force = params.gravity_is_force;
     detect();  // timestep.cpp:29-30
     bool const have_contacts = S && slots;
+    if (have_contacts) launch_publish_u32(stream, coverflow.p, overflow_host);
     contact_dev const c = contacts();
     float const c_at = float(params.collision_compliance / dt2);
     bool const capture = cap_net != nullptr && cap_stride > 0;
@@ -1012,24 +1045,29 @@ void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
     }
 }
 
-// Buffers of the snapshot (allocated outside any graph capture).
-void sbs_gpu::prepare_snapshot()
+// Buffers of a snapshot slot (allocated outside any graph capture).
+void sbs_gpu::prepare_snapshot(int slot)
 {
-    if (!snap_xn.p) {
-        snap_xn.alloc(V);
-        snap_vel.alloc(V);
+    snap_slot& sn = snaps[slot];
+    if (!sn.xn.p) {
+        sn.xn.alloc(V);
+        sn.vel.alloc(V);
     }
-    if (snormal.p && !snap_normal.p) snap_normal.alloc(S);
-    for (auto& g : groups) {
-        if (g.lambda.n && !g.snap_lambda.p) g.snap_lambda.alloc(g.lambda.n);
-        if (g.ws_hist.p && !g.snap_hist.p) g.snap_hist.alloc(g.ws_hist.n);
+    if (snormal.p && !sn.normal.p) sn.normal.alloc(S);
+    sn.lambda.resize(groups.size());
+    sn.hist.resize(groups.size());
+    for (std::size_t i = 0; i < groups.size(); ++i) {
+        group_t const& g = groups[i];
+        if (g.lambda.n && !sn.lambda[i].p) sn.lambda[i].alloc(g.lambda.n);
+        if (g.ws_hist.p && !sn.hist[i].p) sn.hist[i].alloc(g.ws_hist.n);
     }
 }
 
 // Copies of the dynamic state (committed positions, velocities, normals, multipliers and their
 // history) in one launch.
-void sbs_gpu::enqueue_snapshot()
+void sbs_gpu::enqueue_snapshot(int slot)
 {
+    snap_slot& sn = snaps[slot];
     copy_spans sp{};
     auto add = [&](void* d, const void* src, std::size_t bytes) {
         if (!bytes) return;
@@ -1042,25 +1080,27 @@ void sbs_gpu::enqueue_snapshot()
         sp.bytes[sp.n] = bytes;
         ++sp.n;
     };
-    require(snap_xn.p && snap_vel.p && (!snormal.p || snap_normal.p), "snapshot buffers not prepared", SBS_ERR_STATE);
-    for (auto const& g : groups)
-        require((!g.lambda.n || g.snap_lambda.p) && (!g.ws_hist.p || g.snap_hist.p), "snapshot buffers not prepared",
-                SBS_ERR_STATE);
-    add(snap_xn.p, xn.p, V * sizeof(pos4));
-    add(snap_vel.p, vel.p, V * sizeof(float4));
-    if (snormal.p) add(snap_normal.p, snormal.p, S * sizeof(double4));
-    for (auto& g : groups) {
-        if (g.lambda.n) add(g.snap_lambda.p, g.lambda.p, g.lambda.n * sizeof(float));
-        if (g.ws_hist.p) add(g.snap_hist.p, g.ws_hist.p, g.ws_hist.n * sizeof(float2));
+    require(sn.xn.p && sn.vel.p && (!snormal.p || sn.normal.p) && sn.lambda.size() == groups.size(),
+            "snapshot buffers not prepared", SBS_ERR_STATE);
+    for (std::size_t i = 0; i < groups.size(); ++i)
+        require((!groups[i].lambda.n || sn.lambda[i].p) && (!groups[i].ws_hist.p || sn.hist[i].p),
+                "snapshot buffers not prepared", SBS_ERR_STATE);
+    add(sn.xn.p, xn.p, V * sizeof(pos4));
+    add(sn.vel.p, vel.p, V * sizeof(float4));
+    if (snormal.p) add(sn.normal.p, snormal.p, S * sizeof(double4));
+    for (std::size_t i = 0; i < groups.size(); ++i) {
+        group_t const& g = groups[i];
+        if (g.lambda.n) add(sn.lambda[i].p, g.lambda.p, g.lambda.n * sizeof(float));
+        if (g.ws_hist.p) add(sn.hist[i].p, g.ws_hist.p, g.ws_hist.n * sizeof(float2));
     }
     launch_copy_spans(stream, sp);
-    have_snapshot = true;
+    sn.taken = true;
 }
 
-void sbs_gpu::snapshot()
+void sbs_gpu::snapshot(int slot)
 {
-    prepare_snapshot();
-    enqueue_snapshot();
+    prepare_snapshot(slot);
+    enqueue_snapshot(slot);
 }
 
 void sbs_gpu::enqueue_check_snapshot()
@@ -1103,22 +1143,26 @@ void sbs_gpu::restore_check()
                                          hipMemcpyDeviceToDevice, stream), "restore");
 }
 
-void sbs_gpu::restore()
+void sbs_gpu::restore(int slot)
 {
-    require(have_snapshot, "no snapshot has been taken", SBS_ERR_STATE);
-    hip_check(hipMemcpyAsync(xn.p, snap_xn.p, V * sizeof(pos4), hipMemcpyDeviceToDevice, stream), "restore");
-    hip_check(hipMemcpyAsync(xi.p, snap_xn.p, V * sizeof(pos4), hipMemcpyDeviceToDevice, stream), "restore");
-    hip_check(hipMemcpyAsync(vel.p, snap_vel.p, V * sizeof(float4), hipMemcpyDeviceToDevice, stream), "restore");
-    if (snap_normal.p)
-        hip_check(hipMemcpyAsync(snormal.p, snap_normal.p, S * sizeof(double4), hipMemcpyDeviceToDevice, stream),
+    snap_slot const& sn = snaps[slot];
+    require(sn.taken, "no snapshot has been taken in this slot", SBS_ERR_STATE);
+    hip_check(hipMemcpyAsync(xn.p, sn.xn.p, V * sizeof(pos4), hipMemcpyDeviceToDevice, stream), "restore");
+    hip_check(hipMemcpyAsync(xi.p, sn.xn.p, V * sizeof(pos4), hipMemcpyDeviceToDevice, stream), "restore");
+    hip_check(hipMemcpyAsync(vel.p, sn.vel.p, V * sizeof(float4), hipMemcpyDeviceToDevice, stream), "restore");
+    if (sn.normal.p)
+        hip_check(hipMemcpyAsync(snormal.p, sn.normal.p, S * sizeof(double4), hipMemcpyDeviceToDevice, stream),
                   "restore");
-    for (auto& g : groups) {
-        if (g.snap_lambda.p)
-            hip_check(hipMemcpyAsync(g.lambda.p, g.snap_lambda.p, g.lambda.n * sizeof(float), hipMemcpyDeviceToDevice,
+    for (std::size_t i = 0; i < groups.size(); ++i) {
+        group_t& g = groups[i];
+        if (i < sn.lambda.size() && sn.lambda[i].p)
+            hip_check(hipMemcpyAsync(g.lambda.p, sn.lambda[i].p, g.lambda.n * sizeof(float), hipMemcpyDeviceToDevice,
                                      stream), "restore");
-        if (g.ws_hist.p && g.snap_hist.p)
-            hip_check(hipMemcpyAsync(g.ws_hist.p, g.snap_hist.p, g.ws_hist.n * sizeof(float2), hipMemcpyDeviceToDevice,
+        if (g.ws_hist.p && i < sn.hist.size() && sn.hist[i].p)
+            hip_check(hipMemcpyAsync(g.ws_hist.p, sn.hist[i].p, g.ws_hist.n * sizeof(float2), hipMemcpyDeviceToDevice,
                                      stream), "restore");
+        else if (g.ws_hist.p)  // created (as zeros) after the snapshot was taken: zeros at snapshot time
+            hip_check(hipMemsetAsync(g.ws_hist.p, 0, g.ws_hist.n * sizeof(float2), stream), "restore");
     }
 }
 
@@ -1149,11 +1193,7 @@ void sbs_gpu::prepare_check()
             if (groups[i].ws_hist.p && !chk_hist[i].p) chk_hist[i].alloc(groups[i].ws_hist.n);
     std::size_t const total = std::max<std::size_t>(residual_blocks(), 1);
     if (check_partial.n < total) {
-        if (graph_exec) {  // a cached frame graph may hold the old buffer
-            hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-            hip_check(hipGraphExecDestroy(graph_exec), "hipGraphExecDestroy");
-            graph_exec = nullptr;
-        }
+        drop_graphs();  // a cached frame graph may hold the old buffer
         check_partial.alloc(total);
     }
     if (!resid_host) {
@@ -1206,14 +1246,18 @@ void sbs_gpu::run_frame(double dt, uint32_t iterations, uint32_t substeps, int w
         key[13] = surface_refresh;
         key[14] = ws_scale_hyd;
         key[15] = check + 2 * fuse_first_contacts;
-        if (!graph_exec || key != graph_key) {
-            if (graph_exec) {
-                // the previous graph may still be running on the stream: let it finish first
+        auto hit = std::find_if(graphs.begin(), graphs.end(), [&](auto const& g) { return g.first == key; });
+        if (hit != graphs.end()) {
+            std::rotate(hit, hit + 1, graphs.end());  // most recently used last
+        } else {
+            if (graphs.size() == kGraphCache) {
+                // the least recently used graph may still be running on the stream
                 hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-                hip_check(hipGraphExecDestroy(graph_exec), "hipGraphExecDestroy");
-                graph_exec = nullptr;
+                (void)hipGraphExecDestroy(graphs.front().second);
+                graphs.erase(graphs.begin());
             }
             hipGraph_t graph;
+            hipGraphExec_t graph_exec = nullptr;
             hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
             try {
                 enqueue_frame(dt, iterations, substeps, warm, check);
@@ -1227,9 +1271,9 @@ void sbs_gpu::run_frame(double dt, uint32_t iterations, uint32_t substeps, int w
             hipError_t const e = hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
             hip_check(e, "hipGraphInstantiate");
-            graph_key = key;
+            graphs.emplace_back(key, graph_exec);
         }
-        hip_check(hipGraphLaunch(graph_exec, stream), "hipGraphLaunch");
+        hip_check(hipGraphLaunch(graphs.back().second, stream), "hipGraphLaunch");
     } else {
         enqueue_frame(dt, iterations, substeps, warm, check);
         hip_check(hipGetLastError(), "kernel launch");
@@ -1625,6 +1669,7 @@ int sbs_gpu_step(sbs_gpu* sim, double dt, uint32_t iterations, uint32_t substeps
         require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
         require(dt > 0.0 && std::isfinite(dt), "dt must be positive");
         require(substeps >= 1, "substeps must be >= 1");
+        sim->check_overflow();
         sim->set_current();
         bool const check = sim->fb_limit > 0.0 && sim->ws_mode != SBS_WARM_START_NONE;
         // with the check the frame itself snapshots the state and ends with its residual
@@ -1649,6 +1694,7 @@ int sbs_gpu_synchronize(sbs_gpu* sim)
     return guarded([&] {
         require(sim != nullptr, "sim is null");
         hip_check(hipStreamSynchronize(sim->stream), "hipStreamSynchronize");
+        sim->check_overflow();
     });
 }
 
@@ -2087,6 +2133,9 @@ int sbs_gpu_set_warm_start(sbs_gpu* sim, int mode, sbs_mlp* net)
             require(net->device == sim->device, "network and simulation are on different devices");
         }
         sim->set_current();
+        // a cached frame graph holds the network's buffers by address: a network created where a
+        // destroyed one lived must not replay the old graph
+        if (mode == SBS_WARM_START_LEARNED) sim->drop_graphs();
         sim->ensure_warm_buffers(false, mode == SBS_WARM_START_LEARNED);
         sim->ws_mode = mode;
         sim->ws_net = mode == SBS_WARM_START_LEARNED ? net : nullptr;
@@ -2138,23 +2187,29 @@ int sbs_gpu_set_warm_start_scales(sbs_gpu* sim, double deviatoric, double hydros
     });
 }
 
-int sbs_gpu_snapshot(sbs_gpu* sim)
+int sbs_gpu_snapshot(sbs_gpu* sim) { return sbs_gpu_snapshot_slot(sim, 0); }
+
+int sbs_gpu_restore(sbs_gpu* sim) { return sbs_gpu_restore_slot(sim, 0); }
+
+int sbs_gpu_snapshot_slot(sbs_gpu* sim, int slot)
 {
     return guarded([&] {
         require(sim != nullptr, "sim is null");
         require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
+        require(slot >= 0 && slot < SBS_SNAPSHOT_SLOTS, "snapshot slot out of range");
         sim->set_current();
-        sim->snapshot();
+        sim->snapshot(slot);
     });
 }
 
-int sbs_gpu_restore(sbs_gpu* sim)
+int sbs_gpu_restore_slot(sbs_gpu* sim, int slot)
 {
     return guarded([&] {
         require(sim != nullptr, "sim is null");
         require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
+        require(slot >= 0 && slot < SBS_SNAPSHOT_SLOTS, "snapshot slot out of range");
         sim->set_current();
-        sim->restore();
+        sim->restore(slot);
     });
 }
 

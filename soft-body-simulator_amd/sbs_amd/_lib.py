@@ -135,6 +135,8 @@ _SIGS = {
     "sbs_gpu_set_warm_start_scales": [_p, _d, _d],
     "sbs_gpu_snapshot": [_p],
     "sbs_gpu_restore": [_p],
+    "sbs_gpu_snapshot_slot": [_p, _i],
+    "sbs_gpu_restore_slot": [_p, _i],
     "sbs_gpu_set_fallback": [_p, _d, _u32],
     "sbs_gpu_fallback_count": [_p, _pu64],
     "sbs_mesh_bar5": [_u32, _u32, _u32, _pd, _pu],
@@ -601,9 +603,13 @@ class Simulation:
         check(self._L.sbs_gpu_energy(self.h, _dp(out)))
         return out
 
+    warm_mode = "none"
+    warm_net = None
+
     def set_warm_start(self, mode, net=None):
         """mode: "none", "previous", "explicit" or "learned" (needs net, an Mlp(13, 2))."""
         check(self._L.sbs_gpu_set_warm_start(self.h, WARM_START[mode], None if net is None else net.h))
+        self.warm_mode, self.warm_net = mode, net
 
     def set_warm_start_scale(self, scale, hydrostatic=None):
         """Warm-start factor of the multipliers; with `hydrostatic`, scale applies to the deviatoric
@@ -616,11 +622,11 @@ class Simulation:
     def capture(self, net, stride):
         check(self._L.sbs_gpu_capture(self.h, None if net is None else net.h, stride))
 
-    def snapshot(self):
-        check(self._L.sbs_gpu_snapshot(self.h))
+    def snapshot(self, slot=0):
+        check(self._L.sbs_gpu_snapshot_slot(self.h, slot))
 
-    def restore(self):
-        check(self._L.sbs_gpu_restore(self.h))
+    def restore(self, slot=0):
+        check(self._L.sbs_gpu_restore_slot(self.h, slot))
 
     def set_fallback(self, limit, iterations=0):
         """Residual limit of a warm-started frame, and the plain solver's iterations per substep

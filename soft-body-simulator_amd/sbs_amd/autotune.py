@@ -38,6 +38,27 @@ def mean_residual(sim, scene, frames, iterations):
     return float(r.mean()) if np.all(np.isfinite(r)) else math.inf
 
 
+def per_state_ratios(sim, scene, frames, iterations, plain_iterations=None, slot=1):
+    """Frame by frame from the current state, with the warm start as configured: the state X_f is
+    copied (snapshot slot `slot`), the plain solver runs one frame from it (plain_iterations, the
+    scene's count by default) and gives r_plain(X_f); X_f is restored and the warm-started solver runs
+    the frame (`iterations`) and gives r_warm(X_f), continuing the warm-started trajectory. Returns
+    (r_warm, r_plain) over the frames: equal residual on the same state means r_warm <= r_plain."""
+    plain_iterations = scene.iterations if plain_iterations is None else plain_iterations
+    mode = sim.warm_mode
+    rw, rp = [], []
+    for _ in range(frames):
+        sim.snapshot(slot)
+        sim.set_warm_start("none")
+        sim.step(scene.dt, plain_iterations, scene.substeps)
+        rp.append(frame_residual(sim, scene))
+        sim.restore(slot)
+        sim.set_warm_start(mode, sim.warm_net)
+        sim.step(scene.dt, iterations, scene.substeps)
+        rw.append(frame_residual(sim, scene))
+    return np.asarray(rw), np.asarray(rp)
+
+
 def time_frames(sim, scene, frames, iterations):
     sim.synchronize()
     t0 = time.perf_counter()
@@ -101,31 +122,38 @@ def train_warm_start(sim, scene, frames=6, iterations=24, scale=0.5, max_rows=4_
     net.fit_normalization()
     t0 = time.perf_counter()
     loss = net.train(steps, batch, lr, seed=seed + 1)
+    el = time.perf_counter() - t0
     log(f"warm-start network: {n} rows (stride {stride}), {steps} Adam steps of {batch} in "
-        f"{time.perf_counter() - t0:.1f}s, last loss {loss:.4f} (normalized MSE)")
-    return net
+        f"{el:.1f}s, last loss {loss:.4f} (normalized MSE)")
+    return net, {"rows": n, "stride": stride, "frames": frames, "steps": steps, "batch": batch, "train_s": round(el, 2),
+                 "last_loss": loss}
+
+
+def training_config(args):
+    """Keyword arguments of train_warm_start from the bench's command line."""
+    out = {}
+    if getattr(args, "train_frames", None):
+        out["frames"] = args.train_frames
+    return out
 
 
 def tune_warm_start(sim, scene, net, target, frames=8, scales=SCALE_GRID, iterations=(2, 3, 4, 5, 6, 7),
-                    modes=("learned", "previous"), verify_frames=32, verify_target=None, limit=None, screen_slack=1.15,
-                    log=print):
-    """For each mode, the cheapest (iterations, scale) whose frame residuals from the current state
-    reach the plain solver's: screened by the mean over `frames` frames (<= screen_slack x target,
-    the plain mean over those frames), then verified over `verify_frames` frames, where the mean
-    must be <= verify_target (the plain solver's mean over the same frames; default target) and no
-    frame may exceed `limit` (default 2 x verify_target; the bench passes twice the plain solver's
-    largest frame residual, its fallback limit). A scale is one factor or a (deviatoric,
-    hydrostatic) pair. Returns {mode: {"iterations", "scale", "residual", "horizon_mean",
-    "largest_frame_residual"} or None} and the trials, and leaves the simulation at the state it
-    started from, warm start off."""
-    verify_target = target if verify_target is None else verify_target
-    limit = 2.0 * verify_target if limit is None else limit
-    sim.snapshot()
+                    modes=("learned", "previous"), screen_slack=1.15, log=print):
+    """For each mode, the cheapest (iterations, scale) that reaches the plain solver's residual on the
+    same states over `frames` frames from the current state. Each candidate is screened by its mean
+    frame residual along its own trajectory (<= screen_slack x target, the plain solver's mean over
+    those frames); the ones that pass are qualified per state (per_state_ratios): every frame's
+    warm-started residual must be at most the plain solver's from the same state. Among the scales
+    that qualify at the smallest iteration count, the one with the smallest largest ratio is taken.
+    A scale is one factor or a (deviatoric, hydrostatic) pair. Returns {mode: {"iterations", "scale",
+    "screen_mean", "max_ratio", "mean_ratio"} or None} and the trials, and leaves the simulation at
+    the state it started from (snapshot slot 0), warm start off."""
+    sim.snapshot(0)
     found = {}
     trials = []
 
     def start(mode, s):
-        sim.restore()
+        sim.restore(0)
         sim.set_warm_start(mode, net if mode == "learned" else None)
         sim.set_warm_start_scale(*_scales(s))
 
@@ -142,17 +170,17 @@ def tune_warm_start(sim, scene, net, target, frames=8, scales=SCALE_GRID, iterat
                     passing.append((r, s))
             for r, s in sorted(passing, key=lambda p: p[0]):
                 start(mode, s)
-                h = frame_residuals(sim, scene, verify_frames, k)
-                ok = bool(np.all(np.isfinite(h))) and h.mean() <= verify_target and h.max() <= limit
-                log(f"  {mode} x{s} {k} iterations over {verify_frames} frames: mean {h.mean():.4g} "
-                    f"(plain {verify_target:.4g}), largest {h.max():.4g} (limit {limit:.4g}){'' if ok else ': rejected'}")
-                if ok:
-                    best = {"iterations": k, "scale": s, "residual": r, "horizon_mean": float(h.mean()),
-                            "largest_frame_residual": float(h.max())}
-                    break
+                rw, rp = per_state_ratios(sim, scene, frames, k)
+                ratio = rw / rp
+                ok = bool(np.all(np.isfinite(ratio))) and float(ratio.max()) <= 1.0
+                log(f"  {mode} x{s} {k} iterations per state over {frames} frames: ratio max {ratio.max():.4g} "
+                    f"mean {ratio.mean():.4g}{'' if ok else ': rejected'}")
+                if ok and (best is None or ratio.max() < best["max_ratio"]):
+                    best = {"iterations": k, "scale": s, "screen_mean": r, "max_ratio": float(ratio.max()),
+                            "mean_ratio": float(ratio.mean())}
             if best is not None:
                 break  # the smallest iteration count that reaches the target
         found[mode] = best
     sim.set_warm_start("none")
-    sim.restore()
+    sim.restore(0)
     return found, trials

@@ -1,0 +1,83 @@
+"""Accuracy of the benchmark's time step at full size: the 1M-tet beam (220 x 28 x 27 hex cells,
+997,920 stable neo-Hookean tets, 4 substeps, ground plane and 16 spheres) stepped 60 frames from rest
+on the device and in the FP64 oracle (colored order, the same coloring, the color stages on up to 16
+host threads, which gives the sequential order's bits). Errors are the largest vertex distance
+divided by the rest bounding-box diagonal, after frames 1 and 60 (timestep.cpp:20-70 stepped 60
+times)."""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import build_oracle
+from sbs_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+CHECK = (1, 60)
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def run(sc, make, iterations, frames=60):
+    sim = make()
+    out = {}
+    for f in range(1, frames + 1):
+        sim.step(sc.dt, iterations, sc.substeps)
+        if f in CHECK:
+            out[f] = sim.read_body(0)[0].copy()
+    return out
+
+
+@pytest.mark.timeout(1200)
+def test_beam_1m_plain_and_warm_started_against_the_fp64_oracle():
+    sc = scenes.beam_1m()
+    diag = sc.bbox_diagonal()
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+
+    def device(mode=None, scale=None):
+        def make():
+            d = scenes.build_device(sc)
+            d.use_graphs(True)
+            if mode:
+                d.set_warm_start(mode)
+                d.set_warm_start_scale(*scale)
+            return d
+        return make
+
+    def oracle(mode=None, scale=None):
+        def make():
+            o = build_oracle(sc)
+            o.set_threads(threads)
+            if mode:
+                o.set_warm_start(mode, *scale)
+            return o
+        return make
+
+    say("1M beam: plain 8 iterations, device")
+    dev_plain = run(sc, device(), sc.iterations)
+    say(f"1M beam: plain 8 iterations, oracle on {threads} threads")
+    orc_plain = run(sc, oracle(), sc.iterations)
+    scale = (0.7, 0.65)
+    say("1M beam: previous-multiplier warm start (0.7, 0.65), 4 iterations, device")
+    dev_prev = run(sc, device("previous", scale), 4)
+    say("1M beam: previous-multiplier warm start (0.7, 0.65), 4 iterations, oracle")
+    orc_prev = run(sc, oracle("previous", scale), 4)
+    err = {}
+    for f in CHECK:
+        err[("plain", f)] = float(np.abs(dev_plain[f] - orc_plain[f]).max()) / diag
+        err[("previous", f)] = float(np.abs(dev_prev[f] - orc_prev[f]).max()) / diag
+        err[("previous_vs_plain", f)] = float(np.abs(dev_prev[f] - orc_plain[f]).max()) / diag
+        err[("oracle_previous_vs_plain", f)] = float(np.abs(orc_prev[f] - orc_plain[f]).max()) / diag
+    for k, v in sorted(err.items()):
+        say(f"  {k[0]} frame {k[1]}: {v:.3e}")
+    for f in CHECK:
+        assert err[("plain", f)] <= TOL, err
+        assert err[("previous", f)] <= TOL, err
+    # the warm start converges elsewhere on h = 0 than the plain solver; its distance from the plain
+    # trajectory is the oracle's own (the device adds no more than the parity error)
+    for f in CHECK:
+        assert abs(err[("previous_vs_plain", f)] - err[("oracle_previous_vs_plain", f)]) <= TOL, err

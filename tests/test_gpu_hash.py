@@ -105,3 +105,21 @@ def test_hashed_and_brute_force_steps_are_bit_identical():
         a.step(sc.dt, sc.iterations, sc.substeps)
         b.step(sc.dt, sc.iterations, sc.substeps)
     assert np.array_equal(a.read_body(0)[0], b.read_body(0)[0])
+
+
+def test_contact_overflow_is_an_error():
+    # a beam corner inside 10 concentric spheres: 10 contacts at one surface vertex, 2 more than the
+    # slots (the beam rests on the ground plane, sd = 0, no contact)
+    sc = scenes.cantilever(4, 2, 2, 0.1, "neohookean", 5e5, 0.45, 1000.0, ground_y=0.0, iterations=2, substeps=1,
+                           name="overflow_beam")
+    for k in range(10):
+        sc.spheres.append(((0.4, 0.2, 0.2), 0.05 + 0.01 * k))
+    say("contact overflow")
+    dev = scenes.build_device(sc)
+    dev.step(sc.dt, sc.iterations, sc.substeps)
+    with pytest.raises(S.SbsError) as e:
+        dev.synchronize()
+    assert e.value.code == 9  # SBS_ERR_CONTACT_OVERFLOW
+    assert dev.contact_overflow() > 0
+    with pytest.raises(S.SbsError):  # sticky: the next frame is refused
+        dev.step(sc.dt, sc.iterations, sc.substeps)

@@ -293,7 +293,7 @@ def test_learned_warm_start_trains_and_runs():
     dev.snapshot()
     target = autotune.mean_residual(dev, sc, 4, sc.iterations)
     dev.restore()
-    net = autotune.train_warm_start(dev, sc, frames=4, iterations=24, steps=300, batch=4096, log=say)
+    net, _ = autotune.train_warm_start(dev, sc, frames=4, iterations=24, steps=300, batch=4096, log=say)
     assert net.dataset_size() > 0
     found, trials = autotune.tune_warm_start(dev, sc, net, target, frames=4, scales=(0.5,), iterations=(4, 8),
                                              log=say)

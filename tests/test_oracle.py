@@ -552,3 +552,26 @@ def test_batched_group_equals_one_group_per_body():
         out.append([sim.read_body(b)[0] for b in range(2)])
     for a, b in zip(*out):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("warm", [0, 1])
+def test_threaded_colored_order_is_bit_identical(warm):
+    # the elements of one color share no particle, so running them on several threads must give the
+    # sequential colored order's bits; the warm-start pass sums its displacements in constraint order
+    from sbs_amd import scenes
+    from helpers import build_oracle
+
+    sc = scenes.cantilever(12, 4, 4, 0.1, "neohookean", 5e5, 0.45, 1000.0, num_spheres=4, iterations=4, substeps=2,
+                           cluster=True, coloring="index")
+    a, b = build_oracle(sc), build_oracle(sc)
+    b.set_threads(4)
+    for s in (a, b):
+        if warm:
+            s.set_warm_start("previous", 0.5)
+    for _ in range(3):
+        a.step(sc.dt, sc.iterations, sc.substeps)
+        b.step(sc.dt, sc.iterations, sc.substeps)
+    xa, va = a.read_body(0)
+    xb, vb = b.read_body(0)
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+    assert not np.array_equal(xa, sc.bodies[0].x0)

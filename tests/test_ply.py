@@ -275,3 +275,26 @@ def test_million_tet_beam_file_matches_reference():
     same_arrays(back, r)
     print(f"\n{len(ours) / 1e6:.1f} MB: write {t1 - t0:.3f}s (reference {t2 - t1:.3f}s), "
           f"read {t3 - t2:.3f}s (reference {t4 - t3:.3f}s)")
+
+
+def test_second_vertex_element_larger_than_the_arrays_is_an_error():
+    # the arrays are sized from the last vertex element naming x, y, z (one item); a second vertex
+    # element of 64 items naming only x would write past them
+    for fmt in ("ascii", "binary_little_endian", "binary_big_endian"):
+        h = header(fmt, "element vertex 1", "property float x", "property float y", "property float z",
+                   "element vertex 64", "property float x")
+        if fmt == "ascii":
+            body = b"1 2 3\n" + b"".join(b"%d\n" % i for i in range(64))
+        else:
+            end = "<" if fmt.endswith("little_endian") else ">"
+            body = np.array([1, 2, 3], end + "f4").tobytes() + np.arange(64, dtype=end + "f4").tobytes()
+        data = h + body
+        assert ply.read_ply_bytes(data) is None
+        with pytest.raises(S.SbsError) as e:
+            ply.read_ply_bytes(data, raise_errors=True)
+        assert e.value.code == 8  # SBS_ERR_FORMAT
+    # the same elements in the other order fit: the arrays hold 64 positions
+    h = header("ascii", "element vertex 64", "property float x", "property float y", "property float z",
+               "element vertex 1", "property float x")
+    g = ply.read_ply_bytes(h + b"".join(b"%d 0 0\n" % i for i in range(64)) + b"7\n", raise_errors=True)
+    assert g.positions.reshape(-1, 3)[0, 0] == 7.0 and g.positions.reshape(-1, 3)[63, 0] == 63.0
