@@ -394,7 +394,8 @@ def main():
                                                   modes=("previous",), log=log)
         found.update(f_prev)
         trials += t_prev
-        net, train_info = autotune.train_warm_start(sim, scene, log=log, **autotune.training_config(args))
+        net, train_info = autotune.train_warm_start(sim, scene, log=log,
+                                                    **autotune.training_config(args, found.get("previous")))
         f_l, t_l = autotune.tune_warm_start(sim, scene, net, float(plain_tune.mean()), frames=tune,
                                             modes=("learned",), log=log)
         found.update(f_l)

@@ -269,6 +269,55 @@ void orc_set_clusters(void* h, uint32_t group, const uint32_t* offsets, uint32_t
 void orc_set_coloring(void* h, int order) { static_cast<simulation_t*>(h)->coloring = order; }
 // Threads of the colored order's stages (1: sequential); the result is the same bit for bit.
 void orc_set_threads(void* h, int threads) { static_cast<simulation_t*>(h)->threads = threads < 1 ? 1 : threads; }
+// The learned warm start's network: FP32 parameters in the device layout, input normalization (48
+// values each) and output scale (nout values).
+void orc_set_network(void* h, const float* params, const float* mean, const float* inv_std, const float* out_scale,
+                     int nin, int nout)
+{
+    auto net = std::make_shared<warm_net_t>();
+    net->set(params, mean, inv_std, out_scale, nin, nout);
+    static_cast<simulation_t*>(h)->net = net;
+}
+
+// Test hooks: the network on n rows of 39 raw features (y: n x nout), and the features of the
+// clusters of group `group` at the current positions (set with orc_set_body_state) for a substep of dt
+// (out: clusters x 39; rows of clusters that are not 6-tet cells are zero).
+void orc_network_forward(void* h, const double* x, uint32_t n, double* y)
+{
+    auto const& net = *static_cast<simulation_t*>(h)->net;
+    for (uint32_t r = 0; r < n; ++r) net.evaluate(x + std::size_t(r) * 39, y + std::size_t(r) * net.nout);
+}
+
+// (lambda, lambda_h, hist, hist_h) of the neo-Hookean constraints of a group (test hook).
+void orc_multipliers(void* h, uint32_t group, double* out)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    auto const& g = s->groups.at(group);
+    for (std::size_t i = 0; i < g.count; ++i) {
+        auto const* T = dynamic_cast<neohookean_constraint_t const*>(s->constraints[g.first + i].get());
+        double* o = out + 4 * i;
+        o[0] = T ? T->lagrange : 0.0;
+        o[1] = T ? T->lagrange_h : 0.0;
+        o[2] = T ? T->hist : 0.0;
+        o[3] = T ? T->hist_h : 0.0;
+    }
+}
+
+uint32_t orc_cluster_features(void* h, uint32_t group, double dt, double* out)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    auto const& g = s->groups.at(group);
+    if (g.cluster_off.empty()) return 0;
+    uint32_t const nc = uint32_t(g.cluster_off.size() - 1);
+    std::vector<double> const per_body = s->tets_per_body(g);
+    for (uint32_t c = 0; c < nc; ++c) {
+        double atd[6], ath[6];
+        if (!s->cluster_features(g, c, dt, per_body, out + std::size_t(c) * 39, atd, ath))
+            std::fill(out + std::size_t(c) * 39, out + std::size_t(c + 1) * 39, 0.0);
+    }
+    return nc;
+}
+
 void orc_set_warm_start(void* h, int mode, double scale, double scale_h)
 {
     static_cast<simulation_t*>(h)->warm_start = mode;

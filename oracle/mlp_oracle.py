@@ -11,15 +11,15 @@ FP32 on the device.
 """
 import numpy as np
 
-IN, H, OUT, MAX_IN, MAX_OUT = 16, 128, 32, 13, 4
-W1, B1 = 0, 16 * 128
-W2 = B1 + 128
-B2 = W2 + 128 * 128
-W3 = B2 + 128
-B3 = W3 + 128 * 128
-W4 = B3 + 128
-B4 = W4 + 128 * 32
-NPARAMS = B4 + 32
+IN, H, OUT, MAX_IN, MAX_OUT = 48, 64, 32, 40, 12
+W1, B1 = 0, IN * H
+W2 = B1 + H
+B2 = W2 + H * H
+W3 = B2 + H
+B3 = W3 + H * H
+W4 = B3 + H
+B4 = W4 + H * OUT
+NPARAMS = B4 + OUT
 
 
 def bf16(a):
@@ -41,7 +41,7 @@ def _round(a, rounding):
 
 
 def normalize(x, mean, inv_std, rounding=True):
-    """(x - mean) * inv_std in FP32 (as the kernels compute it), padded to 16 inputs."""
+    """(x - mean) * inv_std in FP32 (as the kernels compute it), padded to 48 inputs."""
     x = np.asarray(x, dtype=np.float32)
     n, k = x.shape
     z = np.zeros((n, IN), dtype=np.float32)
@@ -50,7 +50,7 @@ def normalize(x, mean, inv_std, rounding=True):
 
 
 def forward(p, xn, rounding=True):
-    """Network outputs (all 32) and the cached activations for normalized inputs xn [n][16]."""
+    """Network outputs (all 32) and the cached activations for normalized inputs xn [n][48]."""
     w1, b1, w2, b2, w3, b3, w4, b4 = unpack(p)
     if rounding:
         w1, w2, w3, w4 = bf16(w1), bf16(w2), bf16(w3), bf16(w4)

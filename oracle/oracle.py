@@ -22,6 +22,7 @@ _p = ctypes.c_void_p
 _pd = ctypes.POINTER(ctypes.c_double)
 _pu = ctypes.POINTER(ctypes.c_uint32)
 _pu64 = ctypes.POINTER(ctypes.c_uint64)
+_pf = ctypes.POINTER(ctypes.c_float)
 
 _SIGS = {
     "orc_create": (_p, []),
@@ -39,6 +40,10 @@ _SIGS = {
     "orc_add_sphere": (None, [_p, _pd, _d]),
     "orc_set_order": (None, [_p, _i, _u32]),
     "orc_set_threads": (None, [_p, _i]),
+    "orc_set_network": (None, [_p, _pf, _pf, _pf, _pf, _i, _i]),
+    "orc_network_forward": (None, [_p, _pd, _u32, _pd]),
+    "orc_cluster_features": (_u32, [_p, _u32, _d, _pd]),
+    "orc_multipliers": (None, [_p, _u32, _pd]),
     "orc_set_clusters": (None, [_p, _u32, _pu, _u32]),
     "orc_set_coloring": (None, [_p, _i]),
     "orc_set_warm_start": (None, [_p, _i, _d, _d]),
@@ -275,6 +280,43 @@ class Simulation:
         `hydrostatic` when given); their displacement is applied first."""
         self._L.orc_set_warm_start(self.h, {"none": 0, "previous": 1, "explicit": 3}[mode], scale,
                                    scale if hydrostatic is None else hydrostatic)
+
+    def set_learned_warm_start(self, net, scale=1.0, hydrostatic=None):
+        """The learned warm start with the network `net` (an sbs_amd Mlp of 39 inputs and 12 outputs:
+        its FP32 parameters and normalization are read from the device), times the scales."""
+        self.set_network(net.get_params(), *net.get_normalization(), nin=net.nin, nout=net.nout)
+        self._L.orc_set_warm_start(self.h, 2, scale, scale if hydrostatic is None else hydrostatic)
+
+    def set_network(self, params, mean, inv_std, out_scale, nin=39, nout=12):
+        """The network of the learned warm start from its FP32 parameters and normalization."""
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        m48, s48 = np.zeros(48, np.float32), np.ones(48, np.float32)
+        m48[:len(mean)] = mean
+        s48[:len(inv_std)] = inv_std
+        o = np.ascontiguousarray(out_scale, dtype=np.float32)
+        f = lambda a: a.ctypes.data_as(_pf)  # noqa: E731
+        self._keep = (p, m48, s48, o)
+        self._L.orc_set_network(self.h, f(p), f(m48), f(s48), f(o), int(nin), int(nout))
+
+    def network_forward(self, x, nout=12):
+        """The oracle's network on raw feature rows x [n][39] (test hook)."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 39)
+        y = np.zeros((len(x), nout))
+        self._L.orc_network_forward(self.h, _dp(x), len(x), _dp(y))
+        return y
+
+    def cluster_features(self, group, dt, num_clusters):
+        """Features [clusters][39] of a clustered neo-Hookean group at the current positions (test hook)."""
+        out = np.zeros((num_clusters, 39))
+        n = self._L.orc_cluster_features(self.h, group, dt, _dp(out))
+        return out[:n]
+
+    def multipliers(self, group, count):
+        """[count][4]: (deviatoric, hydrostatic) multipliers and their values one substep before, per
+        neo-Hookean constraint of the group (test hook)."""
+        out = np.zeros((count, 4))
+        self._L.orc_multipliers(self.h, group, _dp(out))
+        return out
 
     def set_order(self, colored, recolor_passes=2):
         self._L.orc_set_order(self.h, int(colored), recolor_passes)

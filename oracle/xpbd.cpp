@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <stdexcept>
 
 namespace oracle {
@@ -417,6 +418,125 @@ void simulation_t::apply_warm_start()
         for (std::size_t i = 0; i < particles[b].size(); ++i) particles[b][i].xi += d[b][i];
 }
 
+double bf16_round(float x)
+{
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u = ((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16) << 16;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return double(r);
+}
+
+void warm_net_t::set(const float* p, const float* mu, const float* is, const float* os, int ni, int no)
+{
+    nin = ni;
+    nout = no;
+    auto take = [&](std::size_t& at, std::size_t n, bool round) {
+        std::vector<double> v(n);
+        for (std::size_t i = 0; i < n; ++i) v[i] = round ? bf16_round(p[at + i]) : double(p[at + i]);
+        at += n;
+        return v;
+    };
+    std::size_t at = 0;
+    w1 = take(at, std::size_t(IN) * H, true);
+    b1 = take(at, H, false);
+    w2 = take(at, std::size_t(H) * H, true);
+    b2 = take(at, H, false);
+    w3 = take(at, std::size_t(H) * H, true);
+    b3 = take(at, H, false);
+    w4 = take(at, std::size_t(H) * OUT, true);
+    b4 = take(at, OUT, false);
+    mean.assign(mu, mu + IN);
+    inv_std.assign(is, is + IN);
+    out_scale.assign(os, os + no);
+}
+
+void warm_net_t::evaluate(const double* x, double* y) const
+{
+    double a0[IN], a1[H], a2[H], a3[H];
+    for (int k = 0; k < IN; ++k) {
+        float const z = k < nin ? (float(x[k]) - mean[k]) * inv_std[k] : 0.0f;
+        a0[k] = bf16_round(z);
+    }
+    auto layer = [](const double* a, int n, const std::vector<double>& w, const std::vector<double>& b, int m,
+                    double* out) {
+        for (int o = 0; o < m; ++o) {
+            double z = b[o];
+            for (int i = 0; i < n; ++i) z += a[i] * w[std::size_t(i) * m + o];
+            out[o] = z;
+        }
+    };
+    layer(a0, IN, w1, b1, H, a1);
+    for (int o = 0; o < H; ++o) a1[o] = bf16_round(float(std::max(a1[o], 0.0)));
+    layer(a1, H, w2, b2, H, a2);
+    for (int o = 0; o < H; ++o) a2[o] = bf16_round(float(std::max(a2[o], 0.0)));
+    layer(a2, H, w3, b3, H, a3);
+    for (int o = 0; o < H; ++o) a3[o] = bf16_round(float(std::max(a3[o], 0.0)));
+    double out[OUT];
+    layer(a3, H, w4, b4, OUT, out);
+    for (int o = 0; o < nout; ++o) y[o] = double(out_scale[o]) * out[o];
+}
+
+std::vector<double> simulation_t::tets_per_body(group_t const& g) const
+{
+    std::vector<double> n(particles.size(), 0.0);
+    for (std::size_t i = 0; i < g.count; ++i)
+        n[static_cast<neohookean_constraint_t const*>(constraints[g.first + i].get())->bi] += 1.0;
+    return n;
+}
+
+bool simulation_t::cluster_features(group_t const& g, std::size_t c, double dt, std::vector<double> const& per_body,
+                                    double f[39], double atd[6], double ath[6]) const
+{
+    std::size_t const t0 = g.first + g.cluster_off[c], t1 = g.first + g.cluster_off[c + 1];
+    if (t1 - t0 != 6) return false;  // hex cells only
+    double const dt2 = dt * dt;
+    for (int k = 0; k < 6; ++k) {
+        auto const* T = static_cast<neohookean_constraint_t const*>(constraints[t0 + k].get());
+        auto const& P = particles[T->bi];
+        mat3 const Ds = mat3::from_cols(P[T->v[0]].xi - P[T->v[3]].xi, P[T->v[1]].xi - P[T->v[3]].xi,
+                                        P[T->v[2]].xi - P[T->v[3]].xi);
+        mat3 const F = Ds * T->DmInv;
+        double const V0abs = std::abs(T->V0);
+        atd[k] = (1.0 / (T->mu * V0abs)) / dt2;
+        ath[k] = (1.0 / (T->lam * V0abs)) / dt2;
+        f[2 * k] = atd[k] * T->lagrange;
+        f[2 * k + 1] = ath[k] * T->lagrange_h;
+        f[12 + 2 * k] = atd[k] * T->hist;
+        f[13 + 2 * k] = ath[k] * T->hist_h;
+        f[24 + 2 * k] = std::sqrt(frob2(F));
+        f[25 + 2 * k] = determinant(F) - (1.0 + T->mu / T->lam);
+    }
+    f[36] = std::log(atd[0]);
+    f[37] = std::log(ath[0]);
+    f[38] = std::log(per_body[static_cast<neohookean_constraint_t const*>(constraints[t0].get())->bi]);
+    return true;
+}
+
+void simulation_t::apply_learned_multipliers()
+{
+    for (auto const& g : groups) {
+        if (g.kind != 1 || g.cluster_off.empty()) continue;
+        long const nc = long(g.cluster_off.size()) - 1;
+        std::vector<double> const per_body = tets_per_body(g);
+#pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1)
+        for (long c = 0; c < nc; ++c) {
+            double f[39], atd[6], ath[6], y[12];
+            if (!cluster_features(g, std::size_t(c), warm_dt, per_body, f, atd, ath)) continue;
+            net->evaluate(f, y);
+            std::size_t const t0 = g.first + g.cluster_off[c];
+            for (int k = 0; k < 6; ++k) {
+                auto* T = static_cast<neohookean_constraint_t*>(constraints[t0 + k].get());
+                T->hist = T->lagrange;
+                T->hist_h = T->lagrange_h;
+                T->lagrange += y[2 * k] / atd[k];
+                T->lagrange_h += y[2 * k + 1] / ath[k];
+            }
+        }
+    }
+}
+
 void simulation_t::solve(double dt, std::size_t iterations)
 {
     if (order_dirty) build_order();
@@ -424,6 +544,7 @@ void simulation_t::solve(double dt, std::size_t iterations)
     for (auto& c : constraints)
         if (!(warm_start && dynamic_cast<neohookean_constraint_t*>(c.get()))) c->prepare_for_projection();
     warm_dt = dt;
+    if (warm_start == 2) apply_learned_multipliers();
     if (warm_start) apply_warm_start();
     bool const parallel = threads > 1 && colored_order;
     for (std::size_t k = 0; k < iterations; ++k) {

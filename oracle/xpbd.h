@@ -73,6 +73,7 @@ struct neohookean_constraint_t final : constraint_t {
     mat3 DmInv;
     double V0, mu, lam;
     double lagrange_h = 0.0;
+    double hist = 0.0, hist_h = 0.0;  // the multipliers one substep before (learned warm start)
     neohookean_constraint_t(double beta, simulation_t const& sim, uint32_t body, uint32_t const vs[4],
                             double young, double poisson);
     void prepare_impl() override { lagrange_h = 0.0; }
@@ -85,6 +86,22 @@ struct neohookean_constraint_t final : constraint_t {
     void residual_sq(simulation_t const& sim, double dt, double* acc) const override;
     double energy(simulation_t const& sim) const override;
 };
+
+// The learned warm start's network (no reference counterpart): 48 inputs -> 64 -> 64 -> 64 (ReLU)
+// -> 32 outputs, the parameter layout of the device network (W1[48][64] b1 W2[64][64] b2 W3[64][64]
+// b3 W4[64][32] b4). Evaluated in FP64 with the device's BF16 rounding points: the normalized
+// inputs (normalized in FP32), the weights, and the hidden activations after bias and ReLU (through
+// FP32); the biases and the output scale stay FP32 values.
+struct warm_net_t {
+    static constexpr int IN = 48, H = 64, OUT = 32;
+    std::vector<double> w1, b1, w2, b2, w3, b3, w4, b4;
+    std::vector<float> mean, inv_std, out_scale;
+    int nin = 0, nout = 0;
+    void set(const float* params, const float* mean, const float* inv_std, const float* out_scale, int nin, int nout);
+    // y[o] = out_scale[o] * net((x - mean) * inv_std)[o] for o < nout
+    void evaluate(const double* x, double* y) const;
+};
+double bf16_round(float x);  // nearest BF16 (ties to even), as a double
 
 struct distance_constraint_t final : constraint_t {
     uint32_t b1, b2, v1, v2;
@@ -165,6 +182,19 @@ struct simulation_t {
     double warm_dt = 0.0;
     double warm_scale = 1.0;    // factor on the warm-start multipliers (deviatoric)
     double warm_scale_h = 1.0;  // factor on the hydrostatic warm-start multipliers
+    // warm_start == 2 (learned): per hex cluster of a neo-Hookean group (its 6 tets in order) the
+    // network's 39 features at the predicted positions -- alpha~ lambda of the 6 (deviatoric,
+    // hydrostatic) pairs, alpha~ times the pairs one substep before, the 12 constraint values, log
+    // alpha~ of tet 0 (deviatoric, hydrostatic), log of the number of the group's tets in the
+    // cluster's body -- give the change of alpha~ lambda over the substep; lambda_init = lambda +
+    // change / alpha~ (then the scales), and the history takes lambda.
+    std::shared_ptr<warm_net_t> net;
+    void apply_learned_multipliers();
+    // The 39 features of cluster c of group g at the current predicted positions (xi) for a substep
+    // of dt (test hook of apply_learned_multipliers' features); false unless it is a 6-tet cluster.
+    bool cluster_features(group_t const& g, std::size_t c, double dt, std::vector<double> const& per_body,
+                          double f[39], double atd[6], double ath[6]) const;
+    std::vector<double> tets_per_body(group_t const& g) const;  // the group's tets in each body
 
     // GS visiting order of the elastic constraints (indices into constraints)
     std::vector<uint32_t> order;

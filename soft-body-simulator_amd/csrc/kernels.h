@@ -173,6 +173,22 @@ void launch_jp_keys(hipStream_t s, uint32_t n, const uint32_t* prev, uint64_t* k
 
 void launch_copy_pos(hipStream_t s, const pos4* a, pos4* b, uint32_t n);
 
+// The warm-start network's side of a hex group's warm-start pass: per tet (execution order) the
+// multiplier history hist (the pair one substep before the stored one) and the body-size feature;
+// raw [nrows][40] feature rows (capture); the network's packed BF16 weights, biases and
+// normalization (learned mode).
+struct warm_net_dev {
+    float2* hist;
+    const float* body_size;
+    float* raw;
+    const mlp::bf16* pack;
+    const float* bias;
+    const mlp::norm_dev* norm;
+};
+// net_mode of launch_hex_warm_start: the pass alone; also the features of every cluster into raw
+// (training capture); or the network's predicted change added to the multipliers first.
+constexpr int kWarmNetNone = 0, kWarmNetFeatures = 1, kWarmNetLearned = 2;
+
 // Warm-start pass of a hex-cluster neo-Hookean group in two launches instead of one per color:
 // every cluster's slot displacements at the predicted positions xt into disp ([8][nrows]: slot k of
 // the cluster in row r at k * nrows + r), then for each particle p < n, xi[p] += the FP64 sum over
@@ -185,27 +201,27 @@ void launch_copy_pos(hipStream_t s, const pos4* a, pos4* b, uint32_t n);
 // With surface_of (dense list only; particle -> surface vertex or -1) the gather also runs the first
 // iteration's contact projections (compliance c_at) of the surface vertices, in place of that
 // iteration's contact launch. With prow (dense list only), the gather runs over the particle rows.
+// With net_mode kWarmNetFeatures or kWarmNetLearned, the cluster kernel forms the cluster's 39
+// network features at the predicted positions: alpha~ lambda_prev of its 6 tets' (deviatoric,
+// hydrostatic) pairs (12), alpha~ lambda_prev2 (12), the 12 constraint values C at xt, log alpha~
+// of tet 0 (deviatoric, hydrostatic) and the body-size feature, where lambda_prev is the stored
+// multiplier pair (the previous substep's final) and lambda_prev2 net.hist, which receives
+// lambda_prev. kWarmNetFeatures writes them to net.raw[row]; kWarmNetLearned normalizes them, runs
+// the network (matrix cores, two 32-row tiles per workgroup) and adds out_scale * y_k / alpha~_k to
+// multiplier k before the pass scales them.
 void launch_hex_warm_start(hipStream_t s, tet_group_dev g, cluster_dev cl, const pos4* xt, pos4* xi,
                            const uint4* cinfo, uint32_t nrows, float inv_dt2, int expl, float2 scale,
                            float4* disp, const uint32_t* csr_off, const uint32_t* csr, uint32_t n,
                            const int* surface_of = nullptr, contact_dev const& c = {}, float c_at = 0.0f,
-                           const uint32_t* prow = nullptr, uint32_t nprows = 0);
-
-// Multiplier warm start of a neo-Hookean group, per constraint t (execution order) at the
-// predicted positions xi: the 13 features
-//   lambda_prev (dev, hyd), lambda_prev2 (dev, hyd), C_dev, C_hyd, log sum w|grad C_dev|^2,
-//   log sum w|grad C_hyd|^2, log alpha~_dev, log alpha~_hyd, -C_dev / alpha~_dev, -C_hyd / alpha~_hyd,
-//   log of the number of tets of the constraint's body (body_size[t])
-// where lambda_prev is the multiplier pair stored in g.lambda (the previous substep's final) and
-// lambda_prev2 the pair before it (hist, which receives lambda_prev). raw gets the 13 FP32 values
-// ([count][13]) and/or X the normalized BF16 network input ([count][16]); either may be null.
-void launch_ws_features(hipStream_t s, tet_group_dev g, const pos4* xi, float2* hist, const float* body_size,
-                        uint32_t count, float inv_dt2, float* raw, mlp::bf16* X, const mlp::norm_dev* norm, int block);
-// Training rows of constraints t = offset + i * stride, i < rows: ds_x[base + i] = raw[t],
-// ds_t[base + i] = lambda[t] - lambda_prev[t] (the multipliers' change over the substep); a row
-// holding a non-finite value is stored as zeros.
-void launch_ws_capture(hipStream_t s, const float* raw, const float* lambda, uint32_t stride, uint32_t offset,
-                       float* ds_x, float* ds_t, uint64_t base, uint32_t rows);
+                           const uint32_t* prow = nullptr, uint32_t nprows = 0, int net_mode = kWarmNetNone,
+                           warm_net_dev const& net = {});
+// Training rows of a hex group after a captured substep: for j < rows, cluster ci = offset + j *
+// stride (rowof: cluster -> its row of raw and cinfo), ds_x[base + j] = raw[rowof[ci]] (40 values)
+// and ds_t[base + j] = the change of alpha~ lambda of its 12 multipliers over the substep (12
+// values); rows holding a non-finite value are stored as zeros.
+void launch_hex_capture(hipStream_t s, tet_group_dev g, const uint4* cinfo, const uint32_t* rowof, const float* raw,
+                        uint32_t stride, uint32_t offset, float inv_dt2, float* ds_x, float* ds_t, uint64_t base,
+                        uint32_t rows);
 
 // Surface refresh (surface.hip): for every surface vertex s < S (particle surf[s]), normal[s] +=
 // the face normals of its triangles tris[inc[off[s] .. off[s+1])] (surface vertex indices, in

@@ -12,6 +12,7 @@
 
 #include "device_math.h"
 #include "kernels.h"
+#include "mlp_device.h"
 #include "sdf.h"
 
 namespace sbs {
@@ -468,78 +469,6 @@ __global__ void k_copy_pos(const pos4* __restrict__ a, pos4* __restrict__ b, uin
     if (i < n) b[i] = a[i];
 }
 
-// Features of neo-Hookean constraint t at the predicted positions (see launch_ws_features).
-__global__ void __launch_bounds__(256) k_ws_features(tet_group_dev g, const pos4* __restrict__ xi,
-                                                     float2* __restrict__ hist, const float* __restrict__ body_size,
-                                                     uint32_t count, float inv_dt2,
-                                                     float* __restrict__ raw, mlp::bf16* __restrict__ X,
-                                                     const mlp::norm_dev* __restrict__ norm)
-{
-    uint32_t const t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    int4 const id = g.idx[t];
-    float4 const b = g.m1[t];
-    float2 const c = nh_m2(g, t);
-    nh_rest const q = load_nh(g.m0[t], b);
-    pos4 const P[4] = {xi[id.x], xi[id.y], xi[id.z], xi[id.w]};
-    tet_frame const f = make_frame(P, [](int) { return pos4{}; }, false);
-    float2 const lp = reinterpret_cast<const float2*>(g.lambda)[t];
-    float2 const lp2 = hist[t];
-    hist[t] = lp;
-    v3 hd[4], hh[4];
-    float sd, sh;
-    float const Cd = nh_deviatoric(f.e, q, hd, sd);
-    float const Ch = nh_hydrostatic(f.e, q, c.y, hh, sh);
-    float const wd = grad_norm2(f.w, hd, sd), wh = grad_norm2(f.w, hh, sh);
-    float const atd = b.w * inv_dt2, ath = c.x * inv_dt2;
-    float const v[mlp::kMaxInputs] = {lp.x,
-                                      lp.y,
-                                      lp2.x,
-                                      lp2.y,
-                                      Cd,
-                                      Ch,
-                                      logf(wd + 1e-30f),
-                                      logf(wh + 1e-30f),
-                                      logf(atd),
-                                      logf(ath),
-                                      -Cd / atd,
-                                      -Ch / ath,
-                                      body_size[t]};
-    if (raw) {
-#pragma unroll
-        for (int k = 0; k < mlp::kMaxInputs; ++k) raw[std::size_t(t) * mlp::kMaxInputs + k] = v[k];
-    }
-    if (X) {
-        mlp::bf16* o = X + std::size_t(t) * mlp::kIn;
-#pragma unroll
-        for (int k = 0; k < mlp::kIn; ++k)
-            o[k] = static_cast<mlp::bf16>(k < mlp::kMaxInputs ? (v[k] - norm->mean[k]) * norm->inv_std[k] : 0.0f);
-    }
-}
-
-__global__ void k_ws_capture(const float* __restrict__ raw, const float2* __restrict__ lambda, uint32_t stride,
-                             uint32_t offset, float* __restrict__ ds_x, float* __restrict__ ds_t, uint64_t base,
-                             uint32_t rows)
-{
-    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= rows) return;
-    std::size_t const t = std::size_t(offset) + std::size_t(i) * stride;
-    std::size_t const r = base + i;
-    float const* f = raw + t * mlp::kMaxInputs;
-    float2 const lf = lambda[t];
-    float const d0 = lf.x - f[0], d1 = lf.y - f[1];
-    // a row with a non-finite value (a diverged constraint) is stored as zeros
-    bool ok = isfinite(d0) && isfinite(d1);
-#pragma unroll
-    for (int k = 0; k < mlp::kMaxInputs; ++k) ok = ok && isfinite(f[k]);
-#pragma unroll
-    for (int k = 0; k < mlp::kMaxInputs; ++k) ds_x[r * mlp::kMaxInputs + k] = ok ? f[k] : 0.0f;
-    ds_t[r * mlp::kMaxOutputs + 0] = ok ? d0 : 0.0f;
-    ds_t[r * mlp::kMaxOutputs + 1] = ok ? d1 : 0.0f;
-    ds_t[r * mlp::kMaxOutputs + 2] = 0.0f;
-    ds_t[r * mlp::kMaxOutputs + 3] = 0.0f;
-}
-
 // ---------------------------------------------------------------- clustered Gauss-Seidel
 
 // One thread per cluster: the cluster's particles are staged in LDS (slot-major, so that the
@@ -748,6 +677,17 @@ __global__ void __launch_bounds__(BLOCK) k_neohookean_hex_apply(tet_group_dev g,
     for (int k = 0; k < 8; ++k) add_displacement(&xi[ids[k]], D[k]);
 }
 
+// Constraint values of a neo-Hookean tet at edges e (the quantities nh_deviatoric and
+// nh_hydrostatic return, without their gradients).
+__device__ __forceinline__ float2 nh_values(v3 const e[3], nh_rest const& q, float gamma0)
+{
+    float const g00 = fdot(e[0], e[0]), g11 = fdot(e[1], e[1]), g22 = fdot(e[2], e[2]);
+    float const g01 = fdot(e[0], e[1]), g02 = fdot(e[0], e[2]), g12 = fdot(e[1], e[2]);
+    float const off = fmaf(q.q01, g01, fmaf(q.q02, g02, q.q12 * g12));
+    float const c2 = fmaf(q.q00, g00, fmaf(q.q11, g11, fmaf(q.q22, g22, 2.0f * off)));
+    return make_float2(hw_sqrt(fmaxf(c2, 0.0f)), fmaf(q.detr, fdot(e[0], fcross(e[1], e[2])), -gamma0));
+}
+
 // Warm-start displacements of every hex cluster of a group in one launch (all colors): nothing
 // moves during the launch, so every gradient is taken at the predicted positions xi (a copy of
 // them, or the working positions themselves before any displacement is added). Workgroup b (64
@@ -755,20 +695,34 @@ __global__ void __launch_bounds__(BLOCK) k_neohookean_hex_apply(tet_group_dev g,
 // one's index in its color, the color's cluster count and first constraint); the summed
 // displacement of the cluster's slot k goes to disp[k * nrows + 64 b + t] (one coalesced store per
 // slot).
+// MODE kWarmFeatures (capture) and kWarmLearned also form the cluster's network features (see
+// launch_hex_warm_start) and store the multipliers they start from as the history; kWarmFeatures
+// writes the features to raw[row], kWarmLearned runs the network on the workgroup's 64 rows (two
+// 32-row matrix-core tiles, rows staged through LDS) and adds its predicted change to the
+// multipliers before the displacement.
+constexpr int kWarmPlain = kWarmNetNone, kWarmFeatures = kWarmNetFeatures, kWarmLearned = kWarmNetLearned;
+constexpr int kFeatures = 39;  // used network inputs of a hex cluster
+
+template <int MODE>
 __global__ void __launch_bounds__(64) k_hex_warm_disp(tet_group_dev g, cluster_dev cl, const pos4* __restrict__ xi,
                                                       const uint4* __restrict__ cinfo, uint32_t nrows,
-                                                      float inv_dt2, int expl, float2 scale, float4* __restrict__ disp)
+                                                      float inv_dt2, int expl, float2 scale, float4* __restrict__ disp,
+                                                      warm_net_dev net)
 {
     uint4 const info = cinfo[2 * blockIdx.x];
     uint32_t const used = cinfo[2 * blockIdx.x + 1].x;
-    if (threadIdx.x >= used) return;
+    if (MODE != kWarmLearned && threadIdx.x >= used) return;
+    if (used == 0) return;  // a padding workgroup (uniform)
+    bool const active = threadIdx.x < used;
     uint32_t const r = blockIdx.x * 64u + threadIdx.x;
-    uint32_t const i = info.x + threadIdx.x, count = info.y, tbase = info.z, ci = info.w + threadIdx.x;
+    // inactive lanes (learned mode only) run the network on zero rows and touch no memory of theirs
+    uint32_t const i = info.x + (active ? threadIdx.x : 0u), count = info.y, tbase = info.z;
+    uint32_t const ci = info.w + (active ? threadIdx.x : 0u);
     int4 const va = cl.v0[ci], vb = cl.v1[ci];
     int const ids[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
     // every tet's material and multipliers are loaded before the first multiplier is stored back
     float4 A[kHexTets], B[kHexTets];
-    float2 C[kHexTets], L[kHexTets];
+    float2 C[kHexTets], L[kHexTets], H[kHexTets];
 #pragma unroll
     for (int k = 0; k < kHexTets; ++k) {
         uint32_t const t = tbase + uint32_t(k) * count + i;
@@ -776,10 +730,83 @@ __global__ void __launch_bounds__(64) k_hex_warm_disp(tet_group_dev g, cluster_d
         B[k] = g.m1[t];
         C[k] = nh_m2(g, t);
         L[k] = reinterpret_cast<const float2*>(g.lambda)[t];
+        if (MODE != kWarmPlain) H[k] = net.hist[t];
     }
     pos4 P[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) P[k] = xi[ids[k]];
+    if (MODE != kWarmPlain) {
+        // features: alpha~ lambda_prev (12), alpha~ lambda_prev2 (12), C at the predicted positions
+        // (12), log alpha~ of tet 0 (deviatoric, hydrostatic), the body-size feature
+        float f[kFeatures];
+#pragma unroll
+        for (int k = 0; k < kHexTets; ++k) {
+            int const s0 = kHexSlots[k][0], s1 = kHexSlots[k][1], s2 = kHexSlots[k][2], s3 = kHexSlots[k][3];
+            v3 const e[3] = {sub(P[s0], P[s3]), sub(P[s1], P[s3]), sub(P[s2], P[s3])};
+            float2 const cv = nh_values(e, load_nh(A[k], B[k]), C[k].y);
+            float const atd = B[k].w * inv_dt2, ath = C[k].x * inv_dt2;
+            f[2 * k] = atd * L[k].x;
+            f[2 * k + 1] = ath * L[k].y;
+            f[12 + 2 * k] = atd * H[k].x;
+            f[13 + 2 * k] = ath * H[k].y;
+            f[24 + 2 * k] = cv.x;
+            f[25 + 2 * k] = cv.y;
+        }
+        f[36] = logf(B[0].w * inv_dt2);
+        f[37] = logf(C[0].x * inv_dt2);
+        f[38] = net.body_size[tbase + i];
+        if (active) {
+#pragma unroll
+            for (int k = 0; k < kHexTets; ++k) net.hist[tbase + uint32_t(k) * count + i] = L[k];
+        }
+        if (MODE == kWarmFeatures) {
+            float* o = net.raw + std::size_t(r) * mlp::kMaxInputs;
+#pragma unroll
+            for (int m = 0; m < mlp::kMaxInputs; ++m) o[m] = m < kFeatures ? f[m] : 0.0f;
+        } else {
+            using mlp::bf16x8;
+            __shared__ bf16x8 rows_lds[64 * mlp::kIn / 8];  // [row][48] BF16 inputs, then [row][12] FP32 outputs
+            __shared__ float bias_lds[mlp::kBiasTotal];
+            for (int m = threadIdx.x; m < int(mlp::kBiasTotal); m += 64) bias_lds[m] = net.bias[m];
+            {
+                bf16x8 v[mlp::kIn / 8];
+#pragma unroll
+                for (int m = 0; m < mlp::kIn; ++m)
+                    v[m / 8][m % 8] = static_cast<mlp::bf16>(
+                        active && m < kFeatures ? (f[m] - net.norm->mean[m]) * net.norm->inv_std[m] : 0.0f);
+#pragma unroll
+                for (int m = 0; m < mlp::kIn / 8; ++m) rows_lds[threadIdx.x * (mlp::kIn / 8) + m] = v[m];
+            }
+            __syncthreads();
+            int const lane = threadIdx.x, h = lane >> 5, c = lane & 31;
+            bf16x8 xin[2][mlp::kQ1];
+#pragma unroll
+            for (int t2 = 0; t2 < 2; ++t2)
+#pragma unroll
+                for (int q = 0; q < mlp::kQ1; ++q) xin[t2][q] = rows_lds[(32 * t2 + c) * (mlp::kIn / 8) + 2 * q + h];
+            mlp::f32x16 y[2];
+            mlp::forward_tiles<2>(reinterpret_cast<const bf16x8*>(net.pack), bias_lds, xin, lane, y);
+            __syncthreads();
+            float* out = reinterpret_cast<float*>(rows_lds);
+#pragma unroll
+            for (int t2 = 0; t2 < 2; ++t2)
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    int const o = mlp::acc_unit(j, h);
+                    if (o < 2 * kHexTets) out[(32 * t2 + c) * (2 * kHexTets) + o] = y[t2][j];
+                }
+            __syncthreads();
+            // the predicted change of alpha~ lambda over the substep, back to multipliers
+#pragma unroll
+            for (int k = 0; k < kHexTets; ++k) {
+                float const yd = out[threadIdx.x * (2 * kHexTets) + 2 * k] * net.norm->out_scale[2 * k];
+                float const yh = out[threadIdx.x * (2 * kHexTets) + 2 * k + 1] * net.norm->out_scale[2 * k + 1];
+                L[k].x += yd / (B[k].w * inv_dt2);
+                L[k].y += yh / (C[k].x * inv_dt2);
+            }
+            if (!active) return;
+        }
+    }
     v3 D[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) D[k] = mk(0.0f, 0.0f, 0.0f);
@@ -798,6 +825,39 @@ __global__ void __launch_bounds__(64) k_hex_warm_disp(tet_group_dev g, cluster_d
     for (int k = 0; k < kHexTets; ++k) reinterpret_cast<float2*>(g.lambda)[tbase + uint32_t(k) * count + i] = L[k];
 #pragma unroll
     for (int k = 0; k < 8; ++k) disp[std::size_t(k) * nrows + r] = make_float4(D[k].x, D[k].y, D[k].z, 0.0f);
+}
+
+// Training rows of a hex group's captured substep: for j < rows, cluster ci = offset + j * stride,
+// its row rowof[ci] of raw (the features at the substep start) and, as targets, the change of
+// alpha~ lambda of its 12 multipliers over the substep (the final multipliers against feature
+// values 0..11); a row holding a non-finite value is stored as zeros.
+__global__ void k_hex_capture(tet_group_dev g, const uint4* __restrict__ cinfo, const uint32_t* __restrict__ rowof,
+                              const float* __restrict__ raw, uint32_t stride, uint32_t offset, float inv_dt2,
+                              float* __restrict__ ds_x, float* __restrict__ ds_t, uint64_t base, uint32_t rows)
+{
+    uint32_t const j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= rows) return;
+    uint32_t const row = rowof[offset + j * stride];
+    uint4 const info = cinfo[2 * (row / 64)];
+    uint32_t const i = info.x + row % 64, count = info.y, tbase = info.z;
+    const float* f = raw + std::size_t(row) * mlp::kMaxInputs;
+    float t[2 * kHexTets];
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < kHexTets; ++k) {
+        uint32_t const tk = tbase + uint32_t(k) * count + i;
+        float2 const lf = reinterpret_cast<const float2*>(g.lambda)[tk];
+        t[2 * k] = g.m1[tk].w * inv_dt2 * lf.x - f[2 * k];
+        t[2 * k + 1] = nh_m2(g, tk).x * inv_dt2 * lf.y - f[2 * k + 1];
+        ok = ok && isfinite(t[2 * k]) && isfinite(t[2 * k + 1]);
+    }
+#pragma unroll
+    for (int m = 0; m < kFeatures; ++m) ok = ok && isfinite(f[m]);
+    std::size_t const o = base + j;
+#pragma unroll
+    for (int m = 0; m < mlp::kMaxInputs; ++m) ds_x[o * mlp::kMaxInputs + m] = ok ? f[m] : 0.0f;
+#pragma unroll
+    for (int m = 0; m < mlp::kMaxOutputs; ++m) ds_t[o * mlp::kMaxOutputs + m] = ok && m < 2 * kHexTets ? t[m] : 0.0f;
 }
 
 // x_p += the displacements of p's cluster slots, in FP64, in the order of the list (execution
@@ -1505,10 +1565,18 @@ void launch_hex_warm_start(hipStream_t s, tet_group_dev g, cluster_dev cl, const
                            const uint4* cinfo, uint32_t nrows, float inv_dt2, int expl, float2 scale,
                            float4* disp, const uint32_t* csr_off, const uint32_t* csr, uint32_t n,
                            const int* surface_of, contact_dev const& c, float c_at, const uint32_t* prow,
-                           uint32_t nprows)
+                           uint32_t nprows, int net_mode, warm_net_dev const& net)
 {
     if (!nrows) return;
-    k_hex_warm_disp<<<grid_for(nrows, 64), 64, 0, s>>>(g, cl, xt, cinfo, nrows, inv_dt2, expl, scale, disp);
+    if (net_mode == kWarmLearned)
+        k_hex_warm_disp<kWarmLearned><<<grid_for(nrows, 64), 64, 0, s>>>(g, cl, xt, cinfo, nrows, inv_dt2, expl, scale,
+                                                                         disp, net);
+    else if (net_mode == kWarmFeatures)
+        k_hex_warm_disp<kWarmFeatures><<<grid_for(nrows, 64), 64, 0, s>>>(g, cl, xt, cinfo, nrows, inv_dt2, expl,
+                                                                          scale, disp, net);
+    else
+        k_hex_warm_disp<kWarmPlain><<<grid_for(nrows, 64), 64, 0, s>>>(g, cl, xt, cinfo, nrows, inv_dt2, expl, scale,
+                                                                       disp, net);
     if (csr_off)
         k_gather_disp<<<grid_for(n, 256), 256, 0, s>>>(csr_off, csr, disp, xi, n);
     else if (prow && nprows)
@@ -1519,24 +1587,20 @@ void launch_hex_warm_start(hipStream_t s, tet_group_dev g, cluster_dev cl, const
                                                         c, c_at, nullptr);
 }
 
+void launch_hex_capture(hipStream_t s, tet_group_dev g, const uint4* cinfo, const uint32_t* rowof, const float* raw,
+                        uint32_t stride, uint32_t offset, float inv_dt2, float* ds_x, float* ds_t, uint64_t base,
+                        uint32_t rows)
+{
+    if (!rows) return;
+    k_hex_capture<<<grid_for(rows, 256), 256, 0, s>>>(g, cinfo, rowof, raw, stride, offset, inv_dt2, ds_x, ds_t, base,
+                                                      rows);
+}
+
 void launch_copy_pos(hipStream_t s, const pos4* a, pos4* b, uint32_t n)
 {
     if (n) k_copy_pos<<<grid_for(n, 256), 256, 0, s>>>(a, b, n);
 }
 
-void launch_ws_features(hipStream_t s, tet_group_dev g, const pos4* xi, float2* hist, const float* body_size,
-                        uint32_t count, float inv_dt2, float* raw, mlp::bf16* X, const mlp::norm_dev* norm, int block)
-{
-    if (!count) return;
-    k_ws_features<<<grid_for(count, 256), 256, 0, s>>>(g, xi, hist, body_size, count, inv_dt2, raw, X, norm);
-}
 
-void launch_ws_capture(hipStream_t s, const float* raw, const float* lambda, uint32_t stride, uint32_t offset,
-                       float* ds_x, float* ds_t, uint64_t base, uint32_t rows)
-{
-    if (!rows) return;
-    k_ws_capture<<<grid_for(rows, 256), 256, 0, s>>>(raw, reinterpret_cast<const float2*>(lambda), stride, offset,
-                                                     ds_x, ds_t, base, rows);
-}
 
 }  // namespace sbs

@@ -115,8 +115,8 @@ int sbs_mlp_create(int device, uint32_t num_inputs, uint32_t num_outputs, uint64
 {
     return guarded([&] {
         require(out != nullptr, "out is null");
-        require(num_inputs >= 1 && num_inputs <= uint32_t(kMaxInputs), "num_inputs must be 1..13");
-        require(num_outputs >= 1 && num_outputs <= uint32_t(kMaxOutputs), "num_outputs must be 1..4");
+        require(num_inputs >= 1 && num_inputs <= uint32_t(kMaxInputs), "num_inputs must be 1..40");
+        require(num_outputs >= 1 && num_outputs <= uint32_t(kMaxOutputs), "num_outputs must be 1..12");
         *out = nullptr;
         int count = 0;
         hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");

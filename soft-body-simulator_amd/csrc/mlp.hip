@@ -1,40 +1,17 @@
-// gfx950 kernels of the warm-start network: the fused BF16 matrix-core inference pass, the
-// generic BF16 GEMM of the training passes, and the elementwise training kernels (bias + ReLU,
-// output loss gradient, ReLU backward, bias gradients, Adam).
-//
-// Matrix-core layout (v_mfma_f32_32x32x16_bf16, one wave): the A operand is a 32 x 16 tile,
-// lane l holding row l%32 and the 8 k-values 8*(l/32) .. +7; the B operand is 16 x 32, lane l
-// holding column l%32 and the same 8 k-values; the 32 x 32 result leaves lane l with column l%32
-// and rows 8*(j/4) + 4*(l/32) + j%4 for its 16 accumulators j.
-//
-// The inference pass computes every layer transposed, Z^T = W^T A^T: the output units are the
-// rows, the data rows the columns. A lane then holds 16 units of one data row after each layer,
-// and the 8 accumulators j = 8s .. 8s+7 of unit tile t are exactly the B-operand values of the
-// next layer's k-step over units 32t + 16s .. +15, provided the weights of that k-step are
-// ordered so that value i of lane half h is input unit 16q + (i < 4 ? 4h + i : 8 + 4h + i - 4).
-// Activations therefore never leave the registers; the packed weights sit in LDS.
-#include "mlp.h"
+// gfx950 kernels of the warm-start network: the BF16 matrix-core inference pass, the generic BF16
+// GEMM of the training passes, and the elementwise training kernels (bias + ReLU, output loss
+// gradient, ReLU backward, bias gradients, Adam). The forward pass and its operand layout are in
+// mlp_device.h.
+#include "mlp_device.h"
 
 namespace sbs {
 namespace mlp {
 
 namespace {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-
 // Row-major BF16 weight copies, same order as the FP32 parameter vector without the biases.
 constexpr std::size_t kR1 = 0, kR2 = kR1 + kIn * kHidden, kR3 = kR2 + kHidden * kHidden,
                       kR4 = kR3 + kHidden * kHidden, kRTotal = kR4 + kHidden * kOut;
-// Offsets (in bf16x8 units) of the layers in the packed weights.
-constexpr int kP1 = 0, kP2 = int(kPack1 / 8), kP3 = kP2 + int(kPack23 / 8), kP4 = kP3 + int(kPack23 / 8);
-// Offsets of the biases in the contiguous bias array.
-constexpr int kb1 = 0, kb2 = kHidden, kb3 = 2 * kHidden, kb4 = 3 * kHidden;
-
-__device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c)
-{
-    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
-}
 
 __global__ void k_pack(const float* __restrict__ P, bf16* __restrict__ pack, bf16* __restrict__ wrow,
                        float* __restrict__ bias)
@@ -55,7 +32,7 @@ __global__ void k_pack(const float* __restrict__ P, bf16* __restrict__ pack, bf1
             L = 4;
             e -= kPack1 + 2 * kPack23;
         }
-        int const Q = (L == 1) ? 1 : 8;
+        int const Q = (L == 1) ? kQ1 : kQH;
         int const v = int(e % 8), lane = int((e / 8) % 64), q = int((e / 512) % Q), T = int(e / (512 * Q));
         int const h = lane / 32, c = lane % 32;
         int const in_unit = 16 * q + (L == 1 ? 8 * h + v : (v < 4 ? 4 * h + v : 8 + 4 * h + v - 4));
@@ -85,58 +62,8 @@ __global__ void k_pack(const float* __restrict__ P, bf16* __restrict__ pack, bf1
     }
 }
 
-// bias + ReLU of one 32-unit tile of accumulators, rounded to BF16 (lane half h, bias of the
-// layer starting at b).
-__device__ __forceinline__ void relu_tile(f32x16 const& acc, const float* b, int T, int h, bf16x8& lo, bf16x8& hi)
-{
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        int const unit = 32 * T + 8 * (j / 4) + 4 * h + (j % 4);
-        float const z = fmaxf(acc[j] + b[unit], 0.0f);
-        if (j < 8)
-            lo[j] = static_cast<bf16>(z);
-        else
-            hi[j - 8] = static_cast<bf16>(z);
-    }
-}
-
-// One hidden layer 128 -> 128 from register activations act[t][s] (tile t, half s).
-__device__ __forceinline__ void hidden_layer(const bf16x8* sw, int poff, const float* b, int lane,
-                                             bf16x8 (&act)[4][2])
-{
-    int const h = lane >> 5;
-    f32x16 acc[4];
-#pragma unroll
-    for (int T = 0; T < 4; ++T) {
-        acc[T] = f32x16{};
-#pragma unroll
-        for (int q = 0; q < 8; ++q) acc[T] = mfma(sw[poff + (T * 8 + q) * 64 + lane], act[q >> 1][q & 1], acc[T]);
-    }
-#pragma unroll
-    for (int T = 0; T < 4; ++T) relu_tile(acc[T], b, T, h, act[T][0], act[T][1]);
-}
-
-// Forward pass of one 32-row tile; returns the output accumulators (units 0..3 of the row l%32
-// in accumulators 0..3 of the lanes l < 32).
-__device__ __forceinline__ f32x16 forward_tile(const bf16x8* sw, const float* sb, bf16x8 xin, int lane)
-{
-    int const h = lane >> 5;
-    bf16x8 act[4][2];
-#pragma unroll
-    for (int T = 0; T < 4; ++T) {
-        f32x16 const acc = mfma(sw[kP1 + T * 64 + lane], xin, f32x16{});
-        relu_tile(acc, sb + kb1, T, h, act[T][0], act[T][1]);
-    }
-    hidden_layer(sw, kP2, sb + kb2, lane, act);
-    hidden_layer(sw, kP3, sb + kb3, lane, act);
-    f32x16 y{};
-#pragma unroll
-    for (int q = 0; q < 8; ++q) y = mfma(sw[kP4 + q * 64 + lane], act[q >> 1][q & 1], y);
-#pragma unroll
-    for (int o = 0; o < kMaxOutputs; ++o) y[o] += sb[kb4 + o];
-    return y;
-}
-
+// Persistent inference over 32-row tiles of X[n][48] (BF16), weights staged in LDS; epi(row, y,
+// h) receives the output accumulators of a real row for the lane of half h.
 template <class Epilogue>
 __device__ __forceinline__ void infer_rows(const bf16x8* __restrict__ pack, const float* __restrict__ bias,
                                            const bf16x8* __restrict__ X, uint32_t n, Epilogue&& epi)
@@ -150,13 +77,13 @@ __device__ __forceinline__ void infer_rows(const bf16x8* __restrict__ pack, cons
     uint32_t const waves = blockDim.x >> 6;
     uint32_t const tiles = (n + 31) / 32;
     for (uint32_t tile = blockIdx.x * waves + wave; tile < tiles; tile += gridDim.x * waves) {
-        // keeps the weight reads inside the loop (hoisted, they would need ~300 registers)
-        __asm__ volatile("" ::: "memory");
         uint32_t const row = tile * 32 + uint32_t(c);
-        bf16x8 xin{};
-        if (row < n) xin = X[std::size_t(row) * 2 + h];
-        f32x16 const y = forward_tile(sw, sb, xin, lane);
-        if (h == 0 && row < n) epi(row, y);
+        bf16x8 xin[1][kQ1];
+#pragma unroll
+        for (int q = 0; q < kQ1; ++q) xin[0][q] = row < n ? X[std::size_t(row) * (kIn / 8) + 2 * q + h] : bf16x8{};
+        f32x16 y[1];
+        forward_tiles<1>(sw, sb, xin, lane, y);
+        if (row < n) epi(row, y[0], h);
     }
 }
 
@@ -164,20 +91,12 @@ __global__ void __launch_bounds__(256) k_infer(const bf16x8* __restrict__ pack, 
                                                const bf16x8* __restrict__ X, uint32_t n, int nout,
                                                const float* __restrict__ scale, float* __restrict__ Y)
 {
-    infer_rows(pack, bias, X, n, [&](uint32_t row, f32x16 const& y) {
+    infer_rows(pack, bias, X, n, [&](uint32_t row, f32x16 const& y, int h) {
 #pragma unroll
-        for (int o = 0; o < kMaxOutputs; ++o) Y[std::size_t(row) * kMaxOutputs + o] = o < nout ? scale[o] * y[o] : 0.0f;
-    });
-}
-
-__global__ void __launch_bounds__(256) k_infer_lambda(const bf16x8* __restrict__ pack, const float* __restrict__ bias,
-                                                      const bf16x8* __restrict__ X, uint32_t n, int nout,
-                                                      const float* __restrict__ scale, float* __restrict__ lambda)
-{
-    infer_rows(pack, bias, X, n, [&](uint32_t row, f32x16 const& y) {
-#pragma unroll
-        for (int o = 0; o < kMaxOutputs; ++o)
-            if (o < nout) lambda[std::size_t(row) * nout + o] += scale[o] * y[o];
+        for (int j = 0; j < 16; ++j) {
+            int const o = acc_unit(j, h);
+            if (o < kMaxOutputs) Y[std::size_t(row) * kMaxOutputs + o] = o < nout ? scale[o] * y[j] : 0.0f;
+        }
     });
 }
 
@@ -353,14 +272,6 @@ void launch_infer(hipStream_t s, const bf16* pack, const float* bias, const bf16
     if (n == 0) return;
     k_infer<<<blocks, 256, 0, s>>>(reinterpret_cast<const bf16x8*>(pack), bias, reinterpret_cast<const bf16x8*>(X), n,
                                    num_outputs, out_scale, Y);
-}
-
-void launch_infer_lambda(hipStream_t s, const bf16* pack, const float* bias, const bf16* X, uint32_t n,
-                         int num_outputs, const float* out_scale, float* lambda, int blocks)
-{
-    if (n == 0) return;
-    k_infer_lambda<<<blocks, 256, 0, s>>>(reinterpret_cast<const bf16x8*>(pack), bias,
-                                          reinterpret_cast<const bf16x8*>(X), n, num_outputs, out_scale, lambda);
 }
 
 void launch_normalize(hipStream_t s, const float* raw, uint32_t ld, int num_inputs, const uint32_t* rows, uint32_t n,

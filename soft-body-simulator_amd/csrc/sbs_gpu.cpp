@@ -64,9 +64,9 @@ struct group_t {
     dev_buf<float> lambda;
     // multiplier warm start (neo-Hookean groups)
     dev_buf<float2> ws_hist;     // the multiplier pairs one substep before the stored ones
-    dev_buf<float> ws_raw;       // [count][13] features at the substep start (capture)
+    dev_buf<float> ws_raw;       // hex groups: [ws_rows][40] network features at the substep start (capture)
     dev_buf<float> ws_bsize;     // per constraint: log of the number of tets of its body
-    dev_buf<mlp::bf16> ws_X;     // [count][16] network inputs (learned warm start)
+    dev_buf<uint32_t> ws_rowof_dev;  // hex groups: cluster -> its row (capture)
     dev_buf<float4> ws_disp;     // hex groups: slot displacements of the warm-start pass, [8][rows]
     dev_buf<uint32_t> ws_csr_off, ws_csr;  // particle -> slot list
     // hex groups, the all-color launches: rows in 64-row workgroups, workgroup b holding clusters
@@ -87,6 +87,8 @@ struct group_t {
 };
 
 constexpr std::size_t kMaxContactSlots = 8;   // contacts kept per surface vertex
+// shape of the warm-start network: the features and multiplier changes of one hex cluster
+constexpr int kWarmStartInputs = 39, kWarmStartOutputs = 12;
 constexpr std::size_t kHashMinObstacles = 32;  // automatic mode hashes from this many bounded obstacles on
 constexpr uint64_t kMaxHashCells = 1u << 16;   // cells of one hashed obstacle volume, at most
 
@@ -177,6 +179,7 @@ struct sbs_gpu {
     sbs_mlp* ws_net = nullptr;
     sbs_mlp* cap_net = nullptr;
     uint32_t cap_stride = 0;
+    uint64_t cap_substeps = 0;  // captured substeps so far (moves the captured clusters' offset)
     // snapshot slots: copies of the dynamic state (committed positions, velocities, normals, and per
     // group the multipliers and their warm-start history)
     struct snap_slot {
@@ -267,6 +270,12 @@ struct sbs_gpu {
     void build_hash();
 
     void set_current() { hip_check(hipSetDevice(device), "hipSetDevice"); }
+    bool all_nh_hex() const
+    {
+        for (auto const& g : groups)
+            if (g.kind == 1 && g.count && !g.hex) return false;
+        return true;
+    }
     // SBS_ERR_CONTACT_OVERFLOW once a finished frame has dropped a contact (the pinned word is
     // written by the frame after its detection; read without waiting)
     void check_overflow() const
@@ -286,7 +295,7 @@ struct sbs_gpu {
     void run_frame(double dt, uint32_t iterations, uint32_t substeps, int warm, bool check = false);
     void detect();
     void launch_group_colors(group_t const& g, double dt, int mode, const pos4* xref);
-    void ensure_warm_buffers(bool raw, bool X);
+    void ensure_warm_buffers(bool raw);
     void ensure_cinfo(group_t& g);
     void snapshot(int slot);
     void prepare_snapshot(int slot);
@@ -879,28 +888,25 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
         for (auto& g : groups) {
             if (g.kind != 1) continue;
             bool const learned = warm == SBS_WARM_START_LEARNED;
-            if (capture || learned) {
-                timed(SBS_KERNEL_WARM_START, g.count, [&] {
-                    launch_ws_features(stream, g.tet(), xi.p, g.ws_hist.p, g.ws_bsize.p, g.count, float(1.0 / dt2),
-                                       capture ? g.ws_raw.p : nullptr, learned ? g.ws_X.p : nullptr,
-                                       learned ? ws_net->norm.p : nullptr, block);
-                    if (learned)
-                        mlp::launch_infer_lambda(stream, ws_net->pack.p, ws_net->bias.p, g.ws_X.p, g.count, 2,
-                                                 ws_net->out_scale_dev(), g.lambda.p, 512);
-                });
-            }
             if (warm != SBS_WARM_START_NONE) {
                 int const mode = warm == SBS_WARM_START_EXPLICIT ? kApplyExplicit : kApplyWarmStart;
-                if (g.hex)
+                if (g.hex) {
+                    // the network's features (capture) or its predicted multiplier change (learned)
+                    // are formed inside the pass's cluster kernel
+                    int const net_mode = learned ? kWarmNetLearned : (capture ? kWarmNetFeatures : kWarmNetNone);
+                    warm_net_dev wn{g.ws_hist.p, g.ws_bsize.p, capture ? g.ws_raw.p : nullptr,
+                                    learned ? ws_net->pack.p : nullptr, learned ? ws_net->bias.p : nullptr,
+                                    learned ? ws_net->norm.p : nullptr};
                     timed(SBS_KERNEL_WARM_START, g.count, [&] {
                         launch_hex_warm_start(stream, g.tet(), g.clusters(), xtilde, xi.p, g.ws_cinfo.p,
                                               g.ws_rows, float(1.0 / dt2), mode == kApplyExplicit,
                                               make_float2(float(ws_scale), float(ws_scale_hyd)), g.ws_disp.p,
                                               g.ws_csr_off.p, g.ws_csr.p, V, fused0 ? surface_of_dev.p : nullptr, c,
-                                              c_at, prow.p, uint32_t(prow.n));
+                                              c_at, prow.p, uint32_t(prow.n), net_mode, wn);
                     }, 2);
-                else
+                } else {
                     timed(SBS_KERNEL_WARM_START, g.count, [&] { launch_group_colors(g, dt, mode, xt.p); }, g.ncolors);
+                }
                 warmed = true;
             }
         }
@@ -918,11 +924,14 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
         if (capture) {
             for (auto& g : groups) {
                 if (g.kind != 1) continue;
-                uint64_t const want = (uint64_t(g.count) + cap_stride - 1) / cap_stride;
+                // clusters offset + j * stride; the offset moves from substep to substep
+                uint32_t const nc = g.num_elements();
+                uint32_t const offset = uint32_t((uint64_t(cap_substeps++) * 7919u) % cap_stride) % nc;
+                uint64_t const want = (uint64_t(nc - offset) + cap_stride - 1) / cap_stride;
                 uint64_t const room = cap_net->ds_cap - cap_net->ds_size;
                 uint32_t const rows = uint32_t(std::min(want, room));
-                launch_ws_capture(stream, g.ws_raw.p, g.lambda.p, cap_stride, 0, cap_net->ds_x.p, cap_net->ds_t.p,
-                                  cap_net->ds_size, rows);
+                launch_hex_capture(stream, g.tet(), g.ws_cinfo.p, g.ws_rowof_dev.p, g.ws_raw.p, cap_stride, offset,
+                                   float(1.0 / dt2), cap_net->ds_x.p, cap_net->ds_t.p, cap_net->ds_size, rows);
                 cap_net->ds_size += rows;
             }
         }
@@ -974,7 +983,7 @@ void sbs_gpu::ensure_cinfo(group_t& g)
     g.ws_cinfo.upload(rec.data(), rec.size(), stream);
 }
 
-void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
+void sbs_gpu::ensure_warm_buffers(bool raw)
 {
     if (!xt.p) xt.alloc(V);
     std::vector<int8_t> preg;  // particle -> XCD region of the first hex cluster (execution order) holding it
@@ -1025,8 +1034,10 @@ void sbs_gpu::ensure_warm_buffers(bool raw, bool X)
             g.ws_hist.alloc(g.count);
             hip_check(hipMemsetAsync(g.ws_hist.p, 0, g.count * sizeof(float2), stream), "clear history");
         }
-        if (raw && !g.ws_raw.p) g.ws_raw.alloc(std::size_t(g.count) * mlp::kMaxInputs);
-        if (X && !g.ws_X.p) g.ws_X.alloc(std::size_t(g.count) * mlp::kIn);
+        if (raw && g.hex && !g.ws_raw.p) {
+            g.ws_raw.alloc(std::size_t(g.ws_rows) * mlp::kMaxInputs);
+            g.ws_rowof_dev.upload(g.ws_rowof.data(), g.ws_rowof.size(), stream);
+        }
     }
     if (!prow.p && !preg.empty()) {
         // per region its particles in index order (particles of no hex cluster by index eighths),
@@ -1670,6 +1681,8 @@ int sbs_gpu_step(sbs_gpu* sim, double dt, uint32_t iterations, uint32_t substeps
         require(dt > 0.0 && std::isfinite(dt), "dt must be positive");
         require(substeps >= 1, "substeps must be >= 1");
         sim->check_overflow();
+        require(!(sim->cap_net && sim->cap_stride) || sim->ws_mode != SBS_WARM_START_NONE,
+                "capture needs a warm start: its pass forms the features", SBS_ERR_STATE);
         sim->set_current();
         bool const check = sim->fb_limit > 0.0 && sim->ws_mode != SBS_WARM_START_NONE;
         // with the check the frame itself snapshots the state and ends with its residual
@@ -2128,15 +2141,17 @@ int sbs_gpu_set_warm_start(sbs_gpu* sim, int mode, sbs_mlp* net)
         require(mode >= SBS_WARM_START_NONE && mode <= SBS_WARM_START_EXPLICIT, "unknown warm start mode");
         if (mode == SBS_WARM_START_LEARNED) {
             require(net != nullptr, "the learned warm start needs a network");
-            require(net->nin == uint32_t(mlp::kMaxInputs) && net->nout == 2,
-                    "the warm-start network must have 13 inputs and 2 outputs");
+            require(net->nin == uint32_t(kWarmStartInputs) && net->nout == uint32_t(kWarmStartOutputs),
+                    "the warm-start network must have 39 inputs and 12 outputs");
+            require(sim->all_nh_hex(), "the learned warm start needs hex-cell clusters in every neo-Hookean group",
+                    SBS_ERR_STATE);
             require(net->device == sim->device, "network and simulation are on different devices");
         }
         sim->set_current();
         // a cached frame graph holds the network's buffers by address: a network created where a
         // destroyed one lived must not replay the old graph
         if (mode == SBS_WARM_START_LEARNED) sim->drop_graphs();
-        sim->ensure_warm_buffers(false, mode == SBS_WARM_START_LEARNED);
+        sim->ensure_warm_buffers(false);
         sim->ws_mode = mode;
         sim->ws_net = mode == SBS_WARM_START_LEARNED ? net : nullptr;
         hip_check(hipStreamSynchronize(sim->stream), "warm start");
@@ -2156,10 +2171,11 @@ int sbs_gpu_capture(sbs_gpu* sim, sbs_mlp* net, uint32_t stride)
             return;
         }
         require(net != nullptr, "net is null");
-        require(net->nin == uint32_t(mlp::kMaxInputs) && net->nout == 2,
-                "the warm-start network must have 13 inputs and 2 outputs");
+        require(net->nin == uint32_t(kWarmStartInputs) && net->nout == uint32_t(kWarmStartOutputs),
+                "the warm-start network must have 39 inputs and 12 outputs");
+        require(sim->all_nh_hex(), "capture needs hex-cell clusters in every neo-Hookean group", SBS_ERR_STATE);
         require(net->device == sim->device, "network and simulation are on different devices");
-        sim->ensure_warm_buffers(true, false);
+        sim->ensure_warm_buffers(true);
         hip_check(hipStreamSynchronize(net->stream), "capture");
         sim->cap_net = net;
         sim->cap_stride = stride;

@@ -319,7 +319,7 @@ def sort_pairs(keys, vals, bits=32, device=0):
 class Mlp:
     """The warm-start network behind the C ABI (one sbs_mlp handle)."""
 
-    def __init__(self, num_inputs=12, num_outputs=2, seed=0, device=0):
+    def __init__(self, num_inputs=39, num_outputs=12, seed=0, device=0):
         self._L = lib()
         h = _p()
         check(self._L.sbs_mlp_create(device, num_inputs, num_outputs, seed, ctypes.byref(h)))
@@ -607,7 +607,7 @@ class Simulation:
     warm_net = None
 
     def set_warm_start(self, mode, net=None):
-        """mode: "none", "previous", "explicit" or "learned" (needs net, an Mlp(13, 2))."""
+        """mode: "none", "previous", "explicit" or "learned" (needs net, an Mlp(39, 12))."""
         check(self._L.sbs_gpu_set_warm_start(self.h, WARM_START[mode], None if net is None else net.h))
         self.warm_mode, self.warm_net = mode, net
 

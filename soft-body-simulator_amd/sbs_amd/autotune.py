@@ -100,40 +100,103 @@ def _scales(s):
 SCALE_GRID = tuple((d, h) for d in (0.5, 0.55, 0.6, 0.65, 0.7, 0.75) for h in (0.5, 0.55, 0.6, 0.65, 0.7, 0.75))
 
 
-def train_warm_start(sim, scene, frames=6, iterations=24, scale=0.5, max_rows=4_000_000, steps=1500, batch=16384,
-                     lr=1e-3, seed=0, log=print):
-    """Captures training rows from a previous-multiplier warm-started trajectory (`scale`, one factor
-    or a (deviatoric, hydrostatic) pair, `iterations` per substep) starting at the current state,
-    trains a 13-input, 2-output network on the device and restores the state. Returns the network."""
-    rows_per_substep = sum(len(g.conn) for g in scene.groups if g.kind == "neohookean")
-    stride = max(1, -(-rows_per_substep * scene.substeps * frames // max_rows))
-    net = Mlp(13, 2, seed=seed)
-    net.reserve(max_rows)
-    sim.snapshot()
-    sim.set_warm_start("previous")
-    sim.set_warm_start_scale(*_scales(scale))
-    sim.capture(net, stride)
-    for _ in range(frames):
-        sim.step(scene.dt, iterations, scene.substeps)
-    sim.capture(None, 0)
+WARM_NET_INPUTS, WARM_NET_OUTPUTS = 39, 12  # per hex cluster: features in, multiplier changes out
+
+
+def _random_direction(rng):
+    v = rng.standard_normal(3)
+    return v / np.linalg.norm(v)
+
+
+def capture_episodes(sim, scene, net, episodes, frames, iterations, scale, stride, rng, impulse=(0.5, 2.0),
+                     radius=(0.1, 0.5), log=print):
+    """Training rows from `episodes` trajectories of `frames` frames each, all starting at the state in
+    snapshot slot 1, with the previous-multiplier warm start (`iterations`, `scale`): each episode
+    under gravity of the scene's magnitude in a random direction, and with an impulse load at its
+    start (a velocity change of U[impulse] m/s in a random direction added to the movable particles
+    within U[radius] m of a random particle of a random body). Rows go to net's dataset (capture
+    stride `stride`). The scene's gravity is restored at the end; the state is left as the last
+    episode ends."""
+    g = float(np.linalg.norm(scene.gravity))
+    for e in range(episodes):
+        sim.restore(1)
+        d = _random_direction(rng)
+        sim.set_params(gravity=tuple(g * d), gravity_is_force=scene.gravity_is_force,
+                       collision_compliance=scene.collision_compliance)
+        b = int(rng.integers(len(scene.bodies)))
+        x, v = sim.read_body(b)
+        c = x[int(rng.integers(len(x)))]
+        r = float(rng.uniform(*radius))
+        sel = np.linalg.norm(x - c, axis=1) <= r
+        if scene.bodies[b].mass is not None:
+            sel &= np.asarray(scene.bodies[b].mass) > 0
+        v = v.copy()
+        v[sel] += float(rng.uniform(*impulse)) * _random_direction(rng)
+        sim.write_body(b, None, v)
+        sim.set_warm_start("previous")
+        sim.set_warm_start_scale(*_scales(scale))
+        sim.capture(net, stride)
+        for _ in range(frames):
+            sim.step(scene.dt, iterations, scene.substeps)
+        sim.capture(None, 0)
+    sim.set_params(gravity=scene.gravity, gravity_is_force=scene.gravity_is_force,
+                   collision_compliance=scene.collision_compliance)
     sim.set_warm_start("none")
-    sim.restore()
+
+
+def train_warm_start(sim, scene, iterations=4, scale=(0.7, 0.65), frames=2000, episodes=40, val_episodes=4,
+                     max_rows=4_000_000, steps=3000, batch=16384, lr=1e-3, seed=0, log=print):
+    """The learned warm start's network, trained on the device on trajectories the solver generates:
+    `frames` frames in `episodes` episodes from the current state under random gravity directions and
+    impulse loads (capture_episodes), captured with the previous-multiplier warm start (`iterations`,
+    `scale`); one row per sampled hex cluster and substep (the capture stride keeps about max_rows
+    rows). `val_episodes` more episodes with their own loads are held out: the normalized MSE on them
+    is reported beside the training loss. Adam on minibatches of `batch` rows for `steps` steps.
+    Returns the network and a record of the training; the state is restored."""
+    clusters = sum(len(g.clusters) - 1 for g in scene.groups if g.kind == "neohookean" and g.clusters is not None)
+    per_episode = max(1, frames // episodes)
+    stride = max(1, -(-clusters * scene.substeps * per_episode * episodes // max_rows))
+    rng = np.random.default_rng(seed)
+    net = Mlp(WARM_NET_INPUTS, WARM_NET_OUTPUTS, seed=seed)
+    net.reserve(max_rows)
+    val = Mlp(WARM_NET_INPUTS, WARM_NET_OUTPUTS, seed=seed)
+    val.reserve(max(1, max_rows * val_episodes // max(episodes, 1)))
+    sim.snapshot(1)
+    t0 = time.perf_counter()
+    capture_episodes(sim, scene, net, episodes, per_episode, iterations, scale, stride, rng, log=log)
+    capture_episodes(sim, scene, val, val_episodes, per_episode, iterations, scale, stride, rng, log=log)
+    cap_s = time.perf_counter() - t0
+    sim.restore(1)
     n = net.dataset_size()
     net.fit_normalization()
     t0 = time.perf_counter()
     loss = net.train(steps, batch, lr, seed=seed + 1)
-    el = time.perf_counter() - t0
-    log(f"warm-start network: {n} rows (stride {stride}), {steps} Adam steps of {batch} in "
-        f"{el:.1f}s, last loss {loss:.4f} (normalized MSE)")
-    return net, {"rows": n, "stride": stride, "frames": frames, "steps": steps, "batch": batch, "train_s": round(el, 2),
-                 "last_loss": loss}
+    train_s = time.perf_counter() - t0
+    xv, tv = val.dataset()
+    val_loss, _ = net.loss_grad(xv, tv)
+    # the same error for the network that predicts no change (the previous-multiplier start)
+    zero = float(np.mean((tv / net.get_normalization()[2]) ** 2))
+    val.close()
+    log(f"warm-start network: {n} rows from {episodes} x {per_episode} frames (stride {stride}, capture {cap_s:.1f}s), "
+        f"{steps} Adam steps of {batch} in {train_s:.1f}s, loss {loss:.4f}; held out ({len(xv)} rows of "
+        f"{val_episodes} episodes): {val_loss:.4f} (no change predicted: {zero:.4f})")
+    return net, {"rows": n, "stride": stride, "episodes": episodes, "frames_per_episode": per_episode,
+                 "frames": episodes * per_episode, "captured_with": {"iterations": iterations, "scale": scale},
+                 "steps": steps, "batch": batch, "capture_s": round(cap_s, 2), "train_s": round(train_s, 2),
+                 "train_loss": loss, "held_out_rows": int(len(xv)), "held_out_loss": val_loss,
+                 "held_out_loss_no_change": zero,
+                 "loads": "gravity of the scene's magnitude in a random direction per episode, and a velocity "
+                          "impulse U[0.5, 2] m/s on the particles within U[0.1, 0.5] m of a random particle"}
 
 
-def training_config(args):
-    """Keyword arguments of train_warm_start from the bench's command line."""
+def training_config(args, found=None):
+    """Keyword arguments of train_warm_start from the bench's command line and the tuned
+    previous-multiplier configuration (captured trajectories use it)."""
     out = {}
     if getattr(args, "train_frames", None):
         out["frames"] = args.train_frames
+    if found:
+        out["iterations"], out["scale"] = found["iterations"], found["scale"]
     return out
 
 

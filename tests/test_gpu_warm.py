@@ -8,7 +8,7 @@ import mlp_oracle as M
 import sbs_amd as S
 from helpers import build_oracle
 from sbs_amd import scenes
-from test_oracle_warm import nh_energy, random_params
+from test_oracle_warm import NIN, nh_energy, random_params
 
 pytestmark = pytest.mark.gpu
 
@@ -19,19 +19,23 @@ def say(msg):
     print(msg, flush=True)
 
 
+NOUT = 12  # outputs of the warm-start network (a hex cluster's multiplier changes)
+
+
 @pytest.mark.parametrize("n", [1, 33, 1000])
 def test_fused_inference_matches_oracle(n):
     say(f"mlp forward n={n}")
     rng = np.random.default_rng(n)
-    net = S.Mlp(12, 3, seed=5)
+    net = S.Mlp(NIN, NOUT, seed=5)
     p = random_params(rng).astype(np.float32)
     net.set_params(p)
-    mean, inv_std, scale = rng.normal(0, 1, 12), np.exp(rng.normal(0, 0.3, 12)), np.array([0.5, 2.0, 1.5])
+    mean, inv_std = rng.normal(0, 1, NIN), np.exp(rng.normal(0, 0.3, NIN))
+    scale = np.exp(rng.normal(0, 0.5, NOUT))
     net.set_normalization(mean, inv_std, scale)
-    x = rng.normal(0, 2, (n, 12))
+    x = rng.normal(0, 2, (n, NIN))
     y = net.forward(x)
     yo, _ = M.forward(p, M.normalize(x, mean, inv_std, True), True)
-    yo = yo[:, :3] * scale
+    yo = yo[:, :NOUT] * scale
     assert np.all(np.isfinite(y))
     assert np.abs(y - yo).max() <= 1e-2 * max(1.0, np.abs(yo).max())
 
@@ -39,14 +43,15 @@ def test_fused_inference_matches_oracle(n):
 def test_training_gradient_matches_oracle():
     say("mlp loss gradient")
     rng = np.random.default_rng(11)
-    net = S.Mlp(12, 2, seed=1)
+    net = S.Mlp(NIN, NOUT, seed=1)
     p = random_params(rng).astype(np.float32)
     net.set_params(p)
-    net.set_normalization(np.zeros(12), np.ones(12), np.array([1.0, 3.0]))
-    x = rng.normal(0, 1, (517, 12))
-    t = rng.normal(0, 1, (517, 2)) * np.array([1.0, 3.0])
+    os_ = np.linspace(0.5, 3.0, NOUT)
+    net.set_normalization(np.zeros(NIN), np.ones(NIN), os_)
+    x = rng.normal(0, 1, (517, NIN))
+    t = rng.normal(0, 1, (517, NOUT)) * os_
     loss, g = net.loss_grad(x, t)
-    lo, go = M.loss_grad(p, M.normalize(x, np.zeros(12), np.ones(12), True), t / np.array([1.0, 3.0]), 2, True)
+    lo, go = M.loss_grad(p, M.normalize(x, np.zeros(NIN), np.ones(NIN), True), t / os_, NOUT, True)
     assert loss == pytest.approx(lo, rel=1e-3)
     for (a, b) in ((M.W1, M.B1), (M.B1, M.W2), (M.W2, M.B2), (M.B2, M.W3), (M.W3, M.B3), (M.B3, M.W4), (M.W4, M.B4),
                    (M.B4, M.NPARAMS)):
@@ -57,9 +62,9 @@ def test_training_gradient_matches_oracle():
 def test_training_fits_a_smooth_function():
     say("mlp training")
     rng = np.random.default_rng(2)
-    x = rng.normal(0, 1, (20000, 12)).astype(np.float32)
+    x = rng.normal(0, 1, (20000, NIN)).astype(np.float32)
     t = np.stack([np.sin(x[:, 0]) + 0.5 * x[:, 1] * x[:, 2], np.tanh(x[:, 3] - x[:, 4])], 1).astype(np.float32)
-    net = S.Mlp(12, 2, seed=3)
+    net = S.Mlp(NIN, 2, seed=3)
     net.reserve(len(x))
     net.append(x, t)
     assert net.dataset_size() == len(x)
@@ -156,62 +161,156 @@ def test_device_explicit_pass_is_an_explicit_elastic_step():
     assert np.abs(xs - expect).max() <= 1e-4 * np.abs(xs - xt).max()
 
 
+def _cluster_constraint_values(sc, x, dt):
+    """Per hex cluster (insertion order) of group 0: (C_dev, C_hyd) of its 6 tets at positions x
+    (12 values), log alpha~ of tet 0 (deviatoric, hydrostatic), from the rest shape in FP64."""
+    g = sc.groups[0]
+    x0 = np.concatenate([b.x0 for b in sc.bodies])
+    base = np.cumsum([0] + [len(b.x0) for b in sc.bodies])[:-1]
+    body = np.zeros(len(g.conn), dtype=int) if np.ndim(g.body) == 0 else np.asarray(g.body)
+    tets = np.asarray(g.conn).reshape(-1, 4) + base[body][:, None]
+    young = np.broadcast_to(g.young, len(tets))
+    nu = np.broadcast_to(g.poisson, len(tets))
+    mu = young / (2 * (1 + nu))
+    la = young * nu / ((1 + nu) * (1 - 2 * nu))
+    Dm = np.stack([x0[tets[:, k]] - x0[tets[:, 3]] for k in range(3)], axis=2)
+    Ds = np.stack([x[tets[:, k]] - x[tets[:, 3]] for k in range(3)], axis=2)
+    F = Ds @ np.linalg.inv(Dm)
+    V0 = np.abs(np.linalg.det(Dm)) / 6
+    C = np.stack([np.sqrt(np.sum(F * F, axis=(1, 2))), np.linalg.det(F) - 1 - mu / la], 1).reshape(-1, 12)
+    ad, ah = (1 / (mu * V0) / dt ** 2).reshape(-1, 6), (1 / (la * V0) / dt ** 2).reshape(-1, 6)
+    return C, np.log(ad[:, 0]), np.log(ah[:, 0])
+
+
+def _match_rows(rows, ref):
+    """Index of the reference row nearest to each row (a permutation is expected)."""
+    d = ((rows[:, None, :] - ref[None, :, :]) ** 2).sum(-1)
+    m = d.argmin(1)
+    assert len(np.unique(m)) == len(m)
+    return m
+
+
 def test_capture_rows_and_features():
+    # one row per hex cluster and substep (stride 1): the 39 features at the substep start and the
+    # change of alpha~ lambda over the substep, captured along a previous-multiplier trajectory
     sc = _beam(True, iterations=4)
     say("capture")
     dev = scenes.build_device(sc)
-    net = S.Mlp(13, 2)
-    T = sc.num_tets
-    net.reserve(10 * T)
+    nc = len(sc.groups[0].clusters) - 1
+    net = S.Mlp(NIN, NOUT)
+    net.reserve(10 * nc)
+    dev.set_warm_start("previous")
+    dev.set_warm_start_scale(0.5)
     dev.capture(net, 1)
     dev.step(sc.dt, sc.iterations, sc.substeps)  # 2 substeps
     dev.capture(None, 0)
-    assert net.dataset_size() == 2 * T
+    assert net.dataset_size() == 2 * nc
     x, tt = net.dataset()
     x, tt = x.astype(np.float64), tt.astype(np.float64)
-    f0, f1 = x[:T], x[T:]
-    # lambda_prev of substep 2 is lambda_prev + change of substep 1; its lambda_prev2 is substep 1's lambda_prev
-    assert np.allclose(f1[:, 0:2], f0[:, 0:2] + tt[:T], rtol=1e-6, atol=1e-12)
-    assert np.array_equal(f1[:, 2:4], f0[:, 0:2])
-    assert np.all(f0[:, 0:4] == 0)  # multipliers start at zero
+    f0, f1, t0 = x[:nc], x[nc:], tt[:nc]
+    assert np.all(x[:, 39:] == 0) and np.all(tt[:, 12:] == 0)
+    # substep 2 starts from substep 1's final multipliers; its history is substep 1's start
+    assert np.allclose(f1[:, 0:12], f0[:, 0:12] + t0[:, 0:12], rtol=1e-5, atol=1e-6 * np.abs(t0).max())
+    assert np.array_equal(f1[:, 12:24], f0[:, 0:12])
+    assert np.all(f0[:, 0:24] == 0)  # multipliers start at zero
     # features of substep 1 at the predicted positions x~ = x0 + g dt^2 (gravity as acceleration)
     b = sc.bodies[0]
     dt = sc.dt / sc.substeps
     w = np.where(b.mass > 0, 1.0 / np.where(b.mass > 0, b.mass, 1.0), 0.0)
     xt = b.x0 + (w > 0)[:, None] * np.array([0.0, -9.81, 0.0]) * dt * dt
-    order = _execution_order(dev, sc)
-    tets = np.asarray(sc.groups[0].conn).reshape(-1, 4)[order]
-    mu = 5e5 / (2 * 1.45)
-    la = 5e5 * 0.45 / (1.45 * 0.1)
-    Dm = np.stack([b.x0[tets[:, k]] - b.x0[tets[:, 3]] for k in range(3)], axis=2)
-    Ds = np.stack([xt[tets[:, k]] - xt[tets[:, 3]] for k in range(3)], axis=2)
-    F = Ds @ np.linalg.inv(Dm)
-    V0 = np.abs(np.linalg.det(Dm)) / 6
-    Cd = np.sqrt(np.sum(F * F, axis=(1, 2)))
-    Ch = np.linalg.det(F) - 1 - mu / la
-    ad, ah = 1 / (mu * V0) / dt ** 2, 1 / (la * V0) / dt ** 2
-    assert np.allclose(f0[:, 4], Cd, rtol=1e-5)
-    assert np.allclose(f0[:, 5], Ch, rtol=1e-3, atol=1e-5)
-    assert np.allclose(f0[:, 8], np.log(ad), rtol=1e-5) and np.allclose(f0[:, 9], np.log(ah), rtol=1e-5)
-    assert np.allclose(f0[:, 10], -Cd / ad, rtol=1e-5) and np.allclose(f0[:, 11], -Ch / ah, rtol=1e-3, atol=1e-9)
-    assert np.all(f0[:, 12] == np.float32(np.log(T)))  # one body: log of its tet count
+    C, lad, lah = _cluster_constraint_values(sc, xt, dt)
+    m = _match_rows(f0[:, 24:36], C)
+    assert np.allclose(f0[:, 24:36:2], C[m][:, 0::2], rtol=1e-5)
+    assert np.allclose(f0[:, 25:36:2], C[m][:, 1::2], rtol=1e-3, atol=1e-5)
+    assert np.allclose(f0[:, 36], lad[m], rtol=1e-5) and np.allclose(f0[:, 37], lah[m], rtol=1e-5)
+    assert np.all(f0[:, 38] == np.float32(np.log(sc.num_tets)))  # one body: log of its tet count
 
 
 def test_body_size_feature_on_a_mixed_batch():
-    # one network serves bodies of every size: feature 12 is the log of the tet's body's tet count
+    # one network serves bodies of every size: feature 38 is the log of the cluster's body's tet count
     sc = scenes.batch(3, tet_range=(2e3, 4e3), large=(1, 1.2e4), name="batch_3_plus_1_large")
     say("body-size feature")
     dev = scenes.build_device(sc)
-    net = S.Mlp(13, 2)
-    T = sc.num_tets
-    net.reserve(T)
+    nc = len(sc.groups[0].clusters) - 1
+    net = S.Mlp(NIN, NOUT)
+    net.reserve(nc)
+    dev.set_warm_start("previous")
     dev.capture(net, 1)
     dev.step(sc.dt, sc.iterations, 1)
     dev.capture(None, 0)
     x, _ = net.dataset()
-    expected = np.concatenate([np.full(len(b.tets), np.float32(np.log(len(b.tets)))) for b in sc.bodies])
+    expected = np.concatenate([np.full(len(b.tets) // 6, np.float32(np.log(len(b.tets)))) for b in sc.bodies])
     assert len(np.unique(expected)) == len(sc.bodies)
-    assert np.array_equal(np.sort(x[:T, 12]), np.sort(expected))
+    assert np.array_equal(np.sort(x[:nc, 38]), np.sort(expected))
+
+
+def _train_small(dev, sc, scale=(0.5, 0.5), iterations=4, seed=0):
+    from sbs_amd import autotune
+    return autotune.train_warm_start(dev, sc, iterations=iterations, scale=scale, frames=48, episodes=6,
+                                     val_episodes=2, max_rows=200_000, steps=400, batch=4096, seed=seed, log=say)
+
+
+@pytest.mark.parametrize("which", ["beam", "batch"])
+def test_learned_warm_start_matches_oracle(which):
+    # the network-driven pass end to end: features at the predicted positions, BF16 matrix-core
+    # inference, the predicted change added to the multipliers, the warm-start pass and the
+    # iterations, against the oracle evaluating the same network in FP64 (same BF16 rounding points)
+    if which == "beam":
+        sc, frames, scale = _beam(True, iterations=4), 20, (0.5, 0.5)
+    else:
+        sc, frames, scale = scenes.batch(3, tet_range=(2e3, 4e3), large=(1, 1.2e4), iterations=4,
+                                         name="batch_3_plus_1_large"), 20, (0.4, 0.4)
+    say(f"learned warm start vs oracle: {sc.name}")
+    dev = scenes.build_device(sc)
+    for _ in range(2):
+        dev.step(sc.dt, 8, sc.substeps)
+    net, info = _train_small(dev, sc, scale=scale)
+    say(f"training {info['train_loss']:.4f}, held out {info['held_out_loss']:.4f}")
+    # both from rest with the trained network
+    dev = scenes.build_device(sc)
+    dev.set_warm_start("learned", net)
+    dev.set_warm_start_scale(*scale)
+    orc = build_oracle(sc)
+    orc.set_learned_warm_start(net, *scale)
+    diag = max(float(np.linalg.norm(np.ptp(b.x0, axis=0))) for b in sc.bodies)
+    for f in range(1, frames + 1):
+        dev.step(sc.dt, sc.iterations, sc.substeps)
+        orc.step(sc.dt, sc.iterations, sc.substeps)
+        if f in (1, 5, 10, frames):
+            err = max(float(np.abs(dev.read_body(b)[0] - orc.read_body(b)[0]).max()) for b in range(len(sc.bodies)))
+            say(f"frame {f} error {err / diag:.3e}")
+            assert err / diag <= TOL
+
+
+def test_learned_network_gains_over_the_previous_start():
+    # the same iterations and scales with and without the network, on frames the network was not
+    # trained on: the per-frame residual from the same states
+    from sbs_amd import autotune
+    sc = scenes.cantilever(16, 4, 4, 0.1, "neohookean", 5e5, 0.45, 1000.0, num_spheres=3, iterations=8, substeps=4,
+                           cluster=True, coloring="index", name="gain_beam")
+    say("network gain")
+    dev = scenes.build_device(sc)
+    for _ in range(3):
+        dev.step(sc.dt, sc.iterations, sc.substeps)
+    k, scale = 3, (0.6, 0.55)
+    net, info = autotune.train_warm_start(dev, sc, iterations=k, scale=scale, frames=400, episodes=20, val_episodes=4,
+                                          max_rows=1_000_000, steps=1500, batch=8192, seed=3, log=say)
+    say(f"held out {info['held_out_loss']:.4f} vs no change {info['held_out_loss_no_change']:.4f}")
+    assert info["held_out_loss"] < info["held_out_loss_no_change"]
+    # held-out frames: the plain solver carries on for 10 frames first
+    for _ in range(10):
+        dev.step(sc.dt, sc.iterations, sc.substeps)
+    dev.snapshot(0)
+    res = {}
+    for mode in ("previous", "learned"):
+        dev.restore(0)
+        dev.set_warm_start(mode, net if mode == "learned" else None)
+        dev.set_warm_start_scale(*scale)
+        rw, rp = autotune.per_state_ratios(dev, sc, 10, k)
+        res[mode] = rw / rp
+        say(f"{mode}: per-state residual ratio mean {res[mode].mean():.4f} max {res[mode].max():.4f}")
+    assert np.all(np.isfinite(res["learned"]))
+    assert res["learned"].mean() < res["previous"].mean()
 
 
 def _execution_order(dev, sc):
@@ -293,14 +392,13 @@ def test_learned_warm_start_trains_and_runs():
     dev.snapshot()
     target = autotune.mean_residual(dev, sc, 4, sc.iterations)
     dev.restore()
-    net, _ = autotune.train_warm_start(dev, sc, frames=4, iterations=24, steps=300, batch=4096, log=say)
-    assert net.dataset_size() > 0
+    net, info = _train_small(dev, sc)
+    assert net.dataset_size() > 0 and info["held_out_rows"] > 0
+    x_before = dev.read_body(0)[0]
     found, trials = autotune.tune_warm_start(dev, sc, net, target, frames=4, scales=(0.5,), iterations=(4, 8),
                                              log=say)
     # the tuner leaves the state where it started
-    x0 = dev.read_body(0)[0]
-    dev.restore()
-    assert np.array_equal(dev.read_body(0)[0], x0)
+    assert np.array_equal(dev.read_body(0)[0], x_before)
     for t in trials:
         if t["mode"] == "previous":
             assert np.isfinite(t["residual"]), t
@@ -314,6 +412,27 @@ def test_learned_warm_start_trains_and_runs():
     assert np.all(np.isfinite(dev.read_body(0)[0]))
     assert autotune.frame_residual(dev, sc) <= 2.0 * target
     say(f"fallback frames {dev.fallback_count()}")
+
+
+def test_learned_restores_are_bit_identical():
+    # a snapshot taken before the warm-start history existed restores it as zeros: two
+    # restore-then-step runs in learned mode give the same bits
+    sc = _beam(True, iterations=4)
+    say("learned restores")
+    dev = scenes.build_device(sc)
+    dev.step(sc.dt, 8, sc.substeps)
+    dev.snapshot(0)
+    net, _ = _train_small(dev, sc)
+    out = []
+    for _ in range(2):
+        dev.restore(0)
+        dev.set_warm_start("learned", net)
+        dev.set_warm_start_scale(0.5)
+        for _ in range(3):
+            dev.step(sc.dt, sc.iterations, sc.substeps)
+        out.append(dev.read_body(0))
+        dev.set_warm_start("none")
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
 @pytest.mark.parametrize("beta", [0.0, 0.01])

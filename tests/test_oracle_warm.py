@@ -10,15 +10,18 @@ import oracle as O
 from sbs_amd import scenes
 
 
+NIN = 39  # used inputs of the warm-start network (a hex cluster's features)
+
+
 def random_params(rng, scale=1.0):
     p = np.zeros(M.NPARAMS)
-    for (w, b, fan) in ((M.W1, M.B1, 12), (M.W2, M.B2, 128), (M.W3, M.B3, 128), (M.W4, M.B4, 128)):
+    for (w, b, fan) in ((M.W1, M.B1, NIN), (M.W2, M.B2, M.H), (M.W3, M.B3, M.H), (M.W4, M.B4, M.H)):
         p[w:b] = rng.normal(0, scale * np.sqrt(2.0 / fan), b - w)
-    p[M.B1:M.W2] = rng.normal(0, 0.1, 128)
-    p[M.B2:M.W3] = rng.normal(0, 0.1, 128)
-    p[M.B3:M.W4] = rng.normal(0, 0.1, 128)
-    p[M.B4:] = rng.normal(0, 0.1, 32)
-    p[M.W1:M.B1].reshape(16, 128)[12:] = 0.0  # unused inputs
+    p[M.B1:M.W2] = rng.normal(0, 0.1, M.H)
+    p[M.B2:M.W3] = rng.normal(0, 0.1, M.H)
+    p[M.B3:M.W4] = rng.normal(0, 0.1, M.H)
+    p[M.B4:] = rng.normal(0, 0.1, M.OUT)
+    p[M.W1:M.B1].reshape(M.IN, M.H)[NIN:] = 0.0  # unused inputs
     return p
 
 
@@ -33,17 +36,17 @@ def test_bf16_rounding_matches_torch():
     assert M.bf16(np.float32(1.0 + 3 * 2.0 ** -8)) == 1.0 + 2.0 ** -6
 
 
-@pytest.mark.parametrize("nout", [2, 3])
+@pytest.mark.parametrize("nout", [2, 12])
 def test_network_gradient_matches_finite_differences(nout):
     rng = np.random.default_rng(7)
     p = random_params(rng)
-    x = rng.normal(0, 1, (9, 12))
-    xn = M.normalize(x, np.zeros(12), np.ones(12), rounding=False)
+    x = rng.normal(0, 1, (9, NIN))
+    xn = M.normalize(x, np.zeros(NIN), np.ones(NIN), rounding=False)
     t = rng.normal(0, 1, (9, nout))
     loss, g = M.loss_grad(p, xn, t, nout, rounding=False)
-    idx = np.concatenate([rng.choice(np.arange(M.W1, M.W1 + 12 * 128), 60, replace=False),
+    idx = np.concatenate([rng.choice(np.arange(M.W1, M.W1 + NIN * M.H), 60, replace=False),
                           rng.choice(np.arange(M.W2, M.W4), 120, replace=False),
-                          np.arange(M.W4, M.W4 + 128 * 32, 32)[:40] + rng.integers(0, nout),
+                          np.arange(M.W4, M.W4 + M.H * M.OUT, M.OUT)[:40] + rng.integers(0, nout),
                           np.arange(M.B1, M.W2)[:20], np.arange(M.B4, M.B4 + nout)])
     eps = 1e-6
     for i in idx:
@@ -53,15 +56,15 @@ def test_network_gradient_matches_finite_differences(nout):
         fd = (M.loss_grad(pp, xn, t, nout, False)[0] - M.loss_grad(pm, xn, t, nout, False)[0]) / (2 * eps)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
     # outputs past nout carry no gradient
-    assert np.all(g[M.W4:M.B4].reshape(128, 32)[:, nout:] == 0) and np.all(g[M.B4 + nout:] == 0)
+    assert np.all(g[M.W4:M.B4].reshape(M.H, M.OUT)[:, nout:] == 0) and np.all(g[M.B4 + nout:] == 0)
 
 
 def test_rounded_network_stays_close_to_exact():
     rng = np.random.default_rng(3)
     p = random_params(rng)
-    x = rng.normal(0, 1, (64, 12))
-    y1, _ = M.forward(p, M.normalize(x, np.zeros(12), np.ones(12), True), True)
-    y0, _ = M.forward(p, M.normalize(x, np.zeros(12), np.ones(12), False), False)
+    x = rng.normal(0, 1, (64, NIN))
+    y1, _ = M.forward(p, M.normalize(x, np.zeros(NIN), np.ones(NIN), True), True)
+    y0, _ = M.forward(p, M.normalize(x, np.zeros(NIN), np.ones(NIN), False), False)
     assert np.abs(y1 - y0).max() <= 0.03 * np.abs(y0).max()
 
 
@@ -135,3 +138,103 @@ def test_oracle_explicit_warm_start_is_an_explicit_elastic_step():
     expect = xt - dt * dt * w[:, None] * grad
     assert E0 > 0 and np.abs(xs - xt).max() > 1e-5  # the stretched beam does get pulled back
     assert np.abs(xs - expect).max() <= 1e-6 * np.abs(xs - xt).max()
+
+
+def _hex_beam():
+    return scenes.cantilever(5, 2, 2, 0.1, "neohookean", 5e5, 0.45, 1000.0, num_spheres=2, iterations=3, substeps=2,
+                             cluster=True, coloring="index")
+
+
+def _oracle(sc):
+    s = O.Simulation()
+    sc.apply(s)
+    s.set_order(True, 2)
+    return s
+
+
+def test_oracle_network_matches_the_numpy_restatement():
+    rng = np.random.default_rng(11)
+    p = random_params(rng)
+    mean, inv_std = rng.normal(0, 1, NIN), np.exp(rng.normal(0, 0.5, NIN))
+    out_scale = np.exp(rng.normal(0, 0.5, 12)).astype(np.float32)
+    x = rng.normal(0, 2, (40, NIN))
+    s = _oracle(_hex_beam())
+    s.set_network(p.astype(np.float32), mean.astype(np.float32), inv_std.astype(np.float32), out_scale)
+    y = s.network_forward(x)
+    # the parameters the oracle holds are the FP32 values
+    p32 = p.astype(np.float32).astype(np.float64)
+    ref, _ = M.forward(p32, M.normalize(x, mean.astype(np.float32), inv_std.astype(np.float32), True), True)
+    ref = ref[:, :12] * out_scale.astype(np.float64)
+    assert np.allclose(y, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def test_oracle_cluster_features_at_a_deformed_state():
+    sc = _hex_beam()
+    s = _oracle(sc)
+    b, g = sc.bodies[0], sc.groups[0]
+    rng = np.random.default_rng(5)
+    x = b.x0 + rng.normal(0, 0.01, b.x0.shape)
+    s.set_body_state(0, x, np.zeros_like(x))
+    dt = sc.dt / sc.substeps
+    f = s.cluster_features(0, dt, len(g.clusters) - 1)
+    tets = np.asarray(g.conn).reshape(-1, 6, 4)
+    mu = g.young / (2 * (1 + g.poisson))
+    la = g.young * g.poisson / ((1 + g.poisson) * (1 - 2 * g.poisson))
+    Dm = np.stack([b.x0[tets[..., k]] - b.x0[tets[..., 3]] for k in range(3)], axis=-1)
+    Ds = np.stack([x[tets[..., k]] - x[tets[..., 3]] for k in range(3)], axis=-1)
+    F = Ds @ np.linalg.inv(Dm)
+    V0 = np.abs(np.linalg.det(Dm)) / 6
+    atd, ath = 1 / (mu * V0) / dt ** 2, 1 / (la * V0) / dt ** 2
+    assert np.allclose(f[:, 24::2][:, :6], np.sqrt(np.sum(F * F, axis=(-2, -1))), rtol=1e-12)
+    assert np.allclose(f[:, 25::2][:, :6], np.linalg.det(F) - (1 + mu / la), rtol=1e-10, atol=1e-12)
+    assert np.allclose(f[:, 36], np.log(atd[:, 0]), rtol=1e-12) and np.allclose(f[:, 37], np.log(ath[:, 0]), rtol=1e-12)
+    assert np.all(f[:, 38] == np.log(len(g.conn)))
+    assert np.all(f[:, :24] == 0.0)  # multipliers start at zero
+
+
+def test_oracle_learned_warm_start_with_a_silent_network_is_the_previous_start():
+    # a network whose outputs are all zero predicts no change: the learned start is the previous
+    # one, and its history holds the multipliers each substep started from
+    sc = _hex_beam()
+    a, b = _oracle(sc), _oracle(sc)
+    a.set_warm_start("previous", 0.6, 0.5)
+    b.set_network(np.zeros(M.NPARAMS, np.float32), np.zeros(NIN, np.float32), np.ones(NIN, np.float32),
+                  np.ones(12, np.float32))
+    b._L.orc_set_warm_start(b.h, 2, 0.6, 0.5)
+    n = len(sc.groups[0].conn)
+    for _ in range(3):
+        before = a.multipliers(0, n)
+        a.step(sc.dt, sc.iterations, 1)
+        b.step(sc.dt, sc.iterations, 1)
+        assert np.array_equal(a.read_body(0)[0], b.read_body(0)[0])
+        mb = b.multipliers(0, n)
+        assert np.array_equal(mb[:, :2], a.multipliers(0, n)[:, :2])
+        assert np.array_equal(mb[:, 2:], before[:, :2])
+
+
+def test_oracle_learned_features_carry_the_multipliers():
+    # after warm-started substeps the first 24 features are alpha~ times the stored pairs and the
+    # pairs one substep before
+    sc = _hex_beam()
+    s = _oracle(sc)
+    s.set_network(np.zeros(M.NPARAMS, np.float32), np.zeros(NIN, np.float32), np.ones(NIN, np.float32),
+                  np.ones(12, np.float32))
+    s._L.orc_set_warm_start(s.h, 2, 0.6, 0.5)
+    for _ in range(2):
+        s.step(sc.dt, sc.iterations, 1)
+    g = sc.groups[0]
+    dt = sc.dt
+    f = s.cluster_features(0, dt, len(g.clusters) - 1)
+    m = s.multipliers(0, len(g.conn)).reshape(-1, 6, 4)
+    b = sc.bodies[0]
+    tets = np.asarray(g.conn).reshape(-1, 6, 4)
+    mu = g.young / (2 * (1 + g.poisson))
+    la = g.young * g.poisson / ((1 + g.poisson) * (1 - 2 * g.poisson))
+    Dm = np.stack([b.x0[tets[..., k]] - b.x0[tets[..., 3]] for k in range(3)], axis=-1)
+    V0 = np.abs(np.linalg.det(Dm)) / 6
+    atd, ath = 1 / (mu * V0) / dt ** 2, 1 / (la * V0) / dt ** 2
+    assert np.abs(m[..., 0]).max() > 0 and np.abs(m[..., 2]).max() > 0
+    assert np.allclose(f[:, 0:12:2], atd * m[..., 0], rtol=1e-12)
+    assert np.allclose(f[:, 1:12:2], ath * m[..., 1], rtol=1e-12)
+    assert np.allclose(f[:, 12:24:2], atd * m[..., 2], rtol=1e-12)
+    assert np.allclose(f[:, 13:24:2], ath * m[..., 3], rtol=1e-12)
