@@ -188,8 +188,45 @@ int sbs_gpu_set_contact_hash(sbs_gpu* sim, int mode, double cell_size);
 int sbs_gpu_read_contact_hash(sbs_gpu* sim, double* cell_size, uint32_t* table_bits, uint32_t* start, uint32_t* end,
                               uint32_t* entries, uint32_t cap, uint32_t* num_entries, uint32_t* global,
                               uint32_t* num_global);
-/* Contacts dropped so far because a vertex had more than 8. */
+/* Contacts dropped so far because a vertex had more than 8 (and surface triangles that covered more
+ * than 8 cells of the body contact hash). */
 int sbs_gpu_contact_overflow(sbs_gpu* sim, uint64_t* dropped);
+
+/* Contacts between the surfaces of deformable bodies. The reference's collision detection visits
+ * every pair of collision models (src/physics/collision/brute_force_cd_system.cpp:17-25) but a pair
+ * of surface point clouds yields nothing (bvh_model.cpp:30-34): its only contact is a surface
+ * particle against an SDF (include/sbs/physics/collision/contact.h:14), with the contact point and
+ * normal fixed for the frame (contact_handler.cpp:14-54). Here, once per frame from the positions at
+ * its start, each surface vertex p gets up to two more contacts, each with a surface triangle t
+ * closer than `radius` r (|p - c|^2 < r^2 in FP64, c the closest point of t, the lowest triangle
+ * index on ties): the nearest triangle of another body (p inside or outside it), then the nearest
+ * triangle of p's own surface that does not have p as a corner and that p lies more than h/10
+ * outside of (a fold of the surface approaching or resting on it; the flat or convex surface around
+ * p never holds it). Inside and outside come from the sign of (p - c).N, N the angle-weighted
+ * pseudo-normal of the feature of t holding c (Baerentzen and Aanaes 2005). The contact keeps c's
+ * barycentric weights in t and the unit normal n along the line from c to p pointing out of the
+ * surface (t's normal when p lies on t); a body whose surface encloses a negative volume at rest
+ * has its triangles flipped. Every iteration's contact projection takes c at t's corners as they
+ * are then and keeps p on the outer side of the plane through c + h n (h = thickness), acting only
+ * when p would cross it, p moving by its share of the two-sided projection (denominator
+ * w_p + sum_k b_k^2 w_k + alpha~, b_k the weights); the contacts come after p's SDF contacts
+ * (contact index: number of SDFs + triangle index in global surface-triangle order). Broad phase: a
+ * spatial hash of cells of edge cell_size rebuilt every frame (Teschner et al. 2003), each vertex
+ * querying the 27 cells around its own. Before commit; 0 selects the defaults: cell_size twice,
+ * thickness a tenth of and radius equal to the mean surface edge at rest; thickness < radius <=
+ * 0.999 cell_size. */
+int sbs_gpu_set_body_contacts(sbs_gpu* sim, int enable, double cell_size, double thickness, double radius);
+/* The committed setup: cell edge, thickness, radius, table size 2^table_bits, and the triangles as
+ * the detection uses them (4 words each: 3 surface vertices in outward order, body). Any output may
+ * be NULL; all 0 when body contacts are off. */
+int sbs_gpu_read_body_contact_setup(sbs_gpu* sim, double* cell_size, double* thickness, double* radius,
+                                    uint32_t* table_bits, uint32_t* triangles);
+/* Broad-phase candidate pairs at the committed positions: per surface vertex (global surface
+ * numbering, increasing) the distinct triangles of the hash slots of the 27 cells around its own
+ * that do not have it as a corner, in increasing order, as (vertex, triangle) pairs; with pairs NULL
+ * *count gets an upper bound (repeats included), else up to cap pairs are written and *count gets
+ * their number. */
+int sbs_gpu_read_body_contact_candidates(sbs_gpu* sim, uint32_t* pairs, uint64_t cap, uint64_t* count);
 
 /* Partitions a tet group's constraints (in the order they were added) into clusters of
  * consecutive constraints: cluster i is [cluster_offsets[i], cluster_offsets[i+1]), with
@@ -244,9 +281,10 @@ int sbs_gpu_write_body(sbs_gpu* sim, uint32_t body, const double* x, const doubl
 int sbs_gpu_group_colors(sbs_gpu* sim, uint32_t group, uint32_t* num_colors, uint32_t* color_of_constraint);
 
 /* Detects contacts from the current positions (what the next step() will use) and reads them:
- * ids gets 4 uint32 per contact (body, surface vertex, body vertex, sdf index), qn 6 doubles
- * (contact point, normal), ordered by (body, surface vertex, sdf). *count gets the total even if
- * larger than cap. */
+ * ids gets 4 uint32 per contact (body, surface vertex, body vertex, contact index: the SDF, or the
+ * number of SDFs + the surface triangle), qn 6 doubles (the contact point, for a triangle contact
+ * the barycentric weights of the closest point in the triangle; the normal), per (body, surface
+ * vertex) in projection order. *count gets the total even if larger than cap. */
 int sbs_gpu_read_contacts(sbs_gpu* sim, uint32_t* ids, double* qn, uint32_t cap, uint32_t* count);
 
 /* Squared XPBD residual sum (h = C(x) + alpha~ lambda), in FP64, per color of a group, for the

@@ -59,6 +59,23 @@ void orc_set_surface(void* h, uint32_t body, const uint32_t* map, uint32_t n)
     s->surface.at(body).assign(map, map + n);
 }
 
+void orc_set_surface_triangles(void* h, uint32_t body, const uint32_t* tris, uint32_t n)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    if (s->triangles.size() < s->particles.size()) s->triangles.resize(s->particles.size());
+    s->triangles.at(body).assign(tris, tris + 3 * std::size_t(n));
+}
+
+void orc_set_body_contacts(void* h, int enable, double cell, double thickness, double radius, int brute_force)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    s->body_contacts = enable != 0;
+    s->body_cell_request = cell;
+    s->body_thickness_request = thickness;
+    s->body_radius_request = radius;
+    s->body_brute_force = brute_force != 0;
+}
+
 static int add_group(simulation_t* s, int kind, std::size_t first)
 {
     group_t g;
@@ -286,6 +303,23 @@ void orc_network_forward(void* h, const double* x, uint32_t n, double* y)
 {
     auto const& net = *static_cast<simulation_t*>(h)->net;
     for (uint32_t r = 0; r < n; ++r) net.evaluate(x + std::size_t(r) * 39, y + std::size_t(r) * net.nout);
+}
+
+// Green projection branch trace (test hook): on = 1 starts a new trace, 0 stops; read copies up
+// to cap entries and returns the trace length.
+void orc_green_trace(void* h, int on)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    s->green_trace = on ? std::make_unique<std::vector<uint8_t>>() : nullptr;
+}
+
+uint64_t orc_green_trace_read(void* h, uint8_t* out, uint64_t cap)
+{
+    auto* s = static_cast<simulation_t*>(h);
+    if (!s->green_trace) return 0;
+    uint64_t const n = s->green_trace->size();
+    if (out) std::memcpy(out, s->green_trace->data(), std::min(n, cap));
+    return n;
 }
 
 // (lambda, lambda_h, hist, hist_h) of the neo-Hookean constraints of a group (test hook).

@@ -203,47 +203,54 @@ double grid_sdf_t::interpolate(vec3 x, vec3* grad) const
     return phi;
 }
 
-vec3 closest_point_on_triangle(vec3 p, vec3 a, vec3 b, vec3 c, int* feature)
+vec3 closest_point_on_triangle(vec3 p, vec3 a, vec3 b, vec3 c, int* feature, double* bary)
 {
     vec3 const ab = b - a, ac = c - a, ap = p - a;
     double const d1 = dot(ab, ap), d2 = dot(ac, ap);
     if (d1 <= 0.0 && d2 <= 0.0) {
         *feature = 0;
+        if (bary) bary[0] = 1.0, bary[1] = 0.0, bary[2] = 0.0;
         return a;
     }
     vec3 const bp = p - b;
     double const d3 = dot(ab, bp), d4 = dot(ac, bp);
     if (d3 >= 0.0 && d4 <= d3) {
         *feature = 1;
+        if (bary) bary[0] = 0.0, bary[1] = 1.0, bary[2] = 0.0;
         return b;
     }
     double const vc = d1 * d4 - d3 * d2;
     if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
         double const v = d1 / (d1 - d3);
         *feature = 3;
+        if (bary) bary[0] = 1.0 - v, bary[1] = v, bary[2] = 0.0;
         return a + v * ab;
     }
     vec3 const cp = p - c;
     double const d5 = dot(ab, cp), d6 = dot(ac, cp);
     if (d6 >= 0.0 && d5 <= d6) {
         *feature = 2;
+        if (bary) bary[0] = 0.0, bary[1] = 0.0, bary[2] = 1.0;
         return c;
     }
     double const vb = d5 * d2 - d1 * d6;
     if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
         double const w = d2 / (d2 - d6);
         *feature = 5;
+        if (bary) bary[0] = 1.0 - w, bary[1] = 0.0, bary[2] = w;
         return a + w * ac;
     }
     double const va = d3 * d6 - d5 * d4;
     if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
         double const w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
         *feature = 4;
+        if (bary) bary[0] = 0.0, bary[1] = 1.0 - w, bary[2] = w;
         return b + w * (c - b);
     }
     double const den = 1.0 / (va + vb + vc);
     double const v = vb * den, w = vc * den;
     *feature = 6;
+    if (bary) bary[0] = 1.0 - v - w, bary[1] = v, bary[2] = w;
     return a + ab * v + ac * w;
 }
 

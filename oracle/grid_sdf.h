@@ -56,7 +56,8 @@ struct mesh_distance_t {
 };
 
 // Closest point of triangle (a, b, c) to p; feature: 0-2 vertex a/b/c, 3-5 edge ab/bc/ca, 6 face.
-vec3 closest_point_on_triangle(vec3 p, vec3 a, vec3 b, vec3 c, int* feature);
+// bary (optional): the weights of a, b, c that give the closest point.
+vec3 closest_point_on_triangle(vec3 p, vec3 a, vec3 b, vec3 c, int* feature, double* bary = nullptr);
 
 void environment_domain(const double* pos, uint32_t nv, vec3 lo, vec3 hi, vec3* out_lo, vec3* out_hi);
 

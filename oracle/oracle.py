@@ -11,7 +11,7 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+LIB_PATH = os.environ.get("SBS_ORACLE_LIB") or os.path.join(HERE, "_build", "liboracle.so")
 
 _lib = None
 
@@ -40,10 +40,14 @@ _SIGS = {
     "orc_add_sphere": (None, [_p, _pd, _d]),
     "orc_set_order": (None, [_p, _i, _u32]),
     "orc_set_threads": (None, [_p, _i]),
+    "orc_set_surface_triangles": (None, [_p, _u32, _pu, _u32]),
+    "orc_set_body_contacts": (None, [_p, _i, _d, _d, _d, _i]),
     "orc_set_network": (None, [_p, _pf, _pf, _pf, _pf, _i, _i]),
     "orc_network_forward": (None, [_p, _pd, _u32, _pd]),
     "orc_cluster_features": (_u32, [_p, _u32, _d, _pd]),
     "orc_multipliers": (None, [_p, _u32, _pd]),
+    "orc_green_trace": (None, [_p, _i]),
+    "orc_green_trace_read": (ctypes.c_uint64, [_p, ctypes.c_void_p, ctypes.c_uint64]),
     "orc_set_clusters": (None, [_p, _u32, _pu, _u32]),
     "orc_set_coloring": (None, [_p, _i]),
     "orc_set_warm_start": (None, [_p, _i, _d, _d]),
@@ -317,6 +321,25 @@ class Simulation:
         out = np.zeros((count, 4))
         self._L.orc_multipliers(self.h, group, _dp(out))
         return out
+
+    def green_trace(self, on=True):
+        """Start (or stop) recording the branches of every green projection (test hook)."""
+        self._L.orc_green_trace(self.h, int(on))
+
+    def read_green_trace(self):
+        n = self._L.orc_green_trace_read(self.h, None, 0)
+        out = np.zeros(n, dtype=np.uint8)
+        self._L.orc_green_trace_read(self.h, out.ctypes.data, n)
+        return out
+
+    def set_surface_triangles(self, body, tris):
+        t = u32(tris).reshape(-1)
+        self._L.orc_set_surface_triangles(self.h, body, _up(t), len(t) // 3)
+
+    def set_body_contacts(self, enable=True, cell_size=0.0, thickness=0.0, radius=0.0, brute_force=False):
+        """Contacts between bodies (see simulation_t::detect_body_contacts); brute_force tests every
+        triangle instead of the grid's."""
+        self._L.orc_set_body_contacts(self.h, int(enable), cell_size, thickness, radius, int(brute_force))
 
     def set_order(self, colored, recolor_passes=2):
         self._L.orc_set_order(self.h, int(colored), recolor_passes)

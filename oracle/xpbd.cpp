@@ -2,6 +2,8 @@
 #include "xpbd.h"
 
 #include <algorithm>
+#include <array>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -27,6 +29,7 @@ void lame(double young, double poisson, double* mu, double* lam)
 // Green-strain constraint core (green_constraint.cpp:61-131): computes C = |V0| psi and the
 // forces f1..f4 (= -grad C) at positions p. Returns the weighted sum of squared gradients.
 struct green_eval {
+    uint8_t branches = 0;  // bit 0: inverted, bits 1-3: singular value i clamped
     double C, wsum;
     vec3 f[4];
 };
@@ -53,7 +56,11 @@ green_eval green_core(mat3 const& DmInv, double V0, double mu, double lam, vec3 
         U.set_col(2, -U.col(2));
     }
     double const min_singular_value = 0.577;  // (:99-102)
-    for (int i = 0; i < 3; ++i) Fhat.m[i][i] = std::max(Fhat.m[i][i], min_singular_value);
+    uint8_t branches = is_tet_inverted ? 1 : 0;
+    for (int i = 0; i < 3; ++i) {
+        if (Fhat.m[i][i] < min_singular_value) branches |= uint8_t(2 << i);
+        Fhat.m[i][i] = std::max(Fhat.m[i][i], min_singular_value);
+    }
 
     mat3 const I = mat3::identity();
     mat3 const Ehat = 0.5 * (transpose(Fhat) * Fhat - I);  // (:104-106)
@@ -68,6 +75,7 @@ green_eval green_core(mat3 const& DmInv, double V0, double mu, double lam, vec3 
     double const V0abs = std::abs(V0);               // (:115-120)
     mat3 const H = (-V0abs) * Piola * transpose(DmInv);
     green_eval r;
+    r.branches = branches;
     r.f[0] = H.col(0);
     r.f[1] = H.col(1);
     r.f[2] = H.col(2);
@@ -128,6 +136,7 @@ void green_constraint_t::project_positions(simulation_t& sim, double dt)
     double const w[4] = {pp[0]->invmass(), pp[1]->invmass(), pp[2]->invmass(), pp[3]->invmass()};
     green_eval const g = green_core(DmInv, V0, mu, lam, p, w);
     double constexpr epsilon = 1e-20;  // (:67, :130-131)
+    if (sim.green_trace) sim.green_trace->push_back(uint8_t(g.branches | (g.wsum < epsilon ? 16 : 0)));
     if (g.wsum < epsilon) return;
     double gdd = 0;
     for (int k = 0; k < 4; ++k) gdd += dot(g.f[k], pp[k]->xi - pp[k]->xn);  // (:140-144)
@@ -317,6 +326,36 @@ void collision_constraint_t::residual_sq(simulation_t const& sim, double dt, dou
     *acc += h * h;
 }
 
+vec3 body_collision_constraint_t::plane_point(simulation_t const& sim) const
+{
+    vec3 q = w[0] * sim.contact_snap[cb[0]][cv[0]];
+    q = q + w[1] * sim.contact_snap[cb[1]][cv[1]];
+    q = q + w[2] * sim.contact_snap[cb[2]][cv[2]];
+    return q + h * n;
+}
+
+void body_collision_constraint_t::project_positions(simulation_t& sim, double dt)
+{
+    particle_t& p = sim.particles[bi][vi];
+    double const wi = p.invmass();
+    double const C = dot(p.xi - plane_point(sim), n);
+    if (C >= 0.) return;
+    // the triangle point's inverse mass: the vertex takes its share of the two-sided projection
+    double wt = 0.0;
+    for (int k = 0; k < 3; ++k) wt += w[k] * w[k] * sim.particles[cb[k]][cv[k]].invmass();
+    double const alpha_tilde = alpha / (dt * dt);
+    double const dl = -(C + alpha_tilde * lagrange) / (wi + wt + alpha_tilde);
+    lagrange += dl;
+    p.xi += wi * n * dl;
+}
+
+void body_collision_constraint_t::residual_sq(simulation_t const& sim, double dt, double* acc) const
+{
+    double const C = dot(sim.particles[bi][vi].x - plane_point(sim), n);
+    double const hh = std::min(C, 0.0) + (alpha / (dt * dt)) * lagrange;
+    *acc += hh * hh;
+}
+
 void sdf_t::evaluate(vec3 p, double* sd, vec3* grad) const
 {
     if (kind == 0) {
@@ -343,6 +382,7 @@ void simulation_t::detect_contacts()
 {
     contacts.clear();
     collision_constraints.clear();
+    have_body_contacts = false;
     for (uint32_t b = 0; b < particles.size(); ++b) {
         if (b >= surface.size()) continue;
         for (uint32_t k = 0; k < sdfs.size(); ++k) {
@@ -362,6 +402,171 @@ void simulation_t::detect_contacts()
                 collision_constraints.push_back(std::make_unique<collision_constraint_t>(
                     collision_compliance, collision_damping, b, vi, q, nn));
             }
+        }
+    }
+    if (body_contacts) detect_body_contacts();
+}
+
+void simulation_t::detect_body_contacts()
+{
+    // global surface vertices (body-major) and triangles, outward
+    std::vector<std::pair<uint32_t, uint32_t>> sv;  // (body, surface vertex of the body)
+    std::vector<uint32_t> base(particles.size() + 1, 0);
+    for (uint32_t b = 0; b < particles.size(); ++b) {
+        uint32_t const ns = b < surface.size() ? uint32_t(surface[b].size()) : 0u;
+        for (uint32_t s = 0; s < ns; ++s) sv.emplace_back(b, s);
+        base[b + 1] = base[b] + ns;
+    }
+    struct tri_t {
+        uint32_t v[3], body;
+    };
+    std::vector<tri_t> tr;
+    for (uint32_t b = 0; b < particles.size() && b < triangles.size(); ++b)
+        for (std::size_t t = 0; t + 2 < triangles[b].size(); t += 3)
+            tr.push_back({{base[b] + triangles[b][t], base[b] + triangles[b][t + 1], base[b] + triangles[b][t + 2]}, b});
+    if (tr.empty() || sv.empty()) return;
+    auto rest = [&](uint32_t s) { return particles[sv[s].first][surface[sv[s].first][sv[s].second]].x0; };
+    auto cur = [&](uint32_t s) { return particles[sv[s].first][surface[sv[s].first][sv[s].second]].x; };
+    std::vector<double> vol(particles.size(), 0.0);
+    double edge_sum = 0.0;
+    for (auto const& t : tr) {
+        vec3 const a = rest(t.v[0]), b = rest(t.v[1]), c = rest(t.v[2]);
+        vol[t.body] += a.x * (b.y * c.z - b.z * c.y) + a.y * (b.z * c.x - b.x * c.z) + a.z * (b.x * c.y - b.y * c.x);
+        edge_sum += norm(a - b) + norm(b - c) + norm(c - a);
+    }
+    for (auto& t : tr)
+        if (vol[t.body] < 0.0) std::swap(t.v[1], t.v[2]);
+    double const mean_edge = edge_sum / (3.0 * double(tr.size()));
+    // unit normals of the triangles at the current positions, the edge neighbours of each triangle
+    // (edge k runs from corner k to corner k+1) and the angle-weighted pseudo-normals of the surface
+    // vertices (Baerentzen and Aanaes, "Signed distance computation using the angle weighted
+    // pseudonormal", IEEE TVCG 11(3), 2005): the sign of (p - c).N at the closest point c, with N the
+    // pseudo-normal of the closest feature, tells whether p lies inside the closed surface
+    std::vector<vec3> tn(tr.size());
+    for (uint32_t t = 0; t < tr.size(); ++t) {
+        vec3 const a = cur(tr[t].v[0]);
+        vec3 const n = cross(cur(tr[t].v[1]) - a, cur(tr[t].v[2]) - a);
+        double const l2 = norm2(n);
+        double const l = std::sqrt(l2);
+        tn[t] = l2 > 0.0 ? vec3{n.x / l, n.y / l, n.z / l} : vec3{0.0, 0.0, 0.0};
+    }
+    std::vector<std::array<uint32_t, 3>> nb(tr.size(), {UINT32_MAX, UINT32_MAX, UINT32_MAX});
+    {
+        std::map<std::pair<uint32_t, uint32_t>, uint32_t> open;  // directed edge -> (triangle, edge)
+        for (uint32_t t = 0; t < tr.size(); ++t)
+            for (int k = 0; k < 3; ++k) open[{tr[t].v[k], tr[t].v[(k + 1) % 3]}] = 3 * t + uint32_t(k);
+        for (uint32_t t = 0; t < tr.size(); ++t)
+            for (int k = 0; k < 3; ++k) {
+                auto it = open.find({tr[t].v[(k + 1) % 3], tr[t].v[k]});
+                if (it != open.end()) nb[t][k] = it->second / 3;
+            }
+    }
+    std::vector<vec3> vn(sv.size(), vec3{0.0, 0.0, 0.0});
+    for (uint32_t t = 0; t < tr.size(); ++t)
+        for (int k = 0; k < 3; ++k) {
+            vec3 const o = cur(tr[t].v[k]);
+            vec3 const e1 = cur(tr[t].v[(k + 1) % 3]) - o, e2 = cur(tr[t].v[(k + 2) % 3]) - o;
+            double const ang = std::atan2(std::sqrt(norm2(cross(e1, e2))), dot(e1, e2));
+            vn[tr[t].v[k]] = vn[tr[t].v[k]] + ang * tn[t];
+        }
+    auto pseudo_normal = [&](uint32_t t, int feature) {
+        switch (feature) {
+        case 0: return vn[tr[t].v[0]];
+        case 1: return vn[tr[t].v[1]];
+        case 2: return vn[tr[t].v[2]];
+        case 3: return nb[t][0] == UINT32_MAX ? tn[t] : tn[t] + tn[nb[t][0]];
+        case 4: return nb[t][1] == UINT32_MAX ? tn[t] : tn[t] + tn[nb[t][1]];
+        case 5: return nb[t][2] == UINT32_MAX ? tn[t] : tn[t] + tn[nb[t][2]];
+        default: return tn[t];
+        }
+    };
+    double const H = body_cell_request > 0.0 ? body_cell_request : 2.0 * mean_edge;
+    double const h = body_thickness_request > 0.0 ? body_thickness_request : 0.1 * mean_edge;
+    double const r = body_radius_request > 0.0 ? body_radius_request : mean_edge;
+    double const inv_H = 1.0 / H;
+    auto cell = [&](double v) { return (long long)std::floor(v * inv_H); };
+    std::map<std::array<long long, 3>, std::vector<uint32_t>> grid;
+    if (!body_brute_force)
+        for (uint32_t t = 0; t < tr.size(); ++t) {
+            vec3 const a = cur(tr[t].v[0]), b = cur(tr[t].v[1]), c = cur(tr[t].v[2]);
+            long long lo[3], hi[3];
+            double const mn[3] = {std::min({a.x, b.x, c.x}), std::min({a.y, b.y, c.y}), std::min({a.z, b.z, c.z})};
+            double const mx[3] = {std::max({a.x, b.x, c.x}), std::max({a.y, b.y, c.y}), std::max({a.z, b.z, c.z})};
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = cell(mn[d]);
+                hi[d] = cell(mx[d]);
+            }
+            for (long long k = lo[2]; k <= hi[2]; ++k)
+                for (long long j = lo[1]; j <= hi[1]; ++j)
+                    for (long long i = lo[0]; i <= hi[0]; ++i) grid[{i, j, k}].push_back(t);
+        }
+    std::vector<uint32_t> all;
+    if (body_brute_force) {
+        all.resize(tr.size());
+        for (uint32_t t = 0; t < tr.size(); ++t) all[t] = t;
+    }
+    uint32_t const nsdf = uint32_t(sdfs.size());
+    for (uint32_t s = 0; s < sv.size(); ++s) {
+        vec3 const p = cur(s);
+        std::vector<uint32_t> near;  // the triangles of the 27 cells around p's cell
+        if (!body_brute_force) {
+            long long const ci = cell(p.x), cj = cell(p.y), ck = cell(p.z);
+            for (long long k = ck - 1; k <= ck + 1; ++k)
+                for (long long j = cj - 1; j <= cj + 1; ++j)
+                    for (long long i = ci - 1; i <= ci + 1; ++i) {
+                        auto it = grid.find({i, j, k});
+                        if (it != grid.end()) near.insert(near.end(), it->second.begin(), it->second.end());
+                    }
+        }
+        std::vector<uint32_t> const* cand = body_brute_force ? &all : &near;
+        for (int own = 0; own < 2; ++own) {  // another body's nearest triangle, then its own surface's
+            double best = r * r;
+            uint32_t bt = UINT32_MAX;
+            int bf = 6;
+            vec3 bq{};
+            double bw[3] = {0.0, 0.0, 0.0};
+            for (uint32_t t : *cand) {
+                if ((tr[t].body == sv[s].first) != (own != 0)) continue;
+                if (tr[t].v[0] == s || tr[t].v[1] == s || tr[t].v[2] == s) continue;
+                int feature;
+                double wts[3];
+                vec3 const a = cur(tr[t].v[0]), b = cur(tr[t].v[1]), c = cur(tr[t].v[2]);
+                vec3 const q = closest_point_on_triangle(p, a, b, c, &feature, wts);
+                double const d2 = norm2(p - q);
+                if (!(d2 < best || (d2 == best && bt != UINT32_MAX && t < bt))) continue;
+                if (own) {  // its own surface: more than h / 10 outside along the feature's pseudo-normal
+                    vec3 const N = pseudo_normal(t, feature);
+                    if (!(dot(p - q, N) > 0.1 * h * std::sqrt(norm2(N)))) continue;
+                }
+                best = d2;
+                bt = t;
+                bf = feature;
+                bq = q;
+                bw[0] = wts[0], bw[1] = wts[1], bw[2] = wts[2];
+            }
+            if (bt == UINT32_MAX) continue;
+            // along the line from the closest point to the vertex, out of the surface (towards the
+            // closest point when the vertex lies inside another body, by the sign of (p - c).N); the
+            // triangle's normal when the vertex lies on the triangle
+            vec3 n = tn[bt];
+            vec3 const d = p - bq;
+            double const dd = norm2(d);
+            if (dd > 0.0) {
+                double const l = dot(d, pseudo_normal(bt, bf)) < 0.0 ? -std::sqrt(dd) : std::sqrt(dd);
+                n = vec3{d.x / l, d.y / l, d.z / l};
+            }
+            if (!(norm2(n) > 0.0)) continue;  // a degenerate triangle has no plane
+            uint32_t const b = sv[s].first, vi = surface[b][sv[s].second];
+            uint32_t cb[3], cvv[3];
+            for (int k = 0; k < 3; ++k) {
+                cb[k] = sv[tr[bt].v[k]].first;
+                cvv[k] = surface[cb[k]][sv[tr[bt].v[k]].second];
+            }
+            // the contact record reports the barycentric weights in place of the contact point
+            contacts.push_back({b, sv[s].second, vi, nsdf + bt, vec3{bw[0], bw[1], bw[2]}, n});
+            collision_constraints.push_back(std::make_unique<body_collision_constraint_t>(
+                collision_compliance, collision_damping, b, vi, cb, cvv, bw, h, n));
+            have_body_contacts = true;
         }
     }
 }
@@ -548,6 +753,13 @@ void simulation_t::solve(double dt, std::size_t iterations)
     if (warm_start) apply_warm_start();
     bool const parallel = threads > 1 && colored_order;
     for (std::size_t k = 0; k < iterations; ++k) {
+        if (have_body_contacts) {  // the positions this iteration's triangle contacts read
+            contact_snap.resize(particles.size());
+            for (std::size_t b = 0; b < particles.size(); ++b) {
+                contact_snap[b].resize(particles[b].size());
+                for (std::size_t i = 0; i < particles[b].size(); ++i) contact_snap[b][i] = particles[b][i].xi;
+            }
+        }
         for (auto& c : collision_constraints) c->project_positions(*this, dt);
         if (!parallel) {
             for (uint32_t i : order) constraints[i]->project_positions(*this, dt);

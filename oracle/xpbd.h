@@ -122,6 +122,32 @@ struct collision_constraint_t final : constraint_t {
     void residual_sq(simulation_t const& sim, double dt, double* acc) const override;
 };
 
+// A contact of a surface vertex with a surface triangle (no reference counterpart): the triangle's
+// corners (body, vertex), the closest point's barycentric weights w and the normal n, fixed at
+// detection; the plane point q = sum_k w_k c_k + h n is taken at the corners' positions copied at
+// the start of the contact projections of each iteration (simulation_t::contact_snap); otherwise
+// the SDF contact's update (C = (p - q).n, projected when negative) with the triangle point's inverse
+// mass sum_k w_k^2 w(c_k) added to the denominator: the vertex moves by its share of the two-sided
+// projection, the corners take theirs through their own contacts.
+struct body_collision_constraint_t final : constraint_t {
+    uint32_t bi, vi, cb[3], cv[3];
+    double w[3], h;
+    vec3 n;
+    body_collision_constraint_t(double alpha, double beta, uint32_t b, uint32_t v, uint32_t const corner_b[3],
+                                uint32_t const corner_v[3], double const wts[3], double thickness, vec3 n_)
+        : constraint_t(alpha, beta), bi(b), vi(v), h(thickness), n(n_)
+    {
+        for (int k = 0; k < 3; ++k) {
+            cb[k] = corner_b[k];
+            cv[k] = corner_v[k];
+            w[k] = wts[k];
+        }
+    }
+    vec3 plane_point(simulation_t const& sim) const;
+    void project_positions(simulation_t& sim, double dt) override;
+    void residual_sq(simulation_t const& sim, double dt, double* acc) const override;
+};
+
 // Analytic SDF obstacles. Plane: Eigen::Hyperplane(n, point) semantics used by
 // sdf_model_t::from_plane (sdf_model.cpp:52-64): sd = n.p - n.point, grad = n.
 // Sphere: sd = |p - c| - r, grad = (p - c)/|p - c| (no reference counterpart).
@@ -163,8 +189,13 @@ struct group_t {
 struct simulation_t {
     std::vector<std::vector<particle_t>> particles;
     std::vector<std::vector<uint32_t>> surface;  // per body: surface vertex -> body vertex
+    std::vector<std::vector<uint32_t>> triangles;  // per body: boundary triangles, 3 surface vertices each
     std::vector<std::unique_ptr<constraint_t>> constraints;
     std::vector<std::unique_ptr<constraint_t>> collision_constraints;
+    // the positions the triangle contacts of an iteration's contact projections read (copied at their
+    // start; per body, per vertex)
+    std::vector<std::vector<vec3>> contact_snap;
+    bool have_body_contacts = false;
     std::vector<sdf_t> sdfs;
     std::vector<group_t> groups;
     std::vector<contact_t> contacts;  // contacts of the current frame, in generation order
@@ -204,10 +235,28 @@ struct simulation_t {
     std::vector<std::size_t> stage_off;
     std::vector<std::pair<uint32_t, uint32_t>> stage_elems;
     int threads = 1;
+    // test hook: with threads == 1, every green projection appends its branches (bit 0 inverted,
+    // bits 1-3 singular value i clamped to 0.577, bit 4 the epsilon early-out)
+    std::unique_ptr<std::vector<uint8_t>> green_trace;
     bool order_dirty = true;
     void build_order();
 
     void detect_contacts();
+    // Contacts between bodies (no reference counterpart): once per frame, after the SDF contacts,
+    // each surface vertex p (global surface order) gets a contact with the nearest surface triangle
+    // t (global order; of another body with p not more than r inside its plane, or of its own body
+    // without p as a corner and with p more than h outside its plane over its face) whose closest
+    // point c has |p - c|^2 < r^2 (the lowest triangle index on ties): plane point c + h n, n the
+    // unit direction from c to p, negated when p lies behind t's outward normal (t's normal when p
+    // lies on t or t is p's own body's; a body whose surface encloses a negative volume at rest has
+    // its triangles flipped);
+    // contact index: number of SDFs + t. Defaults (0): cell edge twice, h a
+    // tenth of and r equal to the mean surface edge at rest. The broad phase is a uniform grid of the triangles' boxes,
+    // queried in the 27 cells around p's cell (body_brute_force: every triangle), which holds every
+    // triangle closer than the cell edge.
+    bool body_contacts = false, body_brute_force = false;
+    double body_cell_request = 0.0, body_thickness_request = 0.0, body_radius_request = 0.0;
+    void detect_body_contacts();
     void solve(double dt, std::size_t iterations);  // gauss_seidel_solver_t::solve
     void apply_warm_start();
     void step(double dt, std::size_t iterations, std::size_t substeps);  // timestep_t::step

@@ -35,65 +35,6 @@ struct grid_params {
     double dmin[3], cell[3];
 };
 
-__device__ __forceinline__ void closest_point(const double p[3], const double* t, double q[3], int* feature)
-{
-#pragma clang fp contract(off)
-    double ab[3], ac[3], ap[3], bp[3], cp[3];
-    for (int d = 0; d < 3; ++d) {
-        ab[d] = t[3 + d] - t[d];
-        ac[d] = t[6 + d] - t[d];
-        ap[d] = p[d] - t[d];
-    }
-    double const d1 = ab[0] * ap[0] + ab[1] * ap[1] + ab[2] * ap[2];
-    double const d2 = ac[0] * ap[0] + ac[1] * ap[1] + ac[2] * ap[2];
-    if (d1 <= 0.0 && d2 <= 0.0) {
-        *feature = 0;
-        for (int d = 0; d < 3; ++d) q[d] = t[d];
-        return;
-    }
-    for (int d = 0; d < 3; ++d) bp[d] = p[d] - t[3 + d];
-    double const d3 = ab[0] * bp[0] + ab[1] * bp[1] + ab[2] * bp[2];
-    double const d4 = ac[0] * bp[0] + ac[1] * bp[1] + ac[2] * bp[2];
-    if (d3 >= 0.0 && d4 <= d3) {
-        *feature = 1;
-        for (int d = 0; d < 3; ++d) q[d] = t[3 + d];
-        return;
-    }
-    double const vc = d1 * d4 - d3 * d2;
-    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
-        double const v = d1 / (d1 - d3);
-        *feature = 3;
-        for (int d = 0; d < 3; ++d) q[d] = t[d] + v * ab[d];
-        return;
-    }
-    for (int d = 0; d < 3; ++d) cp[d] = p[d] - t[6 + d];
-    double const d5 = ab[0] * cp[0] + ab[1] * cp[1] + ab[2] * cp[2];
-    double const d6 = ac[0] * cp[0] + ac[1] * cp[1] + ac[2] * cp[2];
-    if (d6 >= 0.0 && d5 <= d6) {
-        *feature = 2;
-        for (int d = 0; d < 3; ++d) q[d] = t[6 + d];
-        return;
-    }
-    double const vb = d5 * d2 - d1 * d6;
-    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
-        double const w = d2 / (d2 - d6);
-        *feature = 5;
-        for (int d = 0; d < 3; ++d) q[d] = t[d] + w * ac[d];
-        return;
-    }
-    double const va = d3 * d6 - d5 * d4;
-    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
-        double const w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
-        *feature = 4;
-        for (int d = 0; d < 3; ++d) q[d] = t[3 + d] + w * (t[6 + d] - t[3 + d]);
-        return;
-    }
-    double const den = 1.0 / (va + vb + vc);
-    double const v = vb * den, w = vc * den;
-    *feature = 6;
-    for (int d = 0; d < 3; ++d) q[d] = t[d] + ab[d] * v + ac[d] * w;
-}
-
 // Squared distance from p to the box lo..hi (0 inside).
 __device__ __forceinline__ double box_dist2(const double p[3], const double* box)
 {
@@ -168,7 +109,7 @@ __global__ void __launch_bounds__(kBlock) k_grid_nodes(mesh_dev mesh, grid_param
             for (uint32_t t = 0; t < cnt; ++t) {
                 double q[3];
                 int feat;
-                closest_point(p, tc + 9 * t, q, &feat);
+                tri_closest_point(p, tc + 9 * t, q, &feat);
                 double const e0 = p[0] - q[0], e1 = p[1] - q[1], e2 = p[2] - q[2];
                 double const d2 = e0 * e0 + e1 * e1 + e2 * e2;
                 uint32_t const o = to[t];

@@ -72,6 +72,13 @@ struct contact_dev {
     float* lambda;
     uint32_t* count;       // contacts per surface vertex
     uint32_t* overflow;    // contacts dropped because a vertex had more than `slots`
+    // contacts with surface triangles (index >= nsdf): q.xyz holds the barycentric weights of the
+    // triangle's closest point, and the plane point is taken from the surface positions copied at the
+    // start of the contact launch (snap[s], per surface vertex) plus body_h along n
+    const pos4* snap;
+    const uint4* btris;
+    uint32_t nsdf;
+    double body_h;
 };
 
 void launch_predict(hipStream_t s, pos4* xi, const pos4* xn, float4* v, uint32_t n, float gx, float gy,
@@ -146,6 +153,9 @@ void launch_distance(hipStream_t s, distance_group_dev g, pos4* xi, const pos4* 
 
 void launch_detect(hipStream_t s, contact_dev c, const pos4* x, const sdf_dev* sdfs, uint32_t nsdf, int block);
 void launch_contact(hipStream_t s, contact_dev c, pos4* xi, float alpha_tilde, int first, int block);
+// snap[s] = xi[surf[s]] for s < S: the surface positions the triangle contacts of a contact launch
+// read.
+void launch_snapshot_surface(hipStream_t s, const uint32_t* surf, const pos4* xi, pos4* snap, uint32_t S);
 
 // Residual partial sums: one double per block of `block` threads over [begin, begin+count).
 void launch_green_residual(hipStream_t s, tet_group_dev g, const pos4* x, uint32_t begin, uint32_t count,

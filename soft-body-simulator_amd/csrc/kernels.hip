@@ -391,7 +391,10 @@ __device__ __forceinline__ void neohookean_displacement(Pos const P[4], float4 a
 // The contacts of surface vertex s (all its slots in SDF order) projected onto P, the vertex's
 // position (collision_constraint.cpp:21-41): C = (p - q).n; when C < 0, dl = -(C + alpha~ lm) /
 // (w + alpha~), lm += dl, p += w n dl. first: the first iteration (multipliers start at zero, and
-// are stored as zero for slots without penetration).
+// are stored as zero for slots without penetration). For a contact with a surface triangle, q is the
+// triangle's point of the stored barycentric weights b at the copied surface positions, plus h n, and
+// the denominator adds that point's inverse mass sum_k b_k^2 w_k: the vertex moves by its share of
+// the two-sided projection (the triangle's corners take theirs through their own contacts).
 __device__ __forceinline__ void contact_vertex_at(contact_dev const& c, pos4& P, uint32_t s, float alpha_tilde,
                                                   bool first)
 {
@@ -399,15 +402,24 @@ __device__ __forceinline__ void contact_vertex_at(contact_dev const& c, pos4& P,
     float const w = wof(P);
     for (uint32_t k = 0; k < cnt; ++k) {
         uint32_t const slot = k * c.S + s;
-        pos4 const q = c.q[slot];
+        pos4 q = c.q[slot];
         v3 const n = xyz(c.n[slot]);
+        float wt = 0.0f;
+        if (c.snap && q.w >= double(c.nsdf)) {  // a surface triangle: its closest point as it is now
+            uint4 const tr = c.btris[uint32_t(q.w) - c.nsdf];
+            pos4 const a = c.snap[tr.x], b = c.snap[tr.y], d = c.snap[tr.z];
+            wt = float(q.x * q.x * a.w + q.y * q.y * b.w + q.z * q.z * d.w);
+            double const hx = c.body_h * double(n.x), hy = c.body_h * double(n.y), hz = c.body_h * double(n.z);
+            q = make_double4(q.x * a.x + q.y * b.x + q.z * d.x + hx, q.x * a.y + q.y * b.y + q.z * d.y + hy,
+                             q.x * a.z + q.y * b.z + q.z * d.z + hz, q.w);
+        }
         float const lm = first ? 0.0f : c.lambda[slot];
         float const C = fdot(sub(P, q), n);
         if (C >= 0.0f) {
             if (first) c.lambda[slot] = 0.0f;
             continue;
         }
-        float const dl = -fmaf(alpha_tilde, lm, C) / (w + alpha_tilde);
+        float const dl = -fmaf(alpha_tilde, lm, C) / (w + wt + alpha_tilde);
         c.lambda[slot] = lm + dl;
         float const sd = w * dl;
         P.x += double(sd * n.x);
@@ -1012,6 +1024,13 @@ __global__ void k_detect(contact_dev c, const pos4* __restrict__ x, const sdf_de
 }
 
 // collision_constraint.cpp:21-41, all contacts of one particle in SDF order by one thread.
+__global__ void k_snapshot_surface(const uint32_t* __restrict__ surf, const pos4* __restrict__ xi,
+                                   pos4* __restrict__ snap, uint32_t S)
+{
+    uint32_t const s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S) snap[s] = xi[surf[s]];
+}
+
 __global__ void k_contact(contact_dev c, pos4* xi, float alpha_tilde, int first)
 {
     uint32_t const s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1448,6 +1467,11 @@ void launch_green_hex(hipStream_t s, tet_group_dev g, cluster_dev cl, pos4* xi, 
 void launch_detect(hipStream_t s, contact_dev c, const pos4* x, const sdf_dev* sdfs, uint32_t nsdf, int block)
 {
     if (c.S) k_detect<<<grid_for(c.S, block), block, 0, s>>>(c, x, sdfs, nsdf);
+}
+
+void launch_snapshot_surface(hipStream_t s, const uint32_t* surf, const pos4* xi, pos4* snap, uint32_t S)
+{
+    if (S) k_snapshot_surface<<<grid_for(S, 256), 256, 0, s>>>(surf, xi, snap, S);
 }
 
 void launch_contact(hipStream_t s, contact_dev c, pos4* xi, float alpha_tilde, int first, int block)

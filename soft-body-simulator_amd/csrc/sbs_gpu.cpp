@@ -14,9 +14,11 @@
 #include <numeric>
 #include <string>
 #include <memory>
+#include <unordered_map>
 #include <vector>
 
 #include "abi_util.h"
+#include "body_contact.h"
 #include "contact_hash.h"
 #include "grid_sdf.h"
 #include "kernels.h"
@@ -145,6 +147,25 @@ struct sbs_gpu {
 
     dev_buf<double> partial;
 
+    // contacts between bodies (surface vertex against surface triangle); sbs_gpu_set_body_contacts
+    bool body_contacts = false;
+    double body_cell_request = 0.0, body_thickness_request = 0.0, body_radius_request = 0.0;  // 0: defaults
+    double body_cell = 0.0, body_thickness = 0.0, body_radius = 0.0;
+    int body_bits = 0;
+    dev_buf<uint4> btris;
+    dev_buf<uint32_t> bkeys, bvals, bstart, bend, bws, bsbody;
+    dev_buf<uint4> bnbr;                 // edge neighbours of each triangle
+    dev_buf<uint32_t> bvt_off, bvt;      // per surface vertex: its (triangle, corner) entries
+    dev_buf<pos4> btn, bvn;              // per frame: triangle unit normals, vertex pseudo-normals
+    dev_buf<pos4> csnap;  // surface positions at the start of a contact launch (triangle contacts)
+    body_contact_dev body_view()
+    {
+        return {btris.p, uint32_t(btris.n), 1.0 / body_cell, body_thickness, body_radius, (1u << body_bits) - 1u,
+                uint32_t(sdfs.size()), bkeys.p, bvals.p, bstart.p, bend.p, coverflow.p, bsbody.p,
+                bnbr.p, bvt_off.p, bvt.p, btn.p, bvn.p};
+    }
+    void setup_body_contacts(std::vector<uint4> const& tr, std::vector<pos4> const& x);
+
     int block = 256;
     int cluster_block = 64;
     bool fuse_first_contacts = true;  // the first iteration's contacts in the warm-start gather
@@ -261,7 +282,11 @@ struct sbs_gpu {
         pending.clear();
     }
 
-    contact_dev contacts() { return {surf.p, S, slots, cq.p, cn.p, clam.p, ccount.p, coverflow.p}; }
+    contact_dev contacts()
+    {
+        return {surf.p, S, slots, cq.p, cn.p, clam.p, ccount.p, coverflow.p, csnap.p, btris.p, uint32_t(sdfs.size()),
+                body_thickness};
+    }
     contact_hash_dev hash_view() const
     {
         return {hstart.p, hend.p, hentries.p, hglobal.p, uint32_t(global_ids.size()), (1u << hash_bits) - 1u,
@@ -676,12 +701,13 @@ void sbs_gpu::commit(uint32_t recolor_passes)
     for (auto const& b : bodies)
         for (uint32_t s : b.surface) sv.push_back(b.offset + s);
     S = uint32_t(sv.size());
-    {  // boundary triangles of all bodies in global surface numbering, and each vertex's triangles
-        std::vector<uint4> tr;
+    std::vector<uint4> tr;  // boundary triangles of all bodies in global surface numbering, w = body
+    {  // and each vertex's triangles
         uint32_t base = 0;
-        for (auto const& b : bodies) {
+        for (uint32_t bi = 0; bi < bodies.size(); ++bi) {
+            body_t const& b = bodies[bi];
             for (std::size_t t = 0; t + 2 < b.triangles.size(); t += 3)
-                tr.push_back(make_uint4(base + b.triangles[t], base + b.triangles[t + 1], base + b.triangles[t + 2], 0));
+                tr.push_back(make_uint4(base + b.triangles[t], base + b.triangles[t + 1], base + b.triangles[t + 2], bi));
             base += uint32_t(b.surface.size());
         }
         ntris = uint32_t(tr.size());
@@ -706,7 +732,9 @@ void sbs_gpu::commit(uint32_t recolor_passes)
             hip_check(hipMemsetAsync(snormal.p, 0, S * sizeof(double4), stream), "clear normals");
         }
     }
-    slots = uint32_t(std::min<std::size_t>(sdfs.size(), kMaxContactSlots));
+    // one more contact slot for the nearest surface triangle of another body (or of the vertex's own)
+    bool const bodies_collide = body_contacts && S && !tr.empty();
+    slots = uint32_t(std::min<std::size_t>(sdfs.size() + (bodies_collide ? 2 : 0), kMaxContactSlots));
     if (S) surf.upload(sv.data(), S, stream);
     {
         std::vector<int> so(V, -1);
@@ -726,6 +754,7 @@ void sbs_gpu::commit(uint32_t recolor_passes)
         *overflow_host = 0;
         dsdf.upload(sdfs.data(), sdfs.size(), stream);
         build_hash();
+        if (bodies_collide) setup_body_contacts(tr, x);
     }
     hip_check(hipStreamSynchronize(stream), "commit");
     committed = true;
@@ -739,7 +768,97 @@ void sbs_gpu::detect()
             timed(SBS_KERNEL_DETECT, S, [&] { launch_detect_hashed(stream, c, xn.p, dsdf.p, hash_view(), block); });
         else
             timed(SBS_KERNEL_DETECT, S, [&] { launch_detect(stream, c, xn.p, dsdf.p, uint32_t(sdfs.size()), block); });
+        if (btris.n)  // after the SDF contacts of every vertex: the nearest surface triangle
+            timed(SBS_KERNEL_DETECT, S, [&] {
+                body_contact_dev const b = body_view();
+                build_body_hash(stream, b, surf.p, xn.p, body_bits, bws.p);
+                launch_detect_bodies(stream, c, b, xn.p, block);
+            });
     }
+}
+
+// The surface triangles as contact obstacles: each body's triangles oriented outward (a body whose
+// surface encloses a negative volume at rest has every triangle flipped), the cell edge (default
+// twice the mean surface edge at rest), the thickness (default a tenth of the mean edge) and the
+// radius (default the mean edge), a table of at least 4 slots per triangle, and the per-frame
+// buffers.
+void sbs_gpu::setup_body_contacts(std::vector<uint4> const& tr, std::vector<pos4> const& x)
+{
+    std::vector<uint32_t> sv, sbody;
+    for (uint32_t bi = 0; bi < bodies.size(); ++bi)
+        for (uint32_t s : bodies[bi].surface) {
+            sv.push_back(bodies[bi].offset + s);
+            sbody.push_back(bi);
+        }
+    bsbody.upload(sbody.data(), sbody.size(), stream);
+    std::vector<double> vol(bodies.size(), 0.0);
+    double edge_sum = 0.0;
+    auto P = [&](uint32_t s) { pos4 const& q = x[sv[s]]; return std::array<double, 3>{q.x, q.y, q.z}; };
+    for (auto const& t : tr) {
+        auto const a = P(t.x), b = P(t.y), c = P(t.z);
+        vol[t.w] += a[0] * (b[1] * c[2] - b[2] * c[1]) + a[1] * (b[2] * c[0] - b[0] * c[2]) + a[2] * (b[0] * c[1] - b[1] * c[0]);
+        auto len = [](std::array<double, 3> const& u, std::array<double, 3> const& v) {
+            double const dx = u[0] - v[0], dy = u[1] - v[1], dz = u[2] - v[2];
+            return std::sqrt(dx * dx + dy * dy + dz * dz);
+        };
+        edge_sum += len(a, b) + len(b, c) + len(c, a);
+    }
+    double const mean_edge = edge_sum / (3.0 * double(tr.size()));
+    body_cell = body_cell_request > 0.0 ? body_cell_request : 2.0 * mean_edge;
+    body_thickness = body_thickness_request > 0.0 ? body_thickness_request : 0.1 * mean_edge;
+    body_radius = body_radius_request > 0.0 ? body_radius_request : mean_edge;
+    require(body_thickness < body_radius && body_radius <= 0.999 * body_cell,
+            "body contacts need thickness < radius <= 0.999 cell size");
+    std::vector<uint4> oriented(tr);
+    for (auto& t : oriented)
+        if (vol[t.w] < 0.0) std::swap(t.y, t.z);
+    body_bits = 10;
+    while ((uint64_t(1) << body_bits) < 4 * uint64_t(tr.size()) && body_bits < 28) ++body_bits;
+    std::size_t const n = std::size_t(kTriCells) * tr.size();
+    require(n < (std::size_t(1) << 31), "too many surface triangles for the body contact hash");
+    btris.upload(oriented.data(), oriented.size(), stream);
+    // the triangle across each edge (the oriented edge b->a of the neighbour matches a->b), and each
+    // surface vertex's (triangle, corner) entries in increasing triangle order
+    std::vector<uint4> nbr(oriented.size(), make_uint4(~0u, ~0u, ~0u, 0u));
+    {
+        std::unordered_map<uint64_t, uint32_t> edge;
+        edge.reserve(3 * oriented.size());
+        auto key = [](uint32_t a, uint32_t b) { return (uint64_t(a) << 32) | b; };
+        for (uint32_t t = 0; t < oriented.size(); ++t) {
+            uint32_t const v[3] = {oriented[t].x, oriented[t].y, oriented[t].z};
+            for (int k = 0; k < 3; ++k) edge[key(v[k], v[(k + 1) % 3])] = t;
+        }
+        for (uint32_t t = 0; t < oriented.size(); ++t) {
+            uint32_t const v[3] = {oriented[t].x, oriented[t].y, oriented[t].z};
+            uint32_t o[3];
+            for (int k = 0; k < 3; ++k) {
+                auto it = edge.find(key(v[(k + 1) % 3], v[k]));
+                o[k] = it == edge.end() ? ~0u : it->second;
+            }
+            nbr[t] = make_uint4(o[0], o[1], o[2], 0u);
+        }
+    }
+    std::vector<uint32_t> vt_off(std::size_t(S) + 1, 0), vt(3 * oriented.size());
+    for (auto const& t : oriented) ++vt_off[t.x + 1], ++vt_off[t.y + 1], ++vt_off[t.z + 1];
+    for (std::size_t s = 0; s < S; ++s) vt_off[s + 1] += vt_off[s];
+    {
+        std::vector<uint32_t> fill(vt_off.begin(), vt_off.end() - 1);
+        for (uint32_t t = 0; t < oriented.size(); ++t) {
+            uint32_t const v[3] = {oriented[t].x, oriented[t].y, oriented[t].z};
+            for (uint32_t k = 0; k < 3; ++k) vt[fill[v[k]]++] = 3 * t + k;
+        }
+    }
+    bnbr.upload(nbr.data(), nbr.size(), stream);
+    bvt_off.upload(vt_off.data(), vt_off.size(), stream);
+    bvt.upload(vt.data(), vt.size(), stream);
+    btn.alloc(oriented.size());
+    bvn.alloc(S);
+    bkeys.alloc(n);
+    bvals.alloc(n);
+    bstart.alloc(std::size_t(1) << body_bits);
+    bend.alloc(std::size_t(1) << body_bits);
+    bws.alloc(body_contact_workspace(uint32_t(tr.size())));
+    csnap.alloc(S);
 }
 
 // Obstacles with a finite volume of at most kMaxHashCells cells are hashed; the others (unbounded
@@ -881,7 +1000,7 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
         // the first iteration's contacts ride in the warm-start pass's gather when it is the last
         // launch before them: one group, hex clusters, the dense gather
         bool const fused0 = have_contacts && fuse_first_contacts && iterations > 0 && warm != SBS_WARM_START_NONE &&
-                            groups.size() == 1 &&
+                            groups.size() == 1 && !csnap.p &&
                             groups[0].kind == 1 && groups[0].hex && !groups[0].ws_csr_off.p;
         if (copy_xt) timed(SBS_KERNEL_WARM_START, V, [&] { launch_copy_pos(stream, xi.p, xt.p, V); });
         const pos4* const xtilde = copy_xt ? xt.p : xi.p;
@@ -913,7 +1032,10 @@ void sbs_gpu::enqueue_frame(double dt_frame, uint32_t iterations, uint32_t subst
         for (uint32_t it = 0; it < iterations; ++it) {
             int const first = it == 0;
             if (have_contacts && !(first && fused0 && warmed))
-                timed(SBS_KERNEL_CONTACT, S, [&] { launch_contact(stream, c, xi.p, c_at, first, block); });
+                timed(SBS_KERNEL_CONTACT, S, [&] {
+                    if (csnap.p) launch_snapshot_surface(stream, surf.p, xi.p, csnap.p, S);
+                    launch_contact(stream, c, xi.p, c_at, first, block);
+                });
             for (auto const& g : groups) {
                 int const kind = g.kind == 0 ? SBS_KERNEL_GREEN : (g.kind == 1 ? SBS_KERNEL_NEOHOOKEAN : SBS_KERNEL_DISTANCE);
                 int const mode = (first && !(warmed && g.kind == 1)) ? kSolveFirst : kSolveNext;
@@ -1973,6 +2095,83 @@ int sbs_gpu_read_contact_hash(sbs_gpu* sim, double* cell_size, uint32_t* table_b
                           "read hash");
         }
         if (num_entries) *num_entries = n;
+    });
+}
+
+int sbs_gpu_set_body_contacts(sbs_gpu* sim, int enable, double cell_size, double thickness, double radius)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(!sim->committed, "body contacts are set before sbs_gpu_commit", SBS_ERR_STATE);
+        require(std::isfinite(cell_size) && cell_size >= 0.0 && std::isfinite(thickness) && thickness >= 0.0 &&
+                    std::isfinite(radius) && radius >= 0.0,
+                "cell_size, thickness and radius must be finite and >= 0");
+        sim->body_contacts = enable != 0;
+        sim->body_cell_request = cell_size;
+        sim->body_thickness_request = thickness;
+        sim->body_radius_request = radius;
+    });
+}
+
+int sbs_gpu_read_body_contact_setup(sbs_gpu* sim, double* cell_size, double* thickness, double* radius,
+                                    uint32_t* table_bits, uint32_t* triangles)
+{
+    return guarded([&] {
+        require(sim != nullptr, "sim is null");
+        require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
+        if (cell_size) *cell_size = sim->btris.n ? sim->body_cell : 0.0;
+        if (thickness) *thickness = sim->btris.n ? sim->body_thickness : 0.0;
+        if (radius) *radius = sim->btris.n ? sim->body_radius : 0.0;
+        if (table_bits) *table_bits = uint32_t(sim->btris.n ? sim->body_bits : 0);
+        if (triangles && sim->btris.n)
+            hip_check(hipMemcpy(triangles, sim->btris.p, sim->btris.n * sizeof(uint4), hipMemcpyDeviceToHost),
+                      "read triangles");
+    });
+}
+
+int sbs_gpu_read_body_contact_candidates(sbs_gpu* sim, uint32_t* pairs, uint64_t cap, uint64_t* count)
+{
+    return guarded([&] {
+        require(sim != nullptr && count != nullptr, "null argument");
+        require(sim->committed, "sbs_gpu_commit has not been called", SBS_ERR_STATE);
+        sim->set_current();
+        *count = 0;
+        if (!sim->btris.n) return;
+        body_contact_dev const b = sim->body_view();
+        contact_dev const c = sim->contacts();
+        build_body_hash(sim->stream, b, sim->surf.p, sim->xn.p, sim->body_bits, sim->bws.p);
+        dev_buf<uint32_t> counts, offs, ws;
+        counts.alloc(sim->S);
+        offs.alloc(std::size_t(sim->S) + 1);
+        ws.alloc(scan_workspace(sim->S));
+        launch_body_candidates(sim->stream, c, b, sim->xn.p, nullptr, counts.p, nullptr);
+        exclusive_scan(sim->stream, counts.p, offs.p, sim->S, ws.p);
+        uint32_t total = 0;
+        hip_check(hipMemcpyAsync(&total, offs.p + sim->S, sizeof(uint32_t), hipMemcpyDeviceToHost, sim->stream),
+                  "read candidates");
+        hip_check(hipStreamSynchronize(sim->stream), "read candidates");
+        *count = total;  // with repeats; the canonical count follows when pairs are read
+        if (!total) return;
+        dev_buf<uint32_t> out;
+        out.alloc(2 * std::size_t(total));
+        launch_body_candidates(sim->stream, c, b, sim->xn.p, offs.p, counts.p, out.p);
+        std::vector<uint64_t> raw(total);
+        {
+            std::vector<uint32_t> h(2 * std::size_t(total));
+            hip_check(hipMemcpyAsync(h.data(), out.p, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, sim->stream),
+                      "read candidates");
+            hip_check(hipStreamSynchronize(sim->stream), "read candidates");
+            for (std::size_t i = 0; i < total; ++i) raw[i] = (uint64_t(h[2 * i]) << 32) | h[2 * i + 1];
+        }
+        // the canonical list: (vertex, triangle) increasing, each pair once
+        std::sort(raw.begin(), raw.end());
+        raw.erase(std::unique(raw.begin(), raw.end()), raw.end());
+        *count = raw.size();
+        std::size_t const n = std::min<uint64_t>(raw.size(), cap);
+        for (std::size_t i = 0; i < n; ++i) {
+            pairs[2 * i] = uint32_t(raw[i] >> 32);
+            pairs[2 * i + 1] = uint32_t(raw[i]);
+        }
     });
 }
 

@@ -18,6 +18,7 @@ namespace sbs {
 namespace {
 
 constexpr int kBlock = 256, kItems = 4, kTile = kBlock * kItems;
+static_assert(kTile == kScanTile, "tile size of the workspace formulas");
 
 unsigned blocks_of(std::size_t n) { return unsigned((n + kTile - 1) / kTile); }
 
@@ -167,18 +168,6 @@ __global__ void __launch_bounds__(kBlock) k_digit_scatter(const uint32_t* __rest
 
 }  // namespace
 
-std::size_t scan_workspace(std::size_t n)
-{
-    // per level: tile sums (tiles + 1 words) and their scan (tiles + 1 words), recursively
-    std::size_t ws = 0;
-    while (n > 1) {
-        std::size_t const t = blocks_of(n);
-        ws += 2 * (t + 1);
-        n = t;
-    }
-    return ws + 2;
-}
-
 void exclusive_scan(hipStream_t s, const uint32_t* in, uint32_t* out, std::size_t n, uint32_t* ws)
 {
     if (n == 0) {
@@ -196,12 +185,6 @@ void exclusive_scan(hipStream_t s, const uint32_t* in, uint32_t* out, std::size_
     exclusive_scan(s, sums, offs, tiles, offs + tiles + 1);
     k_add_tiles<<<tiles, kBlock, 0, s>>>(out, n, offs);
     k_set_total<<<1, 1, 0, s>>>(out, n, offs, tiles);
-}
-
-std::size_t sort_workspace(std::size_t n)
-{
-    std::size_t const hist = std::size_t(256) * blocks_of(n);
-    return 2 * n + 2 * (hist + 1) + scan_workspace(hist);
 }
 
 void radix_sort_pairs(hipStream_t s, uint32_t* keys, uint32_t* vals, std::size_t n, int bits, uint32_t* ws)

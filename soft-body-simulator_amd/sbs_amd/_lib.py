@@ -127,6 +127,9 @@ _SIGS = {
     "sbs_gpu_write_surface_ply": [_p, _u32, ctypes.c_char_p, _i],
     "sbs_gpu_read_contact_hash": [_p, _pd, _pu, _pu, _pu, _pu, _u32, _pu, _pu, _pu],
     "sbs_gpu_contact_overflow": [_p, _pu64],
+    "sbs_gpu_set_body_contacts": [_p, _i, _d, _d, _d],
+    "sbs_gpu_read_body_contact_setup": [_p, _pd, _pd, _pd, _pu, _pu],
+    "sbs_gpu_read_body_contact_candidates": [_p, _pu, _u64, _pu64],
     "sbs_util_exclusive_scan": [_i, _pu, _u64, _pu],
     "sbs_util_sort_pairs": [_i, _pu, _pu, _u64, _i, _pu, _pu],
     "sbs_gpu_extract_boundary": [_i, _u32, _pu, _u32, _pu, _pu, _pu, _pu],
@@ -454,9 +457,12 @@ class Simulation:
         self._surface_sizes = getattr(self, "_surface_sizes", {})
         self._surface_sizes[body] = len(s2t)
 
+    num_triangles = 0
+
     def set_surface_triangles(self, body, triangles):
         t = u32(triangles).reshape(-1)
         check(self._L.sbs_gpu_set_surface_triangles(self.h, body, _up(t), len(t) // 3))
+        self.num_triangles += len(t) // 3
 
     def set_surface_refresh(self, enable=True):
         check(self._L.sbs_gpu_set_surface_refresh(self.h, int(enable)))
@@ -653,6 +659,29 @@ class Simulation:
         check(self._L.sbs_gpu_read_contact_hash(self.h, None, None, _up(start) if T else None, _up(end) if T else None,
                                                 _up(entries), ne.value, ctypes.byref(ne), _up(glob), ctypes.byref(ng)))
         return h.value, bits.value, start, end, entries[:ne.value], glob[:ng.value]
+
+    def set_body_contacts(self, enable=True, cell_size=0.0, thickness=0.0, radius=0.0):
+        """Contacts between the bodies' surfaces (before commit; 0 selects the defaults)."""
+        check(self._L.sbs_gpu_set_body_contacts(self.h, int(enable), cell_size, thickness, radius))
+
+    def body_contact_setup(self):
+        """(cell size, thickness, radius, table bits, triangles [n][4]: outward surface vertices, body)."""
+        cell, h, r, bits = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), _u32()
+        check(self._L.sbs_gpu_read_body_contact_setup(self.h, ctypes.byref(cell), ctypes.byref(h), ctypes.byref(r),
+                                                      ctypes.byref(bits), None))
+        tris = np.zeros((self.num_triangles if cell.value > 0 else 0, 4), dtype=np.uint32)
+        if len(tris):
+            check(self._L.sbs_gpu_read_body_contact_setup(self.h, None, None, None, None, _up(tris)))
+        return cell.value, h.value, r.value, bits.value, tris
+
+    def body_contact_candidates(self):
+        """Broad-phase (surface vertex, triangle) pairs at the committed positions."""
+        n = ctypes.c_uint64()
+        check(self._L.sbs_gpu_read_body_contact_candidates(self.h, None, 0, ctypes.byref(n)))
+        out = np.zeros((n.value, 2), dtype=np.uint32)
+        if n.value:
+            check(self._L.sbs_gpu_read_body_contact_candidates(self.h, _up(out), n.value, ctypes.byref(n)))
+        return out
 
     def contact_overflow(self):
         n = ctypes.c_uint64()

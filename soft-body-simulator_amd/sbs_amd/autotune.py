@@ -45,7 +45,7 @@ def per_state_ratios(sim, scene, frames, iterations, plain_iterations=None, slot
     the frame (`iterations`) and gives r_warm(X_f), continuing the warm-started trajectory. Returns
     (r_warm, r_plain) over the frames: equal residual on the same state means r_warm <= r_plain."""
     plain_iterations = scene.iterations if plain_iterations is None else plain_iterations
-    mode = sim.warm_mode
+    mode, net = sim.warm_mode, sim.warm_net
     rw, rp = [], []
     for _ in range(frames):
         sim.snapshot(slot)
@@ -53,7 +53,7 @@ def per_state_ratios(sim, scene, frames, iterations, plain_iterations=None, slot
         sim.step(scene.dt, plain_iterations, scene.substeps)
         rp.append(frame_residual(sim, scene))
         sim.restore(slot)
-        sim.set_warm_start(mode, sim.warm_net)
+        sim.set_warm_start(mode, net)
         sim.step(scene.dt, iterations, scene.substeps)
         rw.append(frame_residual(sim, scene))
     return np.asarray(rw), np.asarray(rp)

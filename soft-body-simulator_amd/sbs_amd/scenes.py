@@ -50,6 +50,7 @@ class Scene:
     substeps: int = 1
     coloring: str = "hash"  # "hash" (device Jones-Plassmann) or "index" (sequential greedy)
     contact_hash: tuple = None  # (mode "auto" | "on" | "off", cell size) of the contact broad phase
+    body_contacts: tuple = None  # (cell size, thickness, radius) of the contacts between bodies (0: defaults)
 
     @property
     def num_tets(self):
@@ -95,6 +96,8 @@ class Scene:
             sim.add_mesh_obstacle(pos, tris, lo, hi, res)
         if self.contact_hash is not None and hasattr(sim, "set_contact_hash"):
             sim.set_contact_hash(*self.contact_hash)
+        if self.body_contacts is not None:
+            sim.set_body_contacts(True, *self.body_contacts)
 
     def set_initial_state(self, sim):
         """Initial velocities of the bodies that have them (after commit on the device)."""
@@ -279,6 +282,159 @@ def batch(num_bodies=512, tet_range=(2e3, 2e5), edge=0.05, seed=0xB0D1E5, densit
     conn = np.concatenate(bconn)
     sc.groups.append(Group("neohookean", np.concatenate(bodies), conn, young=np.concatenate(young),
                            poisson=np.concatenate(poisson), clusters=hex_clusters(len(conn))))
+    sc.planes.append(((0.0, 1.0, 0.0), (0.0, 0.0, 0.0), (-1e30,) * 3, (1e30,) * 3))
+    return sc
+
+
+def stack(edge=0.05, young=2e5, poisson=0.4, iterations=8, substeps=4, gap=0.02, name="stack_4_bodies",
+          body_contacts=(0.0, 0.0, 0.0), boxes=3, ellipsoid=True):
+    """A few bodies falling onto each other: three boxes of 4 x 4 x 4 hex cells stacked with gaps of
+    `gap` m, each shifted sideways by a quarter of its width and turned about y, and an ellipsoid
+    dropped onto the top box's corner; stable neo-Hookean constraints in one batched group with hex
+    clusters, a ground plane at y = 0, contacts between the bodies (cell size, thickness and radius,
+    0 for the defaults)."""
+    sc = Scene(name, gravity_is_force=False, iterations=iterations, substeps=substeps, coloring="index",
+               body_contacts=body_contacts)
+    meshes, y = [], 0.02
+    for k in range(boxes):
+        pos, tets = _lib.box6(4, 4, 4, edge)
+        pos = pos - pos.mean(0)
+        pos = pos @ _rotation((0.0, 1.0, 0.0), 0.3 * k).T
+        pos = pos + np.array([0.05 * k, y - pos[:, 1].min(), 0.03 * k])
+        y = pos[:, 1].max() + gap
+        meshes.append((pos, tets))
+    if ellipsoid:
+        pos, tets = _lib.ellipsoid6(0.12, 0.08, 0.1, edge)
+        pos = pos - pos.mean(0) + np.array([0.15, y + 0.05 - (pos[:, 1].min() - pos[:, 1].mean()), 0.1])
+        meshes.append((pos, tets))
+    bodies, bconn, clusters = [], [], [0]
+    for b, (x0, tets) in enumerate(meshes):
+        mass = _lib.lumped_mass(x0, tets, 1000.0)
+        s2t, tris = _lib.boundary(len(x0), tets)
+        sc.bodies.append(Body(x0=x0, tets=tets, mass=mass, surface=s2t, triangles=tris))
+        bodies.append(np.full(len(tets), b, dtype=np.uint32))
+        bconn.append(tets)
+    conn = np.concatenate(bconn)
+    sc.groups.append(Group("neohookean", np.concatenate(bodies), conn, young=np.full(len(conn), young),
+                           poisson=np.full(len(conn), poisson), clusters=hex_clusters(len(conn))))
+    sc.planes.append(((0.0, 1.0, 0.0), (0.0, 0.0, 0.0), (-1e30,) * 3, (1e30,) * 3))
+    return sc
+
+
+def cells6(mask, edge):
+    """Hex cells of edge `edge` where mask[i, j, k] is set (cell (i, j, k) spans [i, i + 1] x [j, j + 1]
+    x [k, k + 1] edges), each split into the 6 tets of sbs_mesh_box6 (the same monotone lattice paths
+    and vertex order, so neighbouring cells share their faces' diagonals); the used lattice corners
+    numbered in lattice order."""
+    mask = np.asarray(mask, dtype=bool)
+    nx, ny, nz = mask.shape
+    vy, vz = ny + 1, nz + 1
+    order = ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0))
+    tets = []
+    for i, j, k in zip(*np.nonzero(mask)):
+        for p, axes in enumerate(order):
+            c = [int(i), int(j), int(k)]
+            v = [(c[0] * vy + c[1]) * vz + c[2]]
+            for a in axes:
+                c[a] += 1
+                v.append((c[0] * vy + c[1]) * vz + c[2])
+            if p in (1, 2, 5):
+                v[1], v[2] = v[2], v[1]
+            tets.append(v)
+    tets = np.asarray(tets, dtype=np.int64)
+    used = np.unique(tets)
+    remap = np.full((nx + 1) * vy * vz, -1, dtype=np.int64)
+    remap[used] = np.arange(len(used))
+    li, lj, lk = np.unravel_index(used, (nx + 1, vy, vz))
+    pos = np.stack([li, lj, lk], 1).astype(np.float64) * edge
+    return pos, remap[tets].astype(np.uint32)
+
+
+def channel(edge=0.05, young=5e4, poisson=0.4, iterations=16, substeps=8, length=6, gap=2, beta=1e-3,
+            name="channel_self_contact", body_contacts=(0.0, 0.0, 0.0)):
+    """One body folding onto itself: a channel lying on its side on the ground plane, its upper arm
+    `length` x 1 x 3 hex cells and its lower arm 3 cells longer, joined by a wall of `gap` cells at
+    x = 0, the arms `gap` cells apart; the upper arm sags under gravity until its free end rests on
+    the lower arm (a contact of the body's surface with its own non-incident triangles). Stable
+    neo-Hookean constraints with damping `beta` and hex clusters."""
+    sc = Scene(name, gravity_is_force=False, iterations=iterations, substeps=substeps, coloring="index",
+               body_contacts=body_contacts)
+    mask = np.zeros((length + 3, gap + 2, 3), dtype=bool)
+    mask[:, 0, :] = True
+    mask[:length, gap + 1, :] = True
+    mask[0, :, :] = True
+    x0, tets = cells6(mask, edge)
+    x0 = x0 + np.array([0.0, 0.001, 0.0])
+    mass = _lib.lumped_mass(x0, tets, 1000.0)
+    s2t, tris = _lib.boundary(len(x0), tets)
+    sc.bodies.append(Body(x0=x0, tets=tets, mass=mass, surface=s2t, triangles=tris))
+    sc.groups.append(Group("neohookean", 0, tets, beta=beta, young=young, poisson=poisson,
+                           clusters=hex_clusters(len(tets))))
+    sc.planes.append(((0.0, 1.0, 0.0), (0.0, 0.0, 0.0), (-1e30,) * 3, (1e30,) * 3))
+    return sc
+
+
+def tower(edge=0.05, young=2e5, poisson=0.4, iterations=16, substeps=8, gap=0.01, beta=1e-3, ellipsoid=True,
+          name="tower_4_bodies", body_contacts=(0.0, 0.0, 0.0)):
+    """A few bodies resting on each other: a slab of 8 x 2 x 8 hex cells on the ground plane, a box of
+    4 x 3 x 4 cells turned 0.3 rad about y on the slab, a box of 2 x 2 x 2 cells turned 0.6 rad on
+    that box, and an ellipsoid of radii (0.1, 0.06, 0.08) m on the slab beside them, each starting
+    `gap` m above its support; every body's footprint lies inside its support's top face (without
+    friction an overhanging soft body slides off). Stable neo-Hookean constraints with damping `beta`
+    in one batched group with hex clusters, contacts between the bodies."""
+    sc = Scene(name, gravity_is_force=False, iterations=iterations, substeps=substeps, coloring="index",
+               body_contacts=body_contacts)
+    pos, tets = _lib.box6(8, 2, 8, edge)
+    meshes = [(pos - np.array([4 * edge, -0.001, 4 * edge]), tets)]
+    top = meshes[0][0][:, 1].max()
+    for (nx, ny, nz), angle in (((4, 3, 4), 0.3), ((2, 2, 2), 0.6)):
+        pos, tets = _lib.box6(nx, ny, nz, edge)
+        pos = (pos - pos.mean(0)) @ _rotation((0.0, 1.0, 0.0), angle).T
+        pos = pos + np.array([-0.08, top + gap - pos[:, 1].min(), -0.08])
+        meshes.append((pos, tets))
+        top = pos[:, 1].max()
+    if ellipsoid:
+        pos, tets = _lib.ellipsoid6(0.1, 0.06, 0.08, edge)
+        pos = pos - pos.mean(0)
+        meshes.append((pos + np.array([0.1, meshes[0][0][:, 1].max() + gap - pos[:, 1].min(), 0.1]), tets))
+    bodies, bconn = [], []
+    for b, (x0, tets) in enumerate(meshes):
+        mass = _lib.lumped_mass(x0, tets, 1000.0)
+        s2t, tris = _lib.boundary(len(x0), tets)
+        sc.bodies.append(Body(x0=x0, tets=tets, mass=mass, surface=s2t, triangles=tris))
+        bodies.append(np.full(len(tets), b, dtype=np.uint32))
+        bconn.append(tets)
+    conn = np.concatenate(bconn)
+    sc.groups.append(Group("neohookean", np.concatenate(bodies), conn, beta=beta, young=np.full(len(conn), young),
+                           poisson=np.full(len(conn), poisson), clusters=hex_clusters(len(conn))))
+    sc.planes.append(((0.0, 1.0, 0.0), (0.0, 0.0, 0.0), (-1e30,) * 3, (1e30,) * 3))
+    return sc
+
+
+def box_on_box(edge=0.05, young=2e5, poisson=0.4, iterations=8, substeps=4, gap=0.01, drop=0.0, name="box_on_box",
+               body_contacts=(0.0, 0.0, 0.0)):
+    """A box of 3 x 3 x 3 hex cells resting `gap` m above the middle of a wider box of 6 x 2 x 6
+    cells that lies on the ground plane, moving down at the speed of a fall from `drop` m; stable
+    neo-Hookean constraints in one batched group with hex clusters, contacts between the bodies."""
+    sc = Scene(name, gravity_is_force=False, iterations=iterations, substeps=substeps, coloring="index",
+               body_contacts=body_contacts)
+    lo, lt = _lib.box6(6, 2, 6, edge)
+    lo = lo - np.array([3 * edge, 0.0, 3 * edge]) + np.array([0.0, 0.001, 0.0])
+    up, ut = _lib.box6(3, 3, 3, edge)
+    up = up - np.array([1.5 * edge, 0.0, 1.5 * edge]) + np.array([0.0, lo[:, 1].max() + gap, 0.0])
+    bconn, bodies = [], []
+    for b, (x0, tets) in enumerate(((lo, lt), (up, ut))):
+        mass = _lib.lumped_mass(x0, tets, 1000.0)
+        s2t, tris = _lib.boundary(len(x0), tets)
+        v0 = None
+        if b == 1 and drop > 0.0:
+            v0 = np.tile([0.0, -np.sqrt(2 * 9.81 * drop), 0.0], (len(x0), 1))
+        sc.bodies.append(Body(x0=x0, tets=tets, mass=mass, surface=s2t, triangles=tris, v0=v0))
+        bodies.append(np.full(len(tets), b, dtype=np.uint32))
+        bconn.append(tets)
+    conn = np.concatenate(bconn)
+    sc.groups.append(Group("neohookean", np.concatenate(bodies), conn, young=np.full(len(conn), young),
+                           poisson=np.full(len(conn), poisson), clusters=hex_clusters(len(conn))))
     sc.planes.append(((0.0, 1.0, 0.0), (0.0, 0.0, 0.0), (-1e30,) * 3, (1e30,) * 3))
     return sc
 

@@ -15,7 +15,7 @@ from sbs_amd import scenes
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-4
-CHECK = (1, 60)
+CHECK = (1, 10, 20, 30, 40, 50, 60)
 
 
 def say(msg):
@@ -48,10 +48,13 @@ def test_beam_1m_plain_and_warm_started_against_the_fp64_oracle():
             return d
         return make
 
-    def oracle(mode=None, scale=None):
+    def oracle(mode=None, scale=None, rounded=False):
         def make():
             o = build_oracle(sc)
             o.set_threads(threads)
+            if rounded:  # initial positions rounded to FP32: the oracle's own sensitivity
+                x0 = sc.bodies[0].x0
+                o.set_body_state(0, x0.astype(np.float32).astype(np.float64), np.zeros_like(x0))
             if mode:
                 o.set_warm_start(mode, *scale)
             return o
@@ -66,18 +69,26 @@ def test_beam_1m_plain_and_warm_started_against_the_fp64_oracle():
     dev_prev = run(sc, device("previous", scale), 4)
     say("1M beam: previous-multiplier warm start (0.7, 0.65), 4 iterations, oracle")
     orc_prev = run(sc, oracle("previous", scale), 4)
+    say("1M beam: the same from FP32-rounded initial positions, oracle")
+    orc_prev_r = run(sc, oracle("previous", scale, rounded=True), 4)
     err = {}
     for f in CHECK:
         err[("plain", f)] = float(np.abs(dev_plain[f] - orc_plain[f]).max()) / diag
         err[("previous", f)] = float(np.abs(dev_prev[f] - orc_prev[f]).max()) / diag
+        err[("previous_oracle_spread", f)] = float(np.abs(orc_prev_r[f] - orc_prev[f]).max()) / diag
         err[("previous_vs_plain", f)] = float(np.abs(dev_prev[f] - orc_plain[f]).max()) / diag
         err[("oracle_previous_vs_plain", f)] = float(np.abs(orc_prev[f] - orc_plain[f]).max()) / diag
     for k, v in sorted(err.items()):
         say(f"  {k[0]} frame {k[1]}: {v:.3e}")
     for f in CHECK:
         assert err[("plain", f)] <= TOL, err
-        assert err[("previous", f)] <= TOL, err
+        # the warm-started solver amplifies a perturbation of one FP32 rounding of the initial
+        # positions by about 1e5 over 60 frames in the FP64 oracle itself (the beam meets the
+        # obstacles from frame ~25 on): the device is held to 1e-4 or three times that spread
+        assert err[("previous", f)] <= max(TOL, 3.0 * err[("previous_oracle_spread", f)]), (f, err)
+    assert err[("previous", 1)] <= TOL and err[("previous", 10)] <= TOL
     # the warm start converges elsewhere on h = 0 than the plain solver; its distance from the plain
-    # trajectory is the oracle's own (the device adds no more than the parity error)
+    # trajectory is the oracle's own, to within the parity error
     for f in CHECK:
-        assert abs(err[("previous_vs_plain", f)] - err[("oracle_previous_vs_plain", f)]) <= TOL, err
+        assert abs(err[("previous_vs_plain", f)] - err[("oracle_previous_vs_plain", f)]) <= max(
+            TOL, 3.0 * err[("previous_oracle_spread", f)]), err

@@ -169,18 +169,22 @@ def _oracle_self_spread(scene, frames, rel=2.0 ** -24, seed=2):
 
 
 def test_beam_540_within_reference_conditioning():
-    # With nu = 0.45 the reference's green constraint makes Gauss-Seidel chaotic on this beam: the
-    # FP64 oracle disagrees with itself by ~2e-4 (frame 1) and ~5e-2 (frame 60) when its initial
-    # state is perturbed at FP32 rounding level, so no implementation that is not bit-identical
-    # can meet 1e-4. The device must stay inside that envelope.
+    # With nu = 0.45 the reference's green constraint makes Gauss-Seidel expansive on this beam: the
+    # FP64 oracle disagrees with itself by ~2e-4 after frame 1 when its initial state is perturbed
+    # at FP32 rounding level (no branch of the projection flips between the two runs: the growth is
+    # in the sweep itself, about 40x in the first iteration; at nu = 0.3 the same perturbation does
+    # not grow), so no implementation that is not bit-identical can meet 1e-4. The device must stay
+    # inside that envelope at every frame.
     sc = scenes.beam_540()
     say(f"{sc.name} 60 frames vs oracle self-spread")
-    spread = _oracle_self_spread(sc, (1, 60))
-    errs, dev, orc = run_pair(sc, 60, check_frames=(1, 60))
-    print("beam_540 device errors", errs, "oracle self-spread", spread, flush=True)
+    frames = tuple(range(1, 61))
+    spread = _oracle_self_spread(sc, frames)
+    errs, dev, orc = run_pair(sc, 60, check_frames=frames)
+    print("beam_540 device errors", {f: errs[f] for f in (1, 10, 30, 60)}, "oracle self-spread",
+          {f: spread[f] for f in (1, 10, 30, 60)}, flush=True)
     assert spread[1] > TOL  # the documented ill-conditioning
-    for f in (1, 60):
-        assert errs[f] <= max(TOL, 3.0 * spread[f])
+    for f in frames:
+        assert errs[f] <= max(TOL, 3.0 * spread[f]), (f, errs[f], spread[f])
     x, _ = dev.read_body(0)
     pinned = sc.bodies[0].mass == 0
     assert np.array_equal(x[pinned], sc.bodies[0].x0[pinned])  # FP64 positions: exactly at rest

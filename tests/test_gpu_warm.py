@@ -1,6 +1,8 @@
 """GPU checks of the learned warm start: the network kernels against the NumPy restatement
 (oracle/mlp_oracle.py), training on the device, the multiplier warm-start passes against the FP64
 oracle, the captured training rows, and the plain-solver fallback."""
+import os
+
 import numpy as np
 import pytest
 
@@ -196,6 +198,12 @@ def test_capture_rows_and_features():
     sc = _beam(True, iterations=4)
     say("capture")
     dev = scenes.build_device(sc)
+    # a perturbed start, so that every cluster's features differ
+    b = sc.bodies[0]
+    rng = np.random.default_rng(4)
+    w = np.where(b.mass > 0, 1.0 / np.where(b.mass > 0, b.mass, 1.0), 0.0)
+    x = b.x0 + (w > 0)[:, None] * rng.normal(0, 2e-3, b.x0.shape)
+    dev.write_body(0, x, np.zeros_like(x))
     nc = len(sc.groups[0].clusters) - 1
     net = S.Mlp(NIN, NOUT)
     net.reserve(10 * nc)
@@ -205,19 +213,17 @@ def test_capture_rows_and_features():
     dev.step(sc.dt, sc.iterations, sc.substeps)  # 2 substeps
     dev.capture(None, 0)
     assert net.dataset_size() == 2 * nc
-    x, tt = net.dataset()
-    x, tt = x.astype(np.float64), tt.astype(np.float64)
-    f0, f1, t0 = x[:nc], x[nc:], tt[:nc]
-    assert np.all(x[:, 39:] == 0) and np.all(tt[:, 12:] == 0)
+    rows, tt = net.dataset()
+    rows, tt = rows.astype(np.float64), tt.astype(np.float64)
+    f0, f1, t0 = rows[:nc], rows[nc:], tt[:nc]
+    assert np.all(rows[:, 39:] == 0) and np.all(tt[:, 12:] == 0)
     # substep 2 starts from substep 1's final multipliers; its history is substep 1's start
     assert np.allclose(f1[:, 0:12], f0[:, 0:12] + t0[:, 0:12], rtol=1e-5, atol=1e-6 * np.abs(t0).max())
     assert np.array_equal(f1[:, 12:24], f0[:, 0:12])
     assert np.all(f0[:, 0:24] == 0)  # multipliers start at zero
-    # features of substep 1 at the predicted positions x~ = x0 + g dt^2 (gravity as acceleration)
-    b = sc.bodies[0]
+    # features of substep 1 at the predicted positions x~ = x + g dt^2 (gravity as acceleration)
     dt = sc.dt / sc.substeps
-    w = np.where(b.mass > 0, 1.0 / np.where(b.mass > 0, b.mass, 1.0), 0.0)
-    xt = b.x0 + (w > 0)[:, None] * np.array([0.0, -9.81, 0.0]) * dt * dt
+    xt = x + (w > 0)[:, None] * np.array([0.0, -9.81, 0.0]) * dt * dt
     C, lad, lah = _cluster_constraint_values(sc, xt, dt)
     m = _match_rows(f0[:, 24:36], C)
     assert np.allclose(f0[:, 24:36:2], C[m][:, 0::2], rtol=1e-5)
@@ -270,16 +276,31 @@ def test_learned_warm_start_matches_oracle(which):
     dev = scenes.build_device(sc)
     dev.set_warm_start("learned", net)
     dev.set_warm_start_scale(*scale)
-    orc = build_oracle(sc)
-    orc.set_learned_warm_start(net, *scale)
+    # the oracle twice: from the scene's rest state and from it rounded to FP32; the second run's
+    # distance from the first is the oracle's own sensitivity to rounding-sized perturbations (a
+    # BF16 input that lands on the other side of a rounding boundary changes the network's output
+    # by one BF16 step), and the device is held to 1e-4 or three times that spread
+    orcs = []
+    for rounded in (False, True):
+        o = build_oracle(sc)
+        o.set_threads(max(1, min(16, os.cpu_count() or 1)))
+        if rounded:
+            for bi, body in enumerate(sc.bodies):
+                o.set_body_state(bi, body.x0.astype(np.float32).astype(np.float64), np.zeros_like(body.x0))
+        o.set_learned_warm_start(net, *scale)
+        orcs.append(o)
     diag = max(float(np.linalg.norm(np.ptp(b.x0, axis=0))) for b in sc.bodies)
+    nb = len(sc.bodies)
     for f in range(1, frames + 1):
         dev.step(sc.dt, sc.iterations, sc.substeps)
-        orc.step(sc.dt, sc.iterations, sc.substeps)
+        for o in orcs:
+            o.step(sc.dt, sc.iterations, sc.substeps)
         if f in (1, 5, 10, frames):
-            err = max(float(np.abs(dev.read_body(b)[0] - orc.read_body(b)[0]).max()) for b in range(len(sc.bodies)))
-            say(f"frame {f} error {err / diag:.3e}")
-            assert err / diag <= TOL
+            err = max(float(np.abs(dev.read_body(b)[0] - orcs[0].read_body(b)[0]).max()) for b in range(nb)) / diag
+            spread = max(float(np.abs(orcs[1].read_body(b)[0] - orcs[0].read_body(b)[0]).max())
+                         for b in range(nb)) / diag
+            say(f"frame {f} error {err:.3e} (oracle spread {spread:.3e})")
+            assert err <= max(TOL, 3.0 * spread), (f, err, spread)
 
 
 def test_learned_network_gains_over_the_previous_start():
