@@ -338,6 +338,10 @@ def main():
                     help="the first iteration's contacts in their own launch, not in the warm-start gather")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run device copies of known size (to scale PMC byte counters)")
+    ap.add_argument("--profile-frames", type=int, default=0,
+                    help="profiler run: this many plain and learned frames without tuning or timing (see profile_run)")
+    ap.add_argument("--warm-config", default="4:0.7,0.65",
+                    help="the previous-multiplier start the profiler run trains its network on: iterations:scales")
     ap.add_argument("--row", default="step", choices=["step", "boundary", "grid_sdf", "ply"],
                     help="step: the time step (the headline metric); the others measure the surface, grid SDF "
                          "and PLY components on their own workloads")
@@ -364,6 +368,9 @@ def main():
                   "batch_512": scenes.batch_512, "batch_512_large": scenes.batch_512_large}
     scene = make_scene[args.workload]()
     log(f"workload {scene.name}: {scene.num_vertices} vertices, {scene.num_tets} tets")
+    if args.profile_frames:
+        print(json.dumps(profile_run(args, scene)), flush=True)
+        return
     t0 = time.perf_counter()
     sim = scenes.build_device(scene)
     _, ncolors = sim.group_colors(0)
@@ -582,6 +589,44 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def profile_run(args, scene):
+    """A short run for a profiler (rocprofv3 --kernel-trace / --pmc): the launch configuration the
+    bench would pick, `--warmup` plain frames, `--profile-frames` plain frames, then for a
+    neo-Hookean workload a network trained on `--train-frames` captured frames (default 200) of the
+    previous-multiplier start `--warm-config` (iterations:deviatoric,hydrostatic scales) and
+    `--profile-frames` frames of the learned start with those iterations and scales. No tuning and no
+    timing; one JSON line says what ran."""
+    from sbs_amd import autotune, scenes
+
+    sim = scenes.build_device(scene)
+    sim.use_graphs(not args.no_graphs)
+    launch = autotune.tune_launch(sim, scene, frames=2)
+    for _ in range(args.warmup):
+        sim.step(scene.dt, scene.iterations, scene.substeps)
+    for _ in range(args.profile_frames):
+        sim.step(scene.dt, scene.iterations, scene.substeps)
+    sim.synchronize()
+    out = {"profile_run": scene.name, "launch_config": launch, "plain_frames": args.profile_frames,
+           "iterations": scene.iterations}
+    if scene.groups[0].kind == "neohookean" and not args.no_learned:
+        k, scales = args.warm_config.split(":")
+        k, scale = int(k), tuple(float(v) for v in scales.split(","))
+        net, info = autotune.train_warm_start(sim, scene, iterations=k, scale=scale,
+                                              frames=args.train_frames or 200, episodes=4, val_episodes=1,
+                                              steps=500, log=log)
+        sim.set_warm_start("learned", net)
+        sim.set_warm_start_scale(*scale)
+        sim.step(scene.dt, k, scene.substeps)  # graph capture
+        for _ in range(args.profile_frames):
+            sim.step(scene.dt, k, scene.substeps)
+        sim.synchronize()
+        out.update({"learned_frames": args.profile_frames, "learned_iterations": k, "learned_scale": scale,
+                    "held_out_loss": info["held_out_loss"]})
+    out["contact_overflow"] = sim.contact_overflow()
+    sim.close()
+    return out
 
 
 def plain_throughput(scene, kind, args, warmup=3, frames=5, reps=3):
