@@ -407,6 +407,15 @@ int sbs_util_exclusive_scan(int device, const uint32_t* in, uint64_t n, uint32_t
 /* Stable sort of the pairs by the low `bits` bits of the keys (LSD radix sort, 8 bits per pass). */
 int sbs_util_sort_pairs(int device, const uint32_t* keys, const uint32_t* vals, uint64_t n, int bits,
                         uint32_t* out_keys, uint32_t* out_vals);
+/* The same sort without any allocation; no counterpart in the reference. For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops): keys and vals are
+ * device pointers of n words, sorted in place on `stream` (a hipStream_t; NULL: the null stream) and
+ * returned before the sort runs (the result is there when the stream reaches it). At most
+ * SBS_SORT_NOALLOC_MAX_PAIRS pairs; more are rejected. Its work buffers are one device array per GPU
+ * that the library holds until the process ends (about 10.5 MB), shared by every call on that GPU:
+ * calls that overlap on one GPU race, so issue them on one stream. */
+#define SBS_SORT_NOALLOC_MAX_PAIRS (1u << 20)
+int sbs_util_sort_pairs_noalloc(int device, uint32_t* keys, uint32_t* vals, uint64_t n, int bits, void* stream);
 
 /* ---- host mesh utilities (no device) ---------------------------------------------------------------- */
 

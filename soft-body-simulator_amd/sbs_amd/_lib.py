@@ -132,6 +132,7 @@ _SIGS = {
     "sbs_gpu_read_body_contact_candidates": [_p, _pu, _u64, _pu64],
     "sbs_util_exclusive_scan": [_i, _pu, _u64, _pu],
     "sbs_util_sort_pairs": [_i, _pu, _pu, _u64, _i, _pu, _pu],
+    "sbs_util_sort_pairs_noalloc": [_i, _p, _p, _u64, _i, _p],
     "sbs_gpu_extract_boundary": [_i, _u32, _pu, _u32, _pu, _pu, _pu, _pu],
     "sbs_gpu_capture": [_p, _p, _u32],
     "sbs_gpu_set_warm_start_scale": [_p, _d],
