@@ -5,20 +5,23 @@ One "step" is one timestep_t::step() call (one frame = `substeps` substeps of `i
 Gauss-Seidel sweeps over the 997,920 stable neo-Hookean tets, contacts against the ground plane and
 16 spheres). Timed regions replay the frame as a hipGraph; inputs are resident in HBM.
 
-The metric compares warm-started solvers against the plain solver at equal residual. From one
-device snapshot the plain solver runs the timed horizon (`--steps` frames); its mean FP64 frame
-residual over those frames is the target and twice its largest frame residual the fallback limit.
-The autotuner screens warm-start configurations (previous multipliers, then a network trained on the
-device from captured frames; iterations and a pair of multiplier scales) on the first 8 frames and
-verifies the cheapest over the whole horizon: its mean must not exceed the plain solver's and no
-frame may exceed the limit. Every qualifying configuration is timed from the snapshot with its
-per-frame residual check and plain fallback active; `value` is the fastest (the plain solver's when
-none qualifies).
+The metric compares warm-started solvers against the plain solver at equal residual, per state.
+After `--warmup` frames the tuner takes the next `--tune-frames` frames as its horizon: a
+warm-start candidate (previous multipliers, then a network trained on the device from captured
+episodes; iterations and a pair of multiplier scales) qualifies when on every frame its FP64 frame
+residual is at most the plain solver's from the same state. The plain solver then carries the state
+to the start of the timed horizon (`--steps` frames, disjoint from the tuning frames); every leg is
+timed there from one snapshot (`--reps` repeats, median) with the per-frame fallback check on, and
+every warm-started leg is compared with the plain solver per state over the timed frames again; a
+leg with a frame above the plain residual is timed and compared again with one more iteration.
+`value` is the learned leg when all its timed frames reach the plain residual, else the plain
+solver's.
 
 After the timed regions an instrumented region (HIP events around every launch on the library's
 stream) measures the average duration of the tet projection kernel for the roofline figure. The
-CPU baseline times the FP64 oracle in the reference's insertion order on a bounded sample (frames of
-the same scene, or of a 16-body batch for the batches, until about 10 s of CPU work).
+CPU baseline times the FP64 oracle in the reference's insertion order on one pinned core over a
+bounded sample (5 frames after 5 warm-up frames of the same scene, or of a 16-body batch scaled by
+tet count for the batches); the accuracy legs compare the device with the oracle from rest.
 """
 import argparse
 import json
@@ -325,7 +328,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="beam_1m", choices=["beam_1m", "beam_6m", "beam_540", "batch_512", "batch_512_large"])
     ap.add_argument("--no-learned", action="store_true", help="plain solver only")
-    ap.add_argument("--tune-frames", type=int, default=8, help="frames after warm-up the warm starts are tuned on")
+    ap.add_argument("--tune-frames", type=int, default=16, help="frames after warm-up the warm starts are tuned on")
     ap.add_argument("--reps", type=int, default=5, help="repeats of the timed frames (median)")
     ap.add_argument("--accuracy-frames", type=int, default=60,
                     help="frames of the accuracy legs from rest (0: none); errors at frame 1 and the last")
@@ -416,9 +419,10 @@ def main():
     sim.snapshot(0)
 
     def configure(mode, cfg):
+        # mode: a leg's name ("plain" runs without a warm start)
         sim.restore(0)
-        sim.set_warm_start(mode, net if mode == "learned" else None)
-        if mode != "none":
+        sim.set_warm_start("none" if mode == "plain" else mode, net if mode == "learned" else None)
+        if mode != "plain":
             sim.set_warm_start_scale(*autotune._scales(cfg["scale"]))
             sim.set_fallback(limit, scene.iterations)
         else:
@@ -429,7 +433,7 @@ def main():
         if found.get(mode):
             legs[mode] = dict(found[mode])
     # timing: every leg from S1, `reps` repeats of the --steps timed frames, median
-    for mode, cfg in legs.items():
+    def time_leg(mode, cfg):
         configure(mode, cfg)
         sim.step(scene.dt, cfg["iterations"], scene.substeps)  # graph capture
         f0 = sim.fallback_count()
@@ -441,28 +445,44 @@ def main():
         cfg.update({"substeps_per_s": round(args.steps * scene.substeps / med, 2), "seconds": times,
                     "fallback_frames": sim.fallback_count() - f0})
         log(f"{mode}: median of {args.reps} x {args.steps} frames {1e3 * med:.2f} ms -> {cfg['substeps_per_s']:.1f} "
-            f"substeps/s, {cfg['fallback_frames']} fallback frames")
-    plain_sps = legs["plain"]["substeps_per_s"]
-    # equal residual on the timed frames, per state: along each warm-started trajectory from S1,
-    # every frame's residual against the plain solver's from the same state
-    for mode, cfg in legs.items():
-        if mode == "plain":
-            continue
+            f"substeps/s with {cfg['iterations']} iterations, {cfg['fallback_frames']} fallback frames")
+
+    # equal residual on the timed frames, per state: along the warm-started trajectory from S1 (no
+    # fallback), every frame's residual against the plain solver's from the same state
+    def verify_leg(mode, cfg):
         configure(mode, cfg)
         sim.set_fallback(0.0)
         rw, rp = autotune.per_state_ratios(sim, scene, args.steps, cfg["iterations"])
         ratio = rw / rp
+        above = int(np.sum(~(ratio <= 1.0)))
         cfg["timed_frames_per_state"] = {
             "ratio_max": float(np.nanmax(ratio)), "ratio_median": float(np.nanmedian(ratio)),
-            "ratio_min": float(np.nanmin(ratio)), "frames_above_1": int(np.sum(~(ratio <= 1.0))),
-            "frames": int(len(ratio))}
+            "ratio_min": float(np.nanmin(ratio)), "frames_above_1": above, "frames": int(len(ratio)),
+            "ratios": [round(float(r), 4) for r in ratio]}
+        log(f"{mode} per state over the {args.steps} timed frames with {cfg['iterations']} iterations: ratio max "
+            f"{np.nanmax(ratio):.4g}, median {np.nanmedian(ratio):.4g}, {above} frames above 1")
+        return above
+
+    time_leg("plain", legs["plain"])
+    plain_sps = legs["plain"]["substeps_per_s"]
+    # a warm-started leg whose timed frames do not all reach the plain residual per state runs again
+    # with one more iteration (up to one less than the plain solver's)
+    for mode, cfg in legs.items():
+        if mode == "plain":
+            continue
+        cfg["tuned_iterations"] = cfg["iterations"]
+        while True:
+            time_leg(mode, cfg)
+            above = verify_leg(mode, cfg)
+            if above == 0 or cfg["iterations"] + 1 >= scene.iterations:
+                break
+            cfg["iterations"] += 1
+        cfg["equal_residual"] = above == 0
         cfg["speedup_vs_plain"] = round(cfg["substeps_per_s"] / plain_sps, 3)
-        log(f"{mode} per state over the {args.steps} timed frames: ratio max {np.nanmax(ratio):.4g}, median "
-            f"{np.nanmedian(ratio):.4g}, {cfg['timed_frames_per_state']['frames_above_1']} frames above 1")
     sim.set_warm_start("none")
     sim.set_fallback(0.0)
     sim.restore(0)
-    if "learned" in legs:
+    if legs.get("learned", {}).get("equal_residual"):
         solver, head = "learned", legs["learned"]
     else:
         solver, head = "plain", legs["plain"]
@@ -543,7 +563,7 @@ def main():
             "workload": scene.name,
             "solver": {"learned": "learned multiplier warm start + colored Gauss-Seidel XPBD",
                        "plain": "plain GPU XPBD (colored Gauss-Seidel, no warm start): no learned configuration "
-                                "reached the plain residual per state"}[solver],
+                                "reached the plain residual on every timed frame per state"}[solver],
             "tets": scene.num_tets,
             "vertices": scene.num_vertices,
             "substeps_per_step": scene.substeps,
@@ -566,7 +586,9 @@ def main():
             f"them its residual is at most the plain {scene.iterations}-iteration solver's from the same state. "
             f"Timing on the next {args.steps} frames (disjoint), median of {args.reps} repeats, fallback check on "
             f"(limit {limit:.4g} = twice the largest plain tuning frame residual); timed_frames_per_state: the same "
-            f"per-state comparison over the timed frames"),
+            f"per-state comparison over the timed frames (no fallback); a leg with a timed frame above the plain "
+            f"residual is timed and compared again with one more iteration (tuned_iterations: the tuner's count); "
+            f"value: the learned leg when every timed frame reaches the plain residual, else the plain solver"),
         "training": train_info,
         "num_trials": len(trials),
         "contact_overflow": overflow,

@@ -729,12 +729,20 @@ void simulation_t::apply_learned_multipliers()
         for (long c = 0; c < nc; ++c) {
             double f[39], atd[6], ath[6], y[12];
             if (!cluster_features(g, std::size_t(c), warm_dt, per_body, f, atd, ath)) continue;
-            net->evaluate(f, y);
+            // the network corrects carried-over multipliers: a cluster whose multipliers or their
+            // history are all zero (the first two substeps after a reset) keeps the previous start
+            bool carried = false, history = false;
+            for (int m = 0; m < 12; ++m) {
+                carried = carried || float(f[m]) != 0.0f;
+                history = history || float(f[12 + m]) != 0.0f;
+            }
+            if (carried && history) net->evaluate(f, y);
             std::size_t const t0 = g.first + g.cluster_off[c];
             for (int k = 0; k < 6; ++k) {
                 auto* T = static_cast<neohookean_constraint_t*>(constraints[t0 + k].get());
                 T->hist = T->lagrange;
                 T->hist_h = T->lagrange_h;
+                if (!(carried && history)) continue;
                 T->lagrange += y[2 * k] / atd[k];
                 T->lagrange_h += y[2 * k + 1] / ath[k];
             }

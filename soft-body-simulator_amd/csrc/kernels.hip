@@ -808,15 +808,25 @@ __global__ void __launch_bounds__(64) k_hex_warm_disp(tet_group_dev g, cluster_d
                     if (o < 2 * kHexTets) out[(32 * t2 + c) * (2 * kHexTets) + o] = y[t2][j];
                 }
             __syncthreads();
-            // the predicted change of alpha~ lambda over the substep, back to multipliers
-#pragma unroll
-            for (int k = 0; k < kHexTets; ++k) {
-                float const yd = out[threadIdx.x * (2 * kHexTets) + 2 * k] * net.norm->out_scale[2 * k];
-                float const yh = out[threadIdx.x * (2 * kHexTets) + 2 * k + 1] * net.norm->out_scale[2 * k + 1];
-                L[k].x += yd / (B[k].w * inv_dt2);
-                L[k].y += yh / (C[k].x * inv_dt2);
-            }
             if (!active) return;
+            // the predicted change of alpha~ lambda over the substep, back to multipliers; the
+            // network corrects carried-over multipliers: a cluster whose multipliers or their
+            // history are all zero (the first two substeps after a reset) keeps the previous start
+            bool carried = false, history = false;
+#pragma unroll
+            for (int m = 0; m < 2 * kHexTets; ++m) {
+                carried = carried || f[m] != 0.0f;
+                history = history || f[2 * kHexTets + m] != 0.0f;
+            }
+            if (carried && history) {
+#pragma unroll
+                for (int k = 0; k < kHexTets; ++k) {
+                    float const yd = out[threadIdx.x * (2 * kHexTets) + 2 * k] * net.norm->out_scale[2 * k];
+                    float const yh = out[threadIdx.x * (2 * kHexTets) + 2 * k + 1] * net.norm->out_scale[2 * k + 1];
+                    L[k].x += yd / (B[k].w * inv_dt2);
+                    L[k].y += yh / (C[k].x * inv_dt2);
+                }
+            }
         }
     }
     v3 D[8];

@@ -220,10 +220,11 @@ def _random_rotation(rng):
 
 
 def batch(num_bodies=512, tet_range=(2e3, 2e5), edge=0.05, seed=0xB0D1E5, density=1000.0, iterations=8,
-          substeps=4, name=None, large=None):
+          substeps=4, name=None, large=None, young_range=(1e4, 1e6), poisson_range=(0.3, 0.48)):
     """Batched bodies: `num_bodies` boxes and ellipsoids (alternating) of 6-tet hex cells with tet
     counts log-uniform in tet_range, random rotations, dropped from heights U[0.5, 3] m onto a
-    ground plane at y = 0, Young's modulus log-uniform in [1e4, 1e6], Poisson ratio U[0.3, 0.48],
+    ground plane at y = 0, Young's modulus log-uniform in young_range ([1e4, 1e6]), Poisson ratio
+    uniform in poisson_range ([0.3, 0.48]),
     stable neo-Hookean constraints in one batched group with hex-cell clusters (index-order
     coloring), 4 substeps x 8 iterations. large = (count, tets): that many more bodies of about
     `tets` tets each (same shapes and materials), in a row beside the grid of the others, in the
@@ -274,8 +275,8 @@ def batch(num_bodies=512, tet_range=(2e3, 2e5), edge=0.05, seed=0xB0D1E5, densit
         sc.bodies.append(Body(x0=x0, tets=tets, mass=mass, surface=s2t, triangles=tris))
         bodies.append(np.full(len(tets), b, dtype=np.uint32))
         bconn.append(tets)
-        E = float(np.exp(rng.uniform(np.log(1e4), np.log(1e6))))
-        nu = float(rng.uniform(0.3, 0.48))
+        E = float(np.exp(rng.uniform(np.log(young_range[0]), np.log(young_range[1]))))
+        nu = float(rng.uniform(*poisson_range))
         young.append(np.full(len(tets), E))
         poisson.append(np.full(len(tets), nu))
         clusters.append(clusters[-1] + len(tets))

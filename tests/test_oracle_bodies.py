@@ -113,3 +113,38 @@ def test_a_fold_of_one_body_rests_on_its_own_surface():
         assert n[1] > 0.9, (i, n)
     # and stays above the lower arm's top face
     assert x[tip, 1].min() > 0.05
+
+
+def _vertex_under_a_face(pinned_face):
+    """Two single-tet bodies, no gravity, no elastic constraints: body 1's bottom face (y = 0,
+    outward normal -y) above body 0's apex p = (0.25, -0.01, 0.25); thickness 0.05."""
+    sc = Scene("vertex_under_face", gravity=(0.0, 0.0, 0.0), gravity_is_force=False, iterations=1, substeps=1,
+               coloring="index", body_contacts=(2.0, 0.05, 0.5))
+    a = np.array([[0.25, -0.01, 0.25], [0.0, -1.0, 0.0], [1.0, -1.0, 0.0], [0.0, -1.0, 1.0]])
+    b = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0], [0.25, 1.0, 0.25]])
+    tet = np.array([[0, 1, 2, 3]], dtype=np.uint32)
+    for x0, mass in ((a, np.array([1.0, 0.0, 0.0, 0.0])), (b, np.zeros(4) if pinned_face else np.ones(4))):
+        s2t, tris = _lib.boundary(4, tet)
+        sc.bodies.append(Body(x0=x0, tets=tet, mass=mass, surface=s2t, triangles=tris))
+    return sc
+
+
+@pytest.mark.parametrize("pinned_face", [True, False])
+def test_one_contact_projection_against_a_face(pinned_face):
+    # p lies 0.01 below the face, inside the thickness 0.05: its contact plane is 0.05 below the
+    # face (normal -y). Against a pinned face one projection puts p on that plane (the SDF contact's
+    # orthogonal projection); against a free face with unit masses p takes its share 1 / (1 + sum
+    # b_k^2) of the 0.04 gap, b = (0.5, 0.25, 0.25) the closest point's barycentric weights
+    sc = _vertex_under_a_face(pinned_face)
+    o = build_oracle(sc)
+    ids, qn = o.detect_contacts()
+    mine = [(i, q) for i, q in zip(ids, qn) if i[0] == 0 and i[2] == 0]
+    assert len(mine) == 1
+    assert np.allclose(mine[0][1][3:], [0.0, -1.0, 0.0], atol=1e-15)
+    assert np.allclose(sorted(mine[0][1][:3]), [0.25, 0.25, 0.5], atol=1e-15)
+    o.step(sc.dt, 1, 1)
+    y = o.read_body(0)[0][0, 1]
+    share = 1.0 if pinned_face else 1.0 / (1.0 + 0.5 ** 2 + 2 * 0.25 ** 2)
+    alpha_tilde = sc.collision_compliance / sc.dt ** 2
+    assert abs((-0.01 - y) - 0.04 * share) <= 0.04 * 2 * alpha_tilde
+    assert np.allclose(o.read_body(0)[0][0, [0, 2]], [0.25, 0.25], atol=1e-15)

@@ -238,3 +238,21 @@ def test_oracle_learned_features_carry_the_multipliers():
     assert np.allclose(f[:, 1:12:2], ath * m[..., 1], rtol=1e-12)
     assert np.allclose(f[:, 12:24:2], atd * m[..., 2], rtol=1e-12)
     assert np.allclose(f[:, 13:24:2], ath * m[..., 3], rtol=1e-12)
+
+
+def test_oracle_network_waits_for_carried_multipliers():
+    # the network corrects carried-over multipliers: from rest the first substep has none and the
+    # second none from the substep before, so both equal the previous start exactly, whatever the
+    # network says; from the third substep on a loud network changes the result
+    sc = _hex_beam()
+    a, b = _oracle(sc), _oracle(sc)
+    a.set_warm_start("previous", 0.6, 0.5)
+    rng = np.random.default_rng(17)
+    b.set_network(random_params(rng).astype(np.float32), np.zeros(NIN, np.float32), np.ones(NIN, np.float32),
+                  np.full(12, 1e-3, np.float32))
+    b._L.orc_set_warm_start(b.h, 2, 0.6, 0.5)
+    for s in range(3):
+        a.step(sc.dt, sc.iterations, 1)
+        b.step(sc.dt, sc.iterations, 1)
+        same = np.array_equal(a.read_body(0)[0], b.read_body(0)[0])
+        assert same == (s < 2), s
