@@ -398,16 +398,23 @@ def main():
     limit = 2.0 * float(plain_tune.max())
     log(f"warm-up done ({args.warmup} frames); plain {scene.iterations}-iteration frame residual over the {tune} "
         f"tuning frames: mean {plain_tune.mean():.4g}, largest {plain_tune.max():.4g}")
-    found, trials, net, train_info = {}, [], None, None
+    found, trials, nets, train_info = {}, [], {}, None
     if warmable:
         f_prev, t_prev = autotune.tune_warm_start(sim, scene, None, float(plain_tune.mean()), frames=tune,
                                                   modes=("previous",), log=log)
         found.update(f_prev)
         trials += t_prev
-        net, train_info = autotune.train_warm_start(sim, scene, log=log,
-                                                    **autotune.training_config(args, found.get("previous")))
-        f_l, t_l = autotune.tune_warm_start(sim, scene, net, float(plain_tune.mean()), frames=tune,
-                                            modes=("learned",), log=log)
+        # one network per iteration count from the previous start's count up: each is trained on
+        # trajectories of the iteration count it runs with (all from the tuning start)
+        train_cfg = autotune.training_config(args, found.get("previous"))
+        k0 = int(train_cfg.get("iterations", 4))
+        train_info = {}
+        for k in range(k0, scene.iterations):
+            train_cfg["iterations"] = k
+            nets[k], train_info[k] = autotune.train_warm_start(sim, scene, log=log, **train_cfg)
+        f_l, t_l = autotune.tune_warm_start(sim, scene, nets, float(plain_tune.mean()), frames=tune,
+                                            iterations=tuple(range(k0, scene.iterations)), modes=("learned",),
+                                            log=log)
         found.update(f_l)
         trials += t_l
     # the timed horizon starts where the plain solver leaves the tuning horizon (state S1)
@@ -421,7 +428,7 @@ def main():
     def configure(mode, cfg):
         # mode: a leg's name ("plain" runs without a warm start)
         sim.restore(0)
-        sim.set_warm_start("none" if mode == "plain" else mode, net if mode == "learned" else None)
+        sim.set_warm_start("none" if mode == "plain" else mode, nets[cfg["iterations"]] if mode == "learned" else None)
         if mode != "plain":
             sim.set_warm_start_scale(*autotune._scales(cfg["scale"]))
             sim.set_fallback(limit, scene.iterations)
@@ -532,7 +539,7 @@ def main():
             for mode in ("previous", "learned"):
                 if mode in legs:
                     acc_legs.append({"name": mode, "iterations": legs[mode]["iterations"], "mode": mode,
-                                     "scale": legs[mode]["scale"], "net": net})
+                                     "scale": legs[mode]["scale"], "net": nets.get(legs[mode]["iterations"])})
             accuracy = accuracy_legs(scene, acc_legs, launch, frames=args.accuracy_frames,
                                      check=(1, args.accuracy_frames))
             accuracy["definition"] = ("max over vertices of |x_device - x_oracle|, divided by the rest bounding-box "

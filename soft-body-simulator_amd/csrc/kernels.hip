@@ -716,10 +716,10 @@ constexpr int kWarmPlain = kWarmNetNone, kWarmFeatures = kWarmNetFeatures, kWarm
 constexpr int kFeatures = 39;  // used network inputs of a hex cluster
 
 template <int MODE>
-__global__ void __launch_bounds__(64) k_hex_warm_disp(tet_group_dev g, cluster_dev cl, const pos4* __restrict__ xi,
-                                                      const uint4* __restrict__ cinfo, uint32_t nrows,
-                                                      float inv_dt2, int expl, float2 scale, float4* __restrict__ disp,
-                                                      warm_net_dev net)
+__device__ __forceinline__ void hex_warm_disp(tet_group_dev const& g, cluster_dev const& cl, const pos4* xi,
+                                              const uint4* __restrict__ cinfo, uint32_t nrows, float inv_dt2,
+                                              int expl, float2 scale, float4* __restrict__ disp,
+                                              warm_net_dev const& net)
 {
     uint4 const info = cinfo[2 * blockIdx.x];
     uint32_t const used = cinfo[2 * blockIdx.x + 1].x;
@@ -776,6 +776,14 @@ __global__ void __launch_bounds__(64) k_hex_warm_disp(tet_group_dev g, cluster_d
 #pragma unroll
             for (int m = 0; m < mlp::kMaxInputs; ++m) o[m] = m < kFeatures ? f[m] : 0.0f;
         } else {
+            // the network corrects carried-over multipliers: a cluster whose multipliers or their
+            // history are all zero (the first two substeps after a reset) keeps the previous start
+            bool carried = false, history = false;
+#pragma unroll
+            for (int m = 0; m < 2 * kHexTets; ++m) {
+                carried = carried || f[m] != 0.0f;
+                history = history || f[2 * kHexTets + m] != 0.0f;
+            }
             using mlp::bf16x8;
             __shared__ bf16x8 rows_lds[64 * mlp::kIn / 8];  // [row][48] BF16 inputs, then [row][12] FP32 outputs
             __shared__ float bias_lds[mlp::kBiasTotal];
@@ -791,33 +799,39 @@ __global__ void __launch_bounds__(64) k_hex_warm_disp(tet_group_dev g, cluster_d
             }
             __syncthreads();
             int const lane = threadIdx.x, h = lane >> 5, c = lane & 31;
-            bf16x8 xin[2][mlp::kQ1];
-#pragma unroll
-            for (int t2 = 0; t2 < 2; ++t2)
-#pragma unroll
-                for (int q = 0; q < mlp::kQ1; ++q) xin[t2][q] = rows_lds[(32 * t2 + c) * (mlp::kIn / 8) + 2 * q + h];
-            mlp::f32x16 y[2];
-            mlp::forward_tiles<2>(reinterpret_cast<const bf16x8*>(net.pack), bias_lds, xin, lane, y);
-            __syncthreads();
+            // the two 32-row tiles one after the other, not unrolled (each pass reads the weights
+            // again, from L2): half the accumulators and activations live at a time. A tile's
+            // outputs ([row][12] FP32) overwrite only inputs of rows already read (the workgroup is
+            // one wave: tile t's 32 rows of inputs are in registers before its outputs are written;
+            // its outputs occupy bytes [1536 t, 1536 t + 1536), tile 1's inputs start at 3072).
             float* out = reinterpret_cast<float*>(rows_lds);
+#pragma unroll 1
+            for (int t2 = 0; t2 < 2; ++t2) {
+                bf16x8 xin[1][mlp::kQ1];
 #pragma unroll
-            for (int t2 = 0; t2 < 2; ++t2)
+                for (int q = 0; q < mlp::kQ1; ++q) xin[0][q] = rows_lds[(32 * t2 + c) * (mlp::kIn / 8) + 2 * q + h];
+                mlp::f32x16 yt[1];
+                mlp::forward_tiles<1>(reinterpret_cast<const bf16x8*>(net.pack), bias_lds, xin, lane, yt);
 #pragma unroll
                 for (int j = 0; j < 16; ++j) {
                     int const o = mlp::acc_unit(j, h);
-                    if (o < 2 * kHexTets) out[(32 * t2 + c) * (2 * kHexTets) + o] = y[t2][j];
+                    if (o < 2 * kHexTets) out[(32 * t2 + c) * (2 * kHexTets) + o] = yt[0][j];
                 }
+            }
             __syncthreads();
             if (!active) return;
-            // the predicted change of alpha~ lambda over the substep, back to multipliers; the
-            // network corrects carried-over multipliers: a cluster whose multipliers or their
-            // history are all zero (the first two substeps after a reset) keeps the previous start
-            bool carried = false, history = false;
+            // the material and the positions again (the same values): read back after the network
+            // instead of held across it, so that the network runs with the registers it needs
 #pragma unroll
-            for (int m = 0; m < 2 * kHexTets; ++m) {
-                carried = carried || f[m] != 0.0f;
-                history = history || f[2 * kHexTets + m] != 0.0f;
+            for (int k = 0; k < kHexTets; ++k) {
+                uint32_t const t = tbase + uint32_t(k) * count + i;
+                A[k] = g.m0[t];
+                B[k] = g.m1[t];
+                C[k] = nh_m2(g, t);
             }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) P[k] = xi[ids[k]];
+            // the predicted change of alpha~ lambda over the substep, back to multipliers
             if (carried && history) {
 #pragma unroll
                 for (int k = 0; k < kHexTets; ++k) {
@@ -847,6 +861,24 @@ __global__ void __launch_bounds__(64) k_hex_warm_disp(tet_group_dev g, cluster_d
     for (int k = 0; k < kHexTets; ++k) reinterpret_cast<float2*>(g.lambda)[tbase + uint32_t(k) * count + i] = L[k];
 #pragma unroll
     for (int k = 0; k < 8; ++k) disp[std::size_t(k) * nrows + r] = make_float4(D[k].x, D[k].y, D[k].z, 0.0f);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(64) k_hex_warm_disp(tet_group_dev g, cluster_dev cl, const pos4* __restrict__ xi,
+                                                      const uint4* __restrict__ cinfo, uint32_t nrows, float inv_dt2,
+                                                      int expl, float2 scale, float4* __restrict__ disp,
+                                                      warm_net_dev net)
+{
+    hex_warm_disp<MODE>(g, cl, xi, cinfo, nrows, inv_dt2, expl, scale, disp, net);
+}
+
+// The learned variant, held to at most 256 registers a lane (2 waves per SIMD); its positions are
+// read through a pointer without restrict so that the reads after the network are real loads.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+k_hex_warm_learned(tet_group_dev g, cluster_dev cl, const pos4* xi, const uint4* __restrict__ cinfo, uint32_t nrows,
+                   float inv_dt2, int expl, float2 scale, float4* __restrict__ disp, warm_net_dev net)
+{
+    hex_warm_disp<kWarmLearned>(g, cl, xi, cinfo, nrows, inv_dt2, expl, scale, disp, net);
 }
 
 // Training rows of a hex group's captured substep: for j < rows, cluster ci = offset + j * stride,
@@ -1603,7 +1635,7 @@ void launch_hex_warm_start(hipStream_t s, tet_group_dev g, cluster_dev cl, const
 {
     if (!nrows) return;
     if (net_mode == kWarmLearned)
-        k_hex_warm_disp<kWarmLearned><<<grid_for(nrows, 64), 64, 0, s>>>(g, cl, xt, cinfo, nrows, inv_dt2, expl, scale,
+        k_hex_warm_learned<<<grid_for(nrows, 64), 64, 0, s>>>(g, cl, xt, cinfo, nrows, inv_dt2, expl, scale,
                                                                          disp, net);
     else if (net_mode == kWarmFeatures)
         k_hex_warm_disp<kWarmFeatures><<<grid_for(nrows, 64), 64, 0, s>>>(g, cl, xt, cinfo, nrows, inv_dt2, expl,

@@ -56,7 +56,8 @@ __device__ __forceinline__ void relu_tile(f32x16 const& acc, const float* b, int
 // Forward pass of R 32-row tiles that share every weight operand load: xin[r][q] is tile r's
 // first-layer B operand of k-step q. W: packed weights (LDS or global), b: the biases. Returns the
 // output accumulators y[r] (bias added): lane (c, h) holds output units acc_unit(j, h) of row c of
-// tile r.
+// tile r. A scheduling barrier after each layer keeps the next layer's weight loads from moving up:
+// the scheduler would otherwise load all 26 weight operands (104 registers a lane) at the start.
 template <int R, class W>
 __device__ __forceinline__ void forward_tiles(W const* w, const float* b, bf16x8 const (&xin)[R][kQ1], int lane,
                                               f32x16 (&y)[R])
@@ -77,6 +78,7 @@ __device__ __forceinline__ void forward_tiles(W const* w, const float* b, bf16x8
 #pragma unroll
         for (int r = 0; r < R; ++r) relu_tile(acc[r], b + kb1, T, h, act[r][T][0], act[r][T][1]);
     }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int L = 0; L < 2; ++L) {
         int const poff = L == 0 ? kP2 : kP3;
@@ -97,6 +99,7 @@ __device__ __forceinline__ void forward_tiles(W const* w, const float* b, bf16x8
         for (int T = 0; T < kT; ++T)
 #pragma unroll
             for (int r = 0; r < R; ++r) relu_tile(acc[r][T], bl, T, h, act[r][T][0], act[r][T][1]);
+        __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) y[r] = f32x16{};

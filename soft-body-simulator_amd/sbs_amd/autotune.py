@@ -208,31 +208,38 @@ def tune_warm_start(sim, scene, net, target, frames=8, scales=SCALE_GRID, iterat
     those frames); the ones that pass are qualified per state (per_state_ratios): every frame's
     warm-started residual must be at most the plain solver's from the same state. Among the scales
     that qualify at the smallest iteration count, the one with the smallest largest ratio is taken.
-    A scale is one factor or a (deviatoric, hydrostatic) pair. Returns {mode: {"iterations", "scale",
-    "screen_mean", "max_ratio", "mean_ratio"} or None} and the trials, and leaves the simulation at
-    the state it started from (snapshot slot 0), warm start off."""
+    A scale is one factor or a (deviatoric, hydrostatic) pair; net is one network or a dict of networks
+    by iteration count (the learned start at k iterations uses net[k]; counts without one are
+    skipped). Returns {mode: {"iterations", "scale", "screen_mean", "max_ratio", "mean_ratio"} or None}
+    and the trials, and leaves the simulation at the state it started from (snapshot slot 0), warm
+    start off."""
     sim.snapshot(0)
     found = {}
     trials = []
 
-    def start(mode, s):
+    def net_for(k):
+        return net.get(k) if isinstance(net, dict) else net
+
+    def start(mode, s, k):
         sim.restore(0)
-        sim.set_warm_start(mode, net if mode == "learned" else None)
+        sim.set_warm_start(mode, net_for(k) if mode == "learned" else None)
         sim.set_warm_start_scale(*_scales(s))
 
     for mode in modes:
         best = None
         for k in iterations:
+            if mode == "learned" and net_for(k) is None:
+                continue
             passing = []
             for s in scales:
-                start(mode, s)
+                start(mode, s, k)
                 r = mean_residual(sim, scene, frames, k)
                 trials.append({"mode": mode, "iterations": k, "scale": s, "residual": r})
                 log(f"  {mode} x{s} {k} iterations: mean residual {r:.4g} (target {target:.4g})")
                 if r <= screen_slack * target:
                     passing.append((r, s))
             for r, s in sorted(passing, key=lambda p: p[0]):
-                start(mode, s)
+                start(mode, s, k)
                 rw, rp = per_state_ratios(sim, scene, frames, k)
                 ratio = rw / rp
                 ok = bool(np.all(np.isfinite(ratio))) and float(ratio.max()) <= 1.0

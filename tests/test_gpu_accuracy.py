@@ -92,3 +92,65 @@ def test_beam_1m_plain_and_warm_started_against_the_fp64_oracle():
     for f in CHECK:
         assert abs(err[("previous_vs_plain", f)] - err[("oracle_previous_vs_plain", f)]) <= max(
             TOL, 3.0 * err[("previous_oracle_spread", f)]), err
+
+
+@pytest.mark.timeout(1200)
+def test_beam_1m_learned_warm_start_against_the_fp64_oracle():
+    # the learned start as the benchmark trains it (2,000 captured frames in 40 episodes under random
+    # gravity directions and impulse loads, from the state after 5 plain frames, along the
+    # previous-multiplier start with the same iterations and scales), then from rest on the device
+    # and in the oracle evaluating the same network (the device's BF16 rounding points)
+    from sbs_amd import autotune
+
+    sc = scenes.beam_1m()
+    diag = sc.bbox_diagonal()
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    k, scale = 4, (0.7, 0.65)
+    trainer = scenes.build_device(sc)
+    trainer.use_graphs(True)
+    for _ in range(5):
+        trainer.step(sc.dt, sc.iterations, sc.substeps)
+    net, info = autotune.train_warm_start(trainer, sc, iterations=k, scale=scale, log=say)
+    trainer.close()
+
+    def device():
+        d = scenes.build_device(sc)
+        d.use_graphs(True)
+        d.set_warm_start("learned", net)
+        d.set_warm_start_scale(*scale)
+        return d
+
+    def oracle(rounded=False, learned=True):
+        def make():
+            o = build_oracle(sc)
+            o.set_threads(threads)
+            if rounded:
+                x0 = sc.bodies[0].x0
+                o.set_body_state(0, x0.astype(np.float32).astype(np.float64), np.zeros_like(x0))
+            if learned:
+                o.set_learned_warm_start(net, *scale)
+            return o
+        return make
+
+    say("1M beam: learned warm start, device")
+    dev = run(sc, device, k)
+    say("1M beam: learned warm start, oracle")
+    orc = run(sc, oracle(), k)
+    say("1M beam: learned warm start from FP32-rounded initial positions, oracle")
+    orc_r = run(sc, oracle(rounded=True), k)
+    say("1M beam: plain, oracle")
+    orc_plain = run(sc, oracle(learned=False), sc.iterations)
+    err = {}
+    for f in CHECK:
+        err[("learned", f)] = float(np.abs(dev[f] - orc[f]).max()) / diag
+        err[("learned_oracle_spread", f)] = float(np.abs(orc_r[f] - orc[f]).max()) / diag
+        err[("learned_vs_plain", f)] = float(np.abs(dev[f] - orc_plain[f]).max()) / diag
+        err[("oracle_learned_vs_plain", f)] = float(np.abs(orc[f] - orc_plain[f]).max()) / diag
+    for key, v in sorted(err.items()):
+        say(f"  {key[0]} frame {key[1]}: {v:.3e}")
+    for f in CHECK:
+        assert np.isfinite(err[("learned", f)]), (f, err)
+        assert err[("learned", f)] <= max(TOL, 3.0 * err[("learned_oracle_spread", f)]), (f, err)
+        assert abs(err[("learned_vs_plain", f)] - err[("oracle_learned_vs_plain", f)]) <= max(
+            TOL, 3.0 * err[("learned_oracle_spread", f)]), (f, err)
+    assert err[("learned", 1)] <= TOL
