@@ -461,13 +461,22 @@ def main():
         sim.set_fallback(0.0)
         rw, rp = autotune.per_state_ratios(sim, scene, args.steps, cfg["iterations"])
         ratio = rw / rp
-        above = int(np.sum(~(ratio <= 1.0)))
+        above = int(np.sum(~(ratio <= 1.0)))  # a frame whose residual is not finite counts as above
+        fin = ratio[np.isfinite(ratio)]
+
+        def num(v):
+            return float(v) if np.isfinite(v) else None
+
         cfg["timed_frames_per_state"] = {
-            "ratio_max": float(np.nanmax(ratio)), "ratio_median": float(np.nanmedian(ratio)),
-            "ratio_min": float(np.nanmin(ratio)), "frames_above_1": above, "frames": int(len(ratio)),
-            "ratios": [round(float(r), 4) for r in ratio]}
+            "ratio_max": num(ratio.max()) if len(fin) == len(ratio) else None,
+            "ratio_median": num(np.median(fin)) if len(fin) else None,
+            "ratio_min": num(fin.min()) if len(fin) else None, "frames_above_1": above,
+            "frames_not_finite": int(len(ratio) - len(fin)), "frames": int(len(ratio)),
+            "ratios": [round(float(r), 4) if np.isfinite(r) else None for r in ratio]}
+        t = cfg["timed_frames_per_state"]
         log(f"{mode} per state over the {args.steps} timed frames with {cfg['iterations']} iterations: ratio max "
-            f"{np.nanmax(ratio):.4g}, median {np.nanmedian(ratio):.4g}, {above} frames above 1")
+            f"{t['ratio_max']}, median {t['ratio_median']}, {above} frames above 1 ({t['frames_not_finite']} not "
+            f"finite)")
         return above
 
     time_leg("plain", legs["plain"])
@@ -542,6 +551,13 @@ def main():
                                      "scale": legs[mode]["scale"], "net": nets.get(legs[mode]["iterations"])})
             accuracy = accuracy_legs(scene, acc_legs, launch, frames=args.accuracy_frames,
                                      check=(1, args.accuracy_frames))
+            accuracy["qualifier"] = (
+                "the warm-started legs exceed 1e-4 against the oracle running the same solver after about frame 30 "
+                "because those trajectories amplify rounding-sized perturbations: two FP64 oracle runs whose initial "
+                "positions differ by one FP32 rounding end about 1e-3 of the bbox apart at frame 60 "
+                "(tests/test_gpu_accuracy.py measures that spread and holds the device to three times it); the "
+                "plain solver stays within 1e-4, and every warm-started leg sits about 1.6e-2 from the plain FP64 "
+                "solution at frame 60, as the oracle's own warm-started solution does")
             accuracy["definition"] = ("max over vertices of |x_device - x_oracle|, divided by the rest bounding-box "
                                       "diagonal, from rest; <leg>: the oracle runs the same solver (colored order, "
                                       "same warm start); <leg>_vs_plain: against the plain FP64 solver; no fallback")

@@ -188,8 +188,8 @@ int sbs_gpu_set_contact_hash(sbs_gpu* sim, int mode, double cell_size);
 int sbs_gpu_read_contact_hash(sbs_gpu* sim, double* cell_size, uint32_t* table_bits, uint32_t* start, uint32_t* end,
                               uint32_t* entries, uint32_t cap, uint32_t* num_entries, uint32_t* global,
                               uint32_t* num_global);
-/* Contacts dropped so far because a vertex had more than 8 (and surface triangles that covered more
- * than 8 cells of the body contact hash). */
+/* Contacts dropped so far because a surface vertex had more than its slots (one per SDF plus two
+ * with body contacts on, at most 8); a step reports them as SBS_ERR_CONTACT_OVERFLOW. */
 int sbs_gpu_contact_overflow(sbs_gpu* sim, uint64_t* dropped);
 
 /* Contacts between the surfaces of deformable bodies. The reference's collision detection visits
@@ -222,8 +222,9 @@ int sbs_gpu_set_body_contacts(sbs_gpu* sim, int enable, double cell_size, double
 int sbs_gpu_read_body_contact_setup(sbs_gpu* sim, double* cell_size, double* thickness, double* radius,
                                     uint32_t* table_bits, uint32_t* triangles);
 /* Broad-phase candidate pairs at the committed positions: per surface vertex (global surface
- * numbering, increasing) the distinct triangles of the hash slots of the 27 cells around its own
- * that do not have it as a corner, in increasing order, as (vertex, triangle) pairs; with pairs NULL
+ * numbering, increasing) the distinct triangles of the hash slots of the 27 cells around its own,
+ * and the triangles whose box covers more than 8 cells (tested by every vertex), that do not have it
+ * as a corner, in increasing order, as (vertex, triangle) pairs; with pairs NULL
  * *count gets an upper bound (repeats included), else up to cap pairs are written and *count gets
  * their number. */
 int sbs_gpu_read_body_contact_candidates(sbs_gpu* sim, uint32_t* pairs, uint64_t cap, uint64_t* count);

@@ -9,8 +9,8 @@
 
 namespace sbs {
 
-// Cells of the triangle hash one triangle's box may cover; a triangle covering more is counted in
-// `overflow` (the step reports it).
+// Cells of the triangle hash one triangle's box may cover; a triangle covering more (one stretched
+// beyond the cell edge) goes to the list of large triangles that every vertex tests.
 constexpr int kTriCells = 8;
 
 // The surface triangles as contact obstacles: per triangle its 3 surface vertices (global surface
@@ -27,7 +27,8 @@ struct body_contact_dev {
     uint32_t* vals;         // [kTriCells * ntris] its triangle
     uint32_t* start;        // [mask + 1] slot ranges into keys/vals
     uint32_t* end;
-    uint32_t* overflow;     // triangles that covered more than kTriCells cells (cumulative)
+    uint32_t* big;          // [ntris] this frame's triangles covering more than kTriCells cells
+    uint32_t* nbig;         // their number (one word)
     const uint32_t* sbody;  // body of each surface vertex
     const uint4* nbr;       // [ntris] the triangle across edge k (corner k to corner k+1) in x, y, z;
                             // 0xffffffff where the edge has no neighbour
@@ -41,8 +42,9 @@ struct body_contact_dev {
 std::size_t body_contact_workspace(uint32_t ntris);
 
 // The hash of the triangles at positions x: every triangle's box emits its (slot, triangle) pairs
-// (at most kTriCells), a stable radix sort groups them by slot (triangles in increasing order inside
-// a slot) and each slot's range is marked. ws: body_contact_workspace words.
+// (a triangle covering more than kTriCells cells goes to the large-triangle list instead), a stable
+// radix sort groups them by slot (triangles in increasing order inside a slot) and each slot's range
+// is marked. ws: body_contact_workspace words.
 void build_body_hash(hipStream_t s, body_contact_dev const& b, const uint32_t* surf, const pos4* x, int bits,
                      uint32_t* ws);
 
@@ -57,7 +59,7 @@ void build_body_hash(hipStream_t s, body_contact_dev const& b, const uint32_t* s
 void launch_detect_bodies(hipStream_t s, contact_dev c, body_contact_dev const& b, const pos4* x, int block);
 
 // Candidate pairs of the broad phase: per surface vertex the triangle entries of the 27 slots around
-// its cell that are not incident to it (repeats included); count[s] (s < S) first, then with offsets
+// its cell and the large triangles, those not incident to it (repeats included); count[s] (s < S) first, then with offsets
 // (exclusive prefix of the counts) the pairs (s, t) into out[2 * (off[s] + i)].
 void launch_body_candidates(hipStream_t s, contact_dev c, body_contact_dev const& b, const pos4* x,
                             const uint32_t* offsets, uint32_t* counts, uint32_t* out);

@@ -29,12 +29,13 @@
 // iteration.
 //
 // Broad phase: cells of edge H (twice the mean surface edge by default, r < H) hashed into a table
-// of 2^b slots (Teschner et al. 2003). Each triangle's box covers at most kTriCells cells (more is an
-// error reported by the step); its (slot, triangle) pairs are written in triangle order, padding
-// pairs carry the slot key 2^b (sorted past every slot), a stable radix sort groups them by slot,
-// and each slot's range is marked. A triangle closer than r < H to a vertex overlaps a cell of the
-// 3 x 3 x 3 block around the vertex's cell, so the vertex finds it in one of those 27 slots: the
-// nearest triangle found this way is the brute force's. All FP64 without contraction, so the
+// of 2^b slots (Teschner et al. 2003). A triangle whose box covers at most kTriCells cells writes
+// its (slot, triangle) pairs in triangle order, padding pairs carrying the slot key 2^b (sorted past
+// every slot); a triangle covering more (stretched beyond the cell edge) goes to a list of large
+// triangles that every vertex tests. A stable radix sort groups the pairs by slot and each slot's
+// range is marked. A triangle closer than r < H to a vertex overlaps a cell of the 3 x 3 x 3 block
+// around the vertex's cell, so the vertex finds it in one of those 27 slots or in the large list:
+// the nearest triangle found this way is the brute force's. All FP64 without contraction, so the
 // contacts equal the oracle's bit for bit. Every launch has a fixed size (graph capture).
 #include "body_contact.h"
 #include "contact_hash.h"
@@ -94,8 +95,8 @@ __global__ void k_tri_emit(body_contact_dev b, const uint32_t* __restrict__ surf
                     vals[j] = t;
                     ++j;
                 }
-    } else {
-        atomicAdd(b.overflow, 1u);
+    } else {  // a triangle stretched beyond the cell edge: every vertex tests it
+        b.big[atomicAdd(b.nbig, 1u)] = t;
     }
     for (; j < kTriCells; ++j) {
         keys[j] = pad;
@@ -103,10 +104,11 @@ __global__ void k_tri_emit(body_contact_dev b, const uint32_t* __restrict__ surf
     }
 }
 
-__global__ void k_fill2(uint32_t* __restrict__ a, uint32_t* __restrict__ b, uint32_t n)
+__global__ void k_fill2(uint32_t* __restrict__ a, uint32_t* __restrict__ b, uint32_t n, uint32_t* __restrict__ c)
 {
     uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = b[i] = 0u;
+    if (i == 0) *c = 0u;
 }
 
 __global__ void k_body_slot_ranges(body_contact_dev b, uint32_t n)
@@ -119,8 +121,8 @@ __global__ void k_body_slot_ranges(body_contact_dev b, uint32_t n)
     if (i + 1 == n || b.keys[i + 1] != k) b.end[k] = i + 1;
 }
 
-// Every triangle entry of the slots of the 27 cells around p's cell: visit(t) (a triangle may come
-// more than once, from several slots).
+// Every triangle entry of the slots of the 27 cells around p's cell, then every large triangle:
+// visit(t) (a triangle may come more than once, from several slots).
 template <class Visit>
 __device__ __forceinline__ void for_neighbour_triangles(body_contact_dev const& b, pos4 p, Visit&& visit)
 {
@@ -137,6 +139,7 @@ __device__ __forceinline__ void for_neighbour_triangles(body_contact_dev const& 
                     visit(t);
                 }
             }
+    for (uint32_t i = 0, e = *b.nbig; i < e; ++i) visit(b.big[i]);
 }
 
 __global__ void k_tri_unit_normals(body_contact_dev b, const uint32_t* __restrict__ surf, const pos4* __restrict__ x)
@@ -314,7 +317,7 @@ void build_body_hash(hipStream_t s, body_contact_dev const& b, const uint32_t* s
                      uint32_t* ws)
 {
     uint32_t const T = b.mask + 1u;
-    k_fill2<<<grid_for(T, 256), 256, 0, s>>>(b.start, b.end, T);
+    k_fill2<<<grid_for(T, 256), 256, 0, s>>>(b.start, b.end, T, b.nbig);
     if (!b.ntris) return;
     std::size_t const n = std::size_t(kTriCells) * b.ntris;
     k_tri_emit<<<grid_for(b.ntris, 256), 256, 0, s>>>(b, surf, x);

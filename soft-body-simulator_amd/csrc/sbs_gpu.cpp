@@ -153,7 +153,7 @@ struct sbs_gpu {
     double body_cell = 0.0, body_thickness = 0.0, body_radius = 0.0;
     int body_bits = 0;
     dev_buf<uint4> btris;
-    dev_buf<uint32_t> bkeys, bvals, bstart, bend, bws, bsbody;
+    dev_buf<uint32_t> bkeys, bvals, bstart, bend, bws, bsbody, bbig, bnbig;
     dev_buf<uint4> bnbr;                 // edge neighbours of each triangle
     dev_buf<uint32_t> bvt_off, bvt;      // per surface vertex: its (triangle, corner) entries
     dev_buf<pos4> btn, bvn;              // per frame: triangle unit normals, vertex pseudo-normals
@@ -161,7 +161,7 @@ struct sbs_gpu {
     body_contact_dev body_view()
     {
         return {btris.p, uint32_t(btris.n), 1.0 / body_cell, body_thickness, body_radius, (1u << body_bits) - 1u,
-                uint32_t(sdfs.size()), bkeys.p, bvals.p, bstart.p, bend.p, coverflow.p, bsbody.p,
+                uint32_t(sdfs.size()), bkeys.p, bvals.p, bstart.p, bend.p, bbig.p, bnbig.p, bsbody.p,
                 bnbr.p, bvt_off.p, bvt.p, btn.p, bvn.p};
     }
     void setup_body_contacts(std::vector<uint4> const& tr, std::vector<pos4> const& x);
@@ -309,8 +309,8 @@ struct sbs_gpu {
         if (n)
             throw api_error(SBS_ERR_CONTACT_OVERFLOW,
                             std::to_string(n) + " contacts were dropped because a surface vertex had more than " +
-                                std::to_string(kMaxContactSlots) +
-                                " (the state no longer follows the reference's contact set)");
+                                std::to_string(slots) +
+                                " (its contact slots; the state no longer follows the reference's contact set)");
     }
 
     void commit(uint32_t recolor_passes);
@@ -853,6 +853,8 @@ void sbs_gpu::setup_body_contacts(std::vector<uint4> const& tr, std::vector<pos4
     bvt.upload(vt.data(), vt.size(), stream);
     btn.alloc(oriented.size());
     bvn.alloc(S);
+    bbig.alloc(oriented.size());
+    bnbig.alloc(1);
     bkeys.alloc(n);
     bvals.alloc(n);
     bstart.alloc(std::size_t(1) << body_bits);

@@ -220,9 +220,9 @@ def _random_rotation(rng):
 
 
 def batch(num_bodies=512, tet_range=(2e3, 2e5), edge=0.05, seed=0xB0D1E5, density=1000.0, iterations=8,
-          substeps=4, name=None, large=None, young_range=(1e4, 1e6), poisson_range=(0.3, 0.48)):
+          substeps=4, name=None, large=None, young_range=(1e4, 1e6), poisson_range=(0.3, 0.48), drop_range=(0.5, 3.0)):
     """Batched bodies: `num_bodies` boxes and ellipsoids (alternating) of 6-tet hex cells with tet
-    counts log-uniform in tet_range, random rotations, dropped from heights U[0.5, 3] m onto a
+    counts log-uniform in tet_range, random rotations, dropped from heights U[drop_range] (U[0.5, 3] m) onto a
     ground plane at y = 0, Young's modulus log-uniform in young_range ([1e4, 1e6]), Poisson ratio
     uniform in poisson_range ([0.3, 0.48]),
     stable neo-Hookean constraints in one batched group with hex-cell clusters (index-order
@@ -265,7 +265,7 @@ def batch(num_bodies=512, tet_range=(2e3, 2e5), edge=0.05, seed=0xB0D1E5, densit
     bodies, bconn, young, poisson, clusters = [], [], [], [], [0]
     for b, (pos, tets) in enumerate(meshes):
         gx, gz = b % side, b // side
-        drop = rng.uniform(0.5, 3.0)
+        drop = rng.uniform(*drop_range)
         corner = np.array([gx * extent, 0.0, gz * extent])
         if b >= num_bodies:  # the large bodies: one row beyond the grid
             corner = np.array([(b - num_bodies) * big, 0.0, side * extent + 0.5 * big])
@@ -376,13 +376,15 @@ def channel(edge=0.05, young=5e4, poisson=0.4, iterations=16, substeps=8, length
 
 
 def tower(edge=0.05, young=2e5, poisson=0.4, iterations=16, substeps=8, gap=0.01, beta=1e-3, ellipsoid=True,
-          name="tower_4_bodies", body_contacts=(0.0, 0.0, 0.0)):
+          name="tower_4_bodies", body_contacts=(0.0, 0.0, 0.0), big_block=False):
     """A few bodies resting on each other: a slab of 8 x 2 x 8 hex cells on the ground plane, a box of
     4 x 3 x 4 cells turned 0.3 rad about y on the slab, a box of 2 x 2 x 2 cells turned 0.6 rad on
     that box, and an ellipsoid of radii (0.1, 0.06, 0.08) m on the slab beside them, each starting
     `gap` m above its support; every body's footprint lies inside its support's top face (without
     friction an overhanging soft body slides off). Stable neo-Hookean constraints with damping `beta`
-    in one batched group with hex clusters, contacts between the bodies."""
+    in one batched group with hex clusters, contacts between the bodies. big_block: one more body, a
+    single hex cell of edge 0.3 m on the ground beside the slab, whose triangles are larger than
+    the contact hash's cells."""
     sc = Scene(name, gravity_is_force=False, iterations=iterations, substeps=substeps, coloring="index",
                body_contacts=body_contacts)
     pos, tets = _lib.box6(8, 2, 8, edge)
@@ -398,6 +400,9 @@ def tower(edge=0.05, young=2e5, poisson=0.4, iterations=16, substeps=8, gap=0.01
         pos, tets = _lib.ellipsoid6(0.1, 0.06, 0.08, edge)
         pos = pos - pos.mean(0)
         meshes.append((pos + np.array([0.1, meshes[0][0][:, 1].max() + gap - pos[:, 1].min(), 0.1]), tets))
+    if big_block:
+        pos, tets = _lib.box6(1, 1, 1, 0.3)
+        meshes.append((pos + np.array([0.25, 0.001, -0.15]), tets))
     bodies, bconn = [], []
     for b, (x0, tets) in enumerate(meshes):
         mass = _lib.lumped_mass(x0, tets, 1000.0)

@@ -308,6 +308,54 @@ def test_learned_warm_start_matches_oracle(which):
             assert err <= max(TOL, 3.0 * spread), (f, err, spread)
 
 
+def test_learned_batched_with_body_contacts_over_300_frames():
+    # the learned start on a batched group (3 bodies of 2-4k tets and one of about 12k: the
+    # body-size feature) with contacts between bodies and of each body with its own surface on,
+    # 300 frames of drops of 5-20 cm onto the ground from rest, against the oracle running the same
+    # network; bound: 1e-4 of the bbox or three times the oracle's own spread from FP32-rounded
+    # initial positions, at every checked frame. Materials that settle (E 1e5-2e5, damping 1e-3):
+    # jelly-soft bodies (E from 1e4) dropped from up to 3 m bounce and slide chaotically, two oracle
+    # runs one FP32 rounding apart ending a whole bbox apart
+    sc = scenes.batch(3, tet_range=(2e3, 4e3), large=(1, 1.2e4), iterations=4, name="batch_3_plus_1_large",
+                      young_range=(1e5, 2e5), poisson_range=(0.3, 0.4), drop_range=(0.05, 0.2))
+    sc.groups[0].beta = 1e-3
+    sc.body_contacts = (0.0, 0.0, 0.0)
+    scale = (0.4, 0.4)
+    say(f"learned + batched + body contacts, 300 frames: {sc.name}, {sc.num_tets} tets")
+    dev = scenes.build_device(sc)
+    for _ in range(2):
+        dev.step(sc.dt, 8, sc.substeps)
+    net, info = _train_small(dev, sc, scale=scale)
+    dev = scenes.build_device(sc)
+    dev.set_warm_start("learned", net)
+    dev.set_warm_start_scale(*scale)
+    orcs = []
+    for rounded in (False, True):
+        o = build_oracle(sc)
+        o.set_threads(max(1, min(16, os.cpu_count() or 1)))
+        if rounded:
+            for bi, body in enumerate(sc.bodies):
+                o.set_body_state(bi, body.x0.astype(np.float32).astype(np.float64), np.zeros_like(body.x0))
+        o.set_learned_warm_start(net, *scale)
+        orcs.append(o)
+    diag = max(float(np.linalg.norm(np.ptp(b.x0, axis=0))) for b in sc.bodies)
+    nb = len(sc.bodies)
+    for f in range(1, 301):
+        dev.step(sc.dt, sc.iterations, sc.substeps)
+        for o in orcs:
+            o.step(sc.dt, sc.iterations, sc.substeps)
+        if f in (1, 10, 50, 100, 150, 200, 250, 300):
+            err = max(float(np.abs(dev.read_body(b)[0] - orcs[0].read_body(b)[0]).max()) for b in range(nb)) / diag
+            spread = max(float(np.abs(orcs[1].read_body(b)[0] - orcs[0].read_body(b)[0]).max())
+                         for b in range(nb)) / diag
+            say(f"frame {f} error {err:.3e} (oracle spread {spread:.3e})")
+            assert np.isfinite(err) and err <= max(TOL, 3.0 * spread), (f, err, spread)
+    assert dev.contact_overflow() == 0
+    # no body went through the ground (contacts are found once per frame, as in the reference, so
+    # a vertex moving down during a frame dips below the plane until the next frame's contacts)
+    assert min(float(dev.read_body(b)[0][:, 1].min()) for b in range(nb)) > -0.05
+
+
 def test_learned_network_against_the_previous_start():
     # the same iterations and scale with and without the network, on frames the network was not
     # trained on: the network predicts the multipliers' change over a substep far better than "no

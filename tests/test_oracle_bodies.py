@@ -44,9 +44,10 @@ def _nsdf(sc):
     return len(sc.planes) + len(sc.spheres)
 
 
-@pytest.mark.parametrize("name", ["tower", "channel"])
+@pytest.mark.parametrize("name", ["tower", "tower_big_block", "channel"])
 def test_grid_and_brute_force_pick_the_same_triangles(name):
-    sc = scenes.tower() if name == "tower" else scenes.channel(young=1e5, gap=1)
+    sc = {"tower": scenes.tower, "tower_big_block": lambda: scenes.tower(big_block=True),
+          "channel": lambda: scenes.channel(young=1e5, gap=1)}[name]()
     for frames in (0, 12):
         _, ids_g, qn_g = _contacts(sc, False, frames)
         _, ids_b, qn_b = _contacts(sc, True, frames)
