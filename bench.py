@@ -13,7 +13,8 @@ residual is at most the plain solver's from the same state. The plain solver the
 to the start of the timed horizon (`--steps` frames, disjoint from the tuning frames); every leg is
 timed there from one snapshot (`--reps` repeats, median) with the per-frame fallback check on, and
 every warm-started leg is compared with the plain solver per state over the timed frames again; a
-leg with a frame above the plain residual is timed and compared again with one more iteration.
+leg with a frame above the plain residual is timed and compared again with one more iteration (and
+the scale the tuner found best for that count on the tuning frames).
 `value` is the learned leg when all its timed frames reach the plain residual, else the plain
 solver's.
 
@@ -401,7 +402,7 @@ def main():
     found, trials, nets, train_info = {}, [], {}, None
     if warmable:
         f_prev, t_prev = autotune.tune_warm_start(sim, scene, None, float(plain_tune.mean()), frames=tune,
-                                                  modes=("previous",), log=log)
+                                                  modes=("previous",), all_iterations=True, log=log)
         found.update(f_prev)
         trials += t_prev
         # one network per iteration count from the previous start's count up: each is trained on
@@ -414,7 +415,7 @@ def main():
             nets[k], train_info[k] = autotune.train_warm_start(sim, scene, log=log, **train_cfg)
         f_l, t_l = autotune.tune_warm_start(sim, scene, nets, float(plain_tune.mean()), frames=tune,
                                             iterations=tuple(range(k0, scene.iterations)), modes=("learned",),
-                                            log=log)
+                                            all_iterations=True, log=log)
         found.update(f_l)
         trials += t_l
     # the timed horizon starts where the plain solver leaves the tuning horizon (state S1)
@@ -487,12 +488,17 @@ def main():
         if mode == "plain":
             continue
         cfg["tuned_iterations"] = cfg["iterations"]
+        by_k = cfg.get("by_iterations", {})
         while True:
             time_leg(mode, cfg)
             above = verify_leg(mode, cfg)
             if above == 0 or cfg["iterations"] + 1 >= scene.iterations:
                 break
+            # one more iteration, with the scale the tuner found best for that count on the tuning
+            # frames (the current one where none qualified there)
             cfg["iterations"] += 1
+            if cfg["iterations"] in by_k:
+                cfg["scale"] = by_k[cfg["iterations"]]["scale"]
         cfg["equal_residual"] = above == 0
         cfg["speedup_vs_plain"] = round(cfg["substeps_per_s"] / plain_sps, 3)
     sim.set_warm_start("none")
@@ -610,7 +616,8 @@ def main():
             f"Timing on the next {args.steps} frames (disjoint), median of {args.reps} repeats, fallback check on "
             f"(limit {limit:.4g} = twice the largest plain tuning frame residual); timed_frames_per_state: the same "
             f"per-state comparison over the timed frames (no fallback); a leg with a timed frame above the plain "
-            f"residual is timed and compared again with one more iteration (tuned_iterations: the tuner's count); "
+            f"residual is timed and compared again with one more iteration and the scale the tuner found best for "
+            f"that count on the tuning frames (by_iterations; tuned_iterations: the tuner's smallest count); "
             f"value: the learned leg when every timed frame reaches the plain residual, else the plain solver"),
         "training": train_info,
         "num_trials": len(trials),

@@ -116,8 +116,9 @@ def capture_episodes(sim, scene, net, episodes, frames, iterations, scale, strid
     start (a velocity change of U[impulse] m/s in a random direction added to the movable particles
     within U[radius] m of a random particle of a random body). Rows go to net's dataset (capture
     stride `stride`). The scene's gravity is restored at the end; the state is left as the last
-    episode ends."""
+    episode ends. Returns per episode its range of dataset rows and its largest frame residual."""
     g = float(np.linalg.norm(scene.gravity))
+    record = []
     for e in range(episodes):
         sim.restore(1)
         d = _random_direction(rng)
@@ -135,13 +136,48 @@ def capture_episodes(sim, scene, net, episodes, frames, iterations, scale, strid
         sim.write_body(b, None, v)
         sim.set_warm_start("previous")
         sim.set_warm_start_scale(*_scales(scale))
+        first = net.dataset_size()
         sim.capture(net, stride)
+        largest = 0.0
         for _ in range(frames):
             sim.step(scene.dt, iterations, scene.substeps)
+            r = frame_residual(sim, scene)
+            largest = r if not np.isfinite(r) else max(largest, r)
         sim.capture(None, 0)
+        record.append((first, net.dataset_size(), largest))
     sim.set_params(gravity=scene.gravity, gravity_is_force=scene.gravity_is_force,
                    collision_compliance=scene.collision_compliance)
     sim.set_warm_start("none")
+    return record
+
+
+def diverged_episodes(record, factor=20.0):
+    """The (first row, end row) ranges of the episodes in `record` ((first, end, largest frame
+    residual) each) whose trajectory diverged: a largest frame residual that is not finite or above
+    `factor` times the median over the episodes with a finite one."""
+    largest = np.array([r for _, _, r in record], dtype=np.float64)
+    finite = largest[np.isfinite(largest)]
+    if not len(finite):
+        return [(a, b) for a, b, _ in record]
+    limit = factor * float(np.median(finite))
+    return [(a, b) for a, b, r in record if not (np.isfinite(r) and r <= limit)]
+
+
+def drop_diverged_episodes(net, record, factor=20.0):
+    """Removes from net's dataset the rows of the episodes diverged_episodes names (the warm start
+    they were captured along blew up, and its multipliers would set the network's normalization).
+    Returns the number of episodes removed."""
+    bad = diverged_episodes(record, factor)
+    if not bad:
+        return 0
+    x, t = net.dataset()
+    keep = np.ones(len(x), dtype=bool)
+    for a, b in bad:
+        keep[a:b] = False
+    net.clear()
+    if keep.any():
+        net.append(x[keep], t[keep])
+    return len(bad)
 
 
 def train_warm_start(sim, scene, iterations=4, scale=(0.7, 0.65), frames=2000, episodes=40, val_episodes=4,
@@ -163,8 +199,12 @@ def train_warm_start(sim, scene, iterations=4, scale=(0.7, 0.65), frames=2000, e
     val.reserve(max(1, max_rows * val_episodes // max(episodes, 1)))
     sim.snapshot(1)
     t0 = time.perf_counter()
-    capture_episodes(sim, scene, net, episodes, per_episode, iterations, scale, stride, rng, log=log)
-    capture_episodes(sim, scene, val, val_episodes, per_episode, iterations, scale, stride, rng, log=log)
+    rec = capture_episodes(sim, scene, net, episodes, per_episode, iterations, scale, stride, rng, log=log)
+    rec_val = capture_episodes(sim, scene, val, val_episodes, per_episode, iterations, scale, stride, rng, log=log)
+    dropped = drop_diverged_episodes(net, rec)
+    # the held-out episodes are judged by the training episodes' typical residual
+    typical = float(np.median([r for _, _, r in rec if np.isfinite(r)] or [np.inf]))
+    dropped_val = drop_diverged_episodes(val, rec_val + [(0, 0, typical)] * len(rec))
     cap_s = time.perf_counter() - t0
     sim.restore(1)
     n = net.dataset_size()
@@ -177,6 +217,8 @@ def train_warm_start(sim, scene, iterations=4, scale=(0.7, 0.65), frames=2000, e
     # the same error for the network that predicts no change (the previous-multiplier start)
     zero = float(np.mean((tv / net.get_normalization()[2]) ** 2))
     val.close()
+    if dropped or dropped_val:
+        log(f"warm-start network: {dropped} training and {dropped_val} held-out episodes diverged and were dropped")
     log(f"warm-start network: {n} rows from {episodes} x {per_episode} frames (stride {stride}, capture {cap_s:.1f}s), "
         f"{steps} Adam steps of {batch} in {train_s:.1f}s, loss {loss:.4f}; held out ({len(xv)} rows of "
         f"{val_episodes} episodes): {val_loss:.4f} (no change predicted: {zero:.4f})")
@@ -184,7 +226,8 @@ def train_warm_start(sim, scene, iterations=4, scale=(0.7, 0.65), frames=2000, e
                  "frames": episodes * per_episode, "captured_with": {"iterations": iterations, "scale": scale},
                  "steps": steps, "batch": batch, "capture_s": round(cap_s, 2), "train_s": round(train_s, 2),
                  "train_loss": loss, "held_out_rows": int(len(xv)), "held_out_loss": val_loss,
-                 "held_out_loss_no_change": zero,
+                 "held_out_loss_no_change": zero, "episodes_dropped": dropped,
+                 "held_out_episodes_dropped": dropped_val,
                  "loads": "gravity of the scene's magnitude in a random direction per episode, and a velocity "
                           "impulse U[0.5, 2] m/s on the particles within U[0.1, 0.5] m of a random particle"}
 
@@ -201,7 +244,7 @@ def training_config(args, found=None):
 
 
 def tune_warm_start(sim, scene, net, target, frames=8, scales=SCALE_GRID, iterations=(2, 3, 4, 5, 6, 7),
-                    modes=("learned", "previous"), screen_slack=1.15, log=print):
+                    modes=("learned", "previous"), screen_slack=1.15, all_iterations=False, log=print):
     """For each mode, the cheapest (iterations, scale) that reaches the plain solver's residual on the
     same states over `frames` frames from the current state. Each candidate is screened by its mean
     frame residual along its own trajectory (<= screen_slack x target, the plain solver's mean over
@@ -212,7 +255,8 @@ def tune_warm_start(sim, scene, net, target, frames=8, scales=SCALE_GRID, iterat
     by iteration count (the learned start at k iterations uses net[k]; counts without one are
     skipped). Returns {mode: {"iterations", "scale", "screen_mean", "max_ratio", "mean_ratio"} or None}
     and the trials, and leaves the simulation at the state it started from (snapshot slot 0), warm
-    start off."""
+    start off. With all_iterations every count is tried and the result carries "by_iterations": the
+    best qualifying scale of each count that has one."""
     sim.snapshot(0)
     found = {}
     trials = []
@@ -226,10 +270,11 @@ def tune_warm_start(sim, scene, net, target, frames=8, scales=SCALE_GRID, iterat
         sim.set_warm_start_scale(*_scales(s))
 
     for mode in modes:
-        best = None
+        best, by_k = None, {}
         for k in iterations:
             if mode == "learned" and net_for(k) is None:
                 continue
+            best_k = None
             passing = []
             for s in scales:
                 start(mode, s, k)
@@ -245,11 +290,17 @@ def tune_warm_start(sim, scene, net, target, frames=8, scales=SCALE_GRID, iterat
                 ok = bool(np.all(np.isfinite(ratio))) and float(ratio.max()) <= 1.0
                 log(f"  {mode} x{s} {k} iterations per state over {frames} frames: ratio max {ratio.max():.4g} "
                     f"mean {ratio.mean():.4g}{'' if ok else ': rejected'}")
-                if ok and (best is None or ratio.max() < best["max_ratio"]):
-                    best = {"iterations": k, "scale": s, "screen_mean": r, "max_ratio": float(ratio.max()),
-                            "mean_ratio": float(ratio.mean())}
-            if best is not None:
-                break  # the smallest iteration count that reaches the target
+                if ok and (best_k is None or ratio.max() < best_k["max_ratio"]):
+                    best_k = {"iterations": k, "scale": s, "screen_mean": r, "max_ratio": float(ratio.max()),
+                              "mean_ratio": float(ratio.mean())}
+            if best_k is not None:
+                by_k[k] = best_k
+                if best is None:
+                    best = dict(best_k)  # the smallest iteration count that reaches the target
+                if not all_iterations:
+                    break
+        if best is not None and all_iterations:
+            best["by_iterations"] = by_k
         found[mode] = best
     sim.set_warm_start("none")
     sim.restore(0)

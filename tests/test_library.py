@@ -153,3 +153,13 @@ def test_null_handles_are_status_codes():
         assert L.sbs_gpu_last_error().decode(), name
         checked += 1
     assert checked >= 30
+
+
+def test_diverged_capture_episodes_are_named():
+    # episodes (first row, end row, largest frame residual): a blow-up (non-finite, or 20x the
+    # median of the episodes' largest residuals) is named; the others are kept
+    from sbs_amd.autotune import diverged_episodes
+    rec = [(0, 10, 2.0), (10, 20, 2.5), (20, 30, float("inf")), (30, 40, 1.8), (40, 50, 60.0), (50, 60, 3.0)]
+    assert diverged_episodes(rec) == [(20, 30), (40, 50)]
+    assert diverged_episodes(rec[:2]) == []
+    assert diverged_episodes([(0, 5, float("nan"))]) == [(0, 5)]
