@@ -154,3 +154,34 @@ def test_beam_1m_learned_warm_start_against_the_fp64_oracle():
         assert abs(err[("learned_vs_plain", f)] - err[("oracle_learned_vs_plain", f)]) <= max(
             TOL, 3.0 * err[("learned_oracle_spread", f)]), (f, err)
     assert err[("learned", 1)] <= TOL
+
+
+@pytest.mark.timeout(1200)
+def test_beam_6m_plain_against_the_fp64_oracle():
+    # the large-beam configuration at its full size: 400 x 50 x 50 hex cells, 6,000,000 stable
+    # neo-Hookean tets, 4 substeps x 8 iterations, ground plane and 16 spheres; device and oracle
+    # from rest, frames 1, 5 and 10
+    sc = scenes.beam_6m()
+    assert sc.num_tets == 6_000_000
+    diag = sc.bbox_diagonal()
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    check = (1, 5, 10)
+    say("6M beam: plain 8 iterations, device")
+    dev = scenes.build_device(sc)
+    dev.use_graphs(True)
+    xd = {}
+    for f in range(1, 11):
+        dev.step(sc.dt, sc.iterations, sc.substeps)
+        if f in check:
+            xd[f] = dev.read_body(0)[0].copy()
+    assert dev.contact_overflow() == 0
+    dev.close()
+    say(f"6M beam: plain 8 iterations, oracle on {threads} threads")
+    orc = build_oracle(sc)
+    orc.set_threads(threads)
+    for f in range(1, 11):
+        orc.step(sc.dt, sc.iterations, sc.substeps)
+        if f in check:
+            e = float(np.abs(xd[f] - orc.read_body(0)[0]).max()) / diag
+            say(f"  frame {f}: {e:.3e}")
+            assert e <= TOL, (f, e)
