@@ -327,14 +327,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=5, help="untimed frames")
     ap.add_argument("--profile-steps", type=int, default=3, help="frames of the instrumented region")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="beam_1m", choices=["beam_1m", "beam_6m", "beam_540", "batch_512", "batch_512_large"])
+    ap.add_argument("--workload", default="beam_1m",
+                    choices=["beam_1m", "beam_6m", "beam_540", "batch_512", "batch_512_large", "batch_512_contacts"])
     ap.add_argument("--no-learned", action="store_true", help="plain solver only")
     ap.add_argument("--tune-frames", type=int, default=16, help="frames after warm-up the warm starts are tuned on")
     ap.add_argument("--reps", type=int, default=5, help="repeats of the timed frames (median)")
     ap.add_argument("--accuracy-frames", type=int, default=60,
                     help="frames of the accuracy legs from rest (0: none); errors at frame 1 and the last")
     ap.add_argument("--train-frames", type=int, default=None, help="captured frames of the network's training data")
-    ap.add_argument("--also", nargs="*", default=["beam_6m", "batch_512"],
+    ap.add_argument("--also", nargs="*", default=["beam_6m", "batch_512", "batch_512_contacts"],
                     choices=["beam_1m", "beam_6m", "beam_540", "batch_512", "batch_512_large"],
                     help="further workloads measured with the plain solver (no tuning)")
     ap.add_argument("--no-graphs", action="store_true", help="stream launches instead of hipGraph replays")
@@ -369,7 +370,8 @@ def main():
         copy_calibration()
 
     make_scene = {"beam_1m": scenes.beam_1m, "beam_6m": scenes.beam_6m, "beam_540": scenes.beam_540,
-                  "batch_512": scenes.batch_512, "batch_512_large": scenes.batch_512_large}
+                  "batch_512": scenes.batch_512, "batch_512_large": scenes.batch_512_large,
+                  "batch_512_contacts": scenes.batch_512_contacts}
     scene = make_scene[args.workload]()
     log(f"workload {scene.name}: {scene.num_vertices} vertices, {scene.num_tets} tets")
     if args.profile_frames:
@@ -705,8 +707,20 @@ def plain_throughput(scene, kind, args, warmup=3, frames=5, reps=3):
     sim.reset_stats()
     sim.step(scene.dt, scene.iterations, scene.substeps)
     launches, ms, units = sim.kernel_stats(kind)
+    _, detect_ms, _ = sim.kernel_stats(S._lib.KERNEL_DETECT)
     sim.set_timing(False)
     overflow = sim.contact_overflow()
+    body_contacts = None
+    if scene.body_contacts is not None:  # the contacts of the state after the timed frames, by kind
+        import numpy as np
+        ids, _ = sim.read_contacts()
+        tri = sim.body_contact_setup()[4]
+        nsdf = len(scene.planes) + len(scene.spheres) + len(scene.grids)
+        t = ids[ids[:, 3] >= nsdf]
+        other = int(np.sum(tri[t[:, 3] - nsdf, 3] != t[:, 0]))
+        body_contacts = {"surface_triangles": int(len(tri)), "contacts_with_other_bodies": other,
+                         "contacts_with_own_surface": int(len(t)) - other,
+                         "contacts_with_obstacles": int(len(ids) - len(t))}
     sim.close()
     avg = ms / 1e3 / launches
     gbs = BYTES_PER_TET * units / launches / avg / 1e9
@@ -716,7 +730,9 @@ def plain_throughput(scene, kind, args, warmup=3, frames=5, reps=3):
     return {"workload": scene.name, "tets": scene.num_tets, "plain_substeps_per_s": round(sps, 2),
             "launch_config": launch, "projection_avg_launch_us": round(1e6 * avg, 3),
             "roofline_frac": round(gbs / HBM_PEAK_GBS, 4), "achieved_GBps": round(gbs, 1),
-            "contact_overflow": overflow, "frames": f"median of {reps} x {frames} frames after {warmup}"}
+            "contact_overflow": overflow, "frames": f"median of {reps} x {frames} frames after {warmup}",
+            "detect_ms_per_frame": round(detect_ms, 3),
+            "body_contacts": body_contacts}
 
 
 if __name__ == "__main__":

@@ -30,11 +30,17 @@ struct body_contact_dev {
     uint32_t* big;          // [ntris] this frame's triangles covering more than kTriCells cells
     uint32_t* nbig;         // their number (one word)
     const uint32_t* sbody;  // body of each surface vertex
+    const uint32_t* toff;   // [bodies + 1] body k's triangles are toff[k] .. toff[k + 1] - 1
     const uint4* nbr;       // [ntris] the triangle across edge k (corner k to corner k+1) in x, y, z;
                             // 0xffffffff where the edge has no neighbour
     const uint32_t* vt_off; // [S + 1] per surface vertex, its range of vt
     const uint32_t* vt;     // 3 t + k for each triangle t having the vertex as corner k, t increasing
     pos4* tn;               // [ntris] per frame: the triangle's unit normal (zero when degenerate)
+    pos4* sph;              // [ntris] per frame: a sphere holding the triangle (center, radius)
+    int4* tlo;              // [ntris] per frame: the lowest cell of the triangle's box
+    uint4* tpx;             // [ntris] per frame: the corners' particles and the body
+    uint4* erec;            // [kTriCells * ntris] per frame, hashed entry e: its triangle and that triangle's tlo
+    float4* esph;           // [kTriCells * ntris] per frame: entry e's sphere relative to the corner of cell tlo
     pos4* vn;               // [S] per frame: the vertex's angle-weighted pseudo-normal
 };
 

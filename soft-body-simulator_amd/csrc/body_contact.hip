@@ -83,6 +83,7 @@ __global__ void k_tri_emit(body_contact_dev b, const uint32_t* __restrict__ surf
         hi[d] = hash_cell(mx, b.inv_cell);
         cells *= uint64_t(uint32_t(hi[d] - lo[d]) + 1u);
     }
+    b.tlo[t] = make_int4(lo[0], lo[1], lo[2], 0);
     uint32_t* keys = b.keys + std::size_t(t) * kTriCells;
     uint32_t* vals = b.vals + std::size_t(t) * kTriCells;
     uint32_t const pad = b.mask + 1u;
@@ -153,6 +154,42 @@ __global__ void k_tri_unit_normals(body_contact_dev b, const uint32_t* __restric
     double const l2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
     double const l = sqrt(l2);
     b.tn[t] = l2 > 0.0 ? make_double4(n[0] / l, n[1] / l, n[2] / l, 0.0) : make_double4(0.0, 0.0, 0.0, 0.0);
+    // the centroid and the largest corner distance from it, widened by 1e-9 of itself: a bound that
+    // rounding cannot undercut, for the search's early rejections
+    double const cx = (c[0] + c[3] + c[6]) / 3.0, cy = (c[1] + c[4] + c[7]) / 3.0, cz = (c[2] + c[5] + c[8]) / 3.0;
+    double r2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        double const dx = c[3 * k] - cx, dy = c[3 * k + 1] - cy, dz = c[3 * k + 2] - cz;
+        r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+    }
+    b.sph[t] = make_double4(cx, cy, cz, sqrt(r2) * (1.0 + 1e-9));
+    uint4 const tr = b.tris[t];
+    b.tpx[t] = make_uint4(surf[tr.x], surf[tr.y], surf[tr.z], tr.w);
+}
+
+// Per hashed entry e (key below the padding slot), the search's record of it: the triangle and the
+// lowest cell lo of its box (erec), and a sphere holding the triangle in FP32 relative to that
+// cell's corner O = lo * cell edge (esph: the center minus O rounded to FP32, the radius widened by
+// twice the largest per-axis error of that rounding and of the FP64 subtraction before it, then
+// rounded up; twice covers the sqrt(3) of three axes).
+__global__ void k_entry_records(body_contact_dev b, uint32_t n)
+{
+#pragma clang fp contract(off)
+    uint32_t const e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n || b.keys[e] > b.mask) return;
+    uint32_t const t = b.vals[e];
+    int4 const lo = b.tlo[t];
+    pos4 const sp = b.sph[t];
+    double const cs = 1.0 / b.inv_cell;
+    double const d[3] = {sp.x - double(lo.x) * cs, sp.y - double(lo.y) * cs, sp.z - double(lo.z) * cs};
+    float const f[3] = {float(d[0]), float(d[1]), float(d[2])};
+    double err = 0.0;
+    for (int k = 0; k < 3; ++k) err = fmax(err, fabs(double(f[k]) - d[k]) + 1e-15 * fabs(d[k]));
+    double const R = sp.w + 2.0 * err;
+    float Rf = float(R);
+    if (double(Rf) < R) Rf = nextafterf(Rf, INFINITY);
+    b.erec[e] = make_uint4(t, uint32_t(lo.x), uint32_t(lo.y), uint32_t(lo.z));
+    b.esph[e] = make_float4(f[0], f[1], f[2], Rf);
 }
 
 // N(s) = sum over the triangles t having s as corner k, in increasing t, of the corner's angle
@@ -210,81 +247,186 @@ __device__ __forceinline__ void pseudo_normal(body_contact_dev const& b, uint32_
     }
 }
 
-// The nearest non-incident triangle of surface vertex s closer than r among another body's triangles
-// (own false) or its own surface's (own true, with p more than h / 10 outside along the
-// pseudo-normal N of the closest feature: (p - c).N > (h / 10) |N|), the lowest index on ties
-// whatever the visiting order;
-// returns the triangle (or 0xffffffff), its closest point, the point's barycentric weights and its
-// feature (tri_closest_point's numbering).
-__device__ __forceinline__ uint32_t nearest_triangle(contact_dev const& c, body_contact_dev const& b, const pos4* x,
-                                                     uint32_t s, pos4 p, bool own, double best_q[3],
-                                                     double best_w[3], int& best_f)
+constexpr int kDetectBlock = 256;
+
+// Records the contact of surface vertex s (at p) with triangle t, whose closest point q (barycentric
+// weights w) lies on feature f: the normal runs along the line from q to p, away from the surface
+// (towards q when the vertex lies inside another body), the triangle's normal when p lies on it.
+// Returns false when the vertex has no free slot (counted in c.overflow).
+__device__ __forceinline__ bool emit_triangle_contact(contact_dev const& c, body_contact_dev const& b, uint32_t s,
+                                                      pos4 p, uint32_t t, int f, const double q[3], const double w[3])
 {
 #pragma clang fp contract(off)
-    double best = b.radius * b.radius;
-    uint32_t bt = 0xffffffffu;
-    double const pp[3] = {p.x, p.y, p.z};
-    uint32_t const body = b.sbody[s];
-    for_neighbour_triangles(b, p, [&](uint32_t t) {
-        uint4 const tr = b.tris[t];
-        if ((tr.w == body) != own) return;
-        if (tr.x == s || tr.y == s || tr.z == s) return;
-        double corners[9], q[3], wts[3];
-        tri_corners(b, c.surf, x, tr, corners);
-        int feature;
-        tri_closest_point(pp, corners, q, &feature, wts);
-        double const dx = pp[0] - q[0], dy = pp[1] - q[1], dz = pp[2] - q[2];
-        double const d2 = dx * dx + dy * dy + dz * dz;
-        if (!(d2 < best || (d2 == best && bt != 0xffffffffu && t < bt))) return;
-        if (own) {
-            double N[3];
-            pseudo_normal(b, t, feature, N);
-            double const ln = sqrt(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
-            if (!(dx * N[0] + dy * N[1] + dz * N[2] > 0.1 * b.thickness * ln)) return;
-        }
-        best = d2;
-        bt = t;
-        best_f = feature;
-        best_q[0] = q[0], best_q[1] = q[1], best_q[2] = q[2];
-        best_w[0] = wts[0], best_w[1] = wts[1], best_w[2] = wts[2];
-    });
-    return bt;
+    pos4 const tn = b.tn[t];
+    double n[3] = {tn.x, tn.y, tn.z};
+    double const d[3] = {p.x - q[0], p.y - q[1], p.z - q[2]};
+    double const dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (dd > 0.0) {
+        double N[3];
+        pseudo_normal(b, t, f, N);
+        double const l = d[0] * N[0] + d[1] * N[1] + d[2] * N[2] < 0.0 ? -sqrt(dd) : sqrt(dd);
+        n[0] = d[0] / l, n[1] = d[1] / l, n[2] = d[2] / l;
+    }
+    if (!(n[0] * n[0] + n[1] * n[1] + n[2] * n[2] > 0.0)) return true;  // a degenerate triangle
+    uint32_t const slot = c.count[s];
+    if (slot == c.slots) {
+        atomicAdd(c.overflow, 1u);
+        return false;
+    }
+    // the contact keeps the closest point's barycentric weights: its plane follows the triangle
+    store_contact(c, s, slot, b.sdf_base + t, w, n);
+    c.count[s] = slot + 1;
+    return true;
 }
 
-__global__ void k_detect_bodies(contact_dev c, body_contact_dev b, const pos4* __restrict__ x)
+constexpr int kDetectQueue = 32;  // candidates a lane holds before it computes their closest points
+
+// Per surface vertex s at p, in one search, the nearest triangle of another body and the nearest
+// of its own surface (not having s as a corner, with p more than h / 10 outside along the
+// pseudo-normal N of the closest feature: (p - c).N > (h / 10) |N|), each closer than r, the lowest
+// index on ties whatever the visiting order; a contact for each, another body's first.
+// The search visits the triangles hashed into the slots of the cells that the box p +- r (widened
+// by 1e-6 of r against rounding) overlaps, then the large triangles: every triangle with a point
+// within r of p is among them, since that point lies in the box and in the triangle's own box of
+// cells (a subset of the 27-cell neighbourhood of the candidate pairs). A hashed triangle is taken
+// only from the first of those cells (in visiting order) that its own box covers, the corner
+// max(lo, lowest query cell): once, and never lost. A triangle whose bounding sphere lies farther
+// from p than its class's best so far (with a margin of 1e-9 of that sum; the class from the body's
+// triangle range) cannot be nearer and is dropped before its closest point is computed.
+// Each lane scans its entries (one 32-byte record each, no dependent loads) into a queue in LDS
+// until it holds kDetectQueue candidates or the entries end, then computes the queued closest
+// points, each after testing its sphere again against the class's best as it is by then: the FP64 closest-point code runs with most of the wave's lanes busy instead of the
+// fraction whose current entry passed. The two bests are plain scalars of the kernel (registers).
+// Blocks go round robin over the 8 XCDs in launch order; XCD k takes a contiguous run of vertices so
+// that neighbouring vertices, which read the same cells, share its L2.
+__global__ void __launch_bounds__(kDetectBlock) __attribute__((amdgpu_waves_per_eu(3))) k_detect_bodies(contact_dev c, body_contact_dev b, const pos4* __restrict__ x)
 {
 #pragma clang fp contract(off)
-    uint32_t const s = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t queue[kDetectQueue * kDetectBlock];
+    uint32_t const xcd = blockIdx.x % 8u, per = gridDim.x / 8u, extra = gridDim.x % 8u;
+    uint32_t const logical = xcd * per + min(xcd, extra) + blockIdx.x / 8u;
+    uint32_t const s = logical * kDetectBlock + threadIdx.x;
     if (s >= c.S) return;
-    pos4 const p = x[c.surf[s]];
-    for (int own = 0; own < 2; ++own) {  // another body's nearest triangle, then its own surface's
-        double cq[3], wts[3];
-        int feature = 6;
-        uint32_t const t = nearest_triangle(c, b, x, s, p, own != 0, cq, wts, feature);
-        if (t == 0xffffffffu) continue;
-        // along the line from the closest point to the vertex, away from the surface (towards the
-        // closest point when the vertex lies inside another body); the triangle's normal when the
-        // vertex lies on the triangle
-        pos4 const tn = b.tn[t];
-        double n[3] = {tn.x, tn.y, tn.z};
-        double const d[3] = {p.x - cq[0], p.y - cq[1], p.z - cq[2]};
-        double const dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-        if (dd > 0.0) {
-            double N[3];
-            pseudo_normal(b, t, feature, N);
-            double const l = d[0] * N[0] + d[1] * N[1] + d[2] * N[2] < 0.0 ? -sqrt(dd) : sqrt(dd);
-            n[0] = d[0] / l, n[1] = d[1] / l, n[2] = d[2] / l;
+    uint32_t const ps = c.surf[s];
+    pos4 const p = x[ps];
+    double const pp[3] = {p.x, p.y, p.z};
+    uint32_t const body = b.sbody[s];
+    uint32_t const own0 = b.toff[body], nown = b.toff[body + 1] - own0;  // the own surface's triangles
+    // another body's nearest (o) and the own surface's (m): triangle, feature, squared distance and
+    // its root, closest point, barycentric weights
+    uint32_t to = 0xffffffffu, tm = 0xffffffffu;
+    int fo = 6, fm = 6;
+    double bo = b.radius * b.radius, bm = bo, ro = b.radius, rm = b.radius;
+    double qo[3] = {0.0, 0.0, 0.0}, qm[3] = {0.0, 0.0, 0.0}, wo[3] = {0.0, 0.0, 0.0}, wm[3] = {0.0, 0.0, 0.0};
+    double const cs = 1.0 / b.inv_cell;
+    double const rr = b.radius * (1.0 + 1e-6);
+    int const i0 = hash_cell(p.x - rr, b.inv_cell), ni = hash_cell(p.x + rr, b.inv_cell) - i0 + 1;
+    int const j0 = hash_cell(p.y - rr, b.inv_cell), nj = hash_cell(p.y + rr, b.inv_cell) - j0 + 1;
+    int const k0 = hash_cell(p.z - rr, b.inv_cell), nk = hash_cell(p.z + rr, b.inv_cell) - k0 + 1;
+    int const ncells = ni * nj * nk;
+    uint32_t* const qu = queue + threadIdx.x;  // this lane's queue, stride kDetectBlock
+    // the stream: cell `cell` (ncells: the large triangles), entries [e, n), the last triangle taken
+    int cell = -1, ci = 0, cj = 0, ck = 0;
+    uint32_t e = 0, n = 0, prev = 0xffffffffu;
+    bool more = true;
+    while (more) {
+        int cnt = 0;
+        while (cnt <= kDetectQueue - 4) {  // scan: up to 4 entries a step, into the queue
+            if (e == n) {
+                if (++cell > ncells) {
+                    more = false;
+                    break;
+                }
+                prev = 0xffffffffu;
+                if (cell < ncells) {
+                    ci = i0 + cell % ni, cj = j0 + (cell / ni) % nj, ck = k0 + cell / (ni * nj);
+                    uint32_t const slot = hash_slot(ci, cj, ck, b.mask);
+                    e = b.start[slot], n = b.end[slot];
+                } else {
+                    e = 0, n = *b.nbig;
+                }
+                continue;
+            }
+            if (cell < ncells) {
+                uint32_t const m = min(n - e, 4u);
+                uint4 r[4];
+                float4 f[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (uint32_t(u) < m) r[u] = b.erec[e + u], f[u] = b.esph[e + u];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (uint32_t(u) >= m) break;
+                    uint32_t const t = r[u].x;
+                    if (t == prev) continue;  // two cells of one triangle in one slot
+                    prev = t;
+                    int const lx = int(r[u].y), ly = int(r[u].z), lz = int(r[u].w);
+                    if (max(lx, i0) != ci || max(ly, j0) != cj || max(lz, k0) != ck) continue;  // not its first cell
+                    double const ex = (pp[0] - double(lx) * cs) - double(f[u].x);
+                    double const ey = (pp[1] - double(ly) * cs) - double(f[u].y);
+                    double const ez = (pp[2] - double(lz) * cs) - double(f[u].z);
+                    double const l = (double(f[u].w) + (t - own0 < nown ? rm : ro)) * (1.0 + 1e-9);
+                    if (ex * ex + ey * ey + ez * ez > l * l) continue;
+                    qu[cnt * kDetectBlock] = e + u;
+                    ++cnt;
+                }
+                e += m;
+            } else {
+                uint32_t const t = b.big[e++];
+                pos4 const sp = b.sph[t];
+                double const ex = pp[0] - sp.x, ey = pp[1] - sp.y, ez = pp[2] - sp.z;
+                double const l = (sp.w + (t - own0 < nown ? rm : ro)) * (1.0 + 1e-9);
+                if (!(ex * ex + ey * ey + ez * ez > l * l)) {
+                    qu[cnt * kDetectBlock] = 0x80000000u | t;
+                    ++cnt;
+                }
+            }
         }
-        if (!(n[0] * n[0] + n[1] * n[1] + n[2] * n[2] > 0.0)) continue;  // a degenerate triangle
-        uint32_t const k = c.count[s];
-        if (k == c.slots) {
-            atomicAdd(c.overflow, 1u);
-            return;
+        for (int i = 0; i < cnt; ++i) {  // the queued candidates' closest points
+            uint32_t const qe = qu[i * kDetectBlock];
+            uint32_t t;
+            if (qe & 0x80000000u) {  // a large triangle
+                t = qe & 0x7fffffffu;
+            } else {  // a hashed entry: tested again against the class's best as it is now
+                uint4 const r = b.erec[qe];
+                float4 const f = b.esph[qe];
+                t = r.x;
+                double const ex = (pp[0] - double(int(r.y)) * cs) - double(f.x);
+                double const ey = (pp[1] - double(int(r.z)) * cs) - double(f.y);
+                double const ez = (pp[2] - double(int(r.w)) * cs) - double(f.z);
+                double const l = (double(f.w) + (t - own0 < nown ? rm : ro)) * (1.0 + 1e-9);
+                if (ex * ex + ey * ey + ez * ez > l * l) continue;
+            }
+            uint4 const tp = b.tpx[t];
+            bool const mine = tp.w == body;
+            if (mine && (tp.x == ps || tp.y == ps || tp.z == ps)) continue;
+            pos4 const pa = x[tp.x], pb = x[tp.y], pc = x[tp.z];
+            double const corners[9] = {pa.x, pa.y, pa.z, pb.x, pb.y, pb.z, pc.x, pc.y, pc.z};
+            double q[3], wts[3];
+            int feature;
+            tri_closest_point(pp, corners, q, &feature, wts);
+            double const dx = pp[0] - q[0], dy = pp[1] - q[1], dz = pp[2] - q[2];
+            double const d2 = dx * dx + dy * dy + dz * dz;
+            double const best = mine ? bm : bo;
+            uint32_t const bt = mine ? tm : to;
+            if (!(d2 < best || (d2 == best && bt != 0xffffffffu && t < bt))) continue;
+            if (mine) {
+                double N[3];
+                pseudo_normal(b, t, feature, N);
+                double const ln = sqrt(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
+                if (!(dx * N[0] + dy * N[1] + dz * N[2] > 0.1 * b.thickness * ln)) continue;
+                bm = d2, rm = sqrt(d2), tm = t, fm = feature;
+                qm[0] = q[0], qm[1] = q[1], qm[2] = q[2];
+                wm[0] = wts[0], wm[1] = wts[1], wm[2] = wts[2];
+            } else {
+                bo = d2, ro = sqrt(d2), to = t, fo = feature;
+                qo[0] = q[0], qo[1] = q[1], qo[2] = q[2];
+                wo[0] = wts[0], wo[1] = wts[1], wo[2] = wts[2];
+            }
         }
-        // the contact keeps the closest point's barycentric weights: its plane follows the triangle
-        store_contact(c, s, k, b.sdf_base + t, wts, n);
-        c.count[s] = k + 1;
     }
+    if (to != 0xffffffffu && !emit_triangle_contact(c, b, s, p, to, fo, qo, wo)) return;
+    if (tm != 0xffffffffu) emit_triangle_contact(c, b, s, p, tm, fm, qm, wm);
 }
 
 // The non-incident triangle entries of the 27 slots around the vertex (repeats included):
@@ -329,8 +471,10 @@ void launch_detect_bodies(hipStream_t s, contact_dev c, body_contact_dev const& 
 {
     if (!c.S || !b.ntris) return;
     k_tri_unit_normals<<<grid_for(b.ntris, 256), 256, 0, s>>>(b, c.surf, x);
+    uint32_t const n = uint32_t(std::size_t(kTriCells) * b.ntris);
+    k_entry_records<<<grid_for(n, 256), 256, 0, s>>>(b, n);
     k_vertex_pseudonormals<<<grid_for(c.S, 256), 256, 0, s>>>(b, c.surf, x, c.S);
-    k_detect_bodies<<<grid_for(c.S, block), block, 0, s>>>(c, b, x);
+    k_detect_bodies<<<grid_for(c.S, kDetectBlock), kDetectBlock, 0, s>>>(c, b, x);
 }
 
 void launch_body_candidates(hipStream_t s, contact_dev c, body_contact_dev const& b, const pos4* x,

@@ -153,16 +153,19 @@ struct sbs_gpu {
     double body_cell = 0.0, body_thickness = 0.0, body_radius = 0.0;
     int body_bits = 0;
     dev_buf<uint4> btris;
-    dev_buf<uint32_t> bkeys, bvals, bstart, bend, bws, bsbody, bbig, bnbig;
+    dev_buf<uint32_t> bkeys, bvals, bstart, bend, bws, bsbody, bbig, bnbig, btoff;
     dev_buf<uint4> bnbr;                 // edge neighbours of each triangle
     dev_buf<uint32_t> bvt_off, bvt;      // per surface vertex: its (triangle, corner) entries
-    dev_buf<pos4> btn, bvn;              // per frame: triangle unit normals, vertex pseudo-normals
+    dev_buf<pos4> btn, bsph, bvn;        // per frame: triangle unit normals and bounding spheres, vertex pseudo-normals
+    dev_buf<int4> btlo;                  // per frame: each triangle's lowest cell
+    dev_buf<uint4> btpx, berec;          // per frame: corner particles and body; hash entry records
+    dev_buf<float4> besph;               // per frame: hash entry spheres
     dev_buf<pos4> csnap;  // surface positions at the start of a contact launch (triangle contacts)
     body_contact_dev body_view()
     {
         return {btris.p, uint32_t(btris.n), 1.0 / body_cell, body_thickness, body_radius, (1u << body_bits) - 1u,
-                uint32_t(sdfs.size()), bkeys.p, bvals.p, bstart.p, bend.p, bbig.p, bnbig.p, bsbody.p,
-                bnbr.p, bvt_off.p, bvt.p, btn.p, bvn.p};
+                uint32_t(sdfs.size()), bkeys.p, bvals.p, bstart.p, bend.p, bbig.p, bnbig.p, bsbody.p, btoff.p,
+                bnbr.p, bvt_off.p, bvt.p, btn.p, bsph.p, btlo.p, btpx.p, berec.p, besph.p, bvn.p};
     }
     void setup_body_contacts(std::vector<uint4> const& tr, std::vector<pos4> const& x);
 
@@ -791,6 +794,15 @@ void sbs_gpu::setup_body_contacts(std::vector<uint4> const& tr, std::vector<pos4
             sbody.push_back(bi);
         }
     bsbody.upload(sbody.data(), sbody.size(), stream);
+    {  // the triangles come body by body
+        std::vector<uint32_t> toff(bodies.size() + 1, 0);
+        for (std::size_t t = 0; t < tr.size(); ++t) {
+            require(t == 0 || tr[t].w >= tr[t - 1].w, "surface triangles must be grouped by body");
+            ++toff[tr[t].w + 1];
+        }
+        for (std::size_t k = 0; k < bodies.size(); ++k) toff[k + 1] += toff[k];
+        btoff.upload(toff.data(), toff.size(), stream);
+    }
     std::vector<double> vol(bodies.size(), 0.0);
     double edge_sum = 0.0;
     auto P = [&](uint32_t s) { pos4 const& q = x[sv[s]]; return std::array<double, 3>{q.x, q.y, q.z}; };
@@ -852,6 +864,11 @@ void sbs_gpu::setup_body_contacts(std::vector<uint4> const& tr, std::vector<pos4
     bvt_off.upload(vt_off.data(), vt_off.size(), stream);
     bvt.upload(vt.data(), vt.size(), stream);
     btn.alloc(oriented.size());
+    bsph.alloc(oriented.size());
+    btlo.alloc(oriented.size());
+    btpx.alloc(oriented.size());
+    berec.alloc(std::size_t(kTriCells) * oriented.size());
+    besph.alloc(std::size_t(kTriCells) * oriented.size());
     bvn.alloc(S);
     bbig.alloc(oriented.size());
     bnbig.alloc(1);

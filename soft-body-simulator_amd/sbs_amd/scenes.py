@@ -450,6 +450,17 @@ def batch_512(num_bodies=512):
     return batch(num_bodies, name=f"batch_{num_bodies}_bodies")
 
 
+def batch_512_contacts(num_bodies=512):
+    """The 512-body batch's shapes and places, dropped from 0.1-0.5 m, with contacts between bodies
+    and of each body with its own surface. Young's modulus log-uniform in [1e4, 1e5]: with stiffer
+    bodies (the batch's range reaches 1e6) 8 iterations that start every substep from zero
+    multipliers do not hold the rest shape, and the surfaces fold onto themselves in free fall."""
+    sc = batch(num_bodies, name=f"batch_{num_bodies}_bodies_with_body_contacts",
+               young_range=(1e4, 1e5), drop_range=(0.1, 0.5))
+    sc.body_contacts = (0.0, 0.0, 0.0)
+    return sc
+
+
 def batch_512_large(num_bodies=512, large=(8, 5e5)):
     """The 512-body batch plus 8 bodies of about 500k tets in the same batched group
     (about 26M tets); the learned warm start serves all of them with one network."""

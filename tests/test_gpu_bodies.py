@@ -165,3 +165,31 @@ def test_self_folding_channel_matches_the_oracle_over_30_frames():
     tris = dev.body_contact_setup()[4]
     own = ids[ids[:, 3] >= len(sc.planes)]
     assert len(own) > 0 and np.all(tris[own[:, 3] - len(sc.planes), 3] == 0)
+
+
+
+def test_contacts_match_the_oracle_on_folded_bodies():
+    """32 stiff bodies of 2k-20k tets (Young's modulus up to 1e6) after 15 frames of 8 iterations
+    from zero multipliers: their surfaces have folded onto themselves, the densest case of the
+    own-surface search; the contacts from the same positions equal the oracle's."""
+    sc = scenes.batch(32, tet_range=(2e3, 2e4), name="batch_32_folded")
+    sc.body_contacts = (0.0, 0.0, 0.0)
+    say(f"{sc.name}: {len(sc.bodies)} bodies, {sc.num_tets} tets")
+    dev = scenes.build_device(sc)
+    for _ in range(15):
+        dev.step(sc.dt, sc.iterations, sc.substeps)
+    orc = build_oracle(sc)
+    for b in range(len(sc.bodies)):
+        x, v = dev.read_body(b)
+        assert np.all(np.isfinite(x))
+        dev.write_body(b, x, v)
+        orc.set_body_state(b, x, v)
+    ids_d, qn_d = _sorted(*dev.read_contacts())
+    ids_o, qn_o = _sorted(*orc.detect_contacts())
+    nsdf = len(sc.planes)
+    say(f"{len(ids_d)} contacts, {int(np.sum(ids_d[:, 3] >= nsdf))} with triangles")
+    assert np.sum(ids_d[:, 3] >= nsdf) > 1000
+    assert dev.contact_overflow() == 0
+    assert np.array_equal(ids_d, ids_o)
+    assert np.array_equal(qn_d[:, :3], qn_o[:, :3])
+    assert np.allclose(qn_d[:, 3:], qn_o[:, 3:], rtol=0, atol=1e-7)
