@@ -328,7 +328,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=3, help="frames of the instrumented region")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="beam_1m",
-                    choices=["beam_1m", "beam_6m", "beam_540", "batch_512", "batch_512_large", "batch_512_contacts"])
+                    choices=["beam_1m", "beam_6m", "beam_540", "batch_512", "batch_512_large", "batch_512_contacts",
+                             "batch_512_large_contacts"])
     ap.add_argument("--no-learned", action="store_true", help="plain solver only")
     ap.add_argument("--tune-frames", type=int, default=16, help="frames after warm-up the warm starts are tuned on")
     ap.add_argument("--reps", type=int, default=5, help="repeats of the timed frames (median)")
@@ -336,7 +337,8 @@ def main():
                     help="frames of the accuracy legs from rest (0: none); errors at frame 1 and the last")
     ap.add_argument("--train-frames", type=int, default=None, help="captured frames of the network's training data")
     ap.add_argument("--also", nargs="*", default=["beam_6m", "batch_512", "batch_512_contacts"],
-                    choices=["beam_1m", "beam_6m", "beam_540", "batch_512", "batch_512_large"],
+                    choices=["beam_1m", "beam_6m", "beam_540", "batch_512", "batch_512_large", "batch_512_contacts",
+                             "batch_512_large_contacts"],
                     help="further workloads measured with the plain solver (no tuning)")
     ap.add_argument("--no-graphs", action="store_true", help="stream launches instead of hipGraph replays")
     ap.add_argument("--no-contact-fusion", action="store_true",
@@ -371,7 +373,8 @@ def main():
 
     make_scene = {"beam_1m": scenes.beam_1m, "beam_6m": scenes.beam_6m, "beam_540": scenes.beam_540,
                   "batch_512": scenes.batch_512, "batch_512_large": scenes.batch_512_large,
-                  "batch_512_contacts": scenes.batch_512_contacts}
+                  "batch_512_contacts": scenes.batch_512_contacts,
+                  "batch_512_large_contacts": scenes.batch_512_large_contacts}
     scene = make_scene[args.workload]()
     log(f"workload {scene.name}: {scene.num_vertices} vertices, {scene.num_tets} tets")
     if args.profile_frames:
@@ -570,7 +573,10 @@ def main():
                                       "diagonal, from rest; <leg>: the oracle runs the same solver (colored order, "
                                       "same warm start); <leg>_vs_plain: against the plain FP64 solver; no fallback")
             log(f"accuracy {accuracy}")
-        cpu = cpu_baseline(scene, scenes.batch(16) if args.workload in ("batch_512", "batch_512_large") else None)
+        sample = {"batch_512": scenes.batch, "batch_512_large": scenes.batch,
+                  "batch_512_contacts": scenes.batch_512_contacts,
+                  "batch_512_large_contacts": scenes.batch_512_contacts}.get(args.workload)
+        cpu = cpu_baseline(scene, sample(16) if sample else None)
         log(f"cpu baseline {cpu['value']:.3f} substeps/s")
 
     others = {}

@@ -450,13 +450,22 @@ def batch_512(num_bodies=512):
     return batch(num_bodies, name=f"batch_{num_bodies}_bodies")
 
 
-def batch_512_contacts(num_bodies=512):
-    """The 512-body batch's shapes and places, dropped from 0.1-0.5 m, with contacts between bodies
-    and of each body with its own surface. Young's modulus log-uniform in [1e4, 1e5]: with stiffer
-    bodies (the batch's range reaches 1e6) 8 iterations that start every substep from zero
-    multipliers do not hold the rest shape, and the surfaces fold onto themselves in free fall."""
-    sc = batch(num_bodies, name=f"batch_{num_bodies}_bodies_with_body_contacts",
-               young_range=(1e4, 1e5), drop_range=(0.1, 0.5))
+def batch_512_contacts(num_bodies=512, **kw):
+    """The 512-body batch with contacts between bodies and of each body with its own surface (cell
+    2 mean edges, thickness 0.1 and radius 1 mean edge). With the batch's stiffer bodies (Young's
+    modulus up to 1e6) 8 iterations that start every substep from zero multipliers do not hold the
+    rest shape: their surfaces fold onto themselves in free fall, the own-surface search's densest
+    case. kw: batch()'s other arguments."""
+    sc = batch(num_bodies, name=kw.pop("name", f"batch_{num_bodies}_bodies_with_body_contacts"), **kw)
+    sc.body_contacts = (0.0, 0.0, 0.0)
+    return sc
+
+
+def batch_512_large_contacts(num_bodies=512, large=(8, 5e5)):
+    """The combined configuration with self-collision: batch_512_large with contacts between bodies
+    and of each body with its own surface."""
+    sc = batch_512_large(num_bodies, large)
+    sc.name += "_with_body_contacts"
     sc.body_contacts = (0.0, 0.0, 0.0)
     return sc
 
