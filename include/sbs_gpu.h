@@ -212,7 +212,8 @@ int sbs_gpu_contact_overflow(sbs_gpu* sim, uint64_t* dropped);
  * w_p + sum_k b_k^2 w_k + alpha~, b_k the weights); the contacts come after p's SDF contacts
  * (contact index: number of SDFs + triangle index in global surface-triangle order). Broad phase: a
  * spatial hash of cells of edge cell_size rebuilt every frame (Teschner et al. 2003), each vertex
- * querying the 27 cells around its own. Before commit; 0 selects the defaults: cell_size twice,
+ * querying the cells that the box p +- radius overlaps (at most 8 of the 27 around its own), and a
+ * list of the triangles whose box covers more than 8 cells. Before commit; 0 selects the defaults: cell_size twice,
  * thickness a tenth of and radius equal to the mean surface edge at rest; thickness < radius <=
  * 0.999 cell_size. */
 int sbs_gpu_set_body_contacts(sbs_gpu* sim, int enable, double cell_size, double thickness, double radius);

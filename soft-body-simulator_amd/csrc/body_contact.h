@@ -55,8 +55,8 @@ void build_body_hash(hipStream_t s, body_contact_dev const& b, const uint32_t* s
                      uint32_t* ws);
 
 // Per surface vertex s (at x), after the contacts already in c (its SDF contacts): one contact
-// with the nearest triangle of another body in the 27 slots around its cell with
-// |p - closest point|^2 < r^2, then one with the nearest such triangle of its own surface that does
+// with the nearest triangle of another body with |p - closest point|^2 < r^2 (among the triangles
+// hashed into the cells that the box p +- r overlaps, and the large ones), then one with the nearest such triangle of its own surface that does
 // not have s as a corner and that p lies more than h / 10 outside of along the closest feature's
 // pseudo-normal (the lowest index on ties): the closest point's barycentric weights in the triangle
 // (in c.q) and the normal n along the line from the closest point c to p, pointing out of the

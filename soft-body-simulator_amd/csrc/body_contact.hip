@@ -34,8 +34,9 @@
 // every slot); a triangle covering more (stretched beyond the cell edge) goes to a list of large
 // triangles that every vertex tests. A stable radix sort groups the pairs by slot and each slot's
 // range is marked. A triangle closer than r < H to a vertex overlaps a cell of the 3 x 3 x 3 block
-// around the vertex's cell, so the vertex finds it in one of those 27 slots or in the large list:
-// the nearest triangle found this way is the brute force's. All FP64 without contraction, so the
+// around the vertex's cell (the candidate pairs list those 27 slots), and more precisely one of the
+// cells that the box p +- r overlaps (the search visits those), or it is in the large list: the
+// nearest triangle found this way is the brute force's. All FP64 without contraction, so the
 // contacts equal the oracle's bit for bit. Every launch has a fixed size (graph capture).
 #include "body_contact.h"
 #include "contact_hash.h"
