@@ -281,6 +281,9 @@ __device__ __forceinline__ bool emit_triangle_contact(contact_dev const& c, body
 }
 
 constexpr int kDetectQueue = 32;  // candidates a lane holds before it computes their closest points
+constexpr int kDeferQueue = 16;   // own-surface candidates farther than kNear r, held back
+constexpr double kNear = 0.25;
+constexpr uint32_t kDenseEntries = 512;  // hashed entries in a vertex's cells above which it holds back
 
 // Per surface vertex s at p, in one search, the nearest triangle of another body and the nearest
 // of its own surface (not having s as a corner, with p more than h / 10 outside along the
@@ -298,12 +301,18 @@ constexpr int kDetectQueue = 32;  // candidates a lane holds before it computes 
 // until it holds kDetectQueue candidates or the entries end, then computes the queued closest
 // points, each after testing its sphere again against the class's best as it is by then: the FP64 closest-point code runs with most of the wave's lanes busy instead of the
 // fraction whose current entry passed. The two bests are plain scalars of the kernel (registers).
+// Where a vertex's cells hold more than kDenseEntries entries (a surface folded onto itself),
+// own-surface triangles farther than r1 = kNear r from p go last: they wait in a second queue (or,
+// when more of them come than it holds, a second scan collects them), and are skipped altogether
+// when the own surface's best ends within r1, which no triangle farther than r1 can beat; there the
+// nearest accepted triangle is usually close, and most triangles within r are never computed.
 // Blocks go round robin over the 8 XCDs in launch order; XCD k takes a contiguous run of vertices so
 // that neighbouring vertices, which read the same cells, share its L2.
 __global__ void __launch_bounds__(kDetectBlock) __attribute__((amdgpu_waves_per_eu(3))) k_detect_bodies(contact_dev c, body_contact_dev b, const pos4* __restrict__ x)
 {
 #pragma clang fp contract(off)
     __shared__ uint32_t queue[kDetectQueue * kDetectBlock];
+    __shared__ uint32_t deferred[kDeferQueue * kDetectBlock];
     uint32_t const xcd = blockIdx.x % 8u, per = gridDim.x / 8u, extra = gridDim.x % 8u;
     uint32_t const logical = xcd * per + min(xcd, extra) + blockIdx.x / 8u;
     uint32_t const s = logical * kDetectBlock + threadIdx.x;
@@ -313,26 +322,46 @@ __global__ void __launch_bounds__(kDetectBlock) __attribute__((amdgpu_waves_per_
     double const pp[3] = {p.x, p.y, p.z};
     uint32_t const body = b.sbody[s];
     uint32_t const own0 = b.toff[body], nown = b.toff[body + 1] - own0;  // the own surface's triangles
-    // another body's nearest (o) and the own surface's (m): triangle, feature, squared distance and
-    // its root, closest point, barycentric weights
+    // another body's nearest (o) and the own surface's (m): triangle, squared distance and its root
+    // (the closest point is computed again for the two winners at the end)
     uint32_t to = 0xffffffffu, tm = 0xffffffffu;
-    int fo = 6, fm = 6;
     double bo = b.radius * b.radius, bm = bo, ro = b.radius, rm = b.radius;
-    double qo[3] = {0.0, 0.0, 0.0}, qm[3] = {0.0, 0.0, 0.0}, wo[3] = {0.0, 0.0, 0.0}, wm[3] = {0.0, 0.0, 0.0};
     double const cs = 1.0 / b.inv_cell;
     double const rr = b.radius * (1.0 + 1e-6);
     int const i0 = hash_cell(p.x - rr, b.inv_cell), ni = hash_cell(p.x + rr, b.inv_cell) - i0 + 1;
     int const j0 = hash_cell(p.y - rr, b.inv_cell), nj = hash_cell(p.y + rr, b.inv_cell) - j0 + 1;
     int const k0 = hash_cell(p.z - rr, b.inv_cell), nk = hash_cell(p.z + rr, b.inv_cell) - k0 + 1;
     int const ncells = ni * nj * nk;
-    uint32_t* const qu = queue + threadIdx.x;  // this lane's queue, stride kDetectBlock
+    uint32_t* const qu = queue + threadIdx.x;     // this lane's queue, stride kDetectBlock
+    uint32_t* const dq = deferred + threadIdx.x;  // its held-back own-surface candidates
+    double const r1 = kNear * b.radius;
+    uint32_t entries = 0;
+    for (int k = 0; k < ncells; ++k) {
+        uint32_t const slot = hash_slot(i0 + k % ni, j0 + (k / ni) % nj, k0 + k / (ni * nj), b.mask);
+        entries += b.end[slot] - b.start[slot];
+    }
+    bool const dense = entries > kDenseEntries;
+    int dcnt = 0;
+    bool dover = false;  // more were held back than dq holds
+    // mode 0: every candidate, the far own-surface ones held back; 1: a second scan for those;
+    // 2: the held-back queue is being computed
+    int mode = 0;
     // the stream: cell `cell` (ncells: the large triangles), entries [e, n), the last triangle taken
     int cell = -1, ci = 0, cj = 0, ck = 0;
     uint32_t e = 0, n = 0, prev = 0xffffffffu;
     bool more = true;
-    while (more) {
+    for (;;) {
         int cnt = 0;
-        while (cnt <= kDetectQueue - 4) {  // scan: up to 4 entries a step, into the queue
+        const uint32_t* src = qu;
+        if (!more) {
+            if (mode != 0 || !(rm > r1)) break;  // done, or nothing held back can win
+            if (dover) {  // scan again for the held-back candidates
+                mode = 1, more = true, cell = -1, e = 0, n = 0;
+                continue;
+            }
+            src = dq, cnt = dcnt, mode = 2;
+        }
+        while (more && cnt <= kDetectQueue - 4) {  // scan: up to 4 entries a step, into the queue
             if (e == n) {
                 if (++cell > ncells) {
                     more = false;
@@ -366,12 +395,27 @@ __global__ void __launch_bounds__(kDetectBlock) __attribute__((amdgpu_waves_per_
                     double const ex = (pp[0] - double(lx) * cs) - double(f[u].x);
                     double const ey = (pp[1] - double(ly) * cs) - double(f[u].y);
                     double const ez = (pp[2] - double(lz) * cs) - double(f[u].z);
-                    double const l = (double(f[u].w) + (t - own0 < nown ? rm : ro)) * (1.0 + 1e-9);
-                    if (ex * ex + ey * ey + ez * ez > l * l) continue;
+                    double const e2 = ex * ex + ey * ey + ez * ez;
+                    bool const own = t - own0 < nown;
+                    double const l = (double(f[u].w) + (own ? rm : ro)) * (1.0 + 1e-9);
+                    if (e2 > l * l) continue;
+                    double const l1 = (double(f[u].w) + r1) * (1.0 + 1e-9);
+                    bool const far = dense && own && e2 > l1 * l1;  // every point farther than r1
+                    if (mode == 1) {
+                        if (!far) continue;
+                    } else if (far) {
+                        if (dcnt < kDeferQueue)
+                            dq[dcnt++ * kDetectBlock] = e + u;
+                        else
+                            dover = true;
+                        continue;
+                    }
                     qu[cnt * kDetectBlock] = e + u;
                     ++cnt;
                 }
                 e += m;
+            } else if (mode == 1) {  // the large triangles were all computed in the first scan
+                e = n;
             } else {
                 uint32_t const t = b.big[e++];
                 pos4 const sp = b.sph[t];
@@ -384,7 +428,7 @@ __global__ void __launch_bounds__(kDetectBlock) __attribute__((amdgpu_waves_per_
             }
         }
         for (int i = 0; i < cnt; ++i) {  // the queued candidates' closest points
-            uint32_t const qe = qu[i * kDetectBlock];
+            uint32_t const qe = src[i * kDetectBlock];
             uint32_t t;
             if (qe & 0x80000000u) {  // a large triangle
                 t = qe & 0x7fffffffu;
@@ -416,18 +460,24 @@ __global__ void __launch_bounds__(kDetectBlock) __attribute__((amdgpu_waves_per_
                 pseudo_normal(b, t, feature, N);
                 double const ln = sqrt(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
                 if (!(dx * N[0] + dy * N[1] + dz * N[2] > 0.1 * b.thickness * ln)) continue;
-                bm = d2, rm = sqrt(d2), tm = t, fm = feature;
-                qm[0] = q[0], qm[1] = q[1], qm[2] = q[2];
-                wm[0] = wts[0], wm[1] = wts[1], wm[2] = wts[2];
+                bm = d2, rm = sqrt(d2), tm = t;
             } else {
-                bo = d2, ro = sqrt(d2), to = t, fo = feature;
-                qo[0] = q[0], qo[1] = q[1], qo[2] = q[2];
-                wo[0] = wts[0], wo[1] = wts[1], wo[2] = wts[2];
+                bo = d2, ro = sqrt(d2), to = t;
             }
         }
     }
-    if (to != 0xffffffffu && !emit_triangle_contact(c, b, s, p, to, fo, qo, wo)) return;
-    if (tm != 0xffffffffu) emit_triangle_contact(c, b, s, p, tm, fm, qm, wm);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {  // another body's nearest triangle, then its own surface's
+        uint32_t const t = k == 0 ? to : tm;
+        if (t == 0xffffffffu) continue;
+        uint4 const tp = b.tpx[t];
+        pos4 const pa = x[tp.x], pb = x[tp.y], pc = x[tp.z];
+        double const corners[9] = {pa.x, pa.y, pa.z, pb.x, pb.y, pb.z, pc.x, pc.y, pc.z};
+        double q[3], wts[3];
+        int feature;
+        tri_closest_point(pp, corners, q, &feature, wts);
+        if (!emit_triangle_contact(c, b, s, p, t, feature, q, wts)) return;
+    }
 }
 
 // The non-incident triangle entries of the 27 slots around the vertex (repeats included):
