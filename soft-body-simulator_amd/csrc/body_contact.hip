@@ -299,8 +299,9 @@ constexpr uint32_t kDenseEntries = 512;  // hashed entries in a vertex's cells a
 // triangle range) cannot be nearer and is dropped before its closest point is computed.
 // Each lane scans its entries (one 32-byte record each, no dependent loads) into a queue in LDS
 // until it holds kDetectQueue candidates or the entries end, then computes the queued closest
-// points, each after testing its sphere again against the class's best as it is by then: the FP64 closest-point code runs with most of the wave's lanes busy instead of the
-// fraction whose current entry passed. The two bests are plain scalars of the kernel (registers).
+// points, each after testing its sphere again against the class's best as it is by then: the FP64
+// closest-point code runs with most of the wave's lanes busy instead of the fraction whose current
+// entry passed. The two bests are plain scalars of the kernel (registers).
 // Where a vertex's cells hold more than kDenseEntries entries (a surface folded onto itself),
 // own-surface triangles farther than r1 = kNear r from p go last: they wait in a second queue (or,
 // when more of them come than it holds, a second scan collects them), and are skipped altogether
