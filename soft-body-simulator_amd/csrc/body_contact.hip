@@ -285,6 +285,13 @@ constexpr int kDeferQueue = 16;   // own-surface candidates farther than kNear r
 constexpr double kNear = 0.25;
 constexpr uint32_t kDenseEntries = 512;  // hashed entries in a vertex's cells above which it holds back
 
+// x as a float no smaller than x.
+__device__ __forceinline__ float float_up(double x)
+{
+    float const f = float(x);
+    return double(f) < x ? nextafterf(f, INFINITY) : f;
+}
+
 // Per surface vertex s at p, in one search, the nearest triangle of another body and the nearest
 // of its own surface (not having s as a corner, with p more than h / 10 outside along the
 // pseudo-normal N of the closest feature: (p - c).N > (h / 10) |N|), each closer than r, the lowest
@@ -295,8 +302,11 @@ constexpr uint32_t kDenseEntries = 512;  // hashed entries in a vertex's cells a
 // cells (a subset of the 27-cell neighbourhood of the candidate pairs). A hashed triangle is taken
 // only from the first of those cells (in visiting order) that its own box covers, the corner
 // max(lo, lowest query cell): once, and never lost. A triangle whose bounding sphere lies farther
-// from p than its class's best so far (with a margin of 1e-9 of that sum; the class from the body's
-// triangle range) cannot be nearer and is dropped before its closest point is computed.
+// from p than its class's best so far (the class from the body's triangle range) cannot be nearer
+// and is dropped before its closest point is computed. The scan tests that in FP32 relative to the
+// corner of the cell being scanned, with a margin of 1e-5 of the cell edge and of the bound (the
+// FP32 roundings of a center within 8 cells come to ~1e-7 m); the queued candidates are tested
+// again in FP64 (margin 1e-9) just before their closest points.
 // Each lane scans its entries (one 32-byte record each, no dependent loads) into a queue in LDS
 // until it holds kDetectQueue candidates or the entries end, then computes the queued closest
 // points, each after testing its sphere again against the class's best as it is by then: the FP64
@@ -336,6 +346,9 @@ __global__ void __launch_bounds__(kDetectBlock) __attribute__((amdgpu_waves_per_
     uint32_t* const qu = queue + threadIdx.x;     // this lane's queue, stride kDetectBlock
     uint32_t* const dq = deferred + threadIdx.x;  // its held-back own-surface candidates
     double const r1 = kNear * b.radius;
+    float const cs32 = float(cs), slack = 1e-5f * cs32, r1_32 = float_up(r1);
+    float rm32 = float_up(rm), ro32 = rm32;  // the bounds rounded up
+    float pc[3] = {0.0f, 0.0f, 0.0f};       // p relative to the corner of the cell being scanned
     uint32_t entries = 0;
     for (int k = 0; k < ncells; ++k) {
         uint32_t const slot = hash_slot(i0 + k % ni, j0 + (k / ni) % nj, k0 + k / (ni * nj), b.mask);
@@ -373,6 +386,8 @@ __global__ void __launch_bounds__(kDetectBlock) __attribute__((amdgpu_waves_per_
                     ci = i0 + cell % ni, cj = j0 + (cell / ni) % nj, ck = k0 + cell / (ni * nj);
                     uint32_t const slot = hash_slot(ci, cj, ck, b.mask);
                     e = b.start[slot], n = b.end[slot];
+                    pc[0] = float(pp[0] - double(ci) * cs), pc[1] = float(pp[1] - double(cj) * cs);
+                    pc[2] = float(pp[2] - double(ck) * cs);
                 } else {
                     e = 0, n = *b.nbig;
                 }
@@ -393,14 +408,16 @@ __global__ void __launch_bounds__(kDetectBlock) __attribute__((amdgpu_waves_per_
                     prev = t;
                     int const lx = int(r[u].y), ly = int(r[u].z), lz = int(r[u].w);
                     if (max(lx, i0) != ci || max(ly, j0) != cj || max(lz, k0) != ck) continue;  // not its first cell
-                    double const ex = (pp[0] - double(lx) * cs) - double(f[u].x);
-                    double const ey = (pp[1] - double(ly) * cs) - double(f[u].y);
-                    double const ez = (pp[2] - double(lz) * cs) - double(f[u].z);
-                    double const e2 = ex * ex + ey * ey + ez * ez;
+                    // the sphere's center relative to this cell's corner: lo - cell is 0 or, on an
+                    // axis where the triangle's box starts below the query box, negative
+                    float const ex = pc[0] - fmaf(float(lx - ci), cs32, f[u].x);
+                    float const ey = pc[1] - fmaf(float(ly - cj), cs32, f[u].y);
+                    float const ez = pc[2] - fmaf(float(lz - ck), cs32, f[u].z);
+                    float const e2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
                     bool const own = t - own0 < nown;
-                    double const l = (double(f[u].w) + (own ? rm : ro)) * (1.0 + 1e-9);
+                    float const l = fmaf(f[u].w + (own ? rm32 : ro32), 1.0f + 1e-5f, slack);
                     if (e2 > l * l) continue;
-                    double const l1 = (double(f[u].w) + r1) * (1.0 + 1e-9);
+                    float const l1 = fmaf(f[u].w + r1_32, 1.0f + 1e-5f, slack);
                     bool const far = dense && own && e2 > l1 * l1;  // every point farther than r1
                     if (mode == 1) {
                         if (!far) continue;
@@ -461,9 +478,9 @@ __global__ void __launch_bounds__(kDetectBlock) __attribute__((amdgpu_waves_per_
                 pseudo_normal(b, t, feature, N);
                 double const ln = sqrt(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
                 if (!(dx * N[0] + dy * N[1] + dz * N[2] > 0.1 * b.thickness * ln)) continue;
-                bm = d2, rm = sqrt(d2), tm = t;
+                bm = d2, rm = sqrt(d2), tm = t, rm32 = float_up(rm);
             } else {
-                bo = d2, ro = sqrt(d2), to = t;
+                bo = d2, ro = sqrt(d2), to = t, ro32 = float_up(ro);
             }
         }
     }
