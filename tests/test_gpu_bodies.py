@@ -59,6 +59,11 @@ def _restated_candidates(P, tris, cell, bits):
 def _scene(name):
     if name == "tower":
         return scenes.tower()
+    if name == "tower_far":  # 1 km from the origin: the scan's cell-relative FP32 test at large coordinates
+        sc = scenes.tower(name="tower_far")
+        for b in sc.bodies:
+            b.x0 = b.x0 + np.array([1000.25, 0.0, -700.5])
+        return sc
     if name == "tower_big_block":  # a body whose triangles cover more than 8 cells of the hash
         return scenes.tower(big_block=True, name="tower_big_block")
     return scenes.channel(young=1e5, gap=1)
@@ -93,7 +98,8 @@ def _sorted(ids, qn):
     return ids[o], qn[o]
 
 
-@pytest.mark.parametrize("name,frames", [("tower", 0), ("tower", 20), ("tower_big_block", 10), ("channel", 20)])
+@pytest.mark.parametrize("name,frames", [("tower", 0), ("tower", 20), ("tower_far", 20), ("tower_big_block", 10),
+                                         ("channel", 20)])
 def test_contacts_match_the_oracle_from_the_same_positions(name, frames):
     sc = _scene(name)
     say(f"contacts vs oracle: {sc.name} after {frames} frames")
