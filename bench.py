@@ -127,8 +127,8 @@ def accuracy_legs(scene, legs, launch, frames=60, check=(1, 60)):
     after the frames in `check`, from rest. legs: a list of {"name", "iterations", "mode", "scale",
     "net"} (mode "none" for the plain solver; no fallback, so the warm start's own trajectory is
     measured). Each device leg is compared with the oracle running the same solver (colored order,
-    the same warm start), and each warm-started device leg also with the plain oracle leg, the judge
-    of the warm starts ("<name>_vs_plain"). The oracle runs the colored order's stages on
+    the same warm start), and each warm-started device leg also with the plain oracle leg, the
+    solution the warm starts are measured against ("<name>_vs_plain"). The oracle runs the colored order's stages on
     oracle_threads() host threads (bit-identical to one thread)."""
     import numpy as np
 

@@ -202,7 +202,7 @@ def train_warm_start(sim, scene, iterations=4, scale=(0.7, 0.65), frames=2000, e
     rec = capture_episodes(sim, scene, net, episodes, per_episode, iterations, scale, stride, rng, log=log)
     rec_val = capture_episodes(sim, scene, val, val_episodes, per_episode, iterations, scale, stride, rng, log=log)
     dropped = drop_diverged_episodes(net, rec)
-    # the held-out episodes are judged by the training episodes' typical residual
+    # the held-out episodes are measured against the training episodes' typical residual
     typical = float(np.median([r for _, _, r in rec if np.isfinite(r)] or [np.inf]))
     dropped_val = drop_diverged_episodes(val, rec_val + [(0, 0, typical)] * len(rec))
     cap_s = time.perf_counter() - t0
