@@ -721,7 +721,7 @@ def plain_throughput(scene, kind, args, warmup=3, frames=5, reps=3):
         import numpy as np
         ids, _ = sim.read_contacts()
         tri = sim.body_contact_setup()[4]
-        nsdf = len(scene.planes) + len(scene.spheres) + len(scene.grids)
+        nsdf = len(scene.planes) + len(scene.spheres) + len(scene.grids) + len(scene.meshes)  # the SDFs, in order
         t = ids[ids[:, 3] >= nsdf]
         other = int(np.sum(tri[t[:, 3] - nsdf, 3] != t[:, 0]))
         body_contacts = {"surface_triangles": int(len(tri)), "contacts_with_other_bodies": other,
