@@ -386,6 +386,11 @@ def main():
     log(f"committed in {time.perf_counter() - t0:.1f}s, {ncolors} colors")
     kind = {"green": S._lib.KERNEL_GREEN, "neohookean": S._lib.KERNEL_NEOHOOKEAN}[scene.groups[0].kind]
     warmable = scene.groups[0].kind == "neohookean" and not args.no_learned
+    if warmable and scene.body_contacts is not None:
+        # a warm-start trial that diverges stretches the surfaces over so many cells that every
+        # triangle goes to the large list, which every vertex then tests (a frame takes minutes)
+        warmable = False
+        log("no warm-start legs on a scene with body contacts: the plain solver only")
 
     sim.use_graphs(not args.no_graphs)
     sim.set_contact_fusion(not args.no_contact_fusion)
@@ -670,7 +675,7 @@ def profile_run(args, scene):
     sim.synchronize()
     out = {"profile_run": scene.name, "launch_config": launch, "plain_frames": args.profile_frames,
            "iterations": scene.iterations}
-    if scene.groups[0].kind == "neohookean" and not args.no_learned:
+    if scene.groups[0].kind == "neohookean" and not args.no_learned and scene.body_contacts is None:
         k, scales = args.warm_config.split(":")
         k, scale = int(k), tuple(float(v) for v in scales.split(","))
         net, info = autotune.train_warm_start(sim, scene, iterations=k, scale=scale,
