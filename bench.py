@@ -603,9 +603,10 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": scene.name,
-            "solver": {"learned": "learned multiplier warm start + colored Gauss-Seidel XPBD",
-                       "plain": "plain GPU XPBD (colored Gauss-Seidel, no warm start): no learned configuration "
-                                "reached the plain residual on every timed frame per state"}[solver],
+            "solver": ("learned multiplier warm start + colored Gauss-Seidel XPBD" if solver == "learned" else
+                       "plain GPU XPBD (colored Gauss-Seidel, no warm start): " +
+                       ("no learned configuration reached the plain residual on every timed frame per state"
+                        if warmable else "no warm start tried (--no-learned, a Green group or body contacts)")),
             "tets": scene.num_tets,
             "vertices": scene.num_vertices,
             "substeps_per_step": scene.substeps,
