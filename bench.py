@@ -289,6 +289,33 @@ def row_grid_sdf(args):
             "ms_per_call": round(1e3 * dev, 3), "cpu_baseline": cpu}
 
 
+BEAM_SIZES = ((110, 14, 14), (160, 20, 20), (220, 28, 27), (280, 35, 34), (350, 44, 43), (400, 50, 50),
+              (500, 63, 62))
+
+
+def row_beam_sizes(args):
+    """The plain solver's projection kernel on beams of the 1M beam's shape from 130k to 11.7M tets
+    (same material, contacts, 4 substeps x 8 iterations): its average launch and HBM roofline
+    fraction against the clusters per launch, i.e. the waves each SIMD gets per color."""
+    import sbs_amd as S
+    from sbs_amd import scenes
+
+    rows = []
+    for nx, ny, nz in BEAM_SIZES:
+        sc = scenes.cantilever(nx, ny, nz, 0.1, "neohookean", 5e5, 0.45, 1000.0, num_spheres=16, iterations=8,
+                               substeps=4, name=f"beam_{nx}x{ny}x{nz}_neohookean", cluster=True, coloring="index")
+        r = plain_throughput(sc, S._lib.KERNEL_NEOHOOKEAN, args)
+        per_launch = sc.num_tets / 8
+        r["waves_per_simd_per_launch"] = round(per_launch / 6 / 64 / 1024, 3)
+        rows.append(r)
+        log(f"{sc.name}: {sc.num_tets} tets, {r['waves_per_simd_per_launch']} waves per SIMD per launch, "
+            f"{r['projection_avg_launch_us']} us per launch, {r['roofline_frac']} of HBM")
+    return {"metric": "projection kernel roofline fraction against beam size", "unit": "fraction of 8 TB/s",
+            "higher_is_better": True, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": "beams of 220x28x27 cells' proportions", "sizes": [list(s) for s in BEAM_SIZES]},
+            "rows": rows}
+
+
 def row_ply(args):
     """ASCII PLY writing and reading of the 1M-tet beam (179,452 vertices, 997,920 tets) from and
     to memory; beside it the reference's own ply.cpp (built from its sources as a checker, when
@@ -349,7 +376,7 @@ def main():
                     help="profiler run: this many plain and learned frames without tuning or timing (see profile_run)")
     ap.add_argument("--warm-config", default="4:0.7,0.65",
                     help="the previous-multiplier start the profiler run trains its network on: iterations:scales")
-    ap.add_argument("--row", default="step", choices=["step", "boundary", "grid_sdf", "ply"],
+    ap.add_argument("--row", default="step", choices=["step", "boundary", "grid_sdf", "ply", "beam_sizes"],
                     help="step: the time step (the headline metric); the others measure the surface, grid SDF "
                          "and PLY components on their own workloads")
     args = ap.parse_args()
@@ -358,7 +385,8 @@ def main():
               "beam across GPUs is not implemented, so --gpus must be 1.", file=sys.stderr)
         sys.exit(2)
     if args.row != "step":
-        out = {"boundary": row_boundary, "grid_sdf": row_grid_sdf, "ply": row_ply}[args.row](args)
+        out = {"boundary": row_boundary, "grid_sdf": row_grid_sdf, "ply": row_ply,
+               "beam_sizes": row_beam_sizes}[args.row](args)
         out.update({"n_gpus": 1})
         print(json.dumps(out), flush=True)
         return
