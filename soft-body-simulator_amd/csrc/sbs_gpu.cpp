@@ -825,7 +825,9 @@ void sbs_gpu::setup_body_contacts(std::vector<uint4> const& tr, std::vector<pos4
     for (auto& t : oriented)
         if (vol[t.w] < 0.0) std::swap(t.y, t.z);
     body_bits = 10;
-    while ((uint64_t(1) << body_bits) < 4 * uint64_t(tr.size()) && body_bits < 28) ++body_bits;
+    // two slots per triangle: the occupied cells (about 0.15 per triangle) stay sparse, and up to 4M
+    // triangles the keys with their padding bit fit 24 bits, three radix passes
+    while ((uint64_t(1) << body_bits) < 2 * uint64_t(tr.size()) && body_bits < 28) ++body_bits;
     std::size_t const n = std::size_t(kTriCells) * tr.size();
     require(n < (std::size_t(1) << 31), "too many surface triangles for the body contact hash");
     btris.upload(oriented.data(), oriented.size(), stream);
