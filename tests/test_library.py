@@ -163,3 +163,25 @@ def test_diverged_capture_episodes_are_named():
     assert diverged_episodes(rec) == [(20, 30), (40, 50)]
     assert diverged_episodes(rec[:2]) == []
     assert diverged_episodes([(0, 5, float("nan"))]) == [(0, 5)]
+
+
+def test_body_contact_workloads_are_the_batches_with_contacts():
+    # the contact workloads are the batch configurations themselves (same bodies, places and
+    # materials) with contacts between bodies and of each body with its own surface
+    a, b = scenes.batch(8), scenes.batch_512_contacts(8)
+    assert a.body_contacts is None and b.body_contacts == (0.0, 0.0, 0.0)
+    assert all(np.array_equal(x.x0, y.x0) and np.array_equal(x.tets, y.tets) for x, y in zip(a.bodies, b.bodies))
+    assert np.array_equal(a.groups[0].young, b.groups[0].young)
+    big = scenes.batch_512_large_contacts(4, large=(1, 2e3))
+    assert big.body_contacts == (0.0, 0.0, 0.0) and big.name.endswith("_with_body_contacts")
+    assert len(big.bodies) == 5
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2 and "--gpus must be 1" in r.stderr
