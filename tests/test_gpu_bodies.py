@@ -59,6 +59,11 @@ def _restated_candidates(P, tris, cell, bits):
 def _scene(name):
     if name == "tower":
         return scenes.tower()
+    if name == "tower_spheres":  # 4 spheres pressing 1.5 cm into the slab's sides and the top box: 5 SDFs, 7 slots
+        sc = scenes.tower(name="tower_spheres")
+        sc.spheres += [((0.255, 0.05, 0.0), 0.07), ((-0.255, 0.05, 0.05), 0.07), ((0.0, 0.05, -0.255), 0.07),
+                       ((-0.08, 0.43, -0.08), 0.075)]
+        return sc
     if name == "tower_far":  # 1 km from the origin: the scan's cell-relative FP32 test at large coordinates
         sc = scenes.tower(name="tower_far")
         for b in sc.bodies:
@@ -98,8 +103,8 @@ def _sorted(ids, qn):
     return ids[o], qn[o]
 
 
-@pytest.mark.parametrize("name,frames", [("tower", 0), ("tower", 20), ("tower_far", 20), ("tower_big_block", 10),
-                                         ("channel", 20)])
+@pytest.mark.parametrize("name,frames", [("tower", 0), ("tower", 20), ("tower_far", 20), ("tower_spheres", 0),
+                                         ("tower_spheres", 20), ("tower_big_block", 10), ("channel", 20)])
 def test_contacts_match_the_oracle_from_the_same_positions(name, frames):
     sc = _scene(name)
     say(f"contacts vs oracle: {sc.name} after {frames} frames")
@@ -113,8 +118,9 @@ def test_contacts_match_the_oracle_from_the_same_positions(name, frames):
         orc.set_body_state(b, x, v)
     ids_d, qn_d = _sorted(*dev.read_contacts())
     ids_o, qn_o = _sorted(*orc.detect_contacts())
-    nsdf = len(sc.planes)
-    say(f"{len(ids_d)} contacts, {int(np.sum(ids_d[:, 3] >= nsdf))} with triangles")
+    nsdf = len(sc.planes) + len(sc.spheres)
+    say(f"{len(ids_d)} contacts, {int(np.sum(ids_d[:, 3] >= nsdf))} with triangles, "
+        f"{int(np.sum((ids_d[:, 3] >= len(sc.planes)) & (ids_d[:, 3] < nsdf)))} with spheres")
     assert np.sum(ids_d[:, 3] >= nsdf) > 20
     assert dev.contact_overflow() == 0
     assert np.array_equal(ids_d, ids_o)
