@@ -83,3 +83,19 @@ def test_noalloc_sort_rejects_inputs_beyond_its_scratch():
             _lib.check(lib.sbs_util_sort_pairs_noalloc(0, d, d, n, 32, None))
     finally:
         hip.hipFree(d)
+
+
+def test_noalloc_sort_edge_cases():
+    # empty input (null pointers allowed), one pair, zero key bits: nothing to do and no error;
+    # key bits beyond 32 and a device that does not exist: errors, before any launch
+    lib, hip = _hip()
+    _lib.check(lib.sbs_util_sort_pairs_noalloc(0, None, None, 0, 32, None))
+    ok, ov = _noalloc_sort(np.array([7], np.uint32), np.array([3], np.uint32), 32)
+    assert ok.tolist() == [7] and ov.tolist() == [3]
+    keys = np.array([5, 1, 4], np.uint32)
+    ok, ov = _noalloc_sort(keys, np.arange(3, dtype=np.uint32), 0)
+    assert ok.tolist() == [5, 1, 4] and ov.tolist() == [0, 1, 2]
+    with pytest.raises(_lib.SbsError):
+        _lib.check(lib.sbs_util_sort_pairs_noalloc(0, None, None, 0, 33, None))
+    with pytest.raises(_lib.SbsError):
+        _lib.check(lib.sbs_util_sort_pairs_noalloc(-1, None, None, 0, 8, None))
